@@ -1,0 +1,188 @@
+// Internal declarations of libflgpu.so (gfx950).  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/flgpu.h"
+
+namespace flgpu {
+
+// ---- errors ---------------------------------------------------------------
+
+void set_error(const std::string &msg);
+void clear_error();
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define FLGPU_HIP(call)                                                        \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      throw ::flgpu::Error{-2, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define FLGPU_REQUIRE(cond, msg)                                               \
+  do {                                                                         \
+    if (!(cond)) throw ::flgpu::Error{-1, (msg)};                              \
+  } while (0)
+
+// Runs f, converting exceptions to status codes and the thread's last error.
+template <class F>
+int guard(F &&f) {
+  try {
+    clear_error();
+    return f();
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return -3;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return -4;
+  }
+}
+
+// Locks a context for the rest of the scope and makes its device current.
+#define CTX_LOCK(ctx)                                        \
+  std::lock_guard<std::recursive_mutex> lk_((ctx)->lock);    \
+  FLGPU_HIP(hipSetDevice((ctx)->device))
+
+// ---- device buffers ---------------------------------------------------------
+
+void *dev_alloc(size_t bytes);
+void dev_free(void *p);
+inline size_t prec_bytes(int prec) { return prec == FLGPU_F64 ? 8 : 4; }
+
+// ---- launch geometry --------------------------------------------------------
+
+constexpr int kWave = 64;
+constexpr int kSlice = 64;  // SELL slice height = one wavefront
+
+inline unsigned grid_for(long n, int block) {
+  long g = (n + block - 1) / block;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+// Remaps the hardware block index so that consecutive logical blocks run on
+// the same XCD: blocks are dealt round-robin over the 8 XCDs, so block b runs
+// on XCD b % 8; logical = (b % 8) * (G8/8) + b / 8 keeps contiguous row
+// ranges (and their shared vector neighbours) in one XCD's L2.  Logical
+// indices beyond the work are skipped by the caller.
+// The grid must be launched with a multiple of 8 blocks (xcd_grid) so that
+// the map is a bijection of [0, gridDim.x).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G8) {
+  unsigned per = G8 / 8u;
+  return (b % 8u) * per + b / 8u;
+}
+inline unsigned xcd_grid(unsigned g) { return (g + 7u) & ~7u; }
+
+}  // namespace flgpu
+
+// ---- handles ------------------------------------------------------------------
+
+struct flgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::recursive_mutex lock;
+  int cus = 0;
+  double *red_partial = nullptr;  // device, reduction partials
+  double *red_out = nullptr;      // device, reduction results (64 slots)
+  double *host_out = nullptr;     // pinned host copy of red_out
+  std::string name;
+};
+
+struct flgpu_vec {
+  flgpu_ctx *ctx = nullptr;
+  long n = 0;
+  int prec = FLGPU_F64;
+  void *d = nullptr;
+};
+
+// SELL-64: rows grouped in slices of 64 (one wavefront), entries of a slice
+// stored column-major (entry k of lane l at slice_ptr[s] + 64 k + l) so that a
+// wavefront reads 64 consecutive values per step.  `rows` maps slice lanes to
+// matrix rows (nullptr = identity); padding has value 0 and a valid column.
+struct Sell {
+  int nrows = 0;
+  int nslices = 0;
+  int *rows = nullptr;
+  long *slice_ptr = nullptr;
+  int *col = nullptr;
+  void *val = nullptr;
+  long nstored = 0;
+  int maxw = 0;
+};
+
+struct flgpu_mat {
+  flgpu_ctx *ctx = nullptr;
+  int nrows = 0, ncols = 0;
+  long nnz = 0;
+  int prec = FLGPU_F64;
+  int *rowptr = nullptr;  // device CSR
+  int *colidx = nullptr;
+  void *vals = nullptr;
+  int *diagpos = nullptr;  // CSR position of the diagonal (-1 if absent)
+  void *diag = nullptr;    // diagonal values (prec)
+  Sell sell;
+  int ncolours = 0;
+  std::vector<Sell> colour_sell;
+  flgpu_mat *transpose = nullptr;
+  // lattice description (0 if the matrix was not assembled on a lattice)
+  int lat_dim = 0, lat_order = 0, lat_n = 0;
+  double lat_f = 1.0;
+};
+
+namespace flgpu {
+
+// matrices (flgpu_sparse.hip)
+flgpu_mat *mat_new(flgpu_ctx *ctx, int nrows, int ncols, long nnz, int prec);
+void mat_free(flgpu_mat *A);
+void mat_finish(flgpu_mat *A);  // SELL + diagonal from the device CSR
+void mat_refresh_values(flgpu_mat *A);  // SELL/diag/colour values after a CSR value change
+void mat_build_transpose(const flgpu_mat *A);
+flgpu_mat *mat_convert(const flgpu_mat *A, int prec);
+void sell_free(Sell &s);
+Sell sell_build_rows(flgpu_mat *A, const std::vector<int> &rows);
+void set_colouring(flgpu_mat *A, int ncolours, const int *colour_host);
+
+// out[row] = beta*z[row] + alpha*(A x)[row]  (z may be out; beta == 0 ignores z)
+void spmv(const flgpu_mat *A, double alpha, const void *x, double beta,
+          const void *z, void *out);
+void spmv_t(const flgpu_mat *A, double alpha, const void *x, double beta,
+            void *out);
+void jacobi_iterate(const flgpu_mat *A, double damp, void *x, void *r);
+void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x,
+               const void *b, void *r);
+
+// lattices (flgpu_lattice.hip)
+flgpu_mat *lattice_assemble(flgpu_ctx *ctx, int dim, int order, int n, const flgpu_vec *kappa,
+                            int nk, double f, int prec, int mode, flgpu_vec **rhs);
+flgpu_mat *lattice_interpolation(flgpu_ctx *ctx, int dim, int order, int nc, int prec);
+void lattice_colouring(flgpu_mat *A, int dim, int order, int n);
+
+// vectors (flgpu_core.hip)
+flgpu_vec *vec_new(flgpu_ctx *ctx, long n, int prec);
+void vec_free(flgpu_vec *v);
+void vfill(flgpu_ctx *ctx, long n, int prec, void *x, double value);
+void vcopy(flgpu_ctx *ctx, long n, int sprec, const void *src, int dprec,
+           void *dst);
+void vscal(flgpu_ctx *ctx, long n, int prec, double a, void *x);
+void vaxpy(flgpu_ctx *ctx, long n, int prec, double a, const void *x, void *y);
+// result stays on device in ctx->red_out[slot]; dot_host syncs and returns it
+void vdot_dev(flgpu_ctx *ctx, long n, int prec, const void *x, const void *y,
+              int slot);
+double vdot_host(flgpu_ctx *ctx, long n, int prec, const void *x,
+                 const void *y);
+void fetch_slots(flgpu_ctx *ctx, int nslots);  // red_out[0..n) -> host_out
+
+}  // namespace flgpu

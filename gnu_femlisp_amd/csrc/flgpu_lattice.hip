@@ -1,0 +1,819 @@
+// Structured lattices: femlisp's sparsity pattern, cell assembly with
+// Dirichlet elimination, and the constrained :local interpolation, all built
+// on the GPU for uniformly refined box meshes of the unit cube.
+//
+// Restated femlisp sources:
+//   DOF keys / canonical order  src/discretization/sparseas.lisp:46-59,
+//                               src/mesh/meshgen.lisp:200-217 (midpoint order)
+//   pattern                     src/discretization/fedisc.lisp:144-163
+//   element kernel              src/discretization/ellsys-fe.lisp:68-127,183-190,
+//                               src/discretization/fe.lisp:235-244,502-535
+//   quadrature                  src/discretization/quadrature.lisp:149-215
+//   Lagrange basis              src/discretization/lagrange.lisp:41-202
+//   Dirichlet elimination       src/discretization/constraints.lisp:323-411,
+//                               src/discretization/system-fe.lisp:37-129
+//   interpolation               src/discretization/sparseas.lisp:232-272,
+//                               fetransfer.lisp:73-138, constraints.lisp:58-72
+//
+// Layout: a level with n cells per axis and Lagrange order p has M = p n + 1
+// DOFs per axis, DOF index = i_0 M^(d-1) + ... + i_(d-1) (coordinate 0 most
+// significant, femlisp's sort-lexicographically order of DOF midpoints).
+// Every per-axis relation (which DOFs share a cell, interpolation weights) is
+// a 1-D table; d-dimensional rows are tensor products of those tables, so row
+// pointers have a closed form and no scan is needed.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "flgpu_internal.h"
+
+namespace flgpu {
+
+constexpr int kMaxDim = 3;
+
+// ---- host-side reference element ----------------------------------------------
+
+// s-point Gauss rule on [0,1] in femlisp's order: Legendre roots ascending,
+// points (1 - x_i)/2, weights w_i / 2 (quadrature.lisp:149-176).
+static void gauss_rule(int s, double *pts, double *wts) {
+  for (int i = 0; i < s; i++) {
+    double x = -std::cos(M_PI * (i + 0.75) / (s + 0.5));
+    double p = 0, dp = 0;
+    for (int it = 0; it < 100; it++) {
+      double p0 = 1.0, p1 = x;
+      for (int k = 2; k <= s; k++) {
+        double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      p = (s == 1) ? x : p1;
+      double pm1 = (s == 1) ? 1.0 : p0;
+      dp = s * (x * p - pm1) / (x * x - 1.0);
+      double dx = p / dp;
+      x -= dx;
+      if (std::fabs(dx) < 1e-17) break;
+    }
+    {
+      double p0 = 1.0, p1 = x;
+      for (int k = 2; k <= s; k++) {
+        double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      p = (s == 1) ? x : p1;
+      double pm1 = (s == 1) ? 1.0 : p0;
+      dp = s * (x * p - pm1) / (x * x - 1.0);
+    }
+    pts[i] = (1.0 - x) / 2.0;
+    wts[i] = (2.0 / ((1.0 - x * x) * dp * dp)) / 2.0;
+  }
+}
+
+// uniform Lagrange basis of order p on [0,1] and its derivative
+static void lagrange1d(int p, double t, double *phi, double *dphi) {
+  for (int a = 0; a <= p; a++) {
+    double ta = (double)a / p, v = 1.0, d = 0.0;
+    for (int b = 0; b <= p; b++)
+      if (b != a) v *= (t - (double)b / p) / (ta - (double)b / p);
+    for (int m = 0; m <= p; m++) {
+      if (m == a) continue;
+      double term = 1.0 / (ta - (double)m / p);
+      for (int b = 0; b <= p; b++)
+        if (b != a && b != m) term *= (t - (double)b / p) / (ta - (double)b / p);
+      d += term;
+    }
+    phi[a] = v;
+    if (dphi) dphi[a] = d;
+  }
+}
+
+static long ipow(long b, int e) {
+  long r = 1;
+  while (e-- > 0) r *= b;
+  return r;
+}
+
+struct RefElement {
+  int dim, order, s, nq, nl1, nloc, nloc2;
+  double pts[8], wts[8];
+  std::vector<double> T;  // [nq][nloc*nloc]: w_q sum_m dphi_a/dx_m dphi_b/dx_m (unit cube)
+  std::vector<double> m;  // [nloc]: sum_q w_q phi_a(q)  (load vector of f = 1)
+};
+
+static RefElement ref_element(int dim, int order) {
+  RefElement R;
+  R.dim = dim;
+  R.order = order;
+  R.s = order + 1;  // quadrature-rule: (1+ order) points per axis
+  R.nq = (int)ipow(R.s, dim);
+  R.nl1 = order + 1;
+  R.nloc = (int)ipow(R.nl1, dim);
+  R.nloc2 = R.nloc * R.nloc;
+  gauss_rule(R.s, R.pts, R.wts);
+  double phi[8][8], dphi[8][8];
+  for (int q = 0; q < R.s; q++) lagrange1d(order, R.pts[q], phi[q], dphi[q]);
+  R.T.assign((size_t)R.nq * R.nloc2, 0.0);
+  R.m.assign(R.nloc, 0.0);
+  std::vector<double> g((size_t)R.nloc * kMaxDim), v(R.nloc);
+  for (int q = 0; q < R.nq; q++) {
+    int qc[kMaxDim];
+    for (int k = dim - 1, t = q; k >= 0; k--, t /= R.s) qc[k] = t % R.s;
+    double wq = 1.0;
+    for (int k = 0; k < dim; k++) wq *= R.wts[qc[k]];
+    for (int a = 0; a < R.nloc; a++) {
+      int ac[kMaxDim];
+      for (int k = dim - 1, t = a; k >= 0; k--, t /= R.nl1) ac[k] = t % R.nl1;
+      double val = 1.0;
+      for (int k = 0; k < dim; k++) val *= phi[qc[k]][ac[k]];
+      v[a] = val;
+      for (int mm = 0; mm < dim; mm++) {
+        double gg = 1.0;
+        for (int k = 0; k < dim; k++) gg *= (k == mm) ? dphi[qc[k]][ac[k]] : phi[qc[k]][ac[k]];
+        g[(size_t)a * kMaxDim + mm] = gg;
+      }
+    }
+    for (int a = 0; a < R.nloc; a++) {
+      R.m[a] += wq * v[a];
+      for (int b = 0; b < R.nloc; b++) {
+        double gg = 0.0;
+        for (int mm = 0; mm < dim; mm++) gg += g[(size_t)a * kMaxDim + mm] * g[(size_t)b * kMaxDim + mm];
+        R.T[(size_t)q * R.nloc2 + (size_t)a * R.nloc + b] = wq * gg;
+      }
+    }
+  }
+  return R;
+}
+
+// ---- 1-D tables --------------------------------------------------------------------
+
+// Per-axis relation as CSR: for each row coordinate, ascending column
+// coordinates and weights.
+struct Table1D {
+  int nr = 0, nc = 0;
+  std::vector<int> ptr, col;
+  std::vector<double> w;
+};
+
+// DOFs sharing a cell along one axis (the pattern)
+static Table1D table_pattern(int order, int n) {
+  Table1D t;
+  int M = order * n + 1;
+  t.nr = t.nc = M;
+  t.ptr.push_back(0);
+  for (int i = 0; i < M; i++) {
+    int cmin, cmax;
+    if (i % order == 0) {
+      cmin = std::max(i / order - 1, 0);
+      cmax = std::min(i / order, n - 1);
+    } else {
+      cmin = cmax = i / order;
+    }
+    for (int j = order * cmin; j <= order * cmax + order; j++) {
+      t.col.push_back(j);
+      t.w.push_back(1.0);
+    }
+    t.ptr.push_back((int)t.col.size());
+  }
+  return t;
+}
+
+// 1-D interpolation, coarse nc cells -> fine 2 nc cells: coarse basis at the
+// fine node, entries below 1e-10 dropped (fetransfer.lisp:41-43,120-130)
+static Table1D table_interp(int order, int nc) {
+  Table1D t;
+  int Mf = 2 * order * nc + 1, Mc = order * nc + 1;
+  t.nr = Mf;
+  t.nc = Mc;
+  t.ptr.push_back(0);
+  for (int F = 0; F < Mf; F++) {
+    int cell = std::min(F / (2 * order), nc - 1);
+    double tau = (double)(F - 2 * order * cell) / (2.0 * order);
+    double phi[8];
+    lagrange1d(order, tau, phi, nullptr);
+    for (int a = 0; a <= order; a++) {
+      if (std::fabs(phi[a]) < 1e-10) continue;
+      t.col.push_back(order * cell + a);
+      t.w.push_back(phi[a]);
+    }
+    t.ptr.push_back((int)t.col.size());
+  }
+  return t;
+}
+
+static Table1D table_transpose(const Table1D &t) {
+  Table1D r;
+  r.nr = t.nc;
+  r.nc = t.nr;
+  r.ptr.assign(t.nc + 1, 0);
+  for (int c : t.col) r.ptr[c + 1]++;
+  for (int c = 0; c < t.nc; c++) r.ptr[c + 1] += r.ptr[c];
+  r.col.resize(t.col.size());
+  r.w.resize(t.col.size());
+  std::vector<int> fill(r.ptr.begin(), r.ptr.end() - 1);
+  for (int i = 0; i < t.nr; i++)
+    for (int e = t.ptr[i]; e < t.ptr[i + 1]; e++) {
+      int p = fill[t.col[e]]++;
+      r.col[p] = i;
+      r.w[p] = t.w[e];
+    }
+  return r;
+}
+
+struct DevTable {
+  int nr, nc, nnz;
+  int *ptr, *col;
+  double *w;
+};
+
+static DevTable upload_table(flgpu_ctx *ctx, const Table1D &t) {
+  DevTable d;
+  d.nr = t.nr;
+  d.nc = t.nc;
+  d.nnz = (int)t.col.size();
+  d.ptr = (int *)dev_alloc(sizeof(int) * t.ptr.size());
+  d.col = (int *)dev_alloc(sizeof(int) * std::max<size_t>(t.col.size(), 1));
+  d.w = (double *)dev_alloc(sizeof(double) * std::max<size_t>(t.w.size(), 1));
+  FLGPU_HIP(hipMemcpyAsync(d.ptr, t.ptr.data(), sizeof(int) * t.ptr.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+  if (!t.col.empty()) {
+    FLGPU_HIP(hipMemcpyAsync(d.col, t.col.data(), sizeof(int) * t.col.size(),
+                             hipMemcpyHostToDevice, ctx->stream));
+    FLGPU_HIP(hipMemcpyAsync(d.w, t.w.data(), sizeof(double) * t.w.size(), hipMemcpyHostToDevice,
+                             ctx->stream));
+  }
+  return d;
+}
+
+static void free_table(DevTable &d) {
+  dev_free(d.ptr);
+  dev_free(d.col);
+  dev_free(d.w);
+}
+
+// ---- tensor-product CSR ----------------------------------------------------------
+
+// rowptr of row (i_0..i_{d-1}) = sum_k (prod_{l<k} W(i_l)) * P(i_k) * nnz1^(d-1-k)
+// with W(i) = ptr[i+1]-ptr[i], P(i) = ptr[i] (prefix of W), nnz1 = ptr[nr].
+__global__ void k_tp_rowptr(long N, int dim, DevTable t, int *rowptr) {
+  long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > N) return;
+  if (r == N) {
+    long tot = 1;
+    for (int k = 0; k < dim; k++) tot *= t.nnz;
+    rowptr[N] = (int)tot;
+    return;
+  }
+  int c[kMaxDim];
+  long q = r;
+  for (int k = dim - 1; k >= 0; k--) {
+    c[k] = (int)(q % t.nr);
+    q /= t.nr;
+  }
+  long off = 0, pre = 1;
+  for (int k = 0; k < dim; k++) {
+    long post = 1;
+    for (int l = k + 1; l < dim; l++) post *= t.nnz;
+    off += pre * t.ptr[c[k]] * post;
+    pre *= (t.ptr[c[k] + 1] - t.ptr[c[k]]);
+  }
+  rowptr[r] = (int)off;
+}
+
+// Fills columns (and values = product of 1-D weights) of every row.
+// zero_rule: 0 = keep values; 1 = interpolation (zero rows whose row
+// coordinate is on the boundary and columns on the boundary).
+template <class T>
+__global__ void k_tp_fill(long N, int dim, DevTable t, const int *rowptr, int *colidx, T *vals,
+                          int zero_rule) {
+  long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  int c[kMaxDim], lo[kMaxDim], hi[kMaxDim], e[kMaxDim];
+  long q = r;
+  bool rb = false;
+  for (int k = dim - 1; k >= 0; k--) {
+    c[k] = (int)(q % t.nr);
+    q /= t.nr;
+    lo[k] = t.ptr[c[k]];
+    hi[k] = t.ptr[c[k] + 1];
+    e[k] = lo[k];
+    if (c[k] == 0 || c[k] == t.nr - 1) rb = true;
+  }
+  long pos = rowptr[r];
+  for (;;) {
+    long col = 0;
+    double w = 1.0;
+    bool cb = false;
+    for (int k = 0; k < dim; k++) {
+      int cc = t.col[e[k]];
+      col = col * t.nc + cc;
+      w *= t.w[e[k]];
+      if (cc == 0 || cc == t.nc - 1) cb = true;
+    }
+    colidx[pos] = (int)col;
+    if (vals) vals[pos] = (zero_rule == 1 && (rb || cb)) ? (T)0 : (T)w;
+    pos++;
+    int k = dim - 1;
+    while (k >= 0 && e[k] == hi[k] - 1) {
+      e[k] = lo[k];
+      k--;
+    }
+    if (k < 0) break;
+    e[k]++;
+  }
+}
+
+// ---- coefficient sampling -----------------------------------------------------------
+
+// S[c][q] = H^(d-2) kappa(x_q): the quadrature point g of cell c samples the
+// coefficient cell c*r + floor(g*r) per axis, r = nk/n.
+__global__ void k_coef(long ncells, int dim, int n, int s, int nq, const double *__restrict__ g,
+                       const double *__restrict__ kappa, int nk, double scale, double *S) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells * nq) return;
+  long cell = t / nq;
+  int q = (int)(t % nq);
+  int cc[kMaxDim], qc[kMaxDim];
+  long cq = cell;
+  for (int k = dim - 1; k >= 0; k--) {
+    cc[k] = (int)(cq % n);
+    cq /= n;
+  }
+  for (int k = dim - 1, u = q; k >= 0; k--, u /= s) qc[k] = u % s;
+  double kap = 1.0;
+  if (kappa) {
+    int r = nk / n;
+    long idx = 0;
+    for (int k = 0; k < dim; k++) idx = idx * nk + (long)cc[k] * r + (int)floor(g[qc[k]] * r);
+    kap = kappa[idx];
+  }
+  S[t] = scale * kap;
+}
+
+// ---- element contractions E[c][ab] = sum_q S[c][q] T[q][ab] ---------------------------
+
+template <class TE>
+__global__ void k_elem_fma(long ncells, int nq, int nloc2, const double *__restrict__ S,
+                           const double *__restrict__ Tt, TE *E) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncells * nloc2) return;
+  long cell = t / nloc2;
+  int ab = (int)(t % nloc2);
+  double acc = 0.0;
+  for (int q = 0; q < nq; q++) acc += S[cell * nq + q] * Tt[(long)q * nloc2 + ab];
+  E[t] = (TE)acc;
+}
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+// FP64 MFMA 16x16x4: one wavefront computes a 16-cell x 16-entry tile per
+// N-step.  Operand layout of v_mfma_f64_16x16x4f64: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16]; it receives D[4(l/16)+v][l%16], v = 0..3.
+__global__ __launch_bounds__(256) void k_elem_mfma_f64(long ncells, int nq, int nloc2,
+                                                       const double *__restrict__ S,
+                                                       const double *__restrict__ Tt, double *E) {
+  long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  long c0 = tile * 16;
+  if (c0 >= ncells) return;
+  int lane = threadIdx.x & 63;
+  int i = lane & 15, kq = lane >> 4;
+  long arow = c0 + i;
+  for (int n0 = 0; n0 < nloc2; n0 += 16) {
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    int bcol = n0 + i;
+    for (int k0 = 0; k0 < nq; k0 += 4) {
+      int k = k0 + kq;
+      double a = (arow < ncells && k < nq) ? S[arow * nq + k] : 0.0;
+      double b = (k < nq && bcol < nloc2) ? Tt[(long)k * nloc2 + bcol] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    for (int v = 0; v < 4; v++) {
+      long row = c0 + 4 * kq + v;
+      if (row < ncells && bcol < nloc2) E[row * nloc2 + bcol] = acc[v];
+    }
+  }
+}
+
+__device__ __forceinline__ short f2bf(double x) {
+  float f = (float)x;
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (short)(u >> 16);
+}
+
+// BF16 MFMA 16x16x16 (FP32 accumulate): lane l supplies A[l%16][4(l/16)+v]
+// and B[4(l/16)+v][l%16] (v = 0..3); it receives D[4(l/16)+v][l%16].
+__global__ __launch_bounds__(256) void k_elem_mfma_bf16(long ncells, int nq, int nloc2,
+                                                        const double *__restrict__ S,
+                                                        const double *__restrict__ Tt, float *E) {
+  long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  long c0 = tile * 16;
+  if (c0 >= ncells) return;
+  int lane = threadIdx.x & 63;
+  int i = lane & 15, kg = lane >> 4;
+  long arow = c0 + i;
+  for (int n0 = 0; n0 < nloc2; n0 += 16) {
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    int bcol = n0 + i;
+    for (int k0 = 0; k0 < nq; k0 += 16) {
+      v4s a, b;
+      for (int v = 0; v < 4; v++) {
+        int k = k0 + 4 * kg + v;
+        a[v] = f2bf((arow < ncells && k < nq) ? S[arow * nq + k] : 0.0);
+        b[v] = f2bf((k < nq && bcol < nloc2) ? Tt[(long)k * nloc2 + bcol] : 0.0);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    for (int v = 0; v < 4; v++) {
+      long row = c0 + 4 * kg + v;
+      if (row < ncells && bcol < nloc2) E[row * nloc2 + bcol] = acc[v];
+    }
+  }
+}
+
+// ---- gather (assembly of the global matrix from element matrices) -------------------
+
+// One thread per row: entry (P,Q) = sum over cells containing both DOFs, in
+// ascending cell order, of E[cell][loc(P)][loc(Q)]; then Dirichlet
+// elimination (identity on constrained rows, zeros in their columns) and the
+// load vector rhs_P = f H^d sum_cells m[loc(P)].
+template <class T, class TE>
+__global__ void k_gather(long N, int dim, int order, int n, DevTable pat, const int *rowptr,
+                         const TE *__restrict__ E, int nloc, T *vals, double rhs_scale,
+                         const double *__restrict__ mload, T *rhs) {
+  long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  int M = pat.nr, nl1 = order + 1;
+  int P[kMaxDim], lo[kMaxDim], hi[kMaxDim], e[kMaxDim];
+  long q = r;
+  bool rb = false;
+  for (int k = dim - 1; k >= 0; k--) {
+    P[k] = (int)(q % M);
+    q /= M;
+    lo[k] = pat.ptr[P[k]];
+    hi[k] = pat.ptr[P[k] + 1];
+    e[k] = lo[k];
+    if (P[k] == 0 || P[k] == M - 1) rb = true;
+  }
+  long nloc2 = (long)nloc * nloc;
+  long pos = rowptr[r];
+  for (;;) {
+    int Q[kMaxDim], clo[kMaxDim], chi[kMaxDim], c[kMaxDim];
+    bool cb = false, diag = true;
+    for (int k = 0; k < dim; k++) {
+      Q[k] = pat.col[e[k]];
+      if (Q[k] == 0 || Q[k] == M - 1) cb = true;
+      if (Q[k] != P[k]) diag = false;
+      int mx = max(P[k], Q[k]), mn = min(P[k], Q[k]);
+      // cells c with order*c <= mn and mx <= order*c + order
+      clo[k] = (mx <= order) ? 0 : (mx - 1) / order;
+      chi[k] = min(mn / order, n - 1);
+      c[k] = clo[k];
+    }
+    double acc = 0.0;
+    if (!(rb || cb)) {
+      for (;;) {
+        long cell = 0;
+        int la = 0, lb = 0;
+        for (int k = 0; k < dim; k++) {
+          cell = cell * n + c[k];
+          la = la * nl1 + (P[k] - order * c[k]);
+          lb = lb * nl1 + (Q[k] - order * c[k]);
+        }
+        acc += (double)E[cell * nloc2 + (long)la * nloc + lb];
+        int k = dim - 1;
+        while (k >= 0 && c[k] == chi[k]) {
+          c[k] = clo[k];
+          k--;
+        }
+        if (k < 0) break;
+        c[k]++;
+      }
+    } else {
+      acc = (rb && diag) ? 1.0 : 0.0;
+    }
+    vals[pos++] = (T)acc;
+    int k = dim - 1;
+    while (k >= 0 && e[k] == hi[k] - 1) {
+      e[k] = lo[k];
+      k--;
+    }
+    if (k < 0) break;
+    e[k]++;
+  }
+  if (rhs) {
+    double b = 0.0;
+    if (!rb) {
+      int clo[kMaxDim], chi[kMaxDim], c[kMaxDim];
+      for (int k = 0; k < dim; k++) {
+        clo[k] = (P[k] % order == 0) ? max(P[k] / order - 1, 0) : P[k] / order;
+        chi[k] = (P[k] % order == 0) ? min(P[k] / order, n - 1) : P[k] / order;
+        c[k] = clo[k];
+      }
+      for (;;) {
+        int la = 0;
+        for (int k = 0; k < dim; k++) la = la * nl1 + (P[k] - order * c[k]);
+        b += mload[la];
+        int k = dim - 1;
+        while (k >= 0 && c[k] == chi[k]) {
+          c[k] = clo[k];
+          k--;
+        }
+        if (k < 0) break;
+        c[k]++;
+      }
+      b *= rhs_scale;
+    }
+    rhs[r] = (T)b;
+  }
+}
+
+// ---- drivers ---------------------------------------------------------------------------
+
+struct LatticeBuild {
+  RefElement R;
+  Table1D pat1;
+};
+
+static void check_lattice_args(int dim, int order, int n) {
+  FLGPU_REQUIRE(dim >= 1 && dim <= 3, "dim must be 1, 2 or 3");
+  FLGPU_REQUIRE(order == 1 || order == 2, "order must be 1 or 2");
+  FLGPU_REQUIRE(n >= 1, "n must be >= 1");
+  long N = ipow((long)order * n + 1, dim);
+  long nnz = ipow((long)(order == 1 ? 3 * n + 1 : 8 * n + 1), dim);
+  FLGPU_REQUIRE(N < 0x7fffffffL && nnz < 0x7fffffffL, "lattice too large for int32 indices");
+}
+
+static void check_kappa(const flgpu_vec *kappa, int dim, int n, int nk) {
+  if (!kappa) return;
+  FLGPU_REQUIRE(kappa->prec == FLGPU_F64, "kappa must be an FP64 vector");
+  FLGPU_REQUIRE(nk >= n && nk % n == 0, "nk must be a multiple of n");
+  FLGPU_REQUIRE(kappa->n == ipow(nk, dim), "kappa must hold nk^dim values");
+}
+
+// element matrices of all cells of the level into E (TE = double or float)
+static void element_matrices(flgpu_ctx *ctx, const RefElement &R, int n, const flgpu_vec *kappa,
+                             int nk, int mode, int prec, void *E) {
+  int dim = R.dim;
+  long ncells = ipow(n, dim);
+  double H = 1.0 / n;
+  double scale = std::pow(H, dim - 2);
+  double *dS = (double *)dev_alloc(sizeof(double) * ncells * R.nq);
+  double *dT = (double *)dev_alloc(sizeof(double) * R.T.size());
+  double *dg = (double *)dev_alloc(sizeof(double) * R.s);
+  FLGPU_HIP(hipMemcpyAsync(dT, R.T.data(), sizeof(double) * R.T.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+  FLGPU_HIP(hipMemcpyAsync(dg, R.pts, sizeof(double) * R.s, hipMemcpyHostToDevice, ctx->stream));
+  k_coef<<<grid_for(ncells * R.nq, 256), 256, 0, ctx->stream>>>(
+      ncells, dim, n, R.s, R.nq, dg, kappa ? (const double *)kappa->d : nullptr, nk, scale, dS);
+  FLGPU_HIP(hipGetLastError());
+  if (mode == FLGPU_ASM_MFMA) {
+    unsigned g = grid_for((ncells + 15) / 16, 4);
+    if (prec == FLGPU_F64)
+      k_elem_mfma_f64<<<g, 256, 0, ctx->stream>>>(ncells, R.nq, R.nloc2, dS, dT, (double *)E);
+    else
+      k_elem_mfma_bf16<<<g, 256, 0, ctx->stream>>>(ncells, R.nq, R.nloc2, dS, dT, (float *)E);
+  } else {
+    if (prec == FLGPU_F64)
+      k_elem_fma<double><<<grid_for(ncells * R.nloc2, 256), 256, 0, ctx->stream>>>(
+          ncells, R.nq, R.nloc2, dS, dT, (double *)E);
+    else
+      k_elem_fma<float><<<grid_for(ncells * R.nloc2, 256), 256, 0, ctx->stream>>>(
+          ncells, R.nq, R.nloc2, dS, dT, (float *)E);
+  }
+  FLGPU_HIP(hipGetLastError());
+  FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+  dev_free(dS);
+  dev_free(dT);
+  dev_free(dg);
+}
+
+// values (and rhs) of a lattice matrix whose pattern is already built
+static void assemble_values(flgpu_mat *A, const flgpu_vec *kappa, int nk, int mode,
+                            void *rhs_dev) {
+  flgpu_ctx *ctx = A->ctx;
+  int dim = A->lat_dim, order = A->lat_order, n = A->lat_n;
+  RefElement R = ref_element(dim, order);
+  Table1D pat = table_pattern(order, n);
+  long ncells = ipow(n, dim);
+  // E in FP64 for the FP64 path, FP32 for the FP32 (BF16-MFMA) path
+  int eprec = A->prec;
+  void *E = dev_alloc(prec_bytes(eprec) * ncells * R.nloc2);
+  element_matrices(ctx, R, n, kappa, nk, mode, eprec, E);
+  DevTable dpat = upload_table(ctx, pat);
+  double *dm = (double *)dev_alloc(sizeof(double) * R.nloc);
+  FLGPU_HIP(hipMemcpyAsync(dm, R.m.data(), sizeof(double) * R.nloc, hipMemcpyHostToDevice,
+                           ctx->stream));
+  double rscale = A->lat_f * std::pow(1.0 / n, dim);
+  long N = A->nrows;
+  if (A->prec == FLGPU_F64)
+    k_gather<double, double><<<grid_for(N, 128), 128, 0, ctx->stream>>>(
+        N, dim, order, n, dpat, A->rowptr, (const double *)E, R.nloc, (double *)A->vals, rscale,
+        dm, (double *)rhs_dev);
+  else
+    k_gather<float, float><<<grid_for(N, 128), 128, 0, ctx->stream>>>(
+        N, dim, order, n, dpat, A->rowptr, (const float *)E, R.nloc, (float *)A->vals, rscale, dm,
+        (float *)rhs_dev);
+  FLGPU_HIP(hipGetLastError());
+  FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+  free_table(dpat);
+  dev_free(dm);
+  dev_free(E);
+}
+
+flgpu_mat *lattice_pattern(flgpu_ctx *ctx, int dim, int order, int n, int prec) {
+  check_lattice_args(dim, order, n);
+  Table1D pat = table_pattern(order, n);
+  long N = ipow(pat.nr, dim);
+  long nnz = ipow((long)pat.col.size(), dim);
+  flgpu_mat *A = mat_new(ctx, (int)N, (int)N, nnz, prec);
+  A->lat_dim = dim;
+  A->lat_order = order;
+  A->lat_n = n;
+  DevTable d = upload_table(ctx, pat);
+  k_tp_rowptr<<<grid_for(N + 1, 256), 256, 0, ctx->stream>>>(N, dim, d, A->rowptr);
+  k_tp_fill<double><<<grid_for(N, 128), 128, 0, ctx->stream>>>(N, dim, d, A->rowptr, A->colidx,
+                                                               nullptr, 0);
+  FLGPU_HIP(hipGetLastError());
+  FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+  free_table(d);
+  return A;
+}
+
+flgpu_mat *lattice_assemble(flgpu_ctx *ctx, int dim, int order, int n, const flgpu_vec *kappa,
+                            int nk, double f, int prec, int mode, flgpu_vec **rhs) {
+  check_lattice_args(dim, order, n);
+  check_kappa(kappa, dim, n, nk);
+  FLGPU_REQUIRE(mode == FLGPU_ASM_F64 || mode == FLGPU_ASM_MFMA, "unknown assembly mode");
+  flgpu_mat *A = lattice_pattern(ctx, dim, order, n, prec);
+  A->lat_f = f;
+  flgpu_vec *b = nullptr;
+  try {
+    if (rhs) b = vec_new(ctx, A->nrows, prec);
+    assemble_values(A, kappa, nk, mode, b ? b->d : nullptr);
+    mat_finish(A);
+  } catch (...) {
+    vec_free(b);
+    mat_free(A);
+    throw;
+  }
+  if (rhs) *rhs = b;
+  return A;
+}
+
+// lattice colouring: colour = per-axis (coordinate mod (order+1)) digits
+void lattice_colouring(flgpu_mat *A, int dim, int order, int n) {
+  int M = order * n + 1, m = order + 1;
+  int ncol = (int)ipow(m, dim);
+  std::vector<int> colour(A->nrows);
+  for (long r = 0; r < A->nrows; r++) {
+    long q = r;
+    int cc = 0, mul = 1;
+    for (int k = dim - 1; k >= 0; k--) {
+      cc += (int)((q % M) % m) * mul;
+      mul *= m;
+      q /= M;
+    }
+    colour[r] = cc;
+  }
+  std::vector<std::vector<int>> rows(ncol);
+  for (long r = 0; r < A->nrows; r++) rows[colour[r]].push_back((int)r);
+  // rows of one lattice colour never share a cell, so the sweep over a
+  // colour is order independent; no coupling check needed
+  for (auto &s : A->colour_sell) sell_free(s);
+  A->colour_sell.clear();
+  for (int c = 0; c < ncol; c++) A->colour_sell.push_back(sell_build_rows(A, rows[c]));
+  A->ncolours = ncol;
+}
+
+flgpu_mat *lattice_interpolation(flgpu_ctx *ctx, int dim, int order, int nc, int prec) {
+  check_lattice_args(dim, order, 2 * nc);
+  Table1D t = table_interp(order, nc);
+  Table1D tt = table_transpose(t);
+  long Nf = ipow(t.nr, dim), Nc = ipow(t.nc, dim);
+  long nnz = ipow((long)t.col.size(), dim);
+  flgpu_mat *P = mat_new(ctx, (int)Nf, (int)Nc, nnz, prec);
+  flgpu_mat *PT = nullptr;
+  try {
+    DevTable d = upload_table(ctx, t);
+    k_tp_rowptr<<<grid_for(Nf + 1, 256), 256, 0, ctx->stream>>>(Nf, dim, d, P->rowptr);
+    if (prec == FLGPU_F64)
+      k_tp_fill<double><<<grid_for(Nf, 128), 128, 0, ctx->stream>>>(Nf, dim, d, P->rowptr,
+                                                                    P->colidx, (double *)P->vals, 1);
+    else
+      k_tp_fill<float><<<grid_for(Nf, 128), 128, 0, ctx->stream>>>(Nf, dim, d, P->rowptr, P->colidx,
+                                                                   (float *)P->vals, 1);
+    FLGPU_HIP(hipGetLastError());
+    // the transpose directly from the transposed 1-D table: rows = coarse
+    // DOFs, columns = fine DOFs; the zero rule is symmetric in rows/columns
+    PT = mat_new(ctx, (int)Nc, (int)Nf, nnz, prec);
+    DevTable dt = upload_table(ctx, tt);
+    k_tp_rowptr<<<grid_for(Nc + 1, 256), 256, 0, ctx->stream>>>(Nc, dim, dt, PT->rowptr);
+    if (prec == FLGPU_F64)
+      k_tp_fill<double><<<grid_for(Nc, 128), 128, 0, ctx->stream>>>(
+          Nc, dim, dt, PT->rowptr, PT->colidx, (double *)PT->vals, 1);
+    else
+      k_tp_fill<float><<<grid_for(Nc, 128), 128, 0, ctx->stream>>>(Nc, dim, dt, PT->rowptr,
+                                                                   PT->colidx, (float *)PT->vals, 1);
+    FLGPU_HIP(hipGetLastError());
+    FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+    free_table(d);
+    free_table(dt);
+    mat_finish(P);
+    mat_finish(PT);
+    P->transpose = PT;
+  } catch (...) {
+    mat_free(P);
+    mat_free(PT);
+    throw;
+  }
+  return P;
+}
+
+}  // namespace flgpu
+
+using namespace flgpu;
+
+extern "C" int flgpu_lattice_assemble(flgpu_ctx *ctx, int dim, int order, int n,
+                                      const flgpu_vec *kappa, int nk, double f, int prec, int mode,
+                                      flgpu_mat **A, flgpu_vec **rhs) {
+  return guard([&] {
+    FLGPU_REQUIRE(ctx && A, "ctx and A must not be NULL");
+    FLGPU_REQUIRE(!kappa || kappa->ctx == ctx, "kappa belongs to another context");
+    CTX_LOCK(ctx);
+    *A = lattice_assemble(ctx, dim, order, n, kappa, nk, f, prec, mode, rhs);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_lattice_reassemble(flgpu_mat *A, const flgpu_vec *kappa, int nk, int mode) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    FLGPU_REQUIRE(A->lat_dim > 0, "matrix was not assembled on a lattice");
+    FLGPU_REQUIRE(!kappa || kappa->ctx == A->ctx, "kappa belongs to another context");
+    check_kappa(kappa, A->lat_dim, A->lat_n, nk);
+    CTX_LOCK(A->ctx);
+    assemble_values(A, kappa, nk, mode, nullptr);
+    mat_refresh_values(A);
+    FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+    return 0;
+  });
+}
+
+extern "C" int flgpu_lattice_interpolation(flgpu_ctx *ctx, int dim, int order, int nc, int prec,
+                                           flgpu_mat **P) {
+  return guard([&] {
+    FLGPU_REQUIRE(ctx && P, "ctx and P must not be NULL");
+    CTX_LOCK(ctx);
+    *P = lattice_interpolation(ctx, dim, order, nc, prec);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_lattice_colouring(flgpu_mat *A, int dim, int order, int n) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    check_lattice_args(dim, order, n);
+    FLGPU_REQUIRE(A->nrows == ipow((long)order * n + 1, dim) && A->ncols == A->nrows,
+                  "matrix size does not match the lattice");
+    CTX_LOCK(A->ctx);
+    lattice_colouring(A, dim, order, n);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_lattice_element_matrices(flgpu_ctx *ctx, int dim, int order, int n,
+                                              const flgpu_vec *kappa, int nk, int prec, int mode,
+                                              long first_cell, long ncells, double *out) {
+  return guard([&] {
+    FLGPU_REQUIRE(ctx && out, "ctx and out must not be NULL");
+    check_lattice_args(dim, order, n);
+    check_kappa(kappa, dim, n, nk);
+    FLGPU_REQUIRE(prec == FLGPU_F64 || prec == FLGPU_F32, "precision must be F64 or F32");
+    long total = ipow(n, dim);
+    FLGPU_REQUIRE(first_cell >= 0 && ncells >= 0 && first_cell + ncells <= total,
+                  "cell range out of bounds");
+    CTX_LOCK(ctx);
+    RefElement R = ref_element(dim, order);
+    void *E = dev_alloc(prec_bytes(prec) * total * R.nloc2);
+    try {
+      element_matrices(ctx, R, n, kappa, nk, mode, prec, E);
+      long cnt = ncells * R.nloc2;
+      if (prec == FLGPU_F64) {
+        FLGPU_HIP(hipMemcpy(out, (double *)E + first_cell * R.nloc2, sizeof(double) * cnt,
+                            hipMemcpyDeviceToHost));
+      } else {
+        std::vector<float> tmp(cnt);
+        FLGPU_HIP(hipMemcpy(tmp.data(), (float *)E + first_cell * R.nloc2, sizeof(float) * cnt,
+                            hipMemcpyDeviceToHost));
+        for (long i = 0; i < cnt; i++) out[i] = tmp[i];
+      }
+    } catch (...) {
+      dev_free(E);
+      throw;
+    }
+    dev_free(E);
+    return 0;
+  });
+}

@@ -1,0 +1,650 @@
+// Geometric multigrid and the Krylov/solver loop of libflgpu.so.
+//
+// Restated femlisp sources:
+//   <mg-iteration> iterator, lmgc, CS restrict/prolongate
+//                          src/iteration/multigrid.lisp:173-288,303-393
+//   multilevel decomposition src/iteration/geomg.lisp:109-142
+//   product-iterator       src/iteration/linit.lisp:94-120
+//   <lu> coarse iterator   src/iteration/linit.lisp:132-162
+//   <cg>                   src/iteration/krylow.lisp:67-132
+//   <linear-solver> loop   src/iteration/linsolve.lisp:58-73,
+//                          src/iteration/solve.lisp:126-165,
+//                          src/iteration/iterate.lisp:239-250
+//
+// The V-cycle keeps femlisp's bookkeeping (residual-p flags per level, which
+// decide when a residual is recomputed) so that the operator applied is the
+// reference's, including its (iterate q w w) aliasing when asked for.
+// Scalars of the Krylov recurrences stay on the device; the host reads them
+// once per iteration (the defect norm femlisp's loop observes).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "flgpu_internal.h"
+#include "flgpu_learned.h"
+
+namespace flgpu {
+
+struct MGLevel {
+  flgpu_mat *A = nullptr;  // owned
+  flgpu_mat *I = nullptr;  // owned: interpolation level -> level+1 (transpose attached)
+  void *sol = nullptr, *rhs = nullptr, *res = nullptr;  // owned work vectors
+  bool residual_p = false;
+  double *cinv = nullptr;  // base level: dense inverse (FP64, row-major)
+  double *ctmp = nullptr;
+  long N = 0;
+};
+
+}  // namespace flgpu
+
+struct flgpu_mg {
+  flgpu_ctx *ctx = nullptr;
+  flgpu_mg_params p{};
+  std::vector<flgpu::MGLevel> lv;
+  int top = 0;
+  // FP64 staging when an FP32 cycle is applied to FP64 vectors
+  void *xs = nullptr, *bs = nullptr, *rs = nullptr;
+  // active top-level pointers during one cycle
+  void *sol_top = nullptr, *rhs_top = nullptr, *res_top = nullptr;
+};
+
+namespace flgpu {
+
+// ---- coarse solve: x += A^-1 r, r <- A^-1 r (FP64 dense inverse) ----------------------
+
+template <class T>
+__global__ __launch_bounds__(256) void k_dense_gemv(int n, const double *__restrict__ Ainv,
+                                                    const T *__restrict__ r, double *y) {
+  int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  int lane = threadIdx.x & 63;
+  double acc = 0.0;
+  for (int j = lane; j < n; j += 64) acc += Ainv[(long)row * n + j] * (double)r[j];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) y[row] = acc;
+}
+
+template <class T>
+__global__ void k_coarse_update(int n, const double *y, T *x, T *r) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T v = (T)y[i];
+  r[i] = v;
+  x[i] += v;
+}
+
+// Dense LU without pivoting (the coarse matrices are SPD after Dirichlet
+// elimination) and the inverse by forward/backward substitution.
+static std::vector<double> dense_inverse(int n, const std::vector<int> &rp,
+                                         const std::vector<int> &ci,
+                                         const std::vector<double> &v) {
+  std::vector<double> a((size_t)n * n, 0.0), inv((size_t)n * n, 0.0);
+  for (int r = 0; r < n; r++)
+    for (int e = rp[r]; e < rp[r + 1]; e++) a[(size_t)r * n + ci[e]] = v[e];
+  for (int k = 0; k < n; k++) {
+    double piv = a[(size_t)k * n + k];
+    FLGPU_REQUIRE(piv != 0.0, "coarse matrix is singular (zero pivot)");
+    for (int i = k + 1; i < n; i++) {
+      double l = a[(size_t)i * n + k] / piv;
+      a[(size_t)i * n + k] = l;
+      if (l != 0.0) {
+        double *ai = &a[(size_t)i * n], *ak = &a[(size_t)k * n];
+        for (int j = k + 1; j < n; j++) ai[j] -= l * ak[j];
+      }
+    }
+  }
+  std::vector<double> col(n);
+  for (int c = 0; c < n; c++) {
+    for (int i = 0; i < n; i++) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int j = 0; j < i; j++) s -= a[(size_t)i * n + j] * col[j];
+      col[i] = s;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      double s = col[i];
+      for (int j = i + 1; j < n; j++) s -= a[(size_t)i * n + j] * col[j];
+      col[i] = s / a[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; i++) inv[(size_t)i * n + c] = col[i];
+  }
+  return inv;
+}
+
+static void coarse_setup(flgpu_mg *mg, MGLevel &L) {
+  const flgpu_mat *A = L.A;
+  FLGPU_REQUIRE(A->nrows <= 8192, "base level too large for the dense coarse solve (> 8192 DOFs)");
+  int n = A->nrows;
+  std::vector<int> rp(n + 1), ci(A->nnz);
+  std::vector<double> v(A->nnz);
+  FLGPU_REQUIRE(flgpu_csr_download(A, rp.data(), ci.data(), v.data()) == 0,
+                "coarse matrix download failed");
+  std::vector<double> inv = dense_inverse(n, rp, ci, v);
+  L.cinv = (double *)dev_alloc(sizeof(double) * (size_t)n * n);
+  L.ctmp = (double *)dev_alloc(sizeof(double) * n);
+  FLGPU_HIP(hipMemcpyAsync(L.cinv, inv.data(), sizeof(double) * (size_t)n * n,
+                           hipMemcpyHostToDevice, mg->ctx->stream));
+  FLGPU_HIP(hipStreamSynchronize(mg->ctx->stream));
+}
+
+static void coarse_apply(flgpu_mg *mg, MGLevel &L, void *x, void *r) {
+  int n = (int)L.N;
+  hipStream_t s = mg->ctx->stream;
+  if (mg->p.prec == FLGPU_F64) {
+    k_dense_gemv<double><<<grid_for(n, 4), 256, 0, s>>>(n, L.cinv, (const double *)r, L.ctmp);
+    k_coarse_update<double><<<grid_for(n, 256), 256, 0, s>>>(n, L.ctmp, (double *)x, (double *)r);
+  } else {
+    k_dense_gemv<float><<<grid_for(n, 4), 256, 0, s>>>(n, L.cinv, (const float *)r, L.ctmp);
+    k_coarse_update<float><<<grid_for(n, 256), 256, 0, s>>>(n, L.ctmp, (float *)x, (float *)r);
+  }
+  FLGPU_HIP(hipGetLastError());
+}
+
+// ---- V-cycle ---------------------------------------------------------------------------------
+
+static inline void *lsol(flgpu_mg *mg, int l) { return l == mg->top ? mg->sol_top : mg->lv[l].sol; }
+static inline void *lrhs(flgpu_mg *mg, int l) { return l == mg->top ? mg->rhs_top : mg->lv[l].rhs; }
+static inline void *lres(flgpu_mg *mg, int l) { return l == mg->top ? mg->res_top : mg->lv[l].res; }
+
+// ensure-mg-residual (multigrid.lisp:173-177)
+static void ensure_residual(flgpu_mg *mg, int l) {
+  MGLevel &L = mg->lv[l];
+  if (L.residual_p) return;
+  spmv(L.A, -1.0, lsol(mg, l), 1.0, lrhs(mg, l), lres(mg, l));
+  L.residual_p = true;
+}
+
+// smooth (multigrid.lisp:179-189) with the product-iterator of the smoother
+static void smooth(flgpu_mg *mg, int l, bool post) {
+  const flgpu_mg_params &p = mg->p;
+  int steps = post ? p.post_steps : p.pre_steps;
+  if (steps <= 0) return;
+  MGLevel &L = mg->lv[l];
+  void *x = lsol(mg, l), *b = lrhs(mg, l), *r = lres(mg, l);
+  if (p.smoother == FLGPU_SMOOTH_JACOBI) {
+    ensure_residual(mg, l);  // <jacobi> has residual-before t
+    for (int k = 0; k < steps; k++) {
+      jacobi_iterate(L.A, p.omega, x, r);
+      if (steps > 1) spmv(L.A, -1.0, x, 1.0, b, r);  // residual-before and not residual-after
+    }
+  } else {
+    for (int k = 0; k < steps; k++) sor_sweep(L.A, p.omega, post && p.symmetric, x, b, r);
+  }
+  L.residual_p = false;  // residual-after nil
+}
+
+static void lmgc(flgpu_mg *mg, int l) {
+  MGLevel &L = mg->lv[l];
+  if (l == mg->p.base_level) {
+    coarse_apply(mg, L, lsol(mg, l), lres(mg, l));
+    L.residual_p = false;
+    return;
+  }
+  smooth(mg, l, false);
+  // restrict (multigrid.lisp:202-224): res_{l-1} = I^T res_l
+  ensure_residual(mg, l);
+  MGLevel &C = mg->lv[l - 1];
+  spmv(C.I->transpose, 1.0, lres(mg, l), 0.0, C.res, C.res);
+  // :after methods: x_{l-1} = 0, rhs_{l-1} = res_{l-1}, residual-p t
+  vfill(mg->ctx, C.N, mg->p.prec, C.sol, 0.0);
+  vcopy(mg->ctx, C.N, mg->p.prec, C.res, mg->p.prec, C.rhs);
+  C.residual_p = true;
+  lmgc(mg, l - 1);
+  // prolongate (multigrid.lisp:237-252): sol_l += I sol_{l-1}
+  spmv(C.I, 1.0, C.sol, 1.0, lsol(mg, l), lsol(mg, l));
+  L.residual_p = false;
+  smooth(mg, l, true);
+}
+
+// the <mg-iteration> iterator on device pointers of the hierarchy's precision
+static void mg_cycle(flgpu_mg *mg, void *x, void *b, void *r) {
+  mg->sol_top = x;
+  mg->rhs_top = b;
+  mg->res_top = r;
+  mg->lv[mg->top].residual_p = true;
+  lmgc(mg, mg->top);
+}
+
+template <class S, class D>
+__global__ void k_add_conv(long n, const S *x, D *y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += (D)x[i];
+}
+
+// y += x with precision conversion
+static void vadd_converted(flgpu_ctx *ctx, long n, int sp, const void *src, int dp, void *dst) {
+  unsigned g = grid_for(n, 256);
+  if (sp == FLGPU_F32 && dp == FLGPU_F64)
+    k_add_conv<float, double><<<g, 256, 0, ctx->stream>>>(n, (const float *)src, (double *)dst);
+  else if (sp == FLGPU_F64 && dp == FLGPU_F32)
+    k_add_conv<double, float><<<g, 256, 0, ctx->stream>>>(n, (const double *)src, (float *)dst);
+  else if (sp == FLGPU_F64)
+    k_add_conv<double, double><<<g, 256, 0, ctx->stream>>>(n, (const double *)src, (double *)dst);
+  else
+    k_add_conv<float, float><<<g, 256, 0, ctx->stream>>>(n, (const float *)src, (float *)dst);
+  FLGPU_HIP(hipGetLastError());
+}
+
+// x <- x + V(r) (femlisp's iterate on (x, b, r)); vectors of precision vprec
+void mg_apply_ptr(flgpu_mg *mg, int vprec, void *x, void *b, void *r) {
+  if (vprec == mg->p.prec) {
+    mg_cycle(mg, x, b, r);
+    return;
+  }
+  // cycle in the hierarchy's precision on staged copies: the correction
+  // scheme solves A e = r from e = 0 and adds e to x
+  FLGPU_REQUIRE(b != r, "a mixed-precision cycle needs separate rhs and residual vectors");
+  long N = mg->lv[mg->top].N;
+  vfill(mg->ctx, N, mg->p.prec, mg->xs, 0.0);
+  vcopy(mg->ctx, N, vprec, r, mg->p.prec, mg->rs);
+  vcopy(mg->ctx, N, mg->p.prec, mg->rs, mg->p.prec, mg->bs);
+  mg_cycle(mg, mg->xs, mg->bs, mg->rs);
+  vadd_converted(mg->ctx, N, mg->p.prec, mg->xs, vprec, x);
+}
+
+static void check_params(const flgpu_mg_params *p, int nlevels) {
+  FLGPU_REQUIRE(p != nullptr, "params must not be NULL");
+  FLGPU_REQUIRE(p->smoother == FLGPU_SMOOTH_JACOBI || p->smoother == FLGPU_SMOOTH_SOR,
+                "unknown smoother");
+  FLGPU_REQUIRE(p->pre_steps >= 0 && p->post_steps >= 0, "negative smoothing steps");
+  FLGPU_REQUIRE(p->base_level >= 0 && p->base_level < nlevels, "base_level out of range");
+  FLGPU_REQUIRE(p->prec == FLGPU_F64 || p->prec == FLGPU_F32, "unknown precision");
+}
+
+static void mg_alloc_work(flgpu_mg *mg) {
+  for (auto &L : mg->lv) {
+    L.N = L.A->nrows;
+    size_t b = prec_bytes(mg->p.prec) * L.N;
+    L.sol = dev_alloc(b);
+    L.rhs = dev_alloc(b);
+    L.res = dev_alloc(b);
+    vfill(mg->ctx, L.N, mg->p.prec, L.sol, 0.0);
+    vfill(mg->ctx, L.N, mg->p.prec, L.rhs, 0.0);
+    vfill(mg->ctx, L.N, mg->p.prec, L.res, 0.0);
+  }
+  long N = mg->lv[mg->top].N;
+  size_t b = prec_bytes(mg->p.prec) * N;
+  mg->xs = dev_alloc(b);
+  mg->bs = dev_alloc(b);
+  mg->rs = dev_alloc(b);
+  coarse_setup(mg, mg->lv[mg->p.base_level]);
+}
+
+void mg_free(flgpu_mg *mg) {
+  if (!mg) return;
+  for (auto &L : mg->lv) {
+    mat_free(L.A);
+    mat_free(L.I);
+    dev_free(L.sol);
+    dev_free(L.rhs);
+    dev_free(L.res);
+    dev_free(L.cinv);
+    dev_free(L.ctmp);
+  }
+  dev_free(mg->xs);
+  dev_free(mg->bs);
+  dev_free(mg->rs);
+  delete mg;
+}
+
+// ---- Krylov kernels with device scalars ----------------------------------------------------
+
+// y += sign * (num/den) * x
+template <class T>
+__global__ void k_axpy_ratio(long n, const double *num, const double *den, double sign,
+                             const T *__restrict__ x, T *y) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T lam = (T)(sign * (*num / *den));
+  y[i] = lam * x[i] + y[i];
+}
+
+// p <- (a/b) p + q   (<cg>: scal! (/ new-alpha alpha) p, m+! q p)
+template <class T>
+__global__ void k_cg_p(long n, const double *a, const double *b, T *p, const T *__restrict__ q) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T sc = (T)(*a / *b);
+  p[i] = sc * p[i] + q[i];
+}
+
+// p <- q + ((zr - zrold)/alpha) p  (flexible CG, Polak-Ribiere)
+template <class T>
+__global__ void k_fcg_p(long n, const double *zr, const double *zro, const double *alpha, T *p,
+                        const T *__restrict__ q) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T beta = (T)((*zr - *zro) / *alpha);
+  p[i] = q[i] + beta * p[i];
+}
+
+__global__ void k_slot_copy(double *s, int from, int to) { s[to] = s[from]; }
+
+}  // namespace flgpu
+
+using namespace flgpu;
+
+// ---- C-ABI: hierarchy ---------------------------------------------------------------------
+
+extern "C" int flgpu_mg_create(flgpu_ctx *ctx, int nlevels, flgpu_mat *const *A,
+                               flgpu_mat *const *I, const flgpu_mg_params *params,
+                               flgpu_mg **out) {
+  return guard([&] {
+    FLGPU_REQUIRE(ctx && A && out, "ctx, A and out must not be NULL");
+    FLGPU_REQUIRE(nlevels >= 1, "nlevels must be >= 1");
+    FLGPU_REQUIRE(nlevels == 1 || I, "I must not be NULL");
+    check_params(params, nlevels);
+    for (int l = 0; l < nlevels; l++) {
+      FLGPU_REQUIRE(A[l] && A[l]->ctx == ctx, "level matrix missing or of another context");
+      FLGPU_REQUIRE(A[l]->nrows == A[l]->ncols, "level matrices must be square");
+      if (l + 1 < nlevels) {
+        FLGPU_REQUIRE(I[l] && I[l]->ctx == ctx, "interpolation missing or of another context");
+        FLGPU_REQUIRE(A[l + 1] && I[l]->nrows == A[l + 1]->nrows && I[l]->ncols == A[l]->nrows,
+                      "interpolation dimensions do not match the level matrices");
+      }
+    }
+    CTX_LOCK(ctx);
+    flgpu_mg *mg = new flgpu_mg;
+    mg->ctx = ctx;
+    mg->p = *params;
+    mg->top = nlevels - 1;
+    mg->lv.resize(nlevels);
+    try {
+      for (int l = 0; l < nlevels; l++) {
+        mg->lv[l].A = mat_convert(A[l], params->prec);
+        if (params->smoother == FLGPU_SMOOTH_SOR && mg->lv[l].A->ncolours == 0 &&
+            l > params->base_level) {
+          int nc = 0;
+          FLGPU_REQUIRE(flgpu_mat_colour_greedy(mg->lv[l].A, &nc) == 0, "greedy colouring failed");
+        }
+        if (l + 1 < nlevels) {
+          mg->lv[l].I = mat_convert(I[l], params->prec);
+          mat_build_transpose(mg->lv[l].I);
+        }
+      }
+      mg_alloc_work(mg);
+    } catch (...) {
+      mg_free(mg);
+      throw;
+    }
+    *out = mg;
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n0, int nlevels,
+                                       const flgpu_vec *kappa, int nk, double f, int asm_mode,
+                                       const flgpu_mg_params *params, flgpu_mg **out,
+                                       flgpu_vec **rhs) {
+  return guard([&] {
+    FLGPU_REQUIRE(ctx && out, "ctx and out must not be NULL");
+    FLGPU_REQUIRE(nlevels >= 1 && n0 >= 1, "nlevels and n0 must be >= 1");
+    check_params(params, nlevels);
+    FLGPU_REQUIRE(!kappa || kappa->ctx == ctx, "kappa belongs to another context");
+    CTX_LOCK(ctx);
+    flgpu_mg *mg = new flgpu_mg;
+    mg->ctx = ctx;
+    mg->p = *params;
+    mg->top = nlevels - 1;
+    mg->lv.resize(nlevels);
+    flgpu_vec *b = nullptr;
+    try {
+      for (int l = 0; l < nlevels; l++) {
+        int n = n0 << l;
+        bool is_top = (l == nlevels - 1);
+        mg->lv[l].A = lattice_assemble(ctx, dim, order, n, kappa, nk, f, params->prec, asm_mode,
+                                       (is_top && rhs) ? &b : nullptr);
+        if (params->smoother == FLGPU_SMOOTH_SOR && l > params->base_level)
+          lattice_colouring(mg->lv[l].A, dim, order, n);
+        if (!is_top) mg->lv[l].I = lattice_interpolation(ctx, dim, order, n, params->prec);
+      }
+      mg_alloc_work(mg);
+    } catch (...) {
+      vec_free(b);
+      mg_free(mg);
+      throw;
+    }
+    if (rhs) *rhs = b;
+    *out = mg;
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mg_destroy(flgpu_mg *mg) {
+  return guard([&] {
+    if (!mg) return 0;
+    CTX_LOCK(mg->ctx);
+    FLGPU_HIP(hipStreamSynchronize(mg->ctx->stream));
+    mg_free(mg);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mg_top_matrix(flgpu_mg *mg, flgpu_mat **A) {
+  return guard([&] {
+    FLGPU_REQUIRE(mg && A, "arguments must not be NULL");
+    *A = mg->lv[mg->top].A;
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mg_apply(flgpu_mg *mg, flgpu_vec *x, flgpu_vec *b, flgpu_vec *r) {
+  return guard([&] {
+    FLGPU_REQUIRE(mg && x && b && r, "arguments must not be NULL");
+    long N = mg->lv[mg->top].N;
+    FLGPU_REQUIRE(x->n == N && b->n == N && r->n == N, "vector length does not match the top level");
+    FLGPU_REQUIRE(x->prec == b->prec && b->prec == r->prec, "vectors must share one precision");
+    FLGPU_REQUIRE(x->d != r->d && x->d != b->d, "x must differ from b and r");
+    CTX_LOCK(mg->ctx);
+    mg_apply_ptr(mg, x->prec, x->d, b->d, r->d);
+    return 0;
+  });
+}
+
+// ---- C-ABI: solver --------------------------------------------------------------------------
+
+namespace {
+
+struct SolveWork {
+  flgpu_ctx *ctx;
+  long N;
+  int prec;
+  void *r = nullptr, *p = nullptr, *a = nullptr, *w = nullptr, *w2 = nullptr, *q = nullptr,
+       *rold = nullptr;
+  SolveWork(flgpu_ctx *c, long n, int pr) : ctx(c), N(n), prec(pr) {
+    size_t b = prec_bytes(pr) * n;
+    for (void **v : {&r, &p, &a, &w, &w2, &q, &rold}) {
+      *v = dev_alloc(b);
+      vfill(c, n, pr, *v, 0.0);
+    }
+  }
+  ~SolveWork() {
+    for (void *v : {r, p, a, w, w2, q, rold}) dev_free(v);
+  }
+};
+
+}  // namespace
+
+namespace flgpu {
+
+// preconditioner call of <cg> (krylow.lisp:118-121): (copy! r w) (x<-0 q)
+// (funcall iterate q w w); alias = 0 passes a second copy as the residual
+static void precondition(flgpu_mg *mg, const flgpu_solve_params *p, SolveWork &W, void *r,
+                         void *q, LearnedHook *learned) {
+  vcopy(W.ctx, W.N, W.prec, r, W.prec, W.w);
+  vfill(W.ctx, W.N, W.prec, q, 0.0);
+  if (p->alias) {
+    mg_apply_ptr(mg, W.prec, q, W.w, W.w);
+  } else {
+    vcopy(W.ctx, W.N, W.prec, r, W.prec, W.w2);
+    mg_apply_ptr(mg, W.prec, q, W.w, W.w2);
+  }
+  if (learned) learned->correct(r, q);
+}
+
+}  // namespace flgpu
+
+extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_params *p,
+                           const flgpu_vec *b, flgpu_vec *x, flgpu_solve_report *report,
+                           double *defnorms, int max_defnorms) {
+  return guard([&] {
+    FLGPU_REQUIRE(A && p && b && x, "A, p, b and x must not be NULL");
+    FLGPU_REQUIRE(A->nrows == A->ncols, "the system matrix must be square");
+    FLGPU_REQUIRE(b->n == A->nrows && x->n == A->nrows, "vector length mismatch");
+    FLGPU_REQUIRE(b->prec == A->prec && x->prec == A->prec, "vector precision differs from A");
+    FLGPU_REQUIRE(p->outer >= FLGPU_OUTER_MG && p->outer <= FLGPU_OUTER_FCG, "unknown outer iteration");
+    FLGPU_REQUIRE(p->outer == FLGPU_OUTER_CG_PLAIN || mg, "this iteration needs a hierarchy");
+    if (mg) FLGPU_REQUIRE(mg->lv[mg->top].N == A->nrows, "hierarchy does not match A");
+    FLGPU_REQUIRE(!(p->alias && mg && mg->p.prec != A->prec),
+                  "alias mode needs the hierarchy in the precision of A");
+    FLGPU_REQUIRE(!p->learned || p->outer == FLGPU_OUTER_FCG, "the learned correction needs FCG");
+    flgpu_ctx *ctx = A->ctx;
+    CTX_LOCK(ctx);
+    long N = A->nrows;
+    int prec = A->prec;
+    hipStream_t s = ctx->stream;
+    SolveWork W(ctx, N, prec);
+    LearnedHook *learned = nullptr;
+    if (p->learned) learned = learned_hook(mg, A);
+    double *slot = ctx->red_out;
+    unsigned g = grid_for(N, 256);
+    auto t0 = std::chrono::steady_clock::now();
+    vfill(ctx, N, prec, x->d, 0.0);  // initially: x := 0 (solve.lisp:126-132)
+    bool residual_p = false, initialized = false;
+    int step = 0, status = 1;
+    double initial = -1.0, defnorm = 0.0, alpha = 0.0;
+    int fell_back = 0;
+    double best_true = -1.0;
+    int best_step = 0;
+    for (;;) {
+      // intermediate :before: defect and its norm (solve.lisp:139-155)
+      if (!residual_p) {
+        spmv(A, -1.0, x->d, 1.0, b->d, W.r);
+        residual_p = true;
+        vdot_dev(ctx, N, prec, W.r, W.r, 3);
+        fetch_slots(ctx, 4);
+      }
+      defnorm = std::sqrt(ctx->host_out[3]);
+      if (initial < 0) initial = defnorm;
+      if (defnorms && step < max_defnorms) defnorms[step] = defnorm;
+      double reduction = (defnorm == 0.0) ? 0.0 : defnorm / initial;
+      bool ok = (p->stop == FLGPU_STOP_REDUCTION) ? (reduction < p->tol) : (defnorm < p->tol);
+      if (ok && p->true_residual && p->outer != FLGPU_OUTER_MG) {
+        // confirm with the FP64 true residual; replace the recurrence if it lags
+        spmv(A, -1.0, x->d, 1.0, b->d, W.rold);
+        vdot_dev(ctx, N, prec, W.rold, W.rold, 5);
+        fetch_slots(ctx, 6);
+        double tn = std::sqrt(ctx->host_out[5]);
+        double tr = (tn == 0.0) ? 0.0 : tn / initial;
+        ok = (p->stop == FLGPU_STOP_REDUCTION) ? (tr < p->tol) : (tn < p->tol);
+        if (!ok) {
+          vcopy(ctx, N, prec, W.rold, prec, W.r);
+          initialized = false;  // restart the recurrence from the true residual
+        }
+      }
+      if (ok) {
+        status = 0;
+        break;
+      }
+      if (step > p->maxsteps) {
+        status = 1;
+        break;
+      }
+      // learned stall fallback: plain GMG-FCG takes over when the defect has
+      // not dropped 10x within stall_window steps
+      if (learned && p->stall_window > 0) {
+        if (best_true < 0 || defnorm < 0.1 * best_true) {
+          best_true = defnorm;
+          best_step = step;
+        } else if (step - best_step >= p->stall_window) {
+          learned = nullptr;
+          fell_back = 1;
+          initialized = false;
+          best_true = defnorm;
+          best_step = step;
+        }
+      }
+      // next-step
+      switch (p->outer) {
+        case FLGPU_OUTER_MG:
+          mg_apply_ptr(mg, prec, x->d, b->d, W.r);
+          residual_p = false;
+          break;
+        case FLGPU_OUTER_CG:
+        case FLGPU_OUTER_CG_PLAIN:
+        case FLGPU_OUTER_FCG: {
+          bool pc = (p->outer != FLGPU_OUTER_CG_PLAIN);
+          bool fcg = (p->outer == FLGPU_OUTER_FCG);
+          if (!initialized) {  // restart (krylow.lisp:88-96)
+            if (pc) precondition(mg, p, W, W.r, W.p, learned);
+            else vcopy(ctx, N, prec, W.r, prec, W.p);
+            vdot_dev(ctx, N, prec, W.p, W.r, 0);
+            fetch_slots(ctx, 1);
+            alpha = ctx->host_out[0];
+            initialized = true;
+          }
+          if (alpha != 0.0) {
+            spmv(A, 1.0, W.p, 0.0, W.a, W.a);         // gemm! 1 A p 0 a
+            vdot_dev(ctx, N, prec, W.a, W.p, 1);       // beta = a.p
+            if (fcg) vcopy(ctx, N, prec, W.r, prec, W.rold);
+            if (prec == FLGPU_F64) {
+              k_axpy_ratio<double><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, 1.0, (const double *)W.p,
+                                                     (double *)x->d);
+              k_axpy_ratio<double><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, -1.0,
+                                                     (const double *)W.a, (double *)W.r);
+            } else {
+              k_axpy_ratio<float><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, 1.0, (const float *)W.p,
+                                                    (float *)x->d);
+              k_axpy_ratio<float><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, -1.0, (const float *)W.a,
+                                                    (float *)W.r);
+            }
+            FLGPU_HIP(hipGetLastError());
+            if (pc) precondition(mg, p, W, W.r, W.q, learned);
+            else vcopy(ctx, N, prec, W.r, prec, W.q);
+            vdot_dev(ctx, N, prec, W.q, W.r, 2);  // new alpha = q.r
+            vdot_dev(ctx, N, prec, W.r, W.r, 3);  // defect norm^2
+            if (fcg) {
+              vdot_dev(ctx, N, prec, W.q, W.rold, 4);
+              if (prec == FLGPU_F64)
+                k_fcg_p<double><<<g, 256, 0, s>>>(N, slot + 2, slot + 4, slot + 0, (double *)W.p,
+                                                  (const double *)W.q);
+              else
+                k_fcg_p<float><<<g, 256, 0, s>>>(N, slot + 2, slot + 4, slot + 0, (float *)W.p,
+                                                 (const float *)W.q);
+            } else {
+              if (prec == FLGPU_F64)
+                k_cg_p<double><<<g, 256, 0, s>>>(N, slot + 2, slot + 0, (double *)W.p,
+                                                 (const double *)W.q);
+              else
+                k_cg_p<float><<<g, 256, 0, s>>>(N, slot + 2, slot + 0, (float *)W.p,
+                                                (const float *)W.q);
+            }
+            k_slot_copy<<<1, 1, 0, s>>>(slot, 2, 0);
+            FLGPU_HIP(hipGetLastError());
+            fetch_slots(ctx, 4);
+            alpha = ctx->host_out[0];
+          } else {
+            vdot_dev(ctx, N, prec, W.r, W.r, 3);
+            fetch_slots(ctx, 4);
+          }
+          residual_p = true;
+          break;
+        }
+      }
+      step++;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    // true residual in the precision of A
+    spmv(A, -1.0, x->d, 1.0, b->d, W.rold);
+    double tn = std::sqrt(vdot_host(ctx, N, prec, W.rold, W.rold));
+    if (report) {
+      report->status = status;
+      report->steps = step;
+      report->initial_defnorm = initial;
+      report->final_defnorm = defnorm;
+      report->true_residual_norm = tn;
+      report->fell_back = fell_back;
+      report->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return status;
+  });
+}

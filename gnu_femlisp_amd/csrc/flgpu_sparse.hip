@@ -1,0 +1,669 @@
+// Sparse matrices of libflgpu.so: CSR as handed over by femlisp, streamed by
+// the kernels in SELL-64 form (one wavefront per 64-row slice, column-major
+// entries so that each load instruction of a wavefront reads 64 consecutive
+// values and 64 consecutive column indices).
+//
+// Restates femlisp's <sparse-matrix> operations for scalar (1x1-block) DOFs:
+//   gemm-nn!/gemm-tn!      src/matlisp/sparse-matrix.lisp:378-442
+//   compute-residual       src/iteration/linit.lisp:73-80
+//   <jacobi> iterate       src/iteration/linit.lisp:220-250
+//   <sor> iterate          src/iteration/linit.lisp:288-323
+// Row sums accumulate like femlisp's template: start from beta*x_i and add
+// (alpha*y_j)*A_ij in ascending column order (femlisp's order is hash order).
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "flgpu_internal.h"
+
+namespace flgpu {
+
+// ---- SELL construction ----------------------------------------------------------
+
+__global__ void k_slice_width(int nsub, const int *rows, const int *rowptr, int *width) {
+  int li = blockIdx.x * blockDim.x + threadIdx.x;
+  int len = 0;
+  if (li < nsub) {
+    int r = rows ? rows[li] : li;
+    len = rowptr[r + 1] - rowptr[r];
+  }
+  for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_down(len, off, 64));
+  if ((threadIdx.x & 63) == 0 && (li - (int)(threadIdx.x & 63)) < nsub)
+    width[li >> 6] = len;
+}
+
+template <class T>
+__global__ void k_sell_fill(int nsub, const int *rows, const int *rowptr, const int *colidx,
+                            const T *vals, const long *slice_ptr, int *scol, T *sval,
+                            int fill_cols) {
+  int li = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = li >> 6, lane = li & 63;
+  int nslices = (nsub + 63) >> 6;
+  if (s >= nslices) return;
+  long base = slice_ptr[s];
+  int w = (int)((slice_ptr[s + 1] - base) >> 6);
+  int r = -1, b = 0, len = 0;
+  if (li < nsub) {
+    r = rows ? rows[li] : li;
+    b = rowptr[r];
+    len = rowptr[r + 1] - b;
+  }
+  int padc = (len > 0) ? colidx[b + len - 1] : 0;
+  for (int k = 0; k < w; k++) {
+    long p = base + (long)k * 64 + lane;
+    if (k < len) {
+      if (fill_cols) scol[p] = colidx[b + k];
+      sval[p] = vals[b + k];
+    } else {
+      if (fill_cols) scol[p] = padc;
+      sval[p] = (T)0;
+    }
+  }
+}
+
+void sell_free(Sell &s) {
+  dev_free(s.rows);
+  dev_free(s.slice_ptr);
+  dev_free(s.col);
+  dev_free(s.val);
+  s = Sell();
+}
+
+template <class T>
+static void sell_fill(flgpu_ctx *ctx, const flgpu_mat *A, Sell &s, int fill_cols) {
+  if (s.nrows == 0) return;
+  int nthreads = s.nslices * 64;
+  k_sell_fill<T><<<grid_for(nthreads, 256), 256, 0, ctx->stream>>>(
+      s.nrows, s.rows, A->rowptr, A->colidx, (const T *)A->vals, s.slice_ptr, s.col, (T *)s.val,
+      fill_cols);
+  FLGPU_HIP(hipGetLastError());
+}
+
+// rows_host: row ids of the subset (nullptr: all rows)
+static Sell sell_build(flgpu_mat *A, const std::vector<int> *rows_host) {
+  flgpu_ctx *ctx = A->ctx;
+  Sell s;
+  s.nrows = rows_host ? (int)rows_host->size() : A->nrows;
+  s.nslices = (s.nrows + 63) / 64;
+  if (rows_host && s.nrows > 0) {
+    s.rows = (int *)dev_alloc(sizeof(int) * s.nrows);
+    FLGPU_HIP(hipMemcpyAsync(s.rows, rows_host->data(), sizeof(int) * s.nrows,
+                             hipMemcpyHostToDevice, ctx->stream));
+  }
+  std::vector<int> w(s.nslices);
+  std::vector<long> sp(s.nslices + 1, 0);
+  if (s.nslices > 0) {
+    int *dw = (int *)dev_alloc(sizeof(int) * s.nslices);
+    k_slice_width<<<grid_for((long)s.nslices * 64, 256), 256, 0, ctx->stream>>>(
+        s.nrows, s.rows, A->rowptr, dw);
+    FLGPU_HIP(hipGetLastError());
+    FLGPU_HIP(hipMemcpyAsync(w.data(), dw, sizeof(int) * s.nslices, hipMemcpyDeviceToHost,
+                             ctx->stream));
+    FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+    dev_free(dw);
+  }
+  for (int i = 0; i < s.nslices; i++) {
+    sp[i + 1] = sp[i] + 64L * w[i];
+    s.maxw = std::max(s.maxw, w[i]);
+  }
+  s.nstored = sp[s.nslices];
+  s.slice_ptr = (long *)dev_alloc(sizeof(long) * (s.nslices + 1));
+  FLGPU_HIP(hipMemcpyAsync(s.slice_ptr, sp.data(), sizeof(long) * (s.nslices + 1),
+                           hipMemcpyHostToDevice, ctx->stream));
+  s.col = (int *)dev_alloc(sizeof(int) * s.nstored);
+  s.val = dev_alloc(prec_bytes(A->prec) * s.nstored);
+  if (A->prec == FLGPU_F64) sell_fill<double>(ctx, A, s, 1);
+  else sell_fill<float>(ctx, A, s, 1);
+  FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+  return s;
+}
+
+Sell sell_build_rows(flgpu_mat *A, const std::vector<int> &rows) { return sell_build(A, &rows); }
+
+// ---- diagonal -----------------------------------------------------------------------
+
+__global__ void k_diagpos(int n, const int *rowptr, const int *colidx, int *pos) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int p = -1;
+  for (int e = rowptr[r]; e < rowptr[r + 1]; e++)
+    if (colidx[e] == r) p = e;
+  pos[r] = p;
+}
+
+template <class T>
+__global__ void k_diag(int n, const int *pos, const T *vals, T *diag) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int p = pos[r];
+  diag[r] = p >= 0 ? vals[p] : (T)0;
+}
+
+static void diag_refresh(flgpu_mat *A) {
+  int n = std::min(A->nrows, A->ncols);
+  if (n == 0) return;
+  if (A->prec == FLGPU_F64)
+    k_diag<double><<<grid_for(n, 256), 256, 0, A->ctx->stream>>>(n, A->diagpos, (const double *)A->vals,
+                                                                 (double *)A->diag);
+  else
+    k_diag<float><<<grid_for(n, 256), 256, 0, A->ctx->stream>>>(n, A->diagpos, (const float *)A->vals,
+                                                                (float *)A->diag);
+  FLGPU_HIP(hipGetLastError());
+}
+
+flgpu_mat *mat_new(flgpu_ctx *ctx, int nrows, int ncols, long nnz, int prec) {
+  FLGPU_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0, "negative matrix dimensions");
+  FLGPU_REQUIRE(nnz <= 0x7fffffffL, "nnz exceeds the int32 index range");
+  FLGPU_REQUIRE(prec == FLGPU_F64 || prec == FLGPU_F32, "precision must be FLGPU_F64 or FLGPU_F32");
+  flgpu_mat *A = new flgpu_mat;
+  A->ctx = ctx;
+  A->nrows = nrows;
+  A->ncols = ncols;
+  A->nnz = nnz;
+  A->prec = prec;
+  A->rowptr = (int *)dev_alloc(sizeof(int) * (nrows + 1));
+  A->colidx = (int *)dev_alloc(sizeof(int) * nnz);
+  A->vals = dev_alloc(prec_bytes(prec) * nnz);
+  return A;
+}
+
+void mat_finish(flgpu_mat *A) {
+  flgpu_ctx *ctx = A->ctx;
+  int n = std::min(A->nrows, A->ncols);
+  A->diagpos = (int *)dev_alloc(sizeof(int) * std::max(n, 1));
+  A->diag = dev_alloc(prec_bytes(A->prec) * std::max(n, 1));
+  if (n > 0) {
+    k_diagpos<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, A->rowptr, A->colidx, A->diagpos);
+    FLGPU_HIP(hipGetLastError());
+    diag_refresh(A);
+  }
+  A->sell = sell_build(A, nullptr);
+}
+
+void mat_free(flgpu_mat *A) {
+  if (!A) return;
+  dev_free(A->rowptr);
+  dev_free(A->colidx);
+  dev_free(A->vals);
+  dev_free(A->diagpos);
+  dev_free(A->diag);
+  sell_free(A->sell);
+  for (auto &s : A->colour_sell) sell_free(s);
+  if (A->transpose) mat_free(A->transpose);
+  delete A;
+}
+
+void mat_refresh_values(flgpu_mat *A) {
+  diag_refresh(A);
+  if (A->prec == FLGPU_F64) {
+    sell_fill<double>(A->ctx, A, A->sell, 0);
+    for (auto &s : A->colour_sell) sell_fill<double>(A->ctx, A, s, 0);
+  } else {
+    sell_fill<float>(A->ctx, A, A->sell, 0);
+    for (auto &s : A->colour_sell) sell_fill<float>(A->ctx, A, s, 0);
+  }
+  if (A->transpose) {
+    mat_free(A->transpose);
+    A->transpose = nullptr;
+  }
+}
+
+static void download_csr(const flgpu_mat *A, std::vector<int> &rp, std::vector<int> &ci,
+                         std::vector<double> *v) {
+  flgpu_ctx *ctx = A->ctx;
+  rp.resize(A->nrows + 1);
+  ci.resize(A->nnz);
+  FLGPU_HIP(hipMemcpyAsync(rp.data(), A->rowptr, sizeof(int) * (A->nrows + 1),
+                           hipMemcpyDeviceToHost, ctx->stream));
+  FLGPU_HIP(hipMemcpyAsync(ci.data(), A->colidx, sizeof(int) * A->nnz, hipMemcpyDeviceToHost,
+                           ctx->stream));
+  if (v) {
+    v->resize(A->nnz);
+    if (A->prec == FLGPU_F64) {
+      FLGPU_HIP(hipMemcpyAsync(v->data(), A->vals, sizeof(double) * A->nnz,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+      double *tmp = (double *)dev_alloc(sizeof(double) * A->nnz);
+      vcopy(ctx, A->nnz, FLGPU_F32, A->vals, FLGPU_F64, tmp);
+      FLGPU_HIP(hipMemcpyAsync(v->data(), tmp, sizeof(double) * A->nnz, hipMemcpyDeviceToHost,
+                               ctx->stream));
+      FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+      dev_free(tmp);
+    }
+  }
+  FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+static void upload_vals(flgpu_mat *A, const double *vals) {
+  flgpu_ctx *ctx = A->ctx;
+  if (A->nnz == 0) return;
+  if (A->prec == FLGPU_F64) {
+    FLGPU_HIP(hipMemcpyAsync(A->vals, vals, sizeof(double) * A->nnz, hipMemcpyHostToDevice,
+                             ctx->stream));
+    FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+  } else {
+    double *tmp = (double *)dev_alloc(sizeof(double) * A->nnz);
+    FLGPU_HIP(hipMemcpyAsync(tmp, vals, sizeof(double) * A->nnz, hipMemcpyHostToDevice,
+                             ctx->stream));
+    vcopy(ctx, A->nnz, FLGPU_F64, tmp, FLGPU_F32, A->vals);
+    FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+    dev_free(tmp);
+  }
+}
+
+// Transpose via the host (setup only): columns of A become rows, row
+// indices ascending within each new row.
+void mat_build_transpose(const flgpu_mat *A) {
+  if (A->transpose) return;
+  std::vector<int> rp, ci, trp(A->ncols + 1, 0), tci(A->nnz);
+  std::vector<double> v, tv(A->nnz);
+  download_csr(A, rp, ci, &v);
+  for (long e = 0; e < A->nnz; e++) trp[ci[e] + 1]++;
+  for (int c = 0; c < A->ncols; c++) trp[c + 1] += trp[c];
+  std::vector<int> fill(trp.begin(), trp.end() - 1);
+  for (int r = 0; r < A->nrows; r++)
+    for (int e = rp[r]; e < rp[r + 1]; e++) {
+      int p = fill[ci[e]]++;
+      tci[p] = r;
+      tv[p] = v[e];
+    }
+  flgpu_mat *T = mat_new(A->ctx, A->ncols, A->nrows, A->nnz, A->prec);
+  FLGPU_HIP(hipMemcpyAsync(T->rowptr, trp.data(), sizeof(int) * (A->ncols + 1),
+                           hipMemcpyHostToDevice, A->ctx->stream));
+  FLGPU_HIP(hipMemcpyAsync(T->colidx, tci.data(), sizeof(int) * A->nnz, hipMemcpyHostToDevice,
+                           A->ctx->stream));
+  upload_vals(T, tv.data());
+  mat_finish(T);
+  const_cast<flgpu_mat *>(A)->transpose = T;
+}
+
+flgpu_mat *mat_convert(const flgpu_mat *A, int prec) {
+  flgpu_ctx *ctx = A->ctx;
+  flgpu_mat *B = mat_new(ctx, A->nrows, A->ncols, A->nnz, prec);
+  FLGPU_HIP(hipMemcpyAsync(B->rowptr, A->rowptr, sizeof(int) * (A->nrows + 1),
+                           hipMemcpyDeviceToDevice, ctx->stream));
+  if (A->nnz > 0) {
+    FLGPU_HIP(hipMemcpyAsync(B->colidx, A->colidx, sizeof(int) * A->nnz,
+                             hipMemcpyDeviceToDevice, ctx->stream));
+    vcopy(ctx, A->nnz, A->prec, A->vals, prec, B->vals);
+  }
+  B->lat_dim = A->lat_dim;
+  B->lat_order = A->lat_order;
+  B->lat_n = A->lat_n;
+  B->lat_f = A->lat_f;
+  mat_finish(B);
+  if (A->transpose) B->transpose = mat_convert(A->transpose, prec);
+  if (A->ncolours > 0) {
+    // rebuild the colour subsets from A's row lists
+    std::vector<int> colour(A->nrows, 0);
+    for (int c = 0; c < A->ncolours; c++) {
+      const Sell &s = A->colour_sell[c];
+      std::vector<int> rows(s.nrows);
+      if (s.nrows)
+        FLGPU_HIP(hipMemcpy(rows.data(), s.rows, sizeof(int) * s.nrows, hipMemcpyDeviceToHost));
+      for (int r : rows) colour[r] = c;
+    }
+    set_colouring(B, A->ncolours, colour.data());
+  }
+  return B;
+}
+
+void set_colouring(flgpu_mat *A, int ncolours, const int *colour) {
+  FLGPU_REQUIRE(A->nrows == A->ncols, "colouring needs a square matrix");
+  FLGPU_REQUIRE(ncolours >= 1, "ncolours must be >= 1");
+  std::vector<std::vector<int>> rows(ncolours);
+  for (int r = 0; r < A->nrows; r++) {
+    FLGPU_REQUIRE(colour[r] >= 0 && colour[r] < ncolours, "colour out of range");
+    rows[colour[r]].push_back(r);
+  }
+  // same-colour rows must not couple (otherwise the sweep would not be the
+  // sequential <sor> sweep in (colour, index) order)
+  std::vector<int> rp, ci;
+  download_csr(A, rp, ci, nullptr);
+  for (int r = 0; r < A->nrows; r++)
+    for (int e = rp[r]; e < rp[r + 1]; e++)
+      FLGPU_REQUIRE(ci[e] == r || colour[ci[e]] != colour[r],
+                    "colouring couples two rows of one colour");
+  for (auto &s : A->colour_sell) sell_free(s);
+  A->colour_sell.clear();
+  for (int c = 0; c < ncolours; c++) A->colour_sell.push_back(sell_build(A, &rows[c]));
+  A->ncolours = ncolours;
+}
+
+// ---- kernels ----------------------------------------------------------------------
+
+template <class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
+
+// out[row] = beta*z[row] + sum_k (alpha*x[col]) * val  over one SELL structure
+template <class T>
+__global__ __launch_bounds__(256) void k_sell_spmv(int nsub, int nslices, const int *__restrict__ rows,
+                                                   const long *__restrict__ slice_ptr,
+                                                   const int *__restrict__ col,
+                                                   const T *__restrict__ val, T alpha,
+                                                   const T *__restrict__ x, T beta, const T *z,
+                                                   T *out) {
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  int s = (int)lb * 4 + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  int lane = threadIdx.x & 63;
+  int li = s * 64 + lane;
+  long base = slice_ptr[s];
+  int w = (int)((slice_ptr[s + 1] - base) >> 6);
+  if (li >= nsub) return;
+  int row = rows ? rows[li] : li;
+  T acc = (beta == (T)0) ? (T)0 : beta * z[row];
+  const T *vp = val + base + lane;
+  const int *cp = col + base + lane;
+#pragma unroll 4
+  for (int k = 0; k < w; k++) {
+    T a = ld_stream(vp + (long)k * 64);
+    int c = ld_stream(cp + (long)k * 64);
+    acc += (alpha * x[c]) * a;
+  }
+  out[row] = acc;
+}
+
+template <class T>
+static void launch_spmv(flgpu_ctx *ctx, const Sell &s, double alpha, const void *x, double beta,
+                        const void *z, void *out) {
+  if (s.nrows == 0) return;
+  unsigned g = xcd_grid(grid_for(s.nslices, 4));
+  k_sell_spmv<T><<<g, 256, 0, ctx->stream>>>(s.nrows, s.nslices, s.rows, s.slice_ptr, s.col,
+                                             (const T *)s.val, (T)alpha, (const T *)x, (T)beta,
+                                             (const T *)z, (T *)out);
+  FLGPU_HIP(hipGetLastError());
+}
+
+void spmv(const flgpu_mat *A, double alpha, const void *x, double beta, const void *z,
+          void *out) {
+  if (A->prec == FLGPU_F64) launch_spmv<double>(A->ctx, A->sell, alpha, x, beta, z, out);
+  else launch_spmv<float>(A->ctx, A->sell, alpha, x, beta, z, out);
+}
+
+void spmv_t(const flgpu_mat *A, double alpha, const void *x, double beta, void *out) {
+  mat_build_transpose(A);
+  spmv(A->transpose, alpha, x, beta, out, out);
+}
+
+template <class T>
+__global__ void k_jacobi(int n, T damp, const T *__restrict__ diag, T *x, T *r) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T ri = r[i] / diag[i];
+  r[i] = ri;
+  x[i] += damp * ri;
+}
+
+void jacobi_iterate(const flgpu_mat *A, double damp, void *x, void *r) {
+  int n = A->nrows;
+  if (n == 0) return;
+  if (A->prec == FLGPU_F64)
+    k_jacobi<double><<<grid_for(n, 256), 256, 0, A->ctx->stream>>>(
+        n, damp, (const double *)A->diag, (double *)x, (double *)r);
+  else
+    k_jacobi<float><<<grid_for(n, 256), 256, 0, A->ctx->stream>>>(
+        n, (float)damp, (const float *)A->diag, (float *)x, (float *)r);
+  FLGPU_HIP(hipGetLastError());
+}
+
+// One colour of a <sor> sweep: rows of one colour do not couple, so the
+// sequential sweep over them equals this parallel update.
+template <class T>
+__global__ __launch_bounds__(256) void k_sor_colour(int nsub, int nslices, const int *__restrict__ rows,
+                                                    const long *__restrict__ slice_ptr,
+                                                    const int *__restrict__ col,
+                                                    const T *__restrict__ val,
+                                                    const T *__restrict__ diag, T omega, T *x,
+                                                    const T *b, T *r) {
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  int s = (int)lb * 4 + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  int lane = threadIdx.x & 63;
+  int li = s * 64 + lane;
+  long base = slice_ptr[s];
+  int w = (int)((slice_ptr[s + 1] - base) >> 6);
+  if (li >= nsub) return;
+  int row = rows[li];
+  T corr = b[row];
+  const T *vp = val + base + lane;
+  const int *cp = col + base + lane;
+#pragma unroll 4
+  for (int k = 0; k < w; k++) {
+    T a = ld_stream(vp + (long)k * 64);
+    int c = ld_stream(cp + (long)k * 64);
+    corr += (-x[c]) * a;
+  }
+  corr = corr / diag[row];
+  r[row] = corr;
+  x[row] += omega * corr;
+}
+
+void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const void *b, void *r) {
+  FLGPU_REQUIRE(A->ncolours > 0, "<sor> needs a colouring of the matrix");
+  for (int k = 0; k < A->ncolours; k++) {
+    const Sell &s = A->colour_sell[backwards ? A->ncolours - 1 - k : k];
+    if (s.nrows == 0) continue;
+    unsigned g = xcd_grid(grid_for(s.nslices, 4));
+    if (A->prec == FLGPU_F64)
+      k_sor_colour<double><<<g, 256, 0, A->ctx->stream>>>(
+          s.nrows, s.nslices, s.rows, s.slice_ptr, s.col, (const double *)s.val,
+          (const double *)A->diag, omega, (double *)x, (const double *)b, (double *)r);
+    else
+      k_sor_colour<float><<<g, 256, 0, A->ctx->stream>>>(
+          s.nrows, s.nslices, s.rows, s.slice_ptr, s.col, (const float *)s.val,
+          (const float *)A->diag, (float)omega, (float *)x, (const float *)b, (float *)r);
+    FLGPU_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace flgpu
+
+using namespace flgpu;
+
+// ---- C-ABI ---------------------------------------------------------------------------
+
+extern "C" int flgpu_csr_upload(flgpu_ctx *ctx, int nrows, int ncols, long nnz, const int *rowptr,
+                                const int *colidx, const double *vals, int prec,
+                                flgpu_mat **out) {
+  return guard([&] {
+    FLGPU_REQUIRE(ctx && out && rowptr, "ctx, rowptr and out must not be NULL");
+    FLGPU_REQUIRE(nnz == 0 || (colidx && vals), "colidx and vals must not be NULL");
+    FLGPU_REQUIRE(nrows >= 0 && ncols >= 0, "negative dimension");
+    FLGPU_REQUIRE(rowptr[0] == 0 && rowptr[nrows] == nnz, "rowptr must run from 0 to nnz");
+    for (int r = 0; r < nrows; r++) {
+      FLGPU_REQUIRE(rowptr[r + 1] >= rowptr[r], "rowptr must be non-decreasing");
+      for (int e = rowptr[r]; e < rowptr[r + 1]; e++) {
+        FLGPU_REQUIRE(colidx[e] >= 0 && colidx[e] < ncols, "column index out of range");
+        FLGPU_REQUIRE(e == rowptr[r] || colidx[e] > colidx[e - 1],
+                      "column indices must ascend strictly within a row");
+      }
+    }
+    CTX_LOCK(ctx);
+    flgpu_mat *A = mat_new(ctx, nrows, ncols, nnz, prec);
+    try {
+      FLGPU_HIP(hipMemcpyAsync(A->rowptr, rowptr, sizeof(int) * (nrows + 1), hipMemcpyHostToDevice,
+                               ctx->stream));
+      if (nnz)
+        FLGPU_HIP(hipMemcpyAsync(A->colidx, colidx, sizeof(int) * nnz, hipMemcpyHostToDevice,
+                                 ctx->stream));
+      upload_vals(A, vals);
+      mat_finish(A);
+      FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+      mat_free(A);
+      throw;
+    }
+    *out = A;
+    return 0;
+  });
+}
+
+extern "C" int flgpu_csr_update_values(flgpu_mat *A, const double *vals) {
+  return guard([&] {
+    FLGPU_REQUIRE(A && (vals || A->nnz == 0), "arguments must not be NULL");
+    CTX_LOCK(A->ctx);
+    upload_vals(A, vals);
+    mat_refresh_values(A);
+    FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+    return 0;
+  });
+}
+
+extern "C" int flgpu_csr_download(const flgpu_mat *A, int *rowptr, int *colidx, double *vals) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    CTX_LOCK(A->ctx);
+    std::vector<int> rp, ci;
+    std::vector<double> v;
+    download_csr(A, rp, ci, vals ? &v : nullptr);
+    if (rowptr) std::memcpy(rowptr, rp.data(), sizeof(int) * rp.size());
+    if (colidx) std::memcpy(colidx, ci.data(), sizeof(int) * ci.size());
+    if (vals) std::memcpy(vals, v.data(), sizeof(double) * v.size());
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mat_info(const flgpu_mat *A, int *nrows, int *ncols, long *nnz, int *prec) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    if (nrows) *nrows = A->nrows;
+    if (ncols) *ncols = A->ncols;
+    if (nnz) *nnz = A->nnz;
+    if (prec) *prec = A->prec;
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mat_convert(const flgpu_mat *A, int prec, flgpu_mat **out) {
+  return guard([&] {
+    FLGPU_REQUIRE(A && out, "arguments must not be NULL");
+    CTX_LOCK(A->ctx);
+    *out = mat_convert(A, prec);
+    FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mat_destroy(flgpu_mat *A) {
+  return guard([&] {
+    if (!A) return 0;
+    CTX_LOCK(A->ctx);
+    FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+    mat_free(A);
+    return 0;
+  });
+}
+
+static void check_vec(const flgpu_mat *A, const flgpu_vec *v, long n, const char *what) {
+  FLGPU_REQUIRE(v != nullptr, std::string(what) + " must not be NULL");
+  FLGPU_REQUIRE(v->ctx == A->ctx, std::string(what) + " belongs to another context");
+  FLGPU_REQUIRE(v->n == n, std::string(what) + ": length mismatch");
+  FLGPU_REQUIRE(v->prec == A->prec, std::string(what) + ": precision differs from the matrix");
+}
+
+extern "C" int flgpu_gemm_nn(double alpha, const flgpu_mat *A, const flgpu_vec *y, double beta,
+                             flgpu_vec *x) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    check_vec(A, y, A->ncols, "y");
+    check_vec(A, x, A->nrows, "x");
+    FLGPU_REQUIRE(x->d != y->d, "gemm!: x and y must be different vectors");
+    CTX_LOCK(A->ctx);
+    spmv(A, alpha, y->d, beta, x->d, x->d);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_gemm_tn(double alpha, const flgpu_mat *A, const flgpu_vec *y, double beta,
+                             flgpu_vec *x) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    check_vec(A, y, A->nrows, "y");
+    check_vec(A, x, A->ncols, "x");
+    FLGPU_REQUIRE(x->d != y->d, "gemm!: x and y must be different vectors");
+    CTX_LOCK(A->ctx);
+    mat_build_transpose(A);
+    spmv(A->transpose, alpha, y->d, beta, x->d, x->d);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_compute_residual(const flgpu_mat *A, const flgpu_vec *x, const flgpu_vec *b,
+                                      flgpu_vec *r) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    FLGPU_REQUIRE(A->nrows == A->ncols, "compute-residual needs a square matrix");
+    check_vec(A, x, A->ncols, "x");
+    check_vec(A, b, A->nrows, "b");
+    check_vec(A, r, A->nrows, "r");
+    FLGPU_REQUIRE(x->d != r->d, "compute-residual: x and r must be different vectors");
+    CTX_LOCK(A->ctx);
+    spmv(A, -1.0, x->d, 1.0, b->d, r->d);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mat_set_colouring(flgpu_mat *A, int ncolours, const int *colour) {
+  return guard([&] {
+    FLGPU_REQUIRE(A && colour, "arguments must not be NULL");
+    CTX_LOCK(A->ctx);
+    set_colouring(A, ncolours, colour);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mat_colour_greedy(flgpu_mat *A, int *ncolours) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    FLGPU_REQUIRE(A->nrows == A->ncols, "colouring needs a square matrix");
+    CTX_LOCK(A->ctx);
+    std::vector<int> rp, ci;
+    download_csr(A, rp, ci, nullptr);
+    std::vector<int> colour(A->nrows, -1);
+    int nc = 0;
+    std::vector<int> used;
+    for (int r = 0; r < A->nrows; r++) {
+      used.assign(nc + 1, 0);
+      for (int e = rp[r]; e < rp[r + 1]; e++) {
+        int c = ci[e];
+        if (c != r && colour[c] >= 0) used[colour[c]] = 1;
+      }
+      // symmetric pattern assumed for the reverse couplings; check columns
+      int k = 0;
+      while (k < nc && used[k]) k++;
+      colour[r] = k;
+      if (k == nc) nc++;
+    }
+    set_colouring(A, nc, colour.data());
+    if (ncolours) *ncolours = nc;
+    return 0;
+  });
+}
+
+extern "C" int flgpu_jacobi_iterate(const flgpu_mat *A, double damp, flgpu_vec *x, flgpu_vec *r) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    FLGPU_REQUIRE(A->nrows == A->ncols, "<jacobi> needs a square matrix");
+    check_vec(A, x, A->nrows, "x");
+    check_vec(A, r, A->nrows, "r");
+    CTX_LOCK(A->ctx);
+    jacobi_iterate(A, damp, x->d, r->d);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_sor_iterate(const flgpu_mat *A, double omega, int backwards, flgpu_vec *x,
+                                 const flgpu_vec *b, flgpu_vec *r) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    check_vec(A, x, A->nrows, "x");
+    check_vec(A, b, A->nrows, "b");
+    check_vec(A, r, A->nrows, "r");
+    CTX_LOCK(A->ctx);
+    sor_sweep(A, omega, backwards, x->d, b->d, r->d);
+    return 0;
+  });
+}

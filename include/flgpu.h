@@ -1,0 +1,258 @@
+/*
+ * flgpu.h -- C-ABI of libflgpu.so, the MI355X (gfx950) implementation of
+ * femlisp's solver hot path for Lagrange FE discretisations.
+ *
+ * Conventions (those of femlisp's existing foreign bindings, e.g.
+ * interface/superlu.c:51-88 reached through src/alien/superlu.lisp:57-75 and
+ * fl.port:def-function / foreign-call, src/basic/port.lisp:401-547):
+ *   - every entry point returns an int status: 0 = ok, < 0 = argument or
+ *     runtime error, > 0 = numerical failure; flgpu_last_error() returns the
+ *     message of the last failing call on the calling thread;
+ *   - host arrays are caller-owned and only valid during the call (femlisp
+ *     pins them for the call's duration only): the library copies them and
+ *     never keeps host pointers;
+ *   - host values are always double (femlisp's double-float arrays), indices
+ *     int32 (src/matlisp/compressed.lisp:51-53); device storage may be FP64
+ *     or FP32 (FLGPU_F64 / FLGPU_F32);
+ *   - device memory is owned by opaque handles that the caller destroys;
+ *   - calls on distinct contexts may run concurrently from any thread; calls
+ *     on one context are serialised by the context's lock and run in order
+ *     on the context's HIP stream.
+ *
+ * Vector arguments follow femlisp's argument order: gemm!-style calls take
+ * (alpha A y beta x) and update x; iterators take (x b r) and update x (and
+ * r where femlisp does).
+ */
+#ifndef FLGPU_H
+#define FLGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FLGPU_F64 0
+#define FLGPU_F32 1
+
+typedef struct flgpu_ctx flgpu_ctx;
+typedef struct flgpu_vec flgpu_vec;
+typedef struct flgpu_mat flgpu_mat;
+typedef struct flgpu_mg flgpu_mg;
+typedef struct flgpu_unet flgpu_unet;
+
+/* ---- errors / context ---------------------------------------------------- */
+
+/* Message of the last failing call on this thread (cf. the int status that
+ * c_superlu returns, interface/superlu.c:51-88). */
+int flgpu_last_error(char *buf, int len);
+
+/* Context = one device + one HIP stream + a lock.  Replaces femlisp's
+ * per-call foreign library state (src/alien/alien.lisp:42-48). */
+int flgpu_ctx_create(int device, flgpu_ctx **out);
+int flgpu_ctx_destroy(flgpu_ctx *ctx);
+int flgpu_ctx_synchronize(flgpu_ctx *ctx);
+/* name of the device, number of CUs, bytes of device memory */
+int flgpu_ctx_info(flgpu_ctx *ctx, char *name, int len, int *cus, long *mem);
+
+/* ---- vectors (src/matlisp/vector.lisp:91-152,224-326;
+ *      src/matlisp/sparse-vector.lisp:248-280) -------------------------------- */
+
+/* zero-initialised, like make-domain-vector-for (src/matlisp/matrix.lisp:337) */
+int flgpu_vec_create(flgpu_ctx *ctx, long n, int prec, flgpu_vec **out);
+int flgpu_vec_destroy(flgpu_vec *v);
+int flgpu_vec_info(const flgpu_vec *v, long *n, int *prec);
+int flgpu_vec_upload(flgpu_vec *v, const double *host);
+int flgpu_vec_download(const flgpu_vec *v, double *host);
+/* fill! / x<-0 */
+int flgpu_vec_fill(flgpu_vec *v, double value);
+/* copy! x y : y <- x (converts precision) */
+int flgpu_vec_copy(const flgpu_vec *x, flgpu_vec *y);
+/* scal! a x : x <- a x */
+int flgpu_vec_scal(double a, flgpu_vec *x);
+/* axpy! a x y : y <- a x + y */
+int flgpu_vec_axpy(double a, const flgpu_vec *x, flgpu_vec *y);
+/* dot x y; deterministic fixed-order reduction in FP64 */
+int flgpu_vec_dot(const flgpu_vec *x, const flgpu_vec *y, double *out);
+/* norm x 2 / l2-norm */
+int flgpu_vec_norm2(const flgpu_vec *x, double *out);
+
+/* ---- sparse matrices (src/matlisp/sparse-matrix.lisp:378-442) ------------ */
+
+/* Upload a CSR matrix (rows ascending, columns ascending within a row), e.g.
+ * the result of femlisp's sparse-matrix->ccs applied to the transpose
+ * (src/matlisp/sparselu.lisp:226-333) with keys numbered by
+ * sort-lexicographically (src/mesh/meshgen.lisp:214-217).  prec selects the
+ * device storage.  Explicit zeros are kept. */
+int flgpu_csr_upload(flgpu_ctx *ctx, int nrows, int ncols, long nnz,
+                     const int *rowptr, const int *colidx, const double *vals,
+                     int prec, flgpu_mat **out);
+/* replace the values of an existing pattern (vals[nnz] in CSR order) */
+int flgpu_csr_update_values(flgpu_mat *A, const double *vals);
+/* any of the output arrays may be NULL */
+int flgpu_csr_download(const flgpu_mat *A, int *rowptr, int *colidx,
+                       double *vals);
+int flgpu_mat_info(const flgpu_mat *A, int *nrows, int *ncols, long *nnz,
+                   int *prec);
+/* copy of A stored in another precision */
+int flgpu_mat_convert(const flgpu_mat *A, int prec, flgpu_mat **out);
+int flgpu_mat_destroy(flgpu_mat *A);
+
+/* gemm-nn! alpha A y beta x : x <- beta x + alpha A y
+ * (sparse-matrix.lisp:393-405; src/matlisp/matrix.lisp:213-240). */
+int flgpu_gemm_nn(double alpha, const flgpu_mat *A, const flgpu_vec *y,
+                  double beta, flgpu_vec *x);
+/* gemm-tn! alpha A y beta x : x <- beta x + alpha A^T y
+ * (sparse-matrix.lisp:406-433); the transpose is built on first use. */
+int flgpu_gemm_tn(double alpha, const flgpu_mat *A, const flgpu_vec *y,
+                  double beta, flgpu_vec *x);
+/* compute-residual A x b r : r <- b - A x; b may be the same vector as r, in
+ * which case r <- r - A x exactly as femlisp's copy!+gemm! does
+ * (src/iteration/linit.lisp:73-80). */
+int flgpu_compute_residual(const flgpu_mat *A, const flgpu_vec *x,
+                           const flgpu_vec *b, flgpu_vec *r);
+
+/* ---- smoothers (src/iteration/linit.lisp) ----------------------------------- */
+
+/* Ordering of <sor> sweeps: rows sorted by (colour, index), i.e. <sor> with a
+ * :compare function (linit.lisp:258-263,293-295).  Rows of one colour must
+ * not couple.  colour[nrows] in [0, ncolours). */
+int flgpu_mat_set_colouring(flgpu_mat *A, int ncolours, const int *colour);
+/* greedy colouring of the pattern (rows visited in index order) when the
+ * caller has none; returns the number of colours in *ncolours. */
+int flgpu_mat_colour_greedy(flgpu_mat *A, int *ncolours);
+
+/* one <jacobi> iterate (linit.lisp:220-250): r_i <- r_i / A_ii (gesv! in
+ * place), x_i <- x_i + damp r_i.  Needs an up-to-date residual. */
+int flgpu_jacobi_iterate(const flgpu_mat *A, double damp, flgpu_vec *x,
+                         flgpu_vec *r);
+/* one <sor> sweep (linit.lisp:288-323) in colour order (backwards = 1
+ * sweeps the colours last to first): per row corr = b_i - sum_j A_ij x_j,
+ * corr /= A_ii, r_i <- corr, x_i += omega corr.  b may be r. */
+int flgpu_sor_iterate(const flgpu_mat *A, double omega, int backwards,
+                      flgpu_vec *x, const flgpu_vec *b, flgpu_vec *r);
+
+/* ---- structured lattices: GPU assembly (src/discretization/fedisc.lisp:
+ *      73-268, ellsys-fe.lisp:68-218, fe.lisp:235-244,502-535,
+ *      constraints.lisp:323-411, sparseas.lisp:232-272) ------------------ */
+
+/* Assemble -div(kappa grad u) = f with Lagrange Q`order` elements on the unit
+ * `dim`-cube with n cells per axis, zero Dirichlet boundary, femlisp's full
+ * sparsity pattern (all DOF pairs sharing a cell, Dirichlet rows/columns kept
+ * as explicit zeros, identity diagonal), canonical lattice numbering.
+ * kappa: NULL (kappa = 1) or a device vector of nk^dim per-cell values on a
+ * grid of nk cells per axis (nk a multiple of n; a quadrature point samples
+ * the coefficient cell it lies in, the cell on the upper side when it lies on
+ * a face).  mode: FLGPU_ASM_F64 (FP64 FMA element kernel) or FLGPU_ASM_MFMA
+ * (element contractions on MFMA: FP64 MFMA for prec F64, BF16 inputs with
+ * FP32 accumulation for prec F32).  rhs may be NULL. */
+#define FLGPU_ASM_F64 0
+#define FLGPU_ASM_MFMA 1
+int flgpu_lattice_assemble(flgpu_ctx *ctx, int dim, int order, int n,
+                           const flgpu_vec *kappa, int nk, double f, int prec,
+                           int mode, flgpu_mat **A, flgpu_vec **rhs);
+/* Re-assemble the values of an existing lattice matrix for a new kappa. */
+int flgpu_lattice_reassemble(flgpu_mat *A, const flgpu_vec *kappa, int nk,
+                             int mode);
+/* Constrained :local interpolation from the lattice with nc cells per axis to
+ * the one with 2 nc cells (fetransfer.lisp:73-138, constraints.lisp:58-72):
+ * rows fine DOFs, columns coarse DOFs; fine Dirichlet rows and coarse
+ * Dirichlet columns zeroed but kept. */
+int flgpu_lattice_interpolation(flgpu_ctx *ctx, int dim, int order, int nc,
+                                int prec, flgpu_mat **P);
+/* lattice colouring for <sor>: coordinate parity (Q1) / coordinate mod 3 (Q2) */
+int flgpu_lattice_colouring(flgpu_mat *A, int dim, int order, int n);
+/* The element kernel alone (discretize-locally for a batch of cells,
+ * ellsys-fe.lisp:68-127): element matrices of cells first_cell ..
+ * first_cell+ncells-1 (lattice cell order), local DOFs in lattice order
+ * inside the cell, out[ncells][nloc][nloc] as double. */
+int flgpu_lattice_element_matrices(flgpu_ctx *ctx, int dim, int order, int n,
+                                   const flgpu_vec *kappa, int nk, int prec,
+                                   int mode, long first_cell, long ncells,
+                                   double *out);
+
+/* ---- geometric multigrid (src/iteration/multigrid.lisp:55-393,
+ *      geomg.lisp:109-142) ------------------------------------------------- */
+
+#define FLGPU_SMOOTH_JACOBI 0
+#define FLGPU_SMOOTH_SOR 1
+
+typedef struct flgpu_mg_params {
+  int smoother;    /* FLGPU_SMOOTH_JACOBI (damped) or FLGPU_SMOOTH_SOR (colour order) */
+  double omega;    /* <jacobi> damp / <sor> omega */
+  int pre_steps;   /* product-iterator steps, linit.lisp:94-120 */
+  int post_steps;
+  int base_level;  /* coarse-grid solve level: dense FP64 LU/inverse */
+  int symmetric;   /* 1: post-smoothing sweeps the colours backwards */
+  int prec;        /* storage/arithmetic of the V-cycle: FLGPU_F64 / FLGPU_F32 */
+} flgpu_mg_params;
+
+/* Hierarchy from caller matrices: A[0..nlevels-1] level matrices (coarsest
+ * first), I[0..nlevels-2] interpolations level l -> l+1 (rows = DOFs of
+ * level l+1), exactly femlisp's :a-vec/:i-vec blackboard
+ * (multigrid.lisp:139-141).  The matrices are copied (converted to
+ * params->prec); the caller keeps ownership of its handles.  Matrices
+ * without a colouring get a greedy one when the smoother is <sor>. */
+int flgpu_mg_create(flgpu_ctx *ctx, int nlevels, flgpu_mat *const *A,
+                    flgpu_mat *const *I, const flgpu_mg_params *params,
+                    flgpu_mg **out);
+/* Hierarchy assembled on the GPU on the unit cube: level l has n0 * 2^l cells
+ * per axis, l = 0..nlevels-1; every level is re-discretised with the
+ * coefficient sampled at its own quadrature points (geomg.lisp:122-126). */
+int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n0,
+                            int nlevels, const flgpu_vec *kappa, int nk,
+                            double f, int asm_mode,
+                            const flgpu_mg_params *params, flgpu_mg **out,
+                            flgpu_vec **rhs);
+int flgpu_mg_destroy(flgpu_mg *mg);
+/* the top-level matrix of the hierarchy (owned by mg) */
+int flgpu_mg_top_matrix(flgpu_mg *mg, flgpu_mat **A);
+/* The <mg-iteration> iterator (multigrid.lisp:368-391): one V-cycle on
+ * (x, b, r) with r = b - A x up to date; x is updated, r and b are used as
+ * femlisp uses them (b may be the same vector as r: femlisp's <cg> calls
+ * (iterate q w w), krylow.lisp:118-121).  Vectors may be FP64 with an FP32
+ * hierarchy: then x <- x + V(r) with the cycle in FP32 (b must not be r). */
+int flgpu_mg_apply(flgpu_mg *mg, flgpu_vec *x, flgpu_vec *b, flgpu_vec *r);
+
+/* ---- solvers (src/iteration/krylow.lisp:67-132, linsolve.lisp:58-73,
+ *      solve.lisp:119-165, iterate.lisp:239-250) --------------------------- */
+
+#define FLGPU_OUTER_MG 0       /* the V-cycle is the iteration */
+#define FLGPU_OUTER_CG 1       /* <cg> with the V-cycle as preconditioner */
+#define FLGPU_OUTER_CG_PLAIN 2 /* <cg> without preconditioner */
+#define FLGPU_OUTER_FCG 3      /* flexible CG (Polak-Ribiere beta) */
+#define FLGPU_STOP_REDUCTION 0 /* success-if (< :reduction tol) */
+#define FLGPU_STOP_DEFNORM 1   /* success-if (< :defnorm tol) */
+
+typedef struct flgpu_solve_params {
+  int outer;           /* FLGPU_OUTER_* */
+  int alias;           /* 1: precondition as femlisp's <cg> does, (iterate q w w) */
+  int stop;            /* FLGPU_STOP_* */
+  double tol;
+  int maxsteps;        /* failure-if (> :step maxsteps) */
+  int true_residual;   /* 1: recompute r = b - A x in FP64 when the recurrence
+                          residual meets tol, continue if the true one does not */
+  int learned;         /* 1: add the learned coarse correction (FCG only) */
+  int stall_window;    /* learned: fall back to plain GMG-FCG when the true
+                          residual has not dropped by 10x in this many steps */
+} flgpu_solve_params;
+
+typedef struct flgpu_solve_report {
+  int status;                 /* 0 success, 1 failure (step limit) */
+  int steps;
+  double initial_defnorm;
+  double final_defnorm;       /* the norm femlisp observes (recurrence for CG) */
+  double true_residual_norm;  /* ||b - A x||_2 in FP64 at exit */
+  int fell_back;              /* learned: 1 if the plain fallback took over */
+  double seconds;             /* wall time of the iteration loop */
+} flgpu_solve_report;
+
+/* Solve A x = b from x = 0 with femlisp's <linear-solver> loop.  mg may be
+ * NULL for FLGPU_OUTER_CG_PLAIN.  defnorms (may be NULL) receives
+ * ||r_k||_2 for k = 0..steps, at most max_defnorms values. */
+int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_params *p,
+                const flgpu_vec *b, flgpu_vec *x, flgpu_solve_report *report,
+                double *defnorms, int max_defnorms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLGPU_H */

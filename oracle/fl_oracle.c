@@ -1,0 +1,952 @@
+/*
+ * fl_oracle.c -- CPU restatement of femlisp's solver hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (parity checker + CPU baseline).  Not part of the
+ * product library; see fl_oracle.h.  Each routine names the femlisp source it
+ * restates (paths relative to the femlisp tree).
+ */
+#include "fl_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define FLO_MAXDIM 3
+
+/* ------------------------------------------------------------------------ */
+/* lattice                                                                   */
+/* ------------------------------------------------------------------------ */
+
+static long ipow(long b, int e) {
+  long r = 1;
+  while (e-- > 0) r *= b;
+  return r;
+}
+
+long flo_lattice_ndofs(int dim, int order, int n) {
+  return ipow((long)order * n + 1, dim);
+}
+
+/* Range of lattice coordinates that share a cell with coordinate i along one
+ * axis: cells c with order*c <= i <= order*c+order.  A DOF couples with every
+ * DOF of every cell containing it (the pattern femlisp's assemble-interior
+ * creates, src/discretization/fedisc.lisp:144-163). */
+static void axis_range(int order, int n, int i, int *lo, int *hi) {
+  int cmin, cmax;
+  if (i % order == 0) {
+    cmin = i / order - 1;
+    cmax = i / order;
+    if (cmin < 0) cmin = 0;
+    if (cmax > n - 1) cmax = n - 1;
+  } else {
+    cmin = cmax = i / order;
+  }
+  *lo = order * cmin;
+  *hi = order * cmax + order;
+}
+
+long flo_lattice_nnz(int dim, int order, int n) {
+  long per_axis = 0;
+  int M = order * n + 1;
+  for (int i = 0; i < M; i++) {
+    int lo, hi;
+    axis_range(order, n, i, &lo, &hi);
+    per_axis += hi - lo + 1;
+  }
+  return ipow(per_axis, dim);
+}
+
+static void unrank(long idx, int dim, int M, int *c) {
+  for (int k = dim - 1; k >= 0; k--) {
+    c[k] = (int)(idx % M);
+    idx /= M;
+  }
+}
+
+static long rank(const int *c, int dim, int M) {
+  long idx = 0;
+  for (int k = 0; k < dim; k++) idx = idx * M + c[k];
+  return idx;
+}
+
+int flo_lattice_pattern(int dim, int order, int n, int *rowptr, int *colidx) {
+  int M = order * n + 1;
+  long N = ipow(M, dim);
+  long pos = 0;
+  for (long r = 0; r < N; r++) {
+    int c[FLO_MAXDIM], lo[FLO_MAXDIM], hi[FLO_MAXDIM], q[FLO_MAXDIM];
+    unrank(r, dim, M, c);
+    for (int k = 0; k < dim; k++) axis_range(order, n, c[k], &lo[k], &hi[k]);
+    rowptr[r] = (int)pos;
+    for (int k = 0; k < dim; k++) q[k] = lo[k];
+    for (;;) {
+      if (colidx) colidx[pos] = (int)rank(q, dim, M);
+      pos++;
+      int k = dim - 1;
+      while (k >= 0 && q[k] == hi[k]) {
+        q[k] = lo[k];
+        k--;
+      }
+      if (k < 0) break;
+      q[k]++;
+    }
+  }
+  rowptr[N] = (int)pos;
+  return 0;
+}
+
+/* Every DOF on an :external-boundary patch is constrained, without a
+ * dimension filter (src/discretization/system-fe.lisp:43-44). */
+int flo_lattice_boundary(int dim, int order, int n, unsigned char *is_bdry) {
+  int M = order * n + 1;
+  long N = ipow(M, dim);
+  for (long r = 0; r < N; r++) {
+    int c[FLO_MAXDIM];
+    unrank(r, dim, M, c);
+    unsigned char b = 0;
+    for (int k = 0; k < dim; k++)
+      if (c[k] == 0 || c[k] == M - 1) b = 1;
+    is_bdry[r] = b;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* quadrature and Lagrange basis                                             */
+/* ------------------------------------------------------------------------ */
+
+static void legendre(int s, double x, double *p, double *dp) {
+  double p0 = 1.0, p1 = x;
+  if (s == 0) {
+    *p = 1.0;
+    *dp = 0.0;
+    return;
+  }
+  for (int k = 2; k <= s; k++) {
+    double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+    p0 = p1;
+    p1 = p2;
+  }
+  *p = p1;
+  *dp = s * (x * p1 - p0) / (x * x - 1.0);
+}
+
+/* s-point Gauss rule on [0,1] as femlisp builds it: Legendre roots x_i in
+ * ascending order, points (1 - x_i)/2, weights w_i/2
+ * (src/discretization/quadrature.lisp:149-176, gauss-rule-for-weight with
+ * n = 0).  Roots by Newton iteration instead of femlisp's interval method
+ * (quadrature.lisp:122-146); both converge to the double nearest the root up
+ * to the last ulp. */
+int flo_gauss_rule(int s, double *pts, double *wts) {
+  if (s < 1 || s > 16) return -1;
+  for (int i = 0; i < s; i++) {
+    /* i-th root in ascending order */
+    double x = -cos(M_PI * (i + 0.75) / (s + 0.5));
+    for (int it = 0; it < 100; it++) {
+      double p, dp;
+      legendre(s, x, &p, &dp);
+      double dx = p / dp;
+      x -= dx;
+      if (fabs(dx) < 1e-17) break;
+    }
+    double p, dp;
+    legendre(s, x, &p, &dp);
+    double w = 2.0 / ((1.0 - x * x) * dp * dp);
+    pts[i] = (1.0 - x) / 2.0;
+    wts[i] = w / 2.0;
+  }
+  return 0;
+}
+
+/* Uniform Lagrange nodes a/order on [0,1] (lagrange-coords-1d :uniform,
+ * src/discretization/lagrange.lisp:41-45) and the nodal basis with its
+ * derivative in product form. */
+void flo_lagrange_1d(int order, double t, double *phi, double *dphi) {
+  for (int a = 0; a <= order; a++) {
+    double ta = (double)a / order;
+    double v = 1.0;
+    for (int b = 0; b <= order; b++)
+      if (b != a) v *= (t - (double)b / order) / (ta - (double)b / order);
+    phi[a] = v;
+    double d = 0.0;
+    for (int m = 0; m <= order; m++) {
+      if (m == a) continue;
+      double term = 1.0 / (ta - (double)m / order);
+      for (int b = 0; b <= order; b++)
+        if (b != a && b != m) term *= (t - (double)b / order) / (ta - (double)b / order);
+      d += term;
+    }
+    if (dphi) dphi[a] = d;
+  }
+}
+
+/* Local element kernel for an axis-aligned cube cell of width H
+ * (discretize-locally, src/discretization/ellsys-fe.lisp:68-127,183-190):
+ * for every quadrature point q, weight = w_q * |det J| (fe-cell-geometry,
+ * src/discretization/fe.lisp:502-535), gradients G = Ghat J^-1 = Ghat / H,
+ *   K_ij += weight * (G_j . kappa) . G_i,   b_k += weight * phi_k * f.
+ * Quadrature rule: order+1 Gauss points per axis (quadrature-rule,
+ * src/discretization/fe.lisp:235-244). */
+int flo_element_matrix(int dim, int order, double H, const double *kappa_qp,
+                       double f, double *Ke, double *be) {
+  int s = order + 1;
+  double pts[16], wts[16];
+  flo_gauss_rule(s, pts, wts);
+  int nl1 = order + 1;
+  int nloc = (int)ipow(nl1, dim);
+  int nq = (int)ipow(s, dim);
+  double phi[16][8], dphi[16][8];
+  for (int q = 0; q < s; q++) flo_lagrange_1d(order, pts[q], phi[q], dphi[q]);
+  if (Ke) memset(Ke, 0, sizeof(double) * nloc * nloc);
+  if (be) memset(be, 0, sizeof(double) * nloc);
+  double detJ = 1.0;
+  for (int k = 0; k < dim; k++) detJ *= H;
+  double vals[729], grads[729][FLO_MAXDIM];
+  for (int q = 0; q < nq; q++) {
+    int qc[FLO_MAXDIM];
+    unrank(q, dim, s, qc);
+    /* femlisp multiplies volume-at-point by the rule weight
+     * (fe.lisp:527); the product rule weight is the product of the factor
+     * weights (quadrature.lisp:178-184). */
+    double wq = 1.0;
+    for (int k = 0; k < dim; k++) wq *= wts[qc[k]];
+    double weight = detJ * wq;
+    for (int a = 0; a < nloc; a++) {
+      int ac[FLO_MAXDIM];
+      unrank(a, dim, nl1, ac);
+      double v = 1.0;
+      for (int k = 0; k < dim; k++) v *= phi[qc[k]][ac[k]];
+      vals[a] = v;
+      for (int m = 0; m < dim; m++) {
+        double g = 1.0;
+        for (int k = 0; k < dim; k++)
+          g *= (k == m) ? dphi[qc[k]][ac[k]] : phi[qc[k]][ac[k]];
+        grads[a][m] = g / H;
+      }
+    }
+    double kap = kappa_qp ? kappa_qp[q] : 1.0;
+    if (Ke) {
+      for (int i = 0; i < nloc; i++)
+        for (int j = 0; j < nloc; j++) {
+          double gg = 0.0;
+          for (int m = 0; m < dim; m++) gg += grads[j][m] * kap * grads[i][m];
+          Ke[i * nloc + j] += weight * gg;
+        }
+    }
+    if (be)
+      for (int k = 0; k < nloc; k++) be[k] += weight * vals[k] * f;
+  }
+  return 0;
+}
+
+/* coefficient cell sampled by quadrature point g (in (0,1)) of cell c on a
+ * level with n cells per axis; nk = coefficient grid cells per axis */
+static int kappa_axis_index(int c, double g, int n, int nk) {
+  int r = nk / n;
+  return c * r + (int)floor(g * r);
+}
+
+static long row_offset(int dim, int order, int n, const int *P, const int *Q) {
+  long off = 0;
+  for (int k = 0; k < dim; k++) {
+    int lo, hi;
+    axis_range(order, n, P[k], &lo, &hi);
+    off = off * (hi - lo + 1) + (Q[k] - lo);
+  }
+  return off;
+}
+
+/* Global assembly (assemble-interior, src/discretization/fedisc.lisp:108-206;
+ * scatter global-local-matrix-operation, sparseif.lisp:308-385) followed by
+ * Dirichlet elimination A <- S A S + P, b <- S(b - A r) + r with r = 0
+ * (constraints.lisp:323-411, fedisc.lisp:244-253, geomg.lisp:128-136).
+ * Cells are visited in lattice order; femlisp's cell order is hash order, so
+ * the summation order of contributions agrees only up to rounding. */
+int flo_assemble(int dim, int order, int n, const double *kappa, int nk,
+                 double f, double *vals, double *rhs) {
+  if (dim < 1 || dim > FLO_MAXDIM || order < 1 || order > 2 || n < 1) return -1;
+  if (kappa && (nk % n) != 0) return -2;
+  int M = order * n + 1;
+  long N = ipow(M, dim);
+  int *rowptr = (int *)malloc(sizeof(int) * (N + 1));
+  int *colidx = (int *)malloc(sizeof(int) * flo_lattice_nnz(dim, order, n));
+  flo_lattice_pattern(dim, order, n, rowptr, colidx);
+  long nnz = rowptr[N];
+  memset(vals, 0, sizeof(double) * nnz);
+  if (rhs) memset(rhs, 0, sizeof(double) * N);
+  int s = order + 1, nl1 = order + 1;
+  int nloc = (int)ipow(nl1, dim), nq = (int)ipow(s, dim);
+  double pts[16], wts[16];
+  flo_gauss_rule(s, pts, wts);
+  double H = 1.0 / n;
+  double Ke[729], be[27], kq[27];
+  long ncells = ipow(n, dim);
+  for (long cell = 0; cell < ncells; cell++) {
+    int cc[FLO_MAXDIM];
+    unrank(cell, dim, n, cc);
+    for (int q = 0; q < nq; q++) {
+      int qc[FLO_MAXDIM];
+      unrank(q, dim, s, qc);
+      if (kappa) {
+        int kc[FLO_MAXDIM];
+        for (int k = 0; k < dim; k++) kc[k] = kappa_axis_index(cc[k], pts[qc[k]], n, nk);
+        kq[q] = kappa[rank(kc, dim, nk)];
+      } else {
+        kq[q] = 1.0;
+      }
+    }
+    flo_element_matrix(dim, order, H, kq, f, Ke, be);
+    for (int a = 0; a < nloc; a++) {
+      int ac[FLO_MAXDIM], P[FLO_MAXDIM];
+      unrank(a, dim, nl1, ac);
+      for (int k = 0; k < dim; k++) P[k] = order * cc[k] + ac[k];
+      long row = rank(P, dim, M);
+      if (rhs) rhs[row] += be[a];
+      for (int b = 0; b < nloc; b++) {
+        int bc[FLO_MAXDIM], Q[FLO_MAXDIM];
+        unrank(b, dim, nl1, bc);
+        for (int k = 0; k < dim; k++) Q[k] = order * cc[k] + bc[k];
+        vals[rowptr[row] + row_offset(dim, order, n, P, Q)] += Ke[a * nloc + b];
+      }
+    }
+  }
+  unsigned char *bd = (unsigned char *)malloc(N);
+  flo_lattice_boundary(dim, order, n, bd);
+  for (long r = 0; r < N; r++) {
+    for (int e = rowptr[r]; e < rowptr[r + 1]; e++) {
+      int c = colidx[e];
+      if (bd[r] || bd[c]) vals[e] = (bd[r] && c == r) ? 1.0 : 0.0;
+    }
+    if (rhs && bd[r]) rhs[r] = 0.0;
+  }
+  free(bd);
+  free(rowptr);
+  free(colidx);
+  return 0;
+}
+
+/* 1-D interpolation row of fine coordinate F (fine level 2*nc cells):
+ * coarse columns and weights = coarse basis evaluated at the fine node
+ * (compute-local-imatrix, src/discretization/fetransfer.lisp:73-138), entries
+ * below *ipt-tolerance* = 1e-10 dropped (fetransfer.lisp:41-43,130). */
+static int interp_1d(int order, int nc, int F, int *cols, double *w) {
+  int cell = F / (2 * order);
+  if (cell > nc - 1) cell = nc - 1;
+  double tau = (double)(F - 2 * order * cell) / (2.0 * order);
+  double phi[8];
+  flo_lagrange_1d(order, tau, phi, NULL);
+  int cnt = 0;
+  for (int a = 0; a <= order; a++) {
+    if (fabs(phi[a]) < 1e-10) continue;
+    cols[cnt] = order * cell + a;
+    w[cnt] = phi[a];
+    cnt++;
+  }
+  return cnt;
+}
+
+long flo_interpolation(int dim, int order, int nc, int *rowptr, int *colidx,
+                       double *vals) {
+  int Mf = 2 * order * nc + 1, Mc = order * nc + 1;
+  long Nf = ipow(Mf, dim);
+  long pos = 0;
+  for (long r = 0; r < Nf; r++) {
+    int F[FLO_MAXDIM], cnt[FLO_MAXDIM], cols[FLO_MAXDIM][8], q[FLO_MAXDIM];
+    double w[FLO_MAXDIM][8];
+    unrank(r, dim, Mf, F);
+    int fb = 0;
+    for (int k = 0; k < dim; k++) {
+      cnt[k] = interp_1d(order, nc, F[k], cols[k], w[k]);
+      if (F[k] == 0 || F[k] == Mf - 1) fb = 1;
+    }
+    rowptr[r] = (int)pos;
+    for (int k = 0; k < dim; k++) q[k] = 0;
+    for (;;) {
+      if (colidx) {
+        int C[FLO_MAXDIM];
+        double v = 1.0;
+        int cb = 0;
+        for (int k = 0; k < dim; k++) {
+          C[k] = cols[k][q[k]];
+          v *= w[k][q[k]];
+          if (C[k] == 0 || C[k] == Mc - 1) cb = 1;
+        }
+        colidx[pos] = (int)rank(C, dim, Mc);
+        /* remove-projection-range :row-p for fine Dirichlet rows and
+         * :column-p for coarse Dirichlet columns (constraints.lisp:58-72) */
+        vals[pos] = (fb || cb) ? 0.0 : v;
+      }
+      pos++;
+      int k = dim - 1;
+      while (k >= 0 && q[k] == cnt[k] - 1) {
+        q[k] = 0;
+        k--;
+      }
+      if (k < 0) break;
+      q[k]++;
+    }
+  }
+  rowptr[Nf] = (int)pos;
+  return pos;
+}
+
+/* ------------------------------------------------------------------------ */
+/* algebra                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* gemm-nn! on <sparse-matrix> (src/matlisp/sparse-matrix.lisp:378-405):
+ * per row x_i <- beta*x_i, then for every block x_i += (alpha*y_j)*A_ij
+ * (a 1x1 dgemm, src/matlisp/standard-matrix-blas.lisp:209-266). */
+void flo_spmv(int nrows, const int *rowptr, const int *colidx,
+              const double *vals, double alpha, const double *y, double beta,
+              double *x) {
+  for (int i = 0; i < nrows; i++) {
+    double acc = (beta == 0.0) ? 0.0 : beta * x[i];
+    for (int e = rowptr[i]; e < rowptr[i + 1]; e++)
+      acc += (alpha * y[colidx[e]]) * vals[e];
+    x[i] = acc;
+  }
+}
+
+/* gemm-tn! (sparse-matrix.lisp:406-433, column access):
+ * x_c <- beta*x_c + sum_r (alpha*y_r) * A_rc */
+void flo_spmv_t(int nrows, int ncols, const int *rowptr, const int *colidx,
+                const double *vals, double alpha, const double *y,
+                double beta, double *x) {
+  for (int c = 0; c < ncols; c++) x[c] = (beta == 0.0) ? 0.0 : beta * x[c];
+  for (int r = 0; r < nrows; r++)
+    for (int e = rowptr[r]; e < rowptr[r + 1]; e++)
+      x[colidx[e]] += (alpha * y[r]) * vals[e];
+}
+
+void flo_csr_transpose(int nrows, int ncols, const int *rowptr,
+                       const int *colidx, const double *vals, int *trowptr,
+                       int *tcolidx, double *tvals) {
+  int *cnt = (int *)calloc(ncols + 1, sizeof(int));
+  for (int e = 0; e < rowptr[nrows]; e++) cnt[colidx[e] + 1]++;
+  for (int c = 0; c < ncols; c++) cnt[c + 1] += cnt[c];
+  memcpy(trowptr, cnt, sizeof(int) * (ncols + 1));
+  for (int r = 0; r < nrows; r++)
+    for (int e = rowptr[r]; e < rowptr[r + 1]; e++) {
+      int p = cnt[colidx[e]]++;
+      tcolidx[p] = r;
+      if (tvals) tvals[p] = vals[e];
+    }
+  free(cnt);
+}
+
+static double dot(long n, const double *x, const double *y) {
+  double s = 0.0;
+  for (long i = 0; i < n; i++) s += x[i] * y[i];
+  return s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* multigrid hierarchy                                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int nrows, ncols;
+  int *rowptr, *colidx;
+  double *vals;
+} csr_t;
+
+typedef struct {
+  int n;
+  long N;
+  csr_t A;
+  csr_t I;  /* interpolation level -> level+1 (rows: level+1 DOFs) */
+  csr_t IT; /* its transpose */
+  double *diag;
+  unsigned char *bdry;
+  double *lu; /* dense LU of A (base level), row-major */
+  int lu_n;
+  double *sol, *rhs, *res;
+  int residual_p;
+} level_t;
+
+struct flo_mg {
+  int dim, order, n0, levels;
+  level_t *lv;
+  double *rhs_top;
+};
+
+static void csr_free(csr_t *m) {
+  free(m->rowptr);
+  free(m->colidx);
+  free(m->vals);
+  memset(m, 0, sizeof(*m));
+}
+
+flo_mg *flo_mg_create(int dim, int order, int n0, int levels,
+                      const double *kappa, int nk, double f) {
+  if (dim < 1 || dim > 3 || order < 1 || order > 2 || n0 < 1 || levels < 1) return NULL;
+  flo_mg *mg = (flo_mg *)calloc(1, sizeof(flo_mg));
+  mg->dim = dim;
+  mg->order = order;
+  mg->n0 = n0;
+  mg->levels = levels;
+  mg->lv = (level_t *)calloc(levels, sizeof(level_t));
+  int ntop = n0 << (levels - 1);
+  if (kappa && nk % ntop != 0) {
+    free(mg->lv);
+    free(mg);
+    return NULL;
+  }
+  for (int l = 0; l < levels; l++) {
+    level_t *L = &mg->lv[l];
+    L->n = n0 << l;
+    L->N = flo_lattice_ndofs(dim, order, L->n);
+    long nnz = flo_lattice_nnz(dim, order, L->n);
+    L->A.nrows = L->A.ncols = (int)L->N;
+    L->A.rowptr = (int *)malloc(sizeof(int) * (L->N + 1));
+    L->A.colidx = (int *)malloc(sizeof(int) * nnz);
+    L->A.vals = (double *)malloc(sizeof(double) * nnz);
+    flo_lattice_pattern(dim, order, L->n, L->A.rowptr, L->A.colidx);
+    double *rhs = (l == levels - 1) ? (double *)malloc(sizeof(double) * L->N) : NULL;
+    flo_assemble(dim, order, L->n, kappa, nk, f, L->A.vals, rhs);
+    if (rhs) mg->rhs_top = rhs;
+    L->diag = (double *)malloc(sizeof(double) * L->N);
+    for (long r = 0; r < L->N; r++)
+      for (int e = L->A.rowptr[r]; e < L->A.rowptr[r + 1]; e++)
+        if (L->A.colidx[e] == r) L->diag[r] = L->A.vals[e];
+    L->bdry = (unsigned char *)malloc(L->N);
+    flo_lattice_boundary(dim, order, L->n, L->bdry);
+    if (l < levels - 1) {
+      long Nf = flo_lattice_ndofs(dim, order, 2 * L->n);
+      L->I.nrows = (int)Nf;
+      L->I.ncols = (int)L->N;
+      L->I.rowptr = (int *)malloc(sizeof(int) * (Nf + 1));
+      long inz = flo_interpolation(dim, order, L->n, L->I.rowptr, NULL, NULL);
+      L->I.colidx = (int *)malloc(sizeof(int) * inz);
+      L->I.vals = (double *)malloc(sizeof(double) * inz);
+      flo_interpolation(dim, order, L->n, L->I.rowptr, L->I.colidx, L->I.vals);
+      L->IT.nrows = (int)L->N;
+      L->IT.ncols = (int)Nf;
+      L->IT.rowptr = (int *)malloc(sizeof(int) * (L->N + 1));
+      L->IT.colidx = (int *)malloc(sizeof(int) * inz);
+      L->IT.vals = (double *)malloc(sizeof(double) * inz);
+      flo_csr_transpose((int)Nf, (int)L->N, L->I.rowptr, L->I.colidx, L->I.vals,
+                        L->IT.rowptr, L->IT.colidx, L->IT.vals);
+    }
+    L->sol = (double *)calloc(L->N, sizeof(double));
+    L->rhs = (double *)calloc(L->N, sizeof(double));
+    L->res = (double *)calloc(L->N, sizeof(double));
+  }
+  return mg;
+}
+
+void flo_mg_destroy(flo_mg *mg) {
+  if (!mg) return;
+  for (int l = 0; l < mg->levels; l++) {
+    level_t *L = &mg->lv[l];
+    csr_free(&L->A);
+    csr_free(&L->I);
+    csr_free(&L->IT);
+    free(L->diag);
+    free(L->bdry);
+    free(L->lu);
+    free(L->sol);
+    free(L->rhs);
+    free(L->res);
+  }
+  free(mg->rhs_top);
+  free(mg->lv);
+  free(mg);
+}
+
+long flo_mg_ndofs(const flo_mg *mg, int level) {
+  if (level < 0 || level >= mg->levels) return -1;
+  return mg->lv[level].N;
+}
+long flo_mg_nnz(const flo_mg *mg, int level) {
+  if (level < 0 || level >= mg->levels) return -1;
+  return mg->lv[level].A.rowptr[mg->lv[level].N];
+}
+int flo_mg_get_matrix(const flo_mg *mg, int level, int *rowptr, int *colidx,
+                      double *vals) {
+  if (level < 0 || level >= mg->levels) return -1;
+  const level_t *L = &mg->lv[level];
+  long nnz = L->A.rowptr[L->N];
+  if (rowptr) memcpy(rowptr, L->A.rowptr, sizeof(int) * (L->N + 1));
+  if (colidx) memcpy(colidx, L->A.colidx, sizeof(int) * nnz);
+  if (vals) memcpy(vals, L->A.vals, sizeof(double) * nnz);
+  return 0;
+}
+long flo_mg_interp_nnz(const flo_mg *mg, int level) {
+  if (level < 0 || level >= mg->levels - 1) return -1;
+  return mg->lv[level].I.rowptr[mg->lv[level].I.nrows];
+}
+int flo_mg_get_interp(const flo_mg *mg, int level, int *rowptr, int *colidx,
+                      double *vals) {
+  if (level < 0 || level >= mg->levels - 1) return -1;
+  const csr_t *I = &mg->lv[level].I;
+  long nnz = I->rowptr[I->nrows];
+  if (rowptr) memcpy(rowptr, I->rowptr, sizeof(int) * (I->nrows + 1));
+  if (colidx) memcpy(colidx, I->colidx, sizeof(int) * nnz);
+  if (vals) memcpy(vals, I->vals, sizeof(double) * nnz);
+  return 0;
+}
+int flo_mg_get_rhs(const flo_mg *mg, double *rhs) {
+  memcpy(rhs, mg->rhs_top, sizeof(double) * mg->lv[mg->levels - 1].N);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* iterators (src/iteration/linit.lisp)                                      */
+/* ------------------------------------------------------------------------ */
+
+/* compute-residual r = b - A x: copy! b r, then gemm! -1 A x 1 r
+ * (linit.lisp:73-80).  When b and r are the same vector the copy is a no-op
+ * and r <- r - A x. */
+static void compute_residual(const csr_t *A, const double *x, const double *b,
+                             double *r) {
+  if (r != b) memcpy(r, b, sizeof(double) * A->nrows);
+  flo_spmv(A->nrows, A->rowptr, A->colidx, A->vals, -1.0, x, 1.0, r);
+}
+
+/* <jacobi> iterate on <sparse-matrix> (linit.lisp:220-250): per row,
+ * gesv! overwrites r_i with A_ii^-1 r_i, then x_i += damp * r_i. */
+static void jacobi_iterate(const level_t *L, double omega, double *x, double *r) {
+  for (long i = 0; i < L->N; i++) {
+    r[i] = r[i] / L->diag[i];
+    x[i] += omega * r[i];
+  }
+}
+
+/* <sor> iterate on <sparse-matrix> (linit.lisp:288-323), store-p nil: for
+ * each row key in the ordering, corr := r_i := b_i, corr += (-x_j) A_ij over
+ * the row (diagonal included), gesv! corr /= A_ii, x_i += omega corr. */
+static void sor_iterate(const level_t *L, double omega, const long *ordering,
+                        double *x, const double *b, double *r) {
+  const csr_t *A = &L->A;
+  for (long k = 0; k < L->N; k++) {
+    long i = ordering[k];
+    double corr = b[i];
+    for (int e = A->rowptr[i]; e < A->rowptr[i + 1]; e++)
+      corr += (-x[A->colidx[e]]) * A->vals[e];
+    corr = corr / L->diag[i];
+    r[i] = corr;
+    x[i] += omega * corr;
+  }
+}
+
+/* ordering: FLO_ORDER_INDEX = canonical index order; FLO_ORDER_COLOUR =
+ * <sor> :compare by lattice colour (coordinate parity for Q1, coordinate
+ * mod 3 for Q2), then index (linit.lisp:258-263,293-295). */
+static long *make_ordering(int dim, int order, const level_t *L, int mode,
+                           int backwards) {
+  long *ord = (long *)malloc(sizeof(long) * L->N);
+  int M = order * L->n + 1;
+  int m = order + 1; /* colours per axis */
+  if (mode == FLO_ORDER_INDEX) {
+    for (long i = 0; i < L->N; i++) ord[i] = i;
+  } else {
+    int ncol = (int)ipow(m, dim);
+    long pos = 0;
+    for (int col = 0; col < ncol; col++)
+      for (long i = 0; i < L->N; i++) {
+        int c[FLO_MAXDIM];
+        unrank(i, dim, M, c);
+        int cc = 0;
+        for (int k = 0; k < dim; k++) cc = cc * m + c[k] % m;
+        if (cc == col) ord[pos++] = i;
+      }
+  }
+  if (backwards)
+    for (long i = 0; i < L->N / 2; i++) {
+      long t = ord[i];
+      ord[i] = ord[L->N - 1 - i];
+      ord[L->N - 1 - i] = t;
+    }
+  return ord;
+}
+
+/* dense LU without pivoting of the base-level matrix: femlisp factors the
+ * coarse matrix once (<lu> store-p, linit.lisp:132-162; sparse-ldu,
+ * src/matlisp/sparselu.lisp:61-126) and applies getrs!. */
+static void coarse_factor(level_t *L) {
+  int n = (int)L->N;
+  L->lu_n = n;
+  L->lu = (double *)calloc((size_t)n * n, sizeof(double));
+  for (int r = 0; r < n; r++)
+    for (int e = L->A.rowptr[r]; e < L->A.rowptr[r + 1]; e++)
+      L->lu[(size_t)r * n + L->A.colidx[e]] = L->A.vals[e];
+  double *a = L->lu;
+  for (int k = 0; k < n; k++)
+    for (int i = k + 1; i < n; i++) {
+      double lik = a[(size_t)i * n + k] / a[(size_t)k * n + k];
+      a[(size_t)i * n + k] = lik;
+      if (lik != 0.0)
+        for (int j = k + 1; j < n; j++) a[(size_t)i * n + j] -= lik * a[(size_t)k * n + j];
+    }
+}
+
+static void coarse_solve(const level_t *L, double *r) {
+  int n = L->lu_n;
+  const double *a = L->lu;
+  for (int i = 0; i < n; i++) {
+    double s = r[i];
+    for (int j = 0; j < i; j++) s -= a[(size_t)i * n + j] * r[j];
+    r[i] = s;
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = r[i];
+    for (int j = i + 1; j < n; j++) s -= a[(size_t)i * n + j] * r[j];
+    r[i] = s / a[(size_t)i * n + i];
+  }
+}
+
+typedef struct {
+  flo_mg *mg;
+  const flo_params *p;
+  long **pre_ord, **post_ord;
+  int top;
+} mgrun_t;
+
+/* product-iterator (linit.lisp:94-120): nr-steps sweeps, recomputing the
+ * residual after each sweep when residual-before and not residual-after. */
+static void smoother_run(mgrun_t *R, int l, int post) {
+  level_t *L = &R->mg->lv[l];
+  const flo_params *p = R->p;
+  int steps = post ? p->post_steps : p->pre_steps;
+  if (steps <= 0) return;
+  if (p->smoother == FLO_SMOOTH_JACOBI) {
+    /* residual-before t: ensure-mg-residual (multigrid.lisp:173-189) */
+    if (!L->residual_p) {
+      compute_residual(&L->A, L->sol, L->rhs, L->res);
+      L->residual_p = 1;
+    }
+    for (int k = 0; k < steps; k++) {
+      jacobi_iterate(L, p->omega, L->sol, L->res);
+      if (steps > 1) compute_residual(&L->A, L->sol, L->rhs, L->res);
+    }
+  } else {
+    long *ord = post ? R->post_ord[l] : R->pre_ord[l];
+    for (int k = 0; k < steps; k++) sor_iterate(L, p->omega, ord, L->sol, L->rhs, L->res);
+  }
+  /* residual-after nil for both smoothers */
+  L->residual_p = 0;
+}
+
+/* lmgc with the correction scheme, multiplicative, gamma = 1
+ * (multigrid.lisp:202-252,254-288). */
+static void lmgc(mgrun_t *R, int l) {
+  flo_mg *mg = R->mg;
+  level_t *L = &mg->lv[l];
+  if (l == R->p->base_level) {
+    /* coarse-grid iterator <lu>: r <- A^-1 r, x += damp r */
+    coarse_solve(L, L->res);
+    for (long i = 0; i < L->N; i++) L->sol[i] += 1.0 * L->res[i];
+    L->residual_p = 0;
+    return;
+  }
+  smoother_run(R, l, 0);
+  /* restrict: ensure-mg-residual, res_{l-1} = I^T res_l */
+  if (!L->residual_p) {
+    compute_residual(&L->A, L->sol, L->rhs, L->res);
+    L->residual_p = 1;
+  }
+  level_t *C = &mg->lv[l - 1];
+  flo_spmv(C->IT.nrows, C->IT.rowptr, C->IT.colidx, C->IT.vals, 1.0, L->res, 0.0, C->res);
+  /* :after methods: x_c = 0, rhs_c = res_c, residual-p t */
+  memset(C->sol, 0, sizeof(double) * C->N);
+  memcpy(C->rhs, C->res, sizeof(double) * C->N);
+  C->residual_p = 1;
+  lmgc(R, l - 1);
+  /* prolongate: sol_l += I sol_{l-1}; residual-p nil */
+  flo_spmv(C->I.nrows, C->I.rowptr, C->I.colidx, C->I.vals, 1.0, C->sol, 1.0, L->sol);
+  L->residual_p = 0;
+  smoother_run(R, l, 1);
+}
+
+static void mgrun_init(mgrun_t *R, flo_mg *mg, const flo_params *p) {
+  R->mg = mg;
+  R->p = p;
+  R->top = mg->levels - 1;
+  R->pre_ord = (long **)calloc(mg->levels, sizeof(long *));
+  R->post_ord = (long **)calloc(mg->levels, sizeof(long *));
+  if (p->smoother == FLO_SMOOTH_SOR)
+    for (int l = p->base_level + 1; l < mg->levels; l++) {
+      R->pre_ord[l] = make_ordering(mg->dim, mg->order, &mg->lv[l], p->ordering, 0);
+      R->post_ord[l] = p->symmetric
+                           ? make_ordering(mg->dim, mg->order, &mg->lv[l], p->ordering, 1)
+                           : R->pre_ord[l];
+    }
+  level_t *B = &mg->lv[p->base_level];
+  if (!B->lu) coarse_factor(B);
+}
+
+static void mgrun_free(mgrun_t *R) {
+  for (int l = 0; l < R->mg->levels; l++) {
+    if (R->post_ord[l] != R->pre_ord[l]) free(R->post_ord[l]);
+    free(R->pre_ord[l]);
+  }
+  free(R->pre_ord);
+  free(R->post_ord);
+}
+
+/* the <mg-iteration> iterator (multigrid.lisp:368-391): top-level sol/rhs/res
+ * are the caller's vectors, residual-p t. */
+static void mg_iterate(mgrun_t *R, double *x, double *b, double *r) {
+  flo_mg *mg = R->mg;
+  level_t *T = &mg->lv[R->top];
+  double *s0 = T->sol, *b0 = T->rhs, *r0 = T->res;
+  T->sol = x;
+  T->rhs = b;
+  T->res = r;
+  T->residual_p = 1;
+  if (R->top == R->p->base_level) {
+    coarse_solve(T, T->res);
+    for (long i = 0; i < T->N; i++) T->sol[i] += T->res[i];
+  } else {
+    lmgc(R, R->top);
+  }
+  T->sol = s0;
+  T->rhs = b0;
+  T->res = r0;
+}
+
+/* preconditioner call as <cg> performs it (krylow.lisp:118-121):
+ * (copy! r w) (x<-0 q) (funcall iterate q w w).  alias = 0 passes a second
+ * copy of r as the residual vector instead. */
+static void precond(mgrun_t *R, long N, const double *r, double *q, double *w,
+                    double *w2) {
+  memcpy(w, r, sizeof(double) * N);
+  memset(q, 0, sizeof(double) * N);
+  if (R->p->alias) {
+    mg_iterate(R, q, w, w);
+  } else {
+    memcpy(w2, r, sizeof(double) * N);
+    mg_iterate(R, q, w, w2);
+  }
+}
+
+int flo_mg_apply(flo_mg *mg, const flo_params *p, const double *b, double *x) {
+  if (p->base_level < 0 || p->base_level >= mg->levels) return -1;
+  mgrun_t R;
+  mgrun_init(&R, mg, p);
+  long N = mg->lv[R.top].N;
+  double *w = (double *)malloc(sizeof(double) * N);
+  double *w2 = (double *)malloc(sizeof(double) * N);
+  precond(&R, N, b, x, w, w2);
+  free(w);
+  free(w2);
+  mgrun_free(&R);
+  return 0;
+}
+
+/* <linear-solver> loop (iterate.lisp:239-250; solve.lisp:126-155;
+ * linsolve.lisp:58-73; problem.lisp:109-117) around the iteration chosen by
+ * p->outer: the V-cycle itself, <cg> with the V-cycle as preconditioner
+ * (krylow.lisp:72-132), plain <cg>, or flexible CG (Polak-Ribiere beta). */
+int flo_mg_solve(flo_mg *mg, const flo_params *p, const double *b, double *x,
+                 int *steps_out, double *defnorms) {
+  if (p->base_level < 0 || p->base_level >= mg->levels) return -1;
+  mgrun_t R;
+  mgrun_init(&R, mg, p);
+  level_t *T = &mg->lv[R.top];
+  long N = T->N;
+  double *bb = (double *)malloc(sizeof(double) * N);
+  memcpy(bb, b, sizeof(double) * N);
+  double *r = (double *)calloc(N, sizeof(double));
+  double *pv = (double *)calloc(N, sizeof(double));
+  double *a = (double *)calloc(N, sizeof(double));
+  double *w = (double *)calloc(N, sizeof(double));
+  double *w2 = (double *)calloc(N, sizeof(double));
+  double *q = (double *)calloc(N, sizeof(double));
+  double *rold = (double *)calloc(N, sizeof(double));
+  memset(x, 0, sizeof(double) * N);
+  int residual_p = 0, step = 0, initialized = 0, status = 1;
+  double initial = -1.0, alpha = 0.0;
+  for (;;) {
+    /* intermediate :before */
+    if (!residual_p) {
+      compute_residual(&T->A, x, bb, r);
+      residual_p = 1;
+    }
+    double defnorm = sqrt(dot(N, r, r));
+    if (initial < 0) initial = defnorm;
+    if (defnorms) defnorms[step] = defnorm;
+    double reduction = (defnorm == 0.0) ? 0.0 : defnorm / initial;
+    if ((p->stop == FLO_STOP_REDUCTION && reduction < p->tol) ||
+        (p->stop == FLO_STOP_DEFNORM && defnorm < p->tol)) {
+      status = 0;
+      break;
+    }
+    if (step > p->maxsteps) {
+      status = 1;
+      break;
+    }
+    /* next-step */
+    switch (p->outer) {
+      case FLO_OUTER_MG:
+        mg_iterate(&R, x, bb, r);
+        residual_p = 0;
+        break;
+      case FLO_OUTER_CG:
+      case FLO_OUTER_CG_PLAIN: {
+        int pc = (p->outer == FLO_OUTER_CG);
+        if (!initialized) { /* restart */
+          if (pc) precond(&R, N, r, pv, w, w2);
+          else memcpy(pv, r, sizeof(double) * N);
+          alpha = dot(N, pv, r);
+          initialized = 1;
+        }
+        if (alpha != 0.0) {
+          flo_spmv((int)N, T->A.rowptr, T->A.colidx, T->A.vals, 1.0, pv, 0.0, a);
+          double beta = dot(N, a, pv);
+          double lam = alpha / beta;
+          for (long i = 0; i < N; i++) x[i] += lam * pv[i];
+          for (long i = 0; i < N; i++) r[i] += (-lam) * a[i];
+          if (pc) precond(&R, N, r, q, w, w2);
+          else memcpy(q, r, sizeof(double) * N);
+          double new_alpha = dot(N, q, r);
+          double sc = new_alpha / alpha;
+          for (long i = 0; i < N; i++) pv[i] = sc * pv[i] + q[i];
+          alpha = new_alpha;
+        }
+        residual_p = 1;
+        break;
+      }
+      case FLO_OUTER_FCG: {
+        if (!initialized) {
+          precond(&R, N, r, q, w, w2);
+          memcpy(pv, q, sizeof(double) * N);
+          alpha = dot(N, q, r);
+          initialized = 1;
+        }
+        if (alpha != 0.0) {
+          flo_spmv((int)N, T->A.rowptr, T->A.colidx, T->A.vals, 1.0, pv, 0.0, a);
+          double lam = alpha / dot(N, pv, a);
+          for (long i = 0; i < N; i++) x[i] += lam * pv[i];
+          memcpy(rold, r, sizeof(double) * N);
+          for (long i = 0; i < N; i++) r[i] += (-lam) * a[i];
+          precond(&R, N, r, q, w, w2);
+          double zr = dot(N, q, r);
+          double beta = (zr - dot(N, q, rold)) / alpha;
+          for (long i = 0; i < N; i++) pv[i] = q[i] + beta * pv[i];
+          alpha = zr;
+        }
+        residual_p = 1;
+        break;
+      }
+      default:
+        status = -2;
+        goto done;
+    }
+    step++;
+  }
+done:
+  if (steps_out) *steps_out = step;
+  free(bb);
+  free(r);
+  free(pv);
+  free(a);
+  free(w);
+  free(w2);
+  free(q);
+  free(rold);
+  mgrun_free(&R);
+  return status;
+}

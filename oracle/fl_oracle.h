@@ -1,0 +1,142 @@
+/*
+ * fl_oracle.h -- CPU restatement of femlisp's solver hot path for Lagrange
+ * FE discretisations on uniformly refined box meshes.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the CPU
+ * baseline; it is never linked into, loaded by or called from the product
+ * library (libflgpu.so).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may use it.
+ *
+ * Every function cites the femlisp file:line whose semantics it restates.
+ * Arithmetic is IEEE double throughout, as in femlisp (<sparse-matrix> is
+ * double-float, src/matlisp/sparse-matrix.lisp:56-60).
+ *
+ * Canonical DOF numbering: femlisp keys DOFs by mesh cells and has no integer
+ * index; the canonical index used here is the rank of the DOF's midpoint in
+ * femlisp's sort-lexicographically order (src/mesh/meshgen.lisp:200-217,
+ * coordinate 0 most significant), i.e. the lattice index
+ *   idx = i_0 * M^(d-1) + ... + i_{d-1},  M = order*n + 1 nodes per axis.
+ */
+#ifndef FL_ORACLE_H
+#define FL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- lattice (src/mesh/meshgen.lisp:126-138,180-183; src/mesh/refine.lisp:148-149) */
+long flo_lattice_ndofs(int dim, int order, int n);
+long flo_lattice_nnz(int dim, int order, int n);
+/* CSR pattern of femlisp's assembled matrix: every pair of DOFs sharing a
+ * cell, explicit zeros of Dirichlet rows/columns kept, columns ascending.
+ * rowptr[N+1], colidx[nnz]. Returns 0. */
+int flo_lattice_pattern(int dim, int order, int n, int *rowptr, int *colidx);
+/* 1 for DOFs on the external boundary (src/discretization/system-fe.lisp:37-129) */
+int flo_lattice_boundary(int dim, int order, int n, unsigned char *is_bdry);
+
+/* ---- quadrature and basis (src/discretization/quadrature.lisp:122-215,
+ *      src/discretization/lagrange.lisp:41-202) */
+int flo_gauss_rule(int s, double *pts, double *wts);
+void flo_lagrange_1d(int order, double t, double *phi, double *dphi);
+
+/* Coefficient sampling: kappa is piecewise constant on an nk^dim grid of
+ * cells of the unit cube (NULL -> 1).  A quadrature point at global
+ * coordinate (c + g) * H on a level with n cells per axis (H = 1/n) samples
+ * the coefficient cell c*(nk/n) + floor(g*(nk/n)) along each axis; nk must be
+ * a multiple of n.  See flo_assemble. */
+
+/* Element matrix and load vector of one cell of width H for diffusion kappa*I
+ * sampled per quadrature point (kappa_qp[s^dim]) and source f
+ * (src/discretization/ellsys-fe.lisp:120-127,183-190,
+ *  src/discretization/fe.lisp:502-535).  Local DOF order: lattice order inside
+ * the cell (coordinate 0 most significant).  Ke[nloc*nloc], be[nloc]. */
+int flo_element_matrix(int dim, int order, double H, const double *kappa_qp,
+                       double f, double *Ke, double *be);
+
+/* Global assembly on the level with n cells per axis, followed by Dirichlet
+ * elimination A <- S A S + P, b <- S b (zero boundary values)
+ * (src/discretization/fedisc.lisp:108-268, constraints.lisp:323-411).
+ * vals[nnz] in flo_lattice_pattern order; rhs[N] may be NULL. */
+int flo_assemble(int dim, int order, int n, const double *kappa, int nk,
+                 double f, double *vals, double *rhs);
+
+/* Constrained :local interpolation matrix from the level with nc cells per
+ * axis to the level with 2*nc cells: rows fine DOFs, columns coarse DOFs,
+ * entries below 1e-10 dropped, fine Dirichlet rows and coarse Dirichlet
+ * columns zeroed but kept (src/discretization/sparseas.lisp:232-272,
+ * fetransfer.lisp:73-138, constraints.lisp:58-72,
+ * src/matlisp/sparse-matrix.lisp:635-653).  Call with colidx == NULL to get
+ * nnz returned; rowptr is always filled. */
+long flo_interpolation(int dim, int order, int nc, int *rowptr, int *colidx,
+                       double *vals);
+
+/* ---- algebra (src/matlisp/sparse-matrix.lisp:378-442) */
+/* x <- beta*x + alpha*A*y  (gemm-nn!) */
+void flo_spmv(int nrows, const int *rowptr, const int *colidx,
+              const double *vals, double alpha, const double *y, double beta,
+              double *x);
+/* x <- beta*x + alpha*A^T*y  (gemm-tn!), A is nrows x ncols */
+void flo_spmv_t(int nrows, int ncols, const int *rowptr, const int *colidx,
+                const double *vals, double alpha, const double *y,
+                double beta, double *x);
+/* transpose CSR (ncols+1 rowptr of the result); columns ascending */
+void flo_csr_transpose(int nrows, int ncols, const int *rowptr,
+                       const int *colidx, const double *vals, int *trowptr,
+                       int *tcolidx, double *tvals);
+
+/* ---- multigrid hierarchy + solvers (src/iteration/ multigrid, linit, krylow) */
+typedef struct flo_mg flo_mg;
+
+/* Hierarchy on the unit cube with n0 cells per axis on level 0 and
+ * `levels` levels (top level has n0*2^(levels-1) cells per axis);
+ * re-discretisation on every level (src/iteration/geomg.lisp:109-142). */
+flo_mg *flo_mg_create(int dim, int order, int n0, int levels,
+                      const double *kappa, int nk, double f);
+void flo_mg_destroy(flo_mg *mg);
+long flo_mg_ndofs(const flo_mg *mg, int level);
+long flo_mg_nnz(const flo_mg *mg, int level);
+/* copies of the level matrix / interpolation (I_level: level -> level+1) */
+int flo_mg_get_matrix(const flo_mg *mg, int level, int *rowptr, int *colidx,
+                      double *vals);
+long flo_mg_interp_nnz(const flo_mg *mg, int level);
+int flo_mg_get_interp(const flo_mg *mg, int level, int *rowptr, int *colidx,
+                      double *vals);
+int flo_mg_get_rhs(const flo_mg *mg, double *rhs);
+
+enum { FLO_OUTER_MG = 0, FLO_OUTER_CG = 1, FLO_OUTER_CG_PLAIN = 2,
+       FLO_OUTER_FCG = 3 };
+enum { FLO_SMOOTH_JACOBI = 0, FLO_SMOOTH_SOR = 1 };
+enum { FLO_ORDER_INDEX = 0, FLO_ORDER_COLOUR = 1 };
+enum { FLO_STOP_REDUCTION = 0, FLO_STOP_DEFNORM = 1 };
+
+typedef struct flo_params {
+  int outer;        /* FLO_OUTER_* */
+  int smoother;     /* FLO_SMOOTH_* */
+  double omega;     /* <jacobi> damp / <sor> omega */
+  int ordering;     /* FLO_ORDER_* for <sor> */
+  int pre_steps, post_steps;
+  int base_level;
+  int symmetric;    /* 1: post-smoothing sweeps the ordering backwards */
+  int alias;        /* 1: preconditioner called as (iterate q w w) */
+  int stop;         /* FLO_STOP_* */
+  double tol;
+  int maxsteps;     /* failure-if (> :step maxsteps) */
+} flo_params;
+
+/* Solve A x = b from x = 0 with femlisp's <linear-solver> loop
+ * (src/iteration/iterate.lisp:239-250, solve.lisp:126-165,
+ * linsolve.lisp:58-73).  defnorms[0..steps] receives ||r_k||_2 (the norm
+ * femlisp observes: recurrence residual for CG, true residual otherwise).
+ * Returns 0 on success, 1 on failure (step limit), <0 on argument errors. */
+int flo_mg_solve(flo_mg *mg, const flo_params *p, const double *b, double *x,
+                 int *steps, double *defnorms);
+
+/* One application of the preconditioner as femlisp's CG performs it:
+ * x <- 0, then the V-cycle iterate on (x, b, r) where r is b itself when
+ * p->alias, else a separate copy of b.  Used to probe the operator. */
+int flo_mg_apply(flo_mg *mg, const flo_params *p, const double *b, double *x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,320 @@
+"""GPU parity against the oracle: pattern/indexing bit-exact, assembly and
+SpMV within 1e-13 relative, V-cycle operator within 1e-12, converged
+solutions within 1e-10 and identical iteration counts where the ordering is
+defined (Jacobi; <sor> in colour order).  All calls go through the C-ABI."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from gnu_femlisp_amd import flgpu as F
+from gnu_femlisp_amd import iteration as it
+from gnu_femlisp_amd import lattice as lat
+from gnu_femlisp_amd.matlisp import (GpuSparseMatrix, GpuVector, compute_residual,
+                                     default_context, dot, gemm, norm)
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rowrel(v_gpu, v_ref, rp):
+    """max over rows of |row error|_inf / |reference row|_inf"""
+    worst = 0.0
+    for r in range(len(rp) - 1):
+        a, b = rp[r], rp[r + 1]
+        den = np.abs(v_ref[a:b]).max()
+        if den > 0:
+            worst = max(worst, np.abs(v_gpu[a:b] - v_ref[a:b]).max() / den)
+    return worst
+
+
+def relerr(x, y):
+    d = np.linalg.norm(y)
+    return np.linalg.norm(x - y) / (d if d > 0 else 1.0)
+
+
+CASES = [(1, 1, 8), (2, 1, 16), (3, 1, 8), (1, 2, 5), (2, 2, 6), (3, 2, 4), (3, 1, 16)]
+
+
+@pytest.mark.parametrize("dim,order,n", CASES)
+@pytest.mark.parametrize("mode", [F.ASM_F64, F.ASM_MFMA])
+def test_assembly_parity(dim, order, n, mode):
+    say(f"assembly dim={dim} order={order} n={n} mode={mode}")
+    rng = np.random.default_rng(7)
+    nk = 2 * n
+    kap = np.exp(rng.normal(scale=1.5, size=(nk,) * dim))
+    A, b = lat.assemble(dim, order, n, kap, f=1.0, prec=F.F64, mode=mode)
+    rp, ci, v = A.csr()
+    orp, oci, ov, ob = po.assemble(dim, order, n, kap, 1.0)
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)  # bit-exact pattern
+    assert rowrel(v, ov, orp) < 1e-13
+    assert relerr(b.numpy(), ob) < 1e-13
+
+
+@pytest.mark.parametrize("dim,order,n", [(2, 1, 8), (3, 1, 4), (3, 2, 3)])
+def test_element_kernel_parity(dim, order, n):
+    say(f"element dim={dim} order={order} n={n}")
+    rng = np.random.default_rng(11)
+    kap = np.exp(rng.normal(size=(n,) * dim))
+    for mode in (F.ASM_F64, F.ASM_MFMA):
+        E = lat.element_matrices(dim, order, n, kap, prec=F.F64, mode=mode)
+        for c in (0, n ** dim // 2, n ** dim - 1):
+            kq = np.full(((order + 1) ** dim), kap.ravel()[c])
+            K, _ = po.element_matrix(dim, order, 1.0 / n, kq)
+            assert np.abs(E[c] - K).max() <= 1e-13 * np.abs(K).max()
+    # BF16 MFMA with FP32 accumulation: error of bf16 inputs, stated bound
+    E32 = lat.element_matrices(dim, order, n, kap, prec=F.F32, mode=F.ASM_MFMA)
+    E64 = lat.element_matrices(dim, order, n, kap, prec=F.F64, mode=F.ASM_F64)
+    assert np.abs(E32 - E64).max() <= 2e-2 * np.abs(E64).max()
+
+
+@pytest.mark.parametrize("dim,order,nc", [(1, 1, 4), (2, 1, 4), (3, 1, 4), (2, 2, 3), (3, 2, 2)])
+def test_interpolation_parity(dim, order, nc):
+    say(f"interp dim={dim} order={order} nc={nc}")
+    P = lat.interpolation(dim, order, nc)
+    rp, ci, v = P.csr()
+    orp, oci, ov = po.interpolation(dim, order, nc)
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    assert np.array_equal(v, ov)  # dyadic weights: exact
+
+
+def test_spmv_parity_random_csr():
+    say("spmv random")
+    rng = np.random.default_rng(5)
+    for n, dens in ((1, 1.0), (7, 0.5), (300, 0.03), (5000, 0.002)):
+        M = sp.random(n, n, density=dens, random_state=rng, format="csr") + sp.eye(n, format="csr")
+        M.sort_indices()
+        A = GpuSparseMatrix.from_scipy(M)
+        y = rng.normal(size=n)
+        x0 = rng.normal(size=n)
+        gy, gx = GpuVector.from_array(y), GpuVector.from_array(x0)
+        gemm(0.7, A, gy, -1.3, gx)
+        ref = po.spmv(M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data, y, 0.7, -1.3, x0)
+        assert relerr(gx.numpy(), ref) < 1e-13
+        # gemm-tn!
+        gz = GpuVector.from_array(x0)
+        gemm(1.0, A, gy, 0.0, gz, job="tn")
+        assert relerr(gz.numpy(), M.T @ y) < 1e-13
+
+
+def test_empty_and_ragged_rows():
+    say("spmv ragged")
+    rp = np.array([0, 0, 3, 3, 4, 4], np.int32)
+    ci = np.array([0, 2, 4, 1], np.int32)
+    v = np.array([1.0, 2.0, 3.0, 4.0])
+    A = GpuSparseMatrix.from_csr(rp, ci, v, ncols=5)
+    y = GpuVector.from_array(np.arange(1.0, 6.0))
+    x = GpuVector.from_array(np.full(5, 9.0))
+    gemm(1.0, A, y, 0.0, x)
+    assert np.array_equal(x.numpy(), [0.0, 1 + 6 + 15, 0.0, 8.0, 0.0])
+
+
+def test_csr_upload_rejects_bad_input():
+    rp = np.array([0, 2, 1], np.int32)
+    with pytest.raises(F.FlgpuError):
+        GpuSparseMatrix.from_csr(rp, np.array([0, 1], np.int32), np.ones(2))
+    with pytest.raises(F.FlgpuError):
+        GpuSparseMatrix.from_csr(np.array([0, 2], np.int32), np.array([1, 0], np.int32), np.ones(2))
+
+
+@pytest.mark.parametrize("dim,order,n", [(2, 1, 32), (3, 1, 16), (3, 2, 6)])
+def test_lattice_spmv_parity(dim, order, n):
+    say(f"lattice spmv dim={dim} order={order} n={n}")
+    A, b = lat.assemble(dim, order, n)
+    rp, ci, v = A.csr()
+    y = np.random.default_rng(2).uniform(-1, 1, A.ncols)
+    gy = GpuVector.from_array(y)
+    gx = A.make_image_vector()
+    gemm(1.0, A, gy, 0.0, gx)
+    assert relerr(gx.numpy(), po.spmv(rp, ci, v, y)) < 1e-13
+    # compute-residual with b and r the same vector: r <- r - A x
+    gr = GpuVector.from_array(b.numpy())
+    compute_residual(A, gy, gr, gr)
+    assert relerr(gr.numpy(), b.numpy() - po.spmv(rp, ci, v, y)) < 1e-13
+
+
+def test_vector_ops():
+    say("vector ops")
+    rng = np.random.default_rng(9)
+    for n in (0, 1, 1000, 300001):
+        a, c = rng.normal(size=n), rng.normal(size=n)
+        x, y = GpuVector.from_array(a), GpuVector.from_array(c)
+        assert abs(dot(x, y) - a @ c) <= 1e-12 * max(1.0, np.abs(a) @ np.abs(c))
+        assert abs(norm(x) - np.linalg.norm(a)) <= 1e-12 * max(1.0, np.linalg.norm(a))
+        from gnu_femlisp_amd.matlisp import axpy, scal
+        axpy(0.5, x, y)
+        assert np.allclose(y.numpy(), 0.5 * a + c, rtol=1e-15, atol=1e-15)
+        scal(-2.0, y)
+        assert np.allclose(y.numpy(), -2 * (0.5 * a + c), rtol=1e-15, atol=1e-15)
+    # dot is reproducible bit for bit
+    x = GpuVector.from_array(rng.normal(size=100003))
+    assert len({dot(x, x) for _ in range(5)}) == 1
+
+
+def test_jacobi_and_sor_iterates():
+    say("smoother iterates")
+    dim, order, n = 2, 2, 5
+    A, b = lat.assemble(dim, order, n)
+    rp, ci, v = A.csr()
+    M = sp.csr_matrix((v, ci, rp))
+    d = M.diagonal()
+    rng = np.random.default_rng(4)
+    x0 = rng.normal(size=A.nrows)
+    r0 = rng.normal(size=A.nrows)
+    x, r = GpuVector.from_array(x0), GpuVector.from_array(r0)
+    from gnu_femlisp_amd.flgpu import check, lib
+    check(lib().flgpu_jacobi_iterate(A.h, 0.6, x.h, r.h))
+    rr = r0 / d
+    assert np.allclose(r.numpy(), rr, rtol=1e-15) and np.allclose(x.numpy(), x0 + 0.6 * rr, rtol=1e-15)
+    # <sor> in (colour, index) order, restated row by row on the host
+    lat.colour(A, dim, order, n)
+    colour = lat.lattice_colours(dim, order, n)
+    order_rows = np.lexsort((np.arange(A.nrows), colour))
+    bb = rng.normal(size=A.nrows)
+    xs = x0.copy()
+    rs = np.zeros(A.nrows)
+    for i in order_rows:
+        corr = bb[i]
+        for e in range(rp[i], rp[i + 1]):
+            corr += (-xs[ci[e]]) * v[e]
+        corr /= d[i]
+        rs[i] = corr
+        xs[i] += 1.2 * corr
+    gx, gb, gr = GpuVector.from_array(x0), GpuVector.from_array(bb), GpuVector(A.nrows)
+    check(lib().flgpu_sor_iterate(A.h, 1.2, 0, gx.h, gb.h, gr.h))
+    assert relerr(gx.numpy(), xs) < 1e-13 and relerr(gr.numpy(), rs) < 1e-13
+
+
+def _gpu_hier(dim, order, n0, levels, smoother, omega, pre, post, base, symmetric=False,
+              kappa=None, prec=F.F64, mode=F.ASM_F64):
+    sm = it.Jacobi(omega) if smoother == po.SMOOTH_JACOBI else it.SOR(omega)
+    cs = it.GeometricCS(sm, pre, post, base, symmetric=symmetric, prec=prec)
+    return cs, it.Hierarchy.lattice(cs, dim, order, n0, levels, kappa=kappa, mode=mode)
+
+
+@pytest.mark.parametrize("alias", [0, 1])
+@pytest.mark.parametrize("smoother", [po.SMOOTH_JACOBI, po.SMOOTH_SOR])
+def test_vcycle_operator_parity(alias, smoother):
+    say(f"vcycle apply alias={alias} smoother={smoother}")
+    dim, order, n0, levels = 3, 1, 1, 4
+    omega = 0.6 if smoother == po.SMOOTH_JACOBI else 1.0
+    cs, H = _gpu_hier(dim, order, n0, levels, smoother, omega, 2, 2, 1)
+    oh = po.Hierarchy(dim, order, n0, levels)
+    prm = po.params(smoother=smoother, omega=omega, ordering=po.ORDER_COLOUR, pre_steps=2,
+                    post_steps=2, base_level=1, alias=alias)
+    rng = np.random.default_rng(8)
+    r = rng.uniform(-1, 1, oh.ndofs())
+    r[po.boundary(dim, order, 8)] = 0
+    ref = oh.apply(prm, r)
+    q = GpuVector(len(r))
+    w = GpuVector.from_array(r)
+    if alias:
+        H.apply(q, w, w)
+    else:
+        H.apply(q, w, GpuVector.from_array(r))
+    assert relerr(q.numpy(), ref) < 1e-12
+
+
+def test_known_answer_vcycle_rate_1d():
+    """mg-cdr.lisp:85-87 on the GPU: CR 0.39407..."""
+    say("known answer 1d")
+    cs, H = _gpu_hier(1, 1, 1, 4, po.SMOOTH_JACOBI, 0.5, 1, 0, 2)
+    A = H.top_matrix()
+    bb = it.solve(it.LinearSolver(cs, success_if=("<", "defnorm", 1e-12), failure_if=(">", "step", 10)),
+                  {"matrix": A, "rhs": H.rhs, "hierarchy": H})
+    assert bb["steps"] == 11
+    cr = bb["report"]["convergence-rate"]
+    assert cr < 0.3941 and abs(cr - 0.39407) < 1e-5
+
+
+def test_known_answer_tridiagonal_cg():
+    say("known answer 3x3")
+    k = json.load(open(os.path.join(GOLDEN, "femlisp_known_answers.json")))["tridiagonal_3x3"]
+    A = GpuSparseMatrix.from_scipy(sp.csr_matrix(np.array(k["matrix"])))
+    b = GpuVector.from_array(k["rhs"])
+    bb = it.solve(it.LinearSolver(it.CG(), success_if=("<", "defnorm", 1e-12)),
+                  {"matrix": A, "rhs": b})
+    assert np.allclose(bb["solution"].numpy(), k["solution"], atol=1e-12)
+
+
+SOLVE_CASES = [
+    # dim, order, n0, levels, smoother, omega, pre, post, base, outer, symmetric
+    (2, 1, 1, 6, po.SMOOTH_JACOBI, 0.6, 2, 2, 0, po.OUTER_CG, 0),   # config 1
+    (3, 1, 1, 5, po.SMOOTH_SOR, 1.0, 2, 2, 0, po.OUTER_CG, 1),
+    (3, 1, 1, 5, po.SMOOTH_SOR, 1.0, 2, 2, 1, po.OUTER_MG, 0),
+    (3, 2, 1, 3, po.SMOOTH_JACOBI, 0.6, 2, 2, 0, po.OUTER_FCG, 0),
+    (2, 2, 1, 4, po.SMOOTH_SOR, 1.0, 1, 1, 1, po.OUTER_CG, 1),
+]
+
+
+@pytest.mark.parametrize("case", SOLVE_CASES)
+def test_solve_parity(case):
+    dim, order, n0, levels, smoother, omega, pre, post, base, outer, sym = case
+    say(f"solve {case}")
+    rng = np.random.default_rng(12)
+    ntop = n0 << (levels - 1)
+    kap = np.exp(rng.normal(scale=1.0, size=(ntop,) * dim))
+    oh = po.Hierarchy(dim, order, n0, levels, kappa=kap)
+    b = rng.uniform(-1, 1, oh.ndofs())
+    b[po.boundary(dim, order, ntop)] = 0
+    prm = po.params(outer=outer, smoother=smoother, omega=omega, ordering=po.ORDER_COLOUR,
+                    pre_steps=pre, post_steps=post, base_level=base, symmetric=sym, alias=0,
+                    tol=1e-10, maxsteps=200)
+    st, xo, so, dno = oh.solve(prm, b)
+    assert st == 0
+    cs, H = _gpu_hier(dim, order, n0, levels, smoother, omega, pre, post, base, symmetric=bool(sym),
+                      kappa=kap)
+    A = H.top_matrix()
+    itn = {po.OUTER_MG: cs, po.OUTER_CG: it.CG(cs), po.OUTER_FCG: it.FCG(cs)}[outer]
+    bb = it.solve(it.LinearSolver(itn, success_if=("<", "reduction", 1e-10),
+                                  failure_if=(">", "step", 200)),
+                  {"matrix": A, "rhs": GpuVector.from_array(b), "hierarchy": H})
+    assert bb["status"] == "success"
+    assert bb["steps"] == so
+    assert relerr(bb["solution"].numpy(), xo) < 1e-10
+    assert np.allclose(bb["defnorms"], dno, rtol=1e-8)
+
+
+def test_alias_mode_matches_oracle():
+    """femlisp's literal (iterate q w w): same (stalling) defect history."""
+    say("alias cg")
+    oh = po.Hierarchy(2, 1, 1, 5)
+    prm = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_JACOBI, omega=0.6, pre_steps=2,
+                    post_steps=2, base_level=0, alias=1, tol=1e-10, maxsteps=8)
+    st, xo, so, dno = oh.solve(prm)
+    cs, H = _gpu_hier(2, 1, 1, 5, po.SMOOTH_JACOBI, 0.6, 2, 2, 0)
+    bb = it.solve(it.LinearSolver(it.CG(cs, alias=True), failure_if=(">", "step", 8)),
+                  {"matrix": H.top_matrix(), "rhs": H.rhs, "hierarchy": H})
+    assert bb["steps"] == so
+    assert np.allclose(bb["defnorms"], dno, rtol=1e-9)
+
+
+def test_mixed_precision_fcg():
+    """FP32 V-cycle (BF16-MFMA assembled) inside FP64 FCG reaches 1e-10 true
+    residual; the FP64 solution agrees with the FP64 oracle to 1e-9."""
+    say("mixed fcg")
+    dim, order, n0, levels = 3, 1, 1, 5
+    rng = np.random.default_rng(21)
+    kap = np.exp(rng.normal(scale=1.0, size=(16,) * 3))
+    A, b = lat.assemble(dim, order, 16, kap, prec=F.F64)
+    cs, H = _gpu_hier(dim, order, n0, levels, po.SMOOTH_SOR, 1.0, 2, 2, 1, symmetric=True,
+                      kappa=kap, prec=F.F32, mode=F.ASM_MFMA)
+    bb = it.solve(it.LinearSolver(it.FCG(cs), success_if=("<", "reduction", 1e-10),
+                                  failure_if=(">", "step", 100), true_residual=True),
+                  {"matrix": A, "rhs": b, "hierarchy": H})
+    assert bb["status"] == "success"
+    assert bb["true_residual"] / np.linalg.norm(b.numpy()) < 1e-10
+    oh = po.Hierarchy(dim, order, n0, levels, kappa=kap)
+    prm = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_SOR, ordering=po.ORDER_COLOUR,
+                    pre_steps=2, post_steps=2, base_level=1, symmetric=1, tol=1e-12, maxsteps=100)
+    _, xo, _, _ = oh.solve(prm)
+    assert relerr(bb["solution"].numpy(), xo) < 1e-9
