@@ -225,6 +225,8 @@ extern "C" int flgpu_ctx_create(int device, flgpu_ctx **out) {
     c->red_partial = (double *)dev_alloc(sizeof(double) * kRedBlocks);
     c->red_out = (double *)dev_alloc(sizeof(double) * kRedSlots);
     FLGPU_HIP(hipHostMalloc((void **)&c->host_out, sizeof(double) * kRedSlots, hipHostMallocDefault));
+    FLGPU_HIP(hipEventCreate(&c->ev_start));
+    FLGPU_HIP(hipEventCreate(&c->ev_stop));
     *out = c;
     return 0;
   });
@@ -238,6 +240,8 @@ extern "C" int flgpu_ctx_destroy(flgpu_ctx *ctx) {
     dev_free(ctx->red_partial);
     dev_free(ctx->red_out);
     if (ctx->host_out) (void)hipHostFree(ctx->host_out);
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -265,6 +269,28 @@ extern "C" int flgpu_ctx_info(flgpu_ctx *ctx, char *name, int len, int *cus, lon
       FLGPU_HIP(hipMemGetInfo(&fr, &tot));
       *mem = (long)tot;
     }
+    return 0;
+  });
+}
+
+extern "C" int flgpu_ctx_timer_start(flgpu_ctx *ctx) {
+  return guard([&] {
+    FLGPU_REQUIRE(ctx, "ctx must not be NULL");
+    CTX_LOCK(ctx);
+    FLGPU_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    return 0;
+  });
+}
+
+extern "C" int flgpu_ctx_timer_stop(flgpu_ctx *ctx, double *ms) {
+  return guard([&] {
+    FLGPU_REQUIRE(ctx && ms, "arguments must not be NULL");
+    CTX_LOCK(ctx);
+    FLGPU_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+    FLGPU_HIP(hipEventSynchronize(ctx->ev_stop));
+    float t = 0.f;
+    FLGPU_HIP(hipEventElapsedTime(&t, ctx->ev_start, ctx->ev_stop));
+    *ms = t;
     return 0;
   });
 }

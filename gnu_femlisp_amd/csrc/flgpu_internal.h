@@ -98,6 +98,7 @@ struct flgpu_ctx {
   double *red_partial = nullptr;  // device, reduction partials
   double *red_out = nullptr;      // device, reduction results (64 slots)
   double *host_out = nullptr;     // pinned host copy of red_out
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   std::string name;
 };
 

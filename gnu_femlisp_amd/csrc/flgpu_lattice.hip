@@ -369,7 +369,8 @@ typedef short v4s __attribute__((ext_vector_type(4)));
 
 // FP64 MFMA 16x16x4: one wavefront computes a 16-cell x 16-entry tile per
 // N-step.  Operand layout of v_mfma_f64_16x16x4f64: lane l supplies
-// A[l%16][l/16] and B[l/16][l%16]; it receives D[4(l/16)+v][l%16], v = 0..3.
+// A[l%16][l/16] and B[l/16][l%16]; it receives D[l/16 + 4v][l%16], v = 0..3
+// (measured on gfx950: the four result rows of a lane are 4 apart).
 __global__ __launch_bounds__(256) void k_elem_mfma_f64(long ncells, int nq, int nloc2,
                                                        const double *__restrict__ S,
                                                        const double *__restrict__ Tt, double *E) {
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(256) void k_elem_mfma_f64(long ncells, int nq, int 
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
     for (int v = 0; v < 4; v++) {
-      long row = c0 + 4 * kq + v;
+      long row = c0 + kq + 4 * v;
       if (row < ncells && bcol < nloc2) E[row * nloc2 + bcol] = acc[v];
     }
   }

@@ -59,6 +59,8 @@ SIGNATURES = {
     "flgpu_ctx_destroy": (_ci, [_vp]),
     "flgpu_ctx_synchronize": (_ci, [_vp]),
     "flgpu_ctx_info": (_ci, [_vp, ctypes.c_char_p, _ci, _P(_ci), _P(_cl)]),
+    "flgpu_ctx_timer_start": (_ci, [_vp]),
+    "flgpu_ctx_timer_stop": (_ci, [_vp, _P(_cd)]),
     "flgpu_vec_create": (_ci, [_vp, _cl, _ci, _P(_vp)]),
     "flgpu_vec_destroy": (_ci, [_vp]),
     "flgpu_vec_info": (_ci, [_vp, _P(_cl), _P(_ci)]),

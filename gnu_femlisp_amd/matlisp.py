@@ -32,6 +32,14 @@ class Context:
         check(lib().flgpu_ctx_info(self.h, name, 128, ctypes.byref(cus), ctypes.byref(mem)))
         return {"arch": name.value.decode(), "cus": cus.value, "mem": mem.value}
 
+    def timer_start(self):
+        check(lib().flgpu_ctx_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = ctypes.c_double()
+        check(lib().flgpu_ctx_timer_stop(self.h, ctypes.byref(ms)))
+        return ms.value
+
     def synchronize(self):
         check(lib().flgpu_ctx_synchronize(self.h))
 

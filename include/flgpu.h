@@ -52,6 +52,12 @@ int flgpu_ctx_destroy(flgpu_ctx *ctx);
 int flgpu_ctx_synchronize(flgpu_ctx *ctx);
 /* name of the device, number of CUs, bytes of device memory */
 int flgpu_ctx_info(flgpu_ctx *ctx, char *name, int len, int *cus, long *mem);
+/* HIP events on the context's stream around a caller's region of work:
+ * start records the first event, stop records the second, waits for it and
+ * returns the elapsed milliseconds (femlisp's measure-time-for-block,
+ * src/basic/mflop.lisp:89-96, for device work). */
+int flgpu_ctx_timer_start(flgpu_ctx *ctx);
+int flgpu_ctx_timer_stop(flgpu_ctx *ctx, double *ms);
 
 /* ---- vectors (src/matlisp/vector.lisp:91-152,224-326;
  *      src/matlisp/sparse-vector.lisp:248-280) -------------------------------- */

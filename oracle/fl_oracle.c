@@ -187,22 +187,28 @@ void flo_lagrange_1d(int order, double t, double *phi, double *dphi) {
  *   K_ij += weight * (G_j . kappa) . G_i,   b_k += weight * phi_k * f.
  * Quadrature rule: order+1 Gauss points per axis (quadrature-rule,
  * src/discretization/fe.lisp:235-244). */
-int flo_element_matrix(int dim, int order, double H, const double *kappa_qp,
-                       double f, double *Ke, double *be) {
+typedef struct {
+  int dim, nloc, nq;
+  double weight[27];
+  double vals[27][27];
+  double grads[27][27][FLO_MAXDIM];
+} elem_tab;
+
+/* per-level tables: quadrature weights times |det J|, basis values and
+ * physical gradients at the quadrature points of a cell of width H */
+static void elem_tab_init(elem_tab *t, int dim, int order, double H) {
   int s = order + 1;
   double pts[16], wts[16];
   flo_gauss_rule(s, pts, wts);
   int nl1 = order + 1;
-  int nloc = (int)ipow(nl1, dim);
-  int nq = (int)ipow(s, dim);
+  t->dim = dim;
+  t->nloc = (int)ipow(nl1, dim);
+  t->nq = (int)ipow(s, dim);
   double phi[16][8], dphi[16][8];
   for (int q = 0; q < s; q++) flo_lagrange_1d(order, pts[q], phi[q], dphi[q]);
-  if (Ke) memset(Ke, 0, sizeof(double) * nloc * nloc);
-  if (be) memset(be, 0, sizeof(double) * nloc);
   double detJ = 1.0;
   for (int k = 0; k < dim; k++) detJ *= H;
-  double vals[729], grads[729][FLO_MAXDIM];
-  for (int q = 0; q < nq; q++) {
+  for (int q = 0; q < t->nq; q++) {
     int qc[FLO_MAXDIM];
     unrank(q, dim, s, qc);
     /* femlisp multiplies volume-at-point by the rule weight
@@ -210,32 +216,50 @@ int flo_element_matrix(int dim, int order, double H, const double *kappa_qp,
      * weights (quadrature.lisp:178-184). */
     double wq = 1.0;
     for (int k = 0; k < dim; k++) wq *= wts[qc[k]];
-    double weight = detJ * wq;
-    for (int a = 0; a < nloc; a++) {
+    t->weight[q] = detJ * wq;
+    for (int a = 0; a < t->nloc; a++) {
       int ac[FLO_MAXDIM];
       unrank(a, dim, nl1, ac);
       double v = 1.0;
       for (int k = 0; k < dim; k++) v *= phi[qc[k]][ac[k]];
-      vals[a] = v;
+      t->vals[q][a] = v;
       for (int m = 0; m < dim; m++) {
         double g = 1.0;
         for (int k = 0; k < dim; k++)
           g *= (k == m) ? dphi[qc[k]][ac[k]] : phi[qc[k]][ac[k]];
-        grads[a][m] = g / H;
+        t->grads[q][a][m] = g / H;
       }
     }
+  }
+}
+
+static void elem_apply(const elem_tab *t, const double *kappa_qp, double f, double *Ke,
+                       double *be) {
+  int nloc = t->nloc, dim = t->dim;
+  if (Ke) memset(Ke, 0, sizeof(double) * nloc * nloc);
+  if (be) memset(be, 0, sizeof(double) * nloc);
+  for (int q = 0; q < t->nq; q++) {
+    double weight = t->weight[q];
     double kap = kappa_qp ? kappa_qp[q] : 1.0;
     if (Ke) {
       for (int i = 0; i < nloc; i++)
         for (int j = 0; j < nloc; j++) {
           double gg = 0.0;
-          for (int m = 0; m < dim; m++) gg += grads[j][m] * kap * grads[i][m];
+          for (int m = 0; m < dim; m++) gg += t->grads[q][j][m] * kap * t->grads[q][i][m];
           Ke[i * nloc + j] += weight * gg;
         }
     }
     if (be)
-      for (int k = 0; k < nloc; k++) be[k] += weight * vals[k] * f;
+      for (int k = 0; k < nloc; k++) be[k] += weight * t->vals[q][k] * f;
   }
+}
+
+int flo_element_matrix(int dim, int order, double H, const double *kappa_qp,
+                       double f, double *Ke, double *be) {
+  if (dim < 1 || dim > FLO_MAXDIM || order < 1 || order > 2) return -1;
+  elem_tab t;
+  elem_tab_init(&t, dim, order, H);
+  elem_apply(&t, kappa_qp, f, Ke, be);
   return 0;
 }
 
@@ -281,6 +305,8 @@ int flo_assemble(int dim, int order, int n, const double *kappa, int nk,
   double H = 1.0 / n;
   double Ke[729], be[27], kq[27];
   long ncells = ipow(n, dim);
+  elem_tab *tab = (elem_tab *)malloc(sizeof(elem_tab));
+  elem_tab_init(tab, dim, order, H);
   for (long cell = 0; cell < ncells; cell++) {
     int cc[FLO_MAXDIM];
     unrank(cell, dim, n, cc);
@@ -295,7 +321,7 @@ int flo_assemble(int dim, int order, int n, const double *kappa, int nk,
         kq[q] = 1.0;
       }
     }
-    flo_element_matrix(dim, order, H, kq, f, Ke, be);
+    elem_apply(tab, kq, f, Ke, be);
     for (int a = 0; a < nloc; a++) {
       int ac[FLO_MAXDIM], P[FLO_MAXDIM];
       unrank(a, dim, nl1, ac);
@@ -310,6 +336,7 @@ int flo_assemble(int dim, int order, int n, const double *kappa, int nk,
       }
     }
   }
+  free(tab);
   unsigned char *bd = (unsigned char *)malloc(N);
   flo_lattice_boundary(dim, order, n, bd);
   for (long r = 0; r < N; r++) {
