@@ -155,6 +155,7 @@ flgpu_mat *mat_convert(const flgpu_mat *A, int prec);
 void sell_free(Sell &s);
 Sell sell_build_rows(flgpu_mat *A, const std::vector<int> &rows);
 void set_colouring(flgpu_mat *A, int ncolours, const int *colour_host);
+void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac);
 
 // out[row] = beta*z[row] + alpha*(A x)[row]  (z may be out; beta == 0 ignores z)
 void spmv(const flgpu_mat *A, double alpha, const void *x, double beta,
@@ -170,6 +171,9 @@ flgpu_mat *lattice_assemble(flgpu_ctx *ctx, int dim, int order, int n, const flg
                             int nk, double f, int prec, int mode, flgpu_vec **rhs);
 flgpu_mat *lattice_interpolation(flgpu_ctx *ctx, int dim, int order, int nc, int prec);
 void lattice_colouring(flgpu_mat *A, int dim, int order, int n);
+flgpu_mat *lattice_pattern(flgpu_ctx *ctx, int dim, int order, int n, int prec);
+flgpu_mat *lattice_galerkin(const flgpu_mat *P, const flgpu_mat *Afine, int dim, int order,
+                            int nc);
 
 // vectors (flgpu_core.hip)
 flgpu_vec *vec_new(flgpu_ctx *ctx, long n, int prec);

@@ -663,6 +663,49 @@ flgpu_mat *lattice_assemble(flgpu_ctx *ctx, int dim, int order, int n, const flg
   return A;
 }
 
+// + identity on the diagonal of constrained (boundary) rows: m+! P after the
+// elimination of the Dirichlet constraints (geomg.lisp:128-136)
+template <class T>
+__global__ void k_dirichlet_diag(long N, int dim, int M, const int *rowptr, const int *colidx,
+                                 T *vals) {
+  long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  long q = r;
+  bool b = false;
+  for (int k = 0; k < dim; k++) {
+    int c = (int)(q % M);
+    q /= M;
+    if (c == 0 || c == M - 1) b = true;
+  }
+  if (!b) return;
+  for (int e = rowptr[r]; e < rowptr[r + 1]; e++)
+    if (colidx[e] == r) vals[e] += (T)1;
+}
+
+// Coarse lattice matrix as the Galerkin product P^T A_fine P
+// (geomg.lisp:122-123 with :galerkin-p t; amg.lisp:247-280)
+flgpu_mat *lattice_galerkin(const flgpu_mat *P, const flgpu_mat *Afine, int dim, int order,
+                            int nc) {
+  flgpu_mat *Ac = lattice_pattern(Afine->ctx, dim, order, nc, Afine->prec);
+  try {
+    galerkin_product(P, Afine, Ac);
+    long N = Ac->nrows;
+    int M = order * nc + 1;
+    if (Ac->prec == FLGPU_F64)
+      k_dirichlet_diag<double><<<grid_for(N, 256), 256, 0, Ac->ctx->stream>>>(
+          N, dim, M, Ac->rowptr, Ac->colidx, (double *)Ac->vals);
+    else
+      k_dirichlet_diag<float><<<grid_for(N, 256), 256, 0, Ac->ctx->stream>>>(
+          N, dim, M, Ac->rowptr, Ac->colidx, (float *)Ac->vals);
+    FLGPU_HIP(hipGetLastError());
+    mat_finish(Ac);
+  } catch (...) {
+    mat_free(Ac);
+    throw;
+  }
+  return Ac;
+}
+
 // lattice colouring: colour = per-axis (coordinate mod (order+1)) digits
 void lattice_colouring(flgpu_mat *A, int dim, int order, int n) {
   int M = order * n + 1, m = order + 1;

@@ -333,6 +333,9 @@ extern "C" int flgpu_mg_create(flgpu_ctx *ctx, int nlevels, flgpu_mat *const *A,
     FLGPU_REQUIRE(nlevels >= 1, "nlevels must be >= 1");
     FLGPU_REQUIRE(nlevels == 1 || I, "I must not be NULL");
     check_params(params, nlevels);
+    FLGPU_REQUIRE(!params->galerkin,
+                  "galerkin = 1 is for lattice hierarchies; build coarse matrices with "
+                  "flgpu_galerkin_product");
     for (int l = 0; l < nlevels; l++) {
       FLGPU_REQUIRE(A[l] && A[l]->ctx == ctx, "level matrix missing or of another context");
       FLGPU_REQUIRE(A[l]->nrows == A[l]->ncols, "level matrices must be square");
@@ -391,11 +394,18 @@ extern "C" int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n
       for (int l = 0; l < nlevels; l++) {
         int n = n0 << l;
         bool is_top = (l == nlevels - 1);
-        mg->lv[l].A = lattice_assemble(ctx, dim, order, n, kappa, nk, f, params->prec, asm_mode,
-                                       (is_top && rhs) ? &b : nullptr);
+        if (!is_top) mg->lv[l].I = lattice_interpolation(ctx, dim, order, n, params->prec);
+      }
+      for (int l = nlevels - 1; l >= 0; l--) {
+        int n = n0 << l;
+        bool is_top = (l == nlevels - 1);
+        if (is_top || !params->galerkin)
+          mg->lv[l].A = lattice_assemble(ctx, dim, order, n, kappa, nk, f, params->prec, asm_mode,
+                                         (is_top && rhs) ? &b : nullptr);
+        else
+          mg->lv[l].A = lattice_galerkin(mg->lv[l].I, mg->lv[l + 1].A, dim, order, n);
         if (params->smoother == FLGPU_SMOOTH_SOR && l > params->base_level)
           lattice_colouring(mg->lv[l].A, dim, order, n);
-        if (!is_top) mg->lv[l].I = lattice_interpolation(ctx, dim, order, n, params->prec);
       }
       mg_alloc_work(mg);
     } catch (...) {
@@ -423,6 +433,16 @@ extern "C" int flgpu_mg_top_matrix(flgpu_mg *mg, flgpu_mat **A) {
   return guard([&] {
     FLGPU_REQUIRE(mg && A, "arguments must not be NULL");
     *A = mg->lv[mg->top].A;
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mg_level_matrix(flgpu_mg *mg, int level, int interp, flgpu_mat **A) {
+  return guard([&] {
+    FLGPU_REQUIRE(mg && A, "arguments must not be NULL");
+    FLGPU_REQUIRE(level >= 0 && level <= mg->top, "level out of range");
+    FLGPU_REQUIRE(!interp || level < mg->top, "the top level has no interpolation");
+    *A = interp ? mg->lv[level].I : mg->lv[level].A;
     return 0;
   });
 }

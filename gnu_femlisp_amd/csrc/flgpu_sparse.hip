@@ -459,11 +459,107 @@ void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const v
   }
 }
 
+// ---- Galerkin product P^T A P (src/iteration/amg.lisp:247-280) ---------------------
+
+// One thread per coarse row k: for fine l in column k of P (row k of P^T,
+// ascending), entries A_lm of row l, and P's row m: acc[k][n] += (P_lk A_lm)
+// P_mn, accumulated in FP64 into the caller's coarse pattern.  For a fixed
+// coarse entry the contributions arrive in (l, m) order, as in femlisp's
+// loop nest.  A product entry outside the pattern sets *bad.
+template <class T>
+__global__ void k_rap(int nc, const int *__restrict__ ptrp, const int *__restrict__ ptci,
+                      const T *__restrict__ ptv, const int *__restrict__ arp,
+                      const int *__restrict__ aci, const T *__restrict__ av,
+                      const int *__restrict__ prp, const int *__restrict__ pci,
+                      const T *__restrict__ pv, const int *__restrict__ crp,
+                      const int *__restrict__ cci, double *acc, int *bad) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nc) return;
+  int c0 = crp[k], c1 = crp[k + 1];
+  for (int e = c0; e < c1; e++) acc[e] = 0.0;
+  for (int a = ptrp[k]; a < ptrp[k + 1]; a++) {
+    int l = ptci[a];
+    double plk = (double)ptv[a];
+    for (int b = arp[l]; b < arp[l + 1]; b++) {
+      int m = aci[b];
+      double w = plk * (double)av[b];
+      for (int c = prp[m]; c < prp[m + 1]; c++) {
+        int n = pci[c];
+        int lo = c0, hi = c1 - 1;
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (cci[mid] < n) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo >= c1 || cci[lo] != n) {
+          *bad = 1;
+          continue;
+        }
+        acc[lo] += w * (double)pv[c];
+      }
+    }
+  }
+}
+
+template <class T>
+__global__ void k_store(long n, const double *acc, T *v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (T)acc[i];
+}
+
+// Ac's values <- P^T A P (pattern of Ac given); all three share one precision
+void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac) {
+  FLGPU_REQUIRE(P->nrows == A->nrows && A->nrows == A->ncols, "galerkin: P and A do not match");
+  FLGPU_REQUIRE(Ac->nrows == P->ncols && Ac->ncols == P->ncols, "galerkin: Ac does not match P");
+  FLGPU_REQUIRE(P->prec == A->prec && A->prec == Ac->prec, "galerkin: precisions differ");
+  flgpu_ctx *ctx = A->ctx;
+  mat_build_transpose(P);
+  const flgpu_mat *PT = P->transpose;
+  double *acc = (double *)dev_alloc(sizeof(double) * std::max<long>(Ac->nnz, 1));
+  int *bad = (int *)dev_alloc(sizeof(int));
+  FLGPU_HIP(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
+  int nc = Ac->nrows;
+  if (A->prec == FLGPU_F64) {
+    k_rap<double><<<grid_for(nc, 128), 128, 0, ctx->stream>>>(
+        nc, PT->rowptr, PT->colidx, (const double *)PT->vals, A->rowptr, A->colidx,
+        (const double *)A->vals, P->rowptr, P->colidx, (const double *)P->vals, Ac->rowptr,
+        Ac->colidx, acc, bad);
+    k_store<double><<<grid_for(Ac->nnz, 256), 256, 0, ctx->stream>>>(Ac->nnz, acc,
+                                                                     (double *)Ac->vals);
+  } else {
+    k_rap<float><<<grid_for(nc, 128), 128, 0, ctx->stream>>>(
+        nc, PT->rowptr, PT->colidx, (const float *)PT->vals, A->rowptr, A->colidx,
+        (const float *)A->vals, P->rowptr, P->colidx, (const float *)P->vals, Ac->rowptr,
+        Ac->colidx, acc, bad);
+    k_store<float><<<grid_for(Ac->nnz, 256), 256, 0, ctx->stream>>>(Ac->nnz, acc,
+                                                                    (float *)Ac->vals);
+  }
+  FLGPU_HIP(hipGetLastError());
+  int hbad = 0;
+  FLGPU_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+  dev_free(acc);
+  dev_free(bad);
+  FLGPU_REQUIRE(!hbad, "galerkin: P^T A P has entries outside the coarse pattern");
+}
+
 }  // namespace flgpu
 
 using namespace flgpu;
 
 // ---- C-ABI ---------------------------------------------------------------------------
+
+extern "C" int flgpu_galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac) {
+  return guard([&] {
+    FLGPU_REQUIRE(P && A && Ac, "arguments must not be NULL");
+    FLGPU_REQUIRE(P->ctx == A->ctx && A->ctx == Ac->ctx, "matrices of different contexts");
+    CTX_LOCK(A->ctx);
+    galerkin_product(P, A, Ac);
+    mat_refresh_values(Ac);
+    FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+    return 0;
+  });
+}
 
 extern "C" int flgpu_csr_upload(flgpu_ctx *ctx, int nrows, int ncols, long nnz, const int *rowptr,
                                 const int *colidx, const double *vals, int prec,

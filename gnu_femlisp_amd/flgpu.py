@@ -33,7 +33,7 @@ class MgParams(ctypes.Structure):
     _fields_ = [("smoother", ctypes.c_int), ("omega", ctypes.c_double),
                 ("pre_steps", ctypes.c_int), ("post_steps", ctypes.c_int),
                 ("base_level", ctypes.c_int), ("symmetric", ctypes.c_int),
-                ("prec", ctypes.c_int)]
+                ("prec", ctypes.c_int), ("galerkin", ctypes.c_int)]
 
 
 class SolveParams(ctypes.Structure):
@@ -81,6 +81,7 @@ SIGNATURES = {
     "flgpu_gemm_nn": (_ci, [_cd, _vp, _vp, _cd, _vp]),
     "flgpu_gemm_tn": (_ci, [_cd, _vp, _vp, _cd, _vp]),
     "flgpu_compute_residual": (_ci, [_vp, _vp, _vp, _vp]),
+    "flgpu_galerkin_product": (_ci, [_vp, _vp, _vp]),
     "flgpu_mat_set_colouring": (_ci, [_vp, _ci, _vp]),
     "flgpu_mat_colour_greedy": (_ci, [_vp, _P(_ci)]),
     "flgpu_jacobi_iterate": (_ci, [_vp, _cd, _vp, _vp]),
@@ -95,6 +96,7 @@ SIGNATURES = {
                                       _P(_vp), _P(_vp)]),
     "flgpu_mg_destroy": (_ci, [_vp]),
     "flgpu_mg_top_matrix": (_ci, [_vp, _P(_vp)]),
+    "flgpu_mg_level_matrix": (_ci, [_vp, _ci, _ci, _P(_vp)]),
     "flgpu_mg_apply": (_ci, [_vp, _vp, _vp, _vp]),
     "flgpu_solve": (_ci, [_vp, _vp, _P(SolveParams), _vp, _vp, _P(SolveReport), _vp, _ci]),
 }

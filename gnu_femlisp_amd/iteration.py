@@ -61,7 +61,7 @@ class GeometricCS(LinearIteration):
     """
 
     def __init__(self, smoother=None, pre_steps=1, post_steps=1, base_level=0, gamma=1,
-                 symmetric=False, prec=F64, combination="multiplicative"):
+                 symmetric=False, prec=F64, combination="multiplicative", galerkin=False):
         if gamma != 1 or combination != "multiplicative":
             raise NotImplementedError("only gamma = 1 multiplicative V-cycles run on the GPU")
         self.smoother = smoother or GaussSeidel()
@@ -69,13 +69,14 @@ class GeometricCS(LinearIteration):
         self.base_level = base_level
         self.symmetric = symmetric
         self.prec = prec
+        self.galerkin = galerkin
 
     def params(self):
         sm = self.smoother
         kind = _f.SMOOTH_JACOBI if isinstance(sm, Jacobi) else _f.SMOOTH_SOR
         omega = sm.damp if isinstance(sm, Jacobi) else sm.omega
         return MgParams(kind, omega, self.pre_steps, self.post_steps, self.base_level,
-                        int(self.symmetric), self.prec)
+                        int(self.symmetric), self.prec, int(self.galerkin))
 
 
 class CG(LinearIteration):
@@ -127,6 +128,11 @@ class Hierarchy:
     def top_matrix(self):
         h = ctypes.c_void_p()
         check(lib().flgpu_mg_top_matrix(self.h, ctypes.byref(h)))
+        return GpuSparseMatrix(h, self.ctx, owner=False)
+
+    def level_matrix(self, level, interp=False):
+        h = ctypes.c_void_p()
+        check(lib().flgpu_mg_level_matrix(self.h, level, int(interp), ctypes.byref(h)))
         return GpuSparseMatrix(h, self.ctx, owner=False)
 
     def apply(self, x, b, r):

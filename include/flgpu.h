@@ -110,6 +110,11 @@ int flgpu_gemm_nn(double alpha, const flgpu_mat *A, const flgpu_vec *y,
  * (sparse-matrix.lisp:406-433); the transpose is built on first use. */
 int flgpu_gemm_tn(double alpha, const flgpu_mat *A, const flgpu_vec *y,
                   double beta, flgpu_vec *x);
+/* galerkin-product (src/iteration/amg.lisp:247-280): the values of Ac
+ * become P^T A P, accumulated in FP64 into Ac's existing pattern (an entry
+ * of the product outside that pattern is an error).  P is fine x coarse. */
+int flgpu_galerkin_product(const flgpu_mat *P, const flgpu_mat *A,
+                           flgpu_mat *Ac);
 /* compute-residual A x b r : r <- b - A x; b may be the same vector as r, in
  * which case r <- r - A x exactly as femlisp's copy!+gemm! does
  * (src/iteration/linit.lisp:73-80). */
@@ -189,6 +194,8 @@ typedef struct flgpu_mg_params {
   int base_level;  /* coarse-grid solve level: dense FP64 LU/inverse */
   int symmetric;   /* 1: post-smoothing sweeps the colours backwards */
   int prec;        /* storage/arithmetic of the V-cycle: FLGPU_F64 / FLGPU_F32 */
+  int galerkin;    /* lattice hierarchies: 1 = coarse levels P^T A P
+                      (geomg.lisp:122-123 :galerkin-p t), 0 = re-discretised */
 } flgpu_mg_params;
 
 /* Hierarchy from caller matrices: A[0..nlevels-1] level matrices (coarsest
@@ -211,6 +218,9 @@ int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n0,
 int flgpu_mg_destroy(flgpu_mg *mg);
 /* the top-level matrix of the hierarchy (owned by mg) */
 int flgpu_mg_top_matrix(flgpu_mg *mg, flgpu_mat **A);
+/* the matrix (I = 0) or the interpolation to the next level (I = 1) of one
+ * level, owned by mg (the :a-vec / :i-vec entries, multigrid.lisp:108-109) */
+int flgpu_mg_level_matrix(flgpu_mg *mg, int level, int interp, flgpu_mat **A);
 /* The <mg-iteration> iterator (multigrid.lisp:368-391): one V-cycle on
  * (x, b, r) with r = b - A x up to date; x is updated, r and b are used as
  * femlisp uses them (b may be the same vector as r: femlisp's <cg> calls

@@ -505,8 +505,57 @@ static void csr_free(csr_t *m) {
   memset(m, 0, sizeof(*m));
 }
 
+/* galerkin-product R A P with R = P^T (src/iteration/amg.lisp:247-280):
+ * loop over fine rows l, entries A_lm, coarse k with P_lk, and P's row m:
+ * C_kn += (P_lk A_lm) P_mn, into the coarse lattice pattern (the structural
+ * product of the constrained P, whose zeroed entries are stored, with A is
+ * exactly that pattern).  Then coarse Dirichlet elimination + P: the zeroed
+ * columns of P leave constrained rows and columns zero, and the identity is
+ * added on their diagonal (geomg.lisp:128-136). */
+void flo_mg_destroy(flo_mg *mg);
+
+static int galerkin(level_t *C, const level_t *F) {
+  const csr_t *A = &F->A, *P = &C->I;
+  csr_t *Ac = &C->A;
+  long nnz = Ac->rowptr[Ac->nrows];
+  memset(Ac->vals, 0, sizeof(double) * nnz);
+  for (int l = 0; l < A->nrows; l++)
+    for (int e = A->rowptr[l]; e < A->rowptr[l + 1]; e++) {
+      int m = A->colidx[e];
+      double alm = A->vals[e];
+      for (int pk = P->rowptr[l]; pk < P->rowptr[l + 1]; pk++) {
+        int k = P->colidx[pk];
+        double w = P->vals[pk] * alm;
+        for (int pn = P->rowptr[m]; pn < P->rowptr[m + 1]; pn++) {
+          int n = P->colidx[pn];
+          int lo = Ac->rowptr[k], hi = Ac->rowptr[k + 1] - 1;
+          while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            if (Ac->colidx[mid] < n) lo = mid + 1;
+            else hi = mid;
+          }
+          if (Ac->colidx[lo] != n) return -1;
+          Ac->vals[lo] += w * P->vals[pn];
+        }
+      }
+    }
+  for (long r = 0; r < C->N; r++)
+    if (C->bdry[r])
+      for (int e = Ac->rowptr[r]; e < Ac->rowptr[r + 1]; e++)
+        if (Ac->colidx[e] == r) Ac->vals[e] += 1.0;
+  for (long r = 0; r < C->N; r++)
+    for (int e = Ac->rowptr[r]; e < Ac->rowptr[r + 1]; e++)
+      if (Ac->colidx[e] == r) C->diag[r] = Ac->vals[e];
+  return 0;
+}
+
 flo_mg *flo_mg_create(int dim, int order, int n0, int levels,
                       const double *kappa, int nk, double f) {
+  return flo_mg_create_ex(dim, order, n0, levels, kappa, nk, f, 0);
+}
+
+flo_mg *flo_mg_create_ex(int dim, int order, int n0, int levels,
+                         const double *kappa, int nk, double f, int galerkin_p) {
   if (dim < 1 || dim > 3 || order < 1 || order > 2 || n0 < 1 || levels < 1) return NULL;
   flo_mg *mg = (flo_mg *)calloc(1, sizeof(flo_mg));
   mg->dim = dim;
@@ -531,7 +580,8 @@ flo_mg *flo_mg_create(int dim, int order, int n0, int levels,
     L->A.vals = (double *)malloc(sizeof(double) * nnz);
     flo_lattice_pattern(dim, order, L->n, L->A.rowptr, L->A.colidx);
     double *rhs = (l == levels - 1) ? (double *)malloc(sizeof(double) * L->N) : NULL;
-    flo_assemble(dim, order, L->n, kappa, nk, f, L->A.vals, rhs);
+    /* with galerkin-p only the top level is discretised (geomg.lisp:119-126) */
+    if (!galerkin_p || l == levels - 1) flo_assemble(dim, order, L->n, kappa, nk, f, L->A.vals, rhs);
     if (rhs) mg->rhs_top = rhs;
     L->diag = (double *)malloc(sizeof(double) * L->N);
     for (long r = 0; r < L->N; r++)
@@ -560,6 +610,12 @@ flo_mg *flo_mg_create(int dim, int order, int n0, int levels,
     L->rhs = (double *)calloc(L->N, sizeof(double));
     L->res = (double *)calloc(L->N, sizeof(double));
   }
+  if (galerkin_p)
+    for (int l = levels - 2; l >= 0; l--)
+      if (galerkin(&mg->lv[l], &mg->lv[l + 1]) != 0) {
+        flo_mg_destroy(mg);
+        return NULL;
+      }
   return mg;
 }
 

@@ -92,6 +92,11 @@ typedef struct flo_mg flo_mg;
  * re-discretisation on every level (src/iteration/geomg.lisp:109-142). */
 flo_mg *flo_mg_create(int dim, int order, int n0, int levels,
                       const double *kappa, int nk, double f);
+/* as flo_mg_create; galerkin_p = 1 builds the coarse levels as P^T A P
+ * (geomg.lisp:122-123 :galerkin-p t, amg.lisp:247-280) instead of
+ * re-discretising them */
+flo_mg *flo_mg_create_ex(int dim, int order, int n0, int levels,
+                         const double *kappa, int nk, double f, int galerkin_p);
 void flo_mg_destroy(flo_mg *mg);
 long flo_mg_ndofs(const flo_mg *mg, int level);
 long flo_mg_nnz(const flo_mg *mg, int level);

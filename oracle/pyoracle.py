@@ -68,6 +68,8 @@ def lib():
         L.flo_csr_transpose.argtypes = [ci, ci, vp, vp, vp, vp, vp, vp]
         L.flo_mg_create.restype = vp
         L.flo_mg_create.argtypes = [ci, ci, ci, ci, vp, ci, cd]
+        L.flo_mg_create_ex.restype = vp
+        L.flo_mg_create_ex.argtypes = [ci, ci, ci, ci, vp, ci, cd, ci]
         L.flo_mg_destroy.argtypes = [vp]
         L.flo_mg_ndofs.restype = cl
         L.flo_mg_ndofs.argtypes = [vp, ci]
@@ -167,14 +169,15 @@ def spmv(rp, ci, vals, y, alpha=1.0, beta=0.0, x=None):
 class Hierarchy:
     """Multigrid hierarchy of the oracle (re-discretised levels)."""
 
-    def __init__(self, dim, order, n0, levels, kappa=None, f=1.0):
+    def __init__(self, dim, order, n0, levels, kappa=None, f=1.0, galerkin=False):
         self.dim, self.order, self.n0, self.levels = dim, order, n0, levels
         nk = 0
         self._kappa = None
         if kappa is not None:
             self._kappa = np.ascontiguousarray(kappa, np.float64)
             nk = int(round(self._kappa.size ** (1.0 / dim)))
-        self.h = lib().flo_mg_create(dim, order, n0, levels, _p(self._kappa), nk, f)
+        self.h = lib().flo_mg_create_ex(dim, order, n0, levels, _p(self._kappa), nk, f,
+                                        int(galerkin))
         if not self.h:
             raise ValueError("flo_mg_create failed")
 

@@ -284,6 +284,31 @@ def test_solve_parity(case):
     assert np.allclose(bb["defnorms"], dno, rtol=1e-8)
 
 
+@pytest.mark.parametrize("dim,order,levels", [(3, 1, 4), (2, 2, 3), (2, 1, 5)])
+def test_galerkin_hierarchy_parity(dim, order, levels):
+    say(f"galerkin dim={dim} order={order} levels={levels}")
+    ntop = 1 << (levels - 1)
+    kap = np.exp(np.random.default_rng(3).normal(scale=1.5, size=(ntop,) * dim))
+    oh = po.Hierarchy(dim, order, 1, levels, kappa=kap, galerkin=True)
+    cs = it.GeometricCS(it.SOR(1.0), 1, 1, 0, galerkin=True)
+    H = it.Hierarchy.lattice(cs, dim, order, 1, levels, kappa=kap)
+    for l in range(levels):
+        rp, ci, v = H.level_matrix(l).csr()
+        orp, oci, ov = oh.matrix(l)
+        assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+        assert rowrel(v, ov, orp) < 1e-12
+
+
+def test_known_answer_galerkin_rate_3d():
+    """mg-cdr.lisp:100 on the GPU: Galerkin GS V(1,0), CR < 0.15."""
+    say("known answer galerkin")
+    cs = it.GeometricCS(it.GaussSeidel(), 1, 0, 1, galerkin=True)
+    H = it.Hierarchy.lattice(cs, 3, 1, 1, 3)
+    bb = it.solve(it.LinearSolver(cs, success_if=("<", "defnorm", 1e-12), failure_if=(">", "step", 20)),
+                  {"matrix": H.top_matrix(), "rhs": H.rhs, "hierarchy": H})
+    assert bb["report"]["convergence-rate"] < 0.15
+
+
 def test_alias_mode_matches_oracle():
     """femlisp's literal (iterate q w w): same (stalling) defect history."""
     say("alias cg")

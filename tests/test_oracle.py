@@ -52,6 +52,35 @@ def test_vcycle_rate_2d_gs():
         assert (dn[-1] / dn[0]) ** (1.0 / steps) < k["assert_below"]
 
 
+def test_vcycle_rate_3d_galerkin():
+    """mg-cdr.lisp:100: 3D Q1 level 2 with Galerkin coarse operators, GS
+    V(1,0), base-level 1 -> CR < 0.15."""
+    k = known()["vcycle_cr_3d_q1_l2_galerkin"]
+    h = po.Hierarchy(3, 1, 1, 3, galerkin=True)
+    for ordering in (po.ORDER_INDEX, po.ORDER_COLOUR):
+        p = po.params(outer=po.OUTER_MG, smoother=po.SMOOTH_SOR, omega=1.0, ordering=ordering,
+                      pre_steps=1, post_steps=0, base_level=1, stop=po.STOP_DEFNORM, tol=1e-12,
+                      maxsteps=20)
+        st, x, steps, dn = h.solve(p)
+        assert (dn[-1] / dn[0]) ** (1.0 / steps) < k["assert_below"]
+
+
+def test_galerkin_equals_rediscretisation_for_constant_coefficient():
+    """Trilinear interpolation is the natural injection of the coarse Q1 space,
+    so P^T A P equals the re-discretised coarse matrix when kappa is constant;
+    with a variable kappa it is a different (Galerkin) operator."""
+    hr = po.Hierarchy(3, 1, 1, 4)
+    hg = po.Hierarchy(3, 1, 1, 4, galerkin=True)
+    for l in range(3):
+        a, g = hr.matrix(l)[2], hg.matrix(l)[2]
+        assert np.abs(a - g).max() < 1e-13 * np.abs(a).max()
+    kap = np.exp(np.random.default_rng(0).normal(size=(8, 8, 8)))
+    hk = po.Hierarchy(3, 1, 1, 4, kappa=kap, galerkin=True)
+    rp, ci, v = hk.matrix(2)
+    _, _, vr = po.assemble(3, 1, 4, kap)[:3]
+    assert np.abs(v - vr).max() > 1e-3
+
+
 def test_fe_value_1d():
     """ellsys-fe.lisp:249-256: 1D Q1 -u''=1 on level 2 gives u(0.5) = 0.125 exactly."""
     rp, ci, v, b = po.assemble(1, 1, 4)
