@@ -174,6 +174,8 @@ void lattice_colouring(flgpu_mat *A, int dim, int order, int n);
 flgpu_mat *lattice_pattern(flgpu_ctx *ctx, int dim, int order, int n, int prec);
 flgpu_mat *lattice_galerkin(const flgpu_mat *P, const flgpu_mat *Afine, int dim, int order,
                             int nc);
+void lattice_dirichlet_identity(flgpu_mat *A, int dim, int order, int n);
+void lattice_reassemble(flgpu_mat *A, const flgpu_vec *kappa, int nk, int mode);
 
 // vectors (flgpu_core.hip)
 flgpu_vec *vec_new(flgpu_ctx *ctx, long n, int prec);

@@ -682,6 +682,24 @@ __global__ void k_dirichlet_diag(long N, int dim, int M, const int *rowptr, cons
     if (colidx[e] == r) vals[e] += (T)1;
 }
 
+void lattice_dirichlet_identity(flgpu_mat *A, int dim, int order, int n) {
+  long N = A->nrows;
+  int M = order * n + 1;
+  if (A->prec == FLGPU_F64)
+    k_dirichlet_diag<double><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
+        N, dim, M, A->rowptr, A->colidx, (double *)A->vals);
+  else
+    k_dirichlet_diag<float><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
+        N, dim, M, A->rowptr, A->colidx, (float *)A->vals);
+  FLGPU_HIP(hipGetLastError());
+}
+
+void lattice_reassemble(flgpu_mat *A, const flgpu_vec *kappa, int nk, int mode) {
+  check_kappa(kappa, A->lat_dim, A->lat_n, nk);
+  assemble_values(A, kappa, nk, mode, nullptr);
+  mat_refresh_values(A);
+}
+
 // Coarse lattice matrix as the Galerkin product P^T A_fine P
 // (geomg.lisp:122-123 with :galerkin-p t; amg.lisp:247-280)
 flgpu_mat *lattice_galerkin(const flgpu_mat *P, const flgpu_mat *Afine, int dim, int order,

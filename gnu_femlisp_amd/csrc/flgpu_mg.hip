@@ -46,6 +46,12 @@ struct flgpu_mg {
   void *xs = nullptr, *bs = nullptr, *rs = nullptr;
   // active top-level pointers during one cycle
   void *sol_top = nullptr, *rhs_top = nullptr, *res_top = nullptr;
+  // lattice description (re-assembly for a new coefficient)
+  int lat_dim = 0, lat_order = 0, lat_n0 = 0, asm_mode = FLGPU_ASM_F64;
+  // attached learned correction
+  flgpu_unet *unet = nullptr;
+  const flgpu_vec *unet_kappa = nullptr;
+  int unet_n = 0;
 };
 
 namespace flgpu {
@@ -269,6 +275,12 @@ static void mg_alloc_work(flgpu_mg *mg) {
   coarse_setup(mg, mg->lv[mg->p.base_level]);
 }
 
+void mg_unet(flgpu_mg *mg, flgpu_unet **u, const flgpu_vec **kappa, int *n) {
+  *u = mg->unet;
+  *kappa = mg->unet_kappa;
+  *n = mg->unet_n;
+}
+
 void mg_free(flgpu_mg *mg) {
   if (!mg) return;
   for (auto &L : mg->lv) {
@@ -389,6 +401,10 @@ extern "C" int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n
     mg->p = *params;
     mg->top = nlevels - 1;
     mg->lv.resize(nlevels);
+    mg->lat_dim = dim;
+    mg->lat_order = order;
+    mg->lat_n0 = n0;
+    mg->asm_mode = asm_mode;
     flgpu_vec *b = nullptr;
     try {
       for (int l = 0; l < nlevels; l++) {
@@ -415,6 +431,48 @@ extern "C" int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n
     }
     if (rhs) *rhs = b;
     *out = mg;
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mg_reassemble(flgpu_mg *mg, const flgpu_vec *kappa, int nk) {
+  return guard([&] {
+    FLGPU_REQUIRE(mg, "hierarchy must not be NULL");
+    FLGPU_REQUIRE(mg->lat_dim > 0, "the hierarchy was not assembled on a lattice");
+    FLGPU_REQUIRE(!kappa || kappa->ctx == mg->ctx, "kappa belongs to another context");
+    CTX_LOCK(mg->ctx);
+    for (int l = mg->top; l >= 0; l--) {
+      MGLevel &L = mg->lv[l];
+      if (l == mg->top || !mg->p.galerkin) {
+        lattice_reassemble(L.A, kappa, nk, mg->asm_mode);
+      } else {
+        galerkin_product(L.I, mg->lv[l + 1].A, L.A);
+        lattice_dirichlet_identity(L.A, mg->lat_dim, mg->lat_order, mg->lat_n0 << l);
+        mat_refresh_values(L.A);
+      }
+    }
+    MGLevel &B = mg->lv[mg->p.base_level];
+    dev_free(B.cinv);
+    dev_free(B.ctmp);
+    B.cinv = nullptr;
+    B.ctmp = nullptr;
+    coarse_setup(mg, B);
+    FLGPU_HIP(hipStreamSynchronize(mg->ctx->stream));
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mg_attach_unet(flgpu_mg *mg, flgpu_unet *u, const flgpu_vec *kappa, int n) {
+  return guard([&] {
+    FLGPU_REQUIRE(mg, "hierarchy must not be NULL");
+    FLGPU_REQUIRE(mg->lat_dim == 3 && mg->lat_order == 1,
+                  "the learned correction is defined for 3D Q1 lattices");
+    FLGPU_REQUIRE((mg->lat_n0 << mg->top) == n, "n does not match the top level");
+    FLGPU_REQUIRE(!kappa || (kappa->prec == FLGPU_F64 && kappa->n == (long)n * n * n),
+                  "kappa must be an FP64 vector of n^3 cell values");
+    mg->unet = u;
+    mg->unet_kappa = kappa;
+    mg->unet_n = n;
     return 0;
   });
 }
@@ -523,8 +581,9 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
     int prec = A->prec;
     hipStream_t s = ctx->stream;
     SolveWork W(ctx, N, prec);
-    LearnedHook *learned = nullptr;
-    if (p->learned) learned = learned_hook(mg, A);
+    std::unique_ptr<LearnedHook> hook;
+    if (p->learned) hook = learned_hook(mg, A);
+    LearnedHook *learned = hook.get();
     double *slot = ctx->red_out;
     unsigned g = grid_for(N, 256);
     auto t0 = std::chrono::steady_clock::now();

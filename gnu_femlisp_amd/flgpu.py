@@ -99,6 +99,18 @@ SIGNATURES = {
     "flgpu_mg_level_matrix": (_ci, [_vp, _ci, _ci, _P(_vp)]),
     "flgpu_mg_apply": (_ci, [_vp, _vp, _vp, _vp]),
     "flgpu_solve": (_ci, [_vp, _vp, _P(SolveParams), _vp, _vp, _P(SolveReport), _vp, _ci]),
+    "flgpu_mg_reassemble": (_ci, [_vp, _vp, _ci]),
+    "flgpu_unet_create": (_ci, [_vp, _ci, ctypes.c_ulong, _P(_vp)]),
+    "flgpu_unet_destroy": (_ci, [_vp]),
+    "flgpu_unet_nparams": (_ci, [_vp, _P(_cl)]),
+    "flgpu_unet_get_params": (_ci, [_vp, _vp, _vp, _vp, _P(_ci)]),
+    "flgpu_unet_set_params": (_ci, [_vp, _vp, _vp, _vp, _ci]),
+    "flgpu_unet_forward": (_ci, [_vp, _ci, _ci, _vp, _vp]),
+    "flgpu_unet_loss_grad": (_ci, [_vp, _ci, _ci, _vp, _vp, _P(_cd), _vp]),
+    "flgpu_unet_train_step": (_ci, [_vp, _ci, _ci, _vp, _vp, _cd, _P(_cd)]),
+    "flgpu_unet_pack": (_ci, [_vp, _vp, _ci, _vp, _cl, _ci]),
+    "flgpu_vec_gather_rows": (_ci, [_vp, _cl, _vp, _ci, _vp]),
+    "flgpu_mg_attach_unet": (_ci, [_vp, _vp, _vp, _ci]),
 }
 
 _LIB = None

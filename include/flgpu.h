@@ -228,6 +228,49 @@ int flgpu_mg_level_matrix(flgpu_mg *mg, int level, int interp, flgpu_mat **A);
  * hierarchy: then x <- x + V(r) with the cycle in FP32 (b must not be r). */
 int flgpu_mg_apply(flgpu_mg *mg, flgpu_vec *x, flgpu_vec *b, flgpu_vec *r);
 
+/* Re-assemble every level of a lattice hierarchy for a new coefficient
+ * (same pattern, transfers and colouring; coarse inverse recomputed). */
+int flgpu_mg_reassemble(flgpu_mg *mg, const flgpu_vec *kappa, int nk);
+
+/* ---- learned coarse correction (no femlisp counterpart) ------------------
+ *
+ * A 3-level 3D U-Net (16/32/64 channels, 3x3x3 kernels, BF16 weights, FP32
+ * accumulation on MFMA; Adam) mapping a residual (and log kappa) on the n^3
+ * grid of lattice nodes with coordinates < n to an error estimate.  Inside
+ * flexible CG (flgpu_solve with learned = 1) the preconditioner becomes
+ * z = V(r) + beta N(r - A V(r)), beta minimising the energy along N. */
+int flgpu_unet_create(flgpu_ctx *ctx, int cin, unsigned long seed,
+                      flgpu_unet **out);
+int flgpu_unet_destroy(flgpu_unet *u);
+int flgpu_unet_nparams(const flgpu_unet *u, long *np);
+/* parameters and Adam moments as doubles (any of theta/m/v may be NULL on
+ * get; m/v NULL on set resets them); the layout is per layer W[27][ci][co]
+ * then bias[co] */
+int flgpu_unet_get_params(const flgpu_unet *u, double *theta, double *m,
+                          double *v, int *step);
+int flgpu_unet_set_params(flgpu_unet *u, const double *theta, const double *m,
+                          const double *v, int step);
+/* x: FP32 [batch][n^3][cin], y: FP32 [batch][n^3] */
+int flgpu_unet_forward(flgpu_unet *u, int batch, int n, const flgpu_vec *x,
+                       flgpu_vec *y);
+/* mean squared error against t and its gradient (grad[nparams], may be NULL) */
+int flgpu_unet_loss_grad(flgpu_unet *u, int batch, int n, const flgpu_vec *x,
+                         const flgpu_vec *t, double *loss, double *grad);
+/* forward + backward + one Adam step; loss before the update */
+int flgpu_unet_train_step(flgpu_unet *u, int batch, int n, const flgpu_vec *x,
+                          const flgpu_vec *t, double lr, double *loss);
+/* network input (target = 0: [r/rms(r), log(kappa)/4]) or target
+ * (target = 1: e/rms(e)) of sample `sample` from a lattice vector of
+ * (n+1)^3 values */
+int flgpu_unet_pack(const flgpu_vec *r, const flgpu_vec *kappa, int n,
+                    flgpu_vec *x, long sample, int target);
+/* dst row k = src row idx[k] (rows of row_len values) */
+int flgpu_vec_gather_rows(const flgpu_vec *src, long row_len, const long *idx,
+                          int count, flgpu_vec *dst);
+/* attach a trained network (and the n^3 cell coefficient it reads) */
+int flgpu_mg_attach_unet(flgpu_mg *mg, flgpu_unet *u, const flgpu_vec *kappa,
+                         int n);
+
 /* ---- solvers (src/iteration/krylow.lisp:67-132, linsolve.lisp:58-73,
  *      solve.lisp:119-165, iterate.lisp:239-250) --------------------------- */
 

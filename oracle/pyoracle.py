@@ -82,6 +82,12 @@ def lib():
         L.flo_mg_get_rhs.argtypes = [vp, vp]
         L.flo_mg_solve.argtypes = [vp, P(Params), vp, vp, P(ci), vp]
         L.flo_mg_apply.argtypes = [vp, P(Params), vp, vp]
+        L.flo_unet_nparams.restype = cl
+        L.flo_unet_nparams.argtypes = [ci]
+        L.flo_unet_forward.argtypes = [ci, ci, ci, vp, vp, vp, ci]
+        L.flo_unet_loss_grad.restype = cd
+        L.flo_unet_loss_grad.argtypes = [ci, ci, ci, vp, vp, vp, vp, ci]
+        L.flo_adam.argtypes = [cl, vp, vp, vp, vp, ci, cd, cd, cd, cd]
         _LIB = L
     return _LIB
 
@@ -227,3 +233,32 @@ class Hierarchy:
         x = np.zeros(self.ndofs())
         assert lib().flo_mg_apply(self.h, ctypes.byref(prm), _p(b), _p(x)) == 0
         return x
+
+
+# ---- learned-correction network checker (conv_oracle.c) ----------------------
+
+def unet_nparams(cin=2):
+    return lib().flo_unet_nparams(cin)
+
+
+def unet_forward(theta, x, batch, n, cin=2, bf16=True):
+    theta = np.ascontiguousarray(theta, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(batch * n ** 3)
+    assert lib().flo_unet_forward(batch, n, cin, _p(theta), _p(x), _p(y), int(bf16)) == 0
+    return y
+
+
+def unet_loss_grad(theta, x, t, batch, n, cin=2, bf16=True):
+    theta = np.ascontiguousarray(theta, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    t = np.ascontiguousarray(t, np.float64)
+    g = np.zeros(unet_nparams(cin))
+    loss = lib().flo_unet_loss_grad(batch, n, cin, _p(theta), _p(x), _p(t), _p(g), int(bf16))
+    return loss, g
+
+
+def adam(theta, g, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    theta, g, m, v = (np.ascontiguousarray(a, np.float64).copy() for a in (theta, g, m, v))
+    lib().flo_adam(theta.size, _p(theta), _p(g), _p(m), _p(v), t, lr, b1, b2, eps)
+    return theta, m, v
