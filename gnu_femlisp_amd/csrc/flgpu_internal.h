@@ -141,6 +141,11 @@ struct flgpu_mat {
   // lattice description (0 if the matrix was not assembled on a lattice)
   int lat_dim = 0, lat_order = 0, lat_n = 0;
   double lat_f = 1.0;
+  // 3D Q1 with a coefficient constant per cell: element matrix of cell c is
+  // alpha[c] * Kref, so A_ij = kt[|d|] * (sum of alpha over the cells shared
+  // by DOFs i and j), d = lattice offset; kernels apply A from alpha
+  void *alpha = nullptr;  // n^3 cell factors (prec), nullptr if not available
+  double kt[8] = {};      // Kref[0][b], b = bz*4 + by*2 + bx
 };
 
 namespace flgpu {
@@ -151,6 +156,7 @@ void mat_free(flgpu_mat *A);
 void mat_finish(flgpu_mat *A);  // SELL + diagonal from the device CSR
 void mat_refresh_values(flgpu_mat *A);  // SELL/diag/colour values after a CSR value change
 void mat_build_transpose(const flgpu_mat *A);
+void drop_alpha(flgpu_mat *A);
 flgpu_mat *mat_convert(const flgpu_mat *A, int prec);
 void sell_free(Sell &s);
 Sell sell_build_rows(flgpu_mat *A, const std::vector<int> &rows);

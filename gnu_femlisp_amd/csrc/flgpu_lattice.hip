@@ -623,6 +623,40 @@ static void assemble_values(flgpu_mat *A, const flgpu_vec *kappa, int nk, int mo
   dev_free(E);
 }
 
+template <class T>
+__global__ void k_alpha(long ncells, double H, const double *__restrict__ kappa, T *alpha) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells) alpha[c] = (T)(H * (kappa ? kappa[c] : 1.0));
+}
+
+// 3D Q1 with one coefficient value per cell (nk == n): the element matrix of
+// cell c is H kappa_c Kref, Kref = sum_q T_q, and by the reflection symmetry
+// of the cube Kref[a][b] = Kref[0][a xor b].  Stores alpha_c = H kappa_c and
+// Kref[0][.] so that the kernels apply A without streaming its values.
+static void lattice_set_alpha(flgpu_mat *A, const flgpu_vec *kappa, int nk) {
+  int n = A->lat_n;
+  if (A->lat_dim != 3 || A->lat_order != 1 || (kappa && nk != n)) {
+    drop_alpha(A);
+    return;
+  }
+  RefElement R = ref_element(3, 1);
+  for (int b = 0; b < 8; b++) {
+    double s = 0.0;
+    for (int q = 0; q < R.nq; q++) s += R.T[(size_t)q * R.nloc2 + b];
+    A->kt[b] = s;
+  }
+  long ncells = (long)n * n * n;
+  if (!A->alpha) A->alpha = dev_alloc(prec_bytes(A->prec) * ncells);
+  const double *kd = kappa ? (const double *)kappa->d : nullptr;
+  if (A->prec == FLGPU_F64)
+    k_alpha<double><<<grid_for(ncells, 256), 256, 0, A->ctx->stream>>>(ncells, 1.0 / n, kd,
+                                                                       (double *)A->alpha);
+  else
+    k_alpha<float><<<grid_for(ncells, 256), 256, 0, A->ctx->stream>>>(ncells, 1.0 / n, kd,
+                                                                      (float *)A->alpha);
+  FLGPU_HIP(hipGetLastError());
+}
+
 flgpu_mat *lattice_pattern(flgpu_ctx *ctx, int dim, int order, int n, int prec) {
   check_lattice_args(dim, order, n);
   Table1D pat = table_pattern(order, n);
@@ -653,6 +687,7 @@ flgpu_mat *lattice_assemble(flgpu_ctx *ctx, int dim, int order, int n, const flg
   try {
     if (rhs) b = vec_new(ctx, A->nrows, prec);
     assemble_values(A, kappa, nk, mode, b ? b->d : nullptr);
+    lattice_set_alpha(A, kappa, nk);
     mat_finish(A);
   } catch (...) {
     vec_free(b);
@@ -697,6 +732,7 @@ void lattice_dirichlet_identity(flgpu_mat *A, int dim, int order, int n) {
 void lattice_reassemble(flgpu_mat *A, const flgpu_vec *kappa, int nk, int mode) {
   check_kappa(kappa, A->lat_dim, A->lat_n, nk);
   assemble_values(A, kappa, nk, mode, nullptr);
+  lattice_set_alpha(A, kappa, nk);
   mat_refresh_values(A);
 }
 

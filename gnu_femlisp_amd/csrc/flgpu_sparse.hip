@@ -190,7 +190,14 @@ void mat_free(flgpu_mat *A) {
   sell_free(A->sell);
   for (auto &s : A->colour_sell) sell_free(s);
   if (A->transpose) mat_free(A->transpose);
+  dev_free(A->alpha);
   delete A;
+}
+
+// values no longer follow the per-cell form (new values or another ordering)
+void drop_alpha(flgpu_mat *A) {
+  dev_free(A->alpha);
+  A->alpha = nullptr;
 }
 
 void mat_refresh_values(flgpu_mat *A) {
@@ -291,6 +298,12 @@ flgpu_mat *mat_convert(const flgpu_mat *A, int prec) {
   B->lat_order = A->lat_order;
   B->lat_n = A->lat_n;
   B->lat_f = A->lat_f;
+  if (A->alpha) {
+    long nc = (long)A->lat_n * A->lat_n * A->lat_n;
+    B->alpha = dev_alloc(prec_bytes(prec) * nc);
+    vcopy(ctx, nc, A->prec, A->alpha, prec, B->alpha);
+    for (int j = 0; j < 8; j++) B->kt[j] = A->kt[j];
+  }
   mat_finish(B);
   if (A->transpose) B->transpose = mat_convert(A->transpose, prec);
   if (A->ncolours > 0) {
@@ -377,8 +390,115 @@ static void launch_spmv(flgpu_ctx *ctx, const Sell &s, double alpha, const void 
   FLGPU_HIP(hipGetLastError());
 }
 
+// ---- 3D Q1 lattice with a per-cell coefficient: A applied from alpha ---------------
+
+struct Q1K {
+  double k[8];
+};
+
+// acc + sum over interior neighbours j of row i (ascending j) of (coef x_j) A_ij
+// with A_ij = kt[|dz|*4 + |dy|*2 + |dx|] * (sum of alpha over the cells shared
+// by i and j); *dii receives A_ii.  Row i = (z,y,x) must be interior.
+template <class T>
+__device__ __forceinline__ T q1_row(int M, int n, long i, int z, int y, int x,
+                                    const T *__restrict__ alpha, const Q1K &kt, T acc, T coef,
+                                    const T *__restrict__ xv, T *dii) {
+  T a[2][2][2];
+  for (int p = 0; p < 2; p++)
+    for (int q = 0; q < 2; q++)
+      for (int s = 0; s < 2; s++)
+        a[p][q][s] = alpha[((long)(z - 1 + p) * n + (y - 1 + q)) * n + (x - 1 + s)];
+  long M2 = (long)M * M;
+  for (int dz = -1; dz <= 1; dz++) {
+    int zz = z + dz;
+    if (zz == 0 || zz == M - 1) continue;  // Dirichlet column: eliminated
+    int pz0 = dz > 0 ? 1 : 0, pz1 = dz < 0 ? 0 : 1;
+    for (int dy = -1; dy <= 1; dy++) {
+      int yy = y + dy;
+      if (yy == 0 || yy == M - 1) continue;
+      int py0 = dy > 0 ? 1 : 0, py1 = dy < 0 ? 0 : 1;
+      for (int dx = -1; dx <= 1; dx++) {
+        int xx = x + dx;
+        if (xx == 0 || xx == M - 1) continue;
+        int px0 = dx > 0 ? 1 : 0, px1 = dx < 0 ? 0 : 1;
+        T s = (T)0;
+        for (int p = pz0; p <= pz1; p++)
+          for (int q = py0; q <= py1; q++)
+            for (int r = px0; r <= px1; r++) s += a[p][q][r];
+        T aij = (T)kt.k[(dz != 0) * 4 + (dy != 0) * 2 + (dx != 0)] * s;
+        if (dz == 0 && dy == 0 && dx == 0) *dii = aij;
+        acc += (coef * xv[i + dz * M2 + dy * M + dx]) * aij;
+      }
+    }
+  }
+  return acc;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_q1_spmv(int M, int n, const T *__restrict__ alpha, Q1K kt,
+                                                 T coef, const T *__restrict__ x, T beta,
+                                                 const T *z, T *out) {
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  long i = (long)lb * blockDim.x + threadIdx.x;
+  long N = (long)M * M * M;
+  if (i >= N) return;
+  int xx = (int)(i % M), yy = (int)((i / M) % M), zz = (int)(i / ((long)M * M));
+  T acc = (beta == (T)0) ? (T)0 : beta * z[i];
+  bool bd = xx == 0 || yy == 0 || zz == 0 || xx == M - 1 || yy == M - 1 || zz == M - 1;
+  if (bd) {
+    acc += (coef * x[i]) * (T)1;
+  } else {
+    T d;
+    acc = q1_row<T>(M, n, i, zz, yy, xx, alpha, kt, acc, coef, x, &d);
+  }
+  out[i] = acc;
+}
+
+// one colour (cz, cy, cx) of a <sor> sweep from alpha
+template <class T>
+__global__ __launch_bounds__(256) void k_q1_sor(int M, int n, int cz, int cy, int cx,
+                                                const T *__restrict__ alpha, Q1K kt, T omega, T *x,
+                                                const T *b, T *r) {
+  int nx = (M - cx + 1) / 2, ny = (M - cy + 1) / 2, nz = (M - cz + 1) / 2;
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  long t = (long)lb * blockDim.x + threadIdx.x;
+  if (t >= (long)nx * ny * nz) return;
+  int xx = 2 * (int)(t % nx) + cx, yy = 2 * (int)((t / nx) % ny) + cy,
+      zz = 2 * (int)(t / ((long)nx * ny)) + cz;
+  long i = ((long)zz * M + yy) * M + xx;
+  bool bd = xx == 0 || yy == 0 || zz == 0 || xx == M - 1 || yy == M - 1 || zz == M - 1;
+  T corr = b[i];
+  T d = (T)1;
+  if (bd) corr += (-x[i]) * (T)1;
+  else corr = q1_row<T>(M, n, i, zz, yy, xx, alpha, kt, corr, (T)-1, x, &d);
+  corr = corr / d;
+  r[i] = corr;
+  x[i] += omega * corr;
+}
+
+static Q1K q1k(const flgpu_mat *A) {
+  Q1K k;
+  for (int j = 0; j < 8; j++) k.k[j] = A->kt[j];
+  return k;
+}
+
 void spmv(const flgpu_mat *A, double alpha, const void *x, double beta, const void *z,
           void *out) {
+  if (A->alpha) {
+    int M = A->lat_n + 1;
+    long N = (long)M * M * M;
+    unsigned g = xcd_grid(grid_for(N, 256));
+    if (A->prec == FLGPU_F64)
+      k_q1_spmv<double><<<g, 256, 0, A->ctx->stream>>>(M, A->lat_n, (const double *)A->alpha,
+                                                       q1k(A), alpha, (const double *)x, beta,
+                                                       (const double *)z, (double *)out);
+    else
+      k_q1_spmv<float><<<g, 256, 0, A->ctx->stream>>>(M, A->lat_n, (const float *)A->alpha,
+                                                      q1k(A), (float)alpha, (const float *)x,
+                                                      (float)beta, (const float *)z, (float *)out);
+    FLGPU_HIP(hipGetLastError());
+    return;
+  }
   if (A->prec == FLGPU_F64) launch_spmv<double>(A->ctx, A->sell, alpha, x, beta, z, out);
   else launch_spmv<float>(A->ctx, A->sell, alpha, x, beta, z, out);
 }
@@ -442,6 +562,28 @@ __global__ __launch_bounds__(256) void k_sor_colour(int nsub, int nslices, const
 }
 
 void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const void *b, void *r) {
+  if (A->alpha) {
+    // lattice colours in the order of the lattice colouring: cz*4 + cy*2 + cx
+    int M = A->lat_n + 1;
+    for (int k = 0; k < 8; k++) {
+      int c = backwards ? 7 - k : k;
+      int cz = c >> 2, cy = (c >> 1) & 1, cx = c & 1;
+      long cnt = (long)((M - cx + 1) / 2) * ((M - cy + 1) / 2) * ((M - cz + 1) / 2);
+      unsigned g = xcd_grid(grid_for(cnt, 256));
+      if (A->prec == FLGPU_F64)
+        k_q1_sor<double><<<g, 256, 0, A->ctx->stream>>>(M, A->lat_n, cz, cy, cx,
+                                                        (const double *)A->alpha, q1k(A), omega,
+                                                        (double *)x, (const double *)b,
+                                                        (double *)r);
+      else
+        k_q1_sor<float><<<g, 256, 0, A->ctx->stream>>>(M, A->lat_n, cz, cy, cx,
+                                                       (const float *)A->alpha, q1k(A),
+                                                       (float)omega, (float *)x, (const float *)b,
+                                                       (float *)r);
+      FLGPU_HIP(hipGetLastError());
+    }
+    return;
+  }
   FLGPU_REQUIRE(A->ncolours > 0, "<sor> needs a colouring of the matrix");
   for (int k = 0; k < A->ncolours; k++) {
     const Sell &s = A->colour_sell[backwards ? A->ncolours - 1 - k : k];
@@ -601,6 +743,7 @@ extern "C" int flgpu_csr_update_values(flgpu_mat *A, const double *vals) {
   return guard([&] {
     FLGPU_REQUIRE(A && (vals || A->nnz == 0), "arguments must not be NULL");
     CTX_LOCK(A->ctx);
+    drop_alpha(A);
     upload_vals(A, vals);
     mat_refresh_values(A);
     FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
@@ -639,6 +782,20 @@ extern "C" int flgpu_mat_convert(const flgpu_mat *A, int prec, flgpu_mat **out) 
     CTX_LOCK(A->ctx);
     *out = mat_convert(A, prec);
     FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mat_cell_form(flgpu_mat *A, int keep, int *active) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    CTX_LOCK(A->ctx);
+    if (!keep && A->alpha) {
+      FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+      drop_alpha(A);
+      if (A->lat_dim > 0 && A->ncolours == 0) lattice_colouring(A, A->lat_dim, A->lat_order, A->lat_n);
+    }
+    if (active) *active = A->alpha != nullptr;
     return 0;
   });
 }
@@ -706,6 +863,7 @@ extern "C" int flgpu_mat_set_colouring(flgpu_mat *A, int ncolours, const int *co
   return guard([&] {
     FLGPU_REQUIRE(A && colour, "arguments must not be NULL");
     CTX_LOCK(A->ctx);
+    drop_alpha(A);
     set_colouring(A, ncolours, colour);
     return 0;
   });
@@ -733,6 +891,7 @@ extern "C" int flgpu_mat_colour_greedy(flgpu_mat *A, int *ncolours) {
       colour[r] = k;
       if (k == nc) nc++;
     }
+    drop_alpha(A);
     set_colouring(A, nc, colour.data());
     if (ncolours) *ncolours = nc;
     return 0;

@@ -78,6 +78,7 @@ SIGNATURES = {
     "flgpu_mat_info": (_ci, [_vp, _P(_ci), _P(_ci), _P(_cl), _P(_ci)]),
     "flgpu_mat_convert": (_ci, [_vp, _ci, _P(_vp)]),
     "flgpu_mat_destroy": (_ci, [_vp]),
+    "flgpu_mat_cell_form": (_ci, [_vp, _ci, _P(_ci)]),
     "flgpu_gemm_nn": (_ci, [_cd, _vp, _vp, _cd, _vp]),
     "flgpu_gemm_tn": (_ci, [_cd, _vp, _vp, _cd, _vp]),
     "flgpu_compute_residual": (_ci, [_vp, _vp, _vp, _vp]),

@@ -126,14 +126,20 @@ class Hierarchy:
         return self
 
     def top_matrix(self):
+        """The hierarchy's top-level matrix (owned by the hierarchy, which the
+        returned object keeps alive)."""
         h = ctypes.c_void_p()
         check(lib().flgpu_mg_top_matrix(self.h, ctypes.byref(h)))
-        return GpuSparseMatrix(h, self.ctx, owner=False)
+        m = GpuSparseMatrix(h, self.ctx, owner=False)
+        m._parent = self
+        return m
 
     def level_matrix(self, level, interp=False):
         h = ctypes.c_void_p()
         check(lib().flgpu_mg_level_matrix(self.h, level, int(interp), ctypes.byref(h)))
-        return GpuSparseMatrix(h, self.ctx, owner=False)
+        m = GpuSparseMatrix(h, self.ctx, owner=False)
+        m._parent = self
+        return m
 
     def apply(self, x, b, r):
         check(lib().flgpu_mg_apply(self.h, x.h, b.h, r.h))

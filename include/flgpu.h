@@ -100,6 +100,13 @@ int flgpu_mat_info(const flgpu_mat *A, int *nrows, int *ncols, long *nnz,
                    int *prec);
 /* copy of A stored in another precision */
 int flgpu_mat_convert(const flgpu_mat *A, int prec, flgpu_mat **out);
+/* Per-cell form of a 3D Q1 lattice matrix whose coefficient is constant per
+ * cell: the kernels then apply A from one factor per cell instead of
+ * streaming its values (same operator, values are sums of the same element
+ * contributions).  keep = 0 switches it off (the CSR/SELL storage is used);
+ * *active tells whether it is in use.  New values, or a colouring other than
+ * the lattice one, switch it off as well. */
+int flgpu_mat_cell_form(flgpu_mat *A, int keep, int *active);
 int flgpu_mat_destroy(flgpu_mat *A);
 
 /* gemm-nn! alpha A y beta x : x <- beta x + alpha A y

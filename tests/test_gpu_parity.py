@@ -142,6 +142,43 @@ def test_lattice_spmv_parity(dim, order, n):
     assert relerr(gr.numpy(), b.numpy() - po.spmv(rp, ci, v, y)) < 1e-13
 
 
+@pytest.mark.parametrize("prec", [F.F64, F.F32])
+def test_cell_form_matches_csr(prec):
+    """3D Q1 with a per-cell coefficient: the operator applied from the cell
+    factors equals the assembled CSR (SpMV, compute-residual and a <sor>
+    sweep), and the oracle's SpMV."""
+    say(f"cell form prec={prec}")
+    n = 12
+    rng = np.random.default_rng(17)
+    kap = np.exp(rng.normal(scale=1.5, size=(n, n, n)))
+    A, b = lat.assemble(3, 1, n, kap, prec=prec)
+    B, _ = lat.assemble(3, 1, n, kap, prec=prec)
+    assert A.cell_form() and not B.cell_form(keep=False)
+    tol = 1e-13 if prec == F.F64 else 2e-6
+    y = rng.uniform(-1, 1, A.ncols)
+    ya, yb = GpuVector.from_array(y, prec), GpuVector.from_array(y, prec)
+    xa, xb = GpuVector(A.nrows, prec), GpuVector(A.nrows, prec)
+    gemm(1.0, A, ya, 0.0, xa)
+    gemm(1.0, B, yb, 0.0, xb)
+    assert relerr(xa.numpy(), xb.numpy()) < tol
+    rp, ci, v, _ = po.assemble(3, 1, n, kap)
+    if prec == F.F64:
+        assert relerr(xa.numpy(), po.spmv(rp, ci, v, y)) < 1e-13
+    ra, rb = GpuVector.from_array(b.numpy(), prec), GpuVector.from_array(b.numpy(), prec)
+    compute_residual(A, ya, ra, ra)
+    compute_residual(B, yb, rb, rb)
+    assert relerr(ra.numpy(), rb.numpy()) < tol
+    from gnu_femlisp_amd.flgpu import check, lib
+    for bw in (0, 1):
+        xs = [GpuVector.from_array(y, prec) for _ in range(2)]
+        rs = [GpuVector(A.nrows, prec) for _ in range(2)]
+        for M, xv, rv in ((A, xs[0], rs[0]), (B, xs[1], rs[1])):
+            check(lib().flgpu_sor_iterate(M.h, 1.1, bw, xv.h, b.h if prec == F.F64 else
+                                          GpuVector.from_array(b.numpy(), prec).h, rv.h))
+        assert relerr(xs[0].numpy(), xs[1].numpy()) < tol
+        assert relerr(rs[0].numpy(), rs[1].numpy()) < 10 * tol
+
+
 def test_vector_ops():
     say("vector ops")
     rng = np.random.default_rng(9)
