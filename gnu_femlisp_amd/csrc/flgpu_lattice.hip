@@ -403,8 +403,15 @@ __device__ __forceinline__ short f2bf(double x) {
   return (short)(u >> 16);
 }
 
+__device__ __forceinline__ float bf2f(short h) {
+  return __uint_as_float(((unsigned)(unsigned short)h) << 16);
+}
+
 // BF16 MFMA 16x16x16 (FP32 accumulate): lane l supplies A[l%16][4(l/16)+v]
 // and B[4(l/16)+v][l%16] (v = 0..3); it receives D[4(l/16)+v][l%16].
+// Every operand is split into BF16 parts hi + lo (lo = the BF16 rounding of
+// the remainder) and hi*hi + hi*lo + lo*hi is accumulated: three BF16 MFMAs
+// give about 16 significant bits per product instead of 8.
 __global__ __launch_bounds__(256) void k_elem_mfma_bf16(long ncells, int nq, int nloc2,
                                                         const double *__restrict__ S,
                                                         const double *__restrict__ Tt, float *E) {
@@ -418,13 +425,19 @@ __global__ __launch_bounds__(256) void k_elem_mfma_bf16(long ncells, int nq, int
     v4f acc = {0.f, 0.f, 0.f, 0.f};
     int bcol = n0 + i;
     for (int k0 = 0; k0 < nq; k0 += 16) {
-      v4s a, b;
+      v4s ah, al, bh, bl;
       for (int v = 0; v < 4; v++) {
         int k = k0 + 4 * kg + v;
-        a[v] = f2bf((arow < ncells && k < nq) ? S[arow * nq + k] : 0.0);
-        b[v] = f2bf((k < nq && bcol < nloc2) ? Tt[(long)k * nloc2 + bcol] : 0.0);
+        float av = (float)((arow < ncells && k < nq) ? S[arow * nq + k] : 0.0);
+        float bv = (float)((k < nq && bcol < nloc2) ? Tt[(long)k * nloc2 + bcol] : 0.0);
+        ah[v] = f2bf(av);
+        al[v] = f2bf(av - bf2f(ah[v]));
+        bh[v] = f2bf(bv);
+        bl[v] = f2bf(bv - bf2f(bh[v]));
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
     }
     for (int v = 0; v < 4; v++) {
       long row = c0 + 4 * kg + v;

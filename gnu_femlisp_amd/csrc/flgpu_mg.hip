@@ -52,6 +52,14 @@ struct flgpu_mg {
   flgpu_unet *unet = nullptr;
   const flgpu_vec *unet_kappa = nullptr;
   int unet_n = 0;
+  // captured cycles: one HIP graph per (x, b, r, precision) of the caller
+  struct Graph {
+    void *x, *b, *r;
+    int vprec;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+  int use_graphs = 1;
 };
 
 namespace flgpu {
@@ -247,6 +255,49 @@ void mg_apply_ptr(flgpu_mg *mg, int vprec, void *x, void *b, void *r) {
   vadd_converted(mg->ctx, N, mg->p.prec, mg->xs, vprec, x);
 }
 
+// The cycle's launch sequence depends only on the parameters (the residual-p
+// bookkeeping restarts at every cycle), so it is captured once per vector
+// triple into a HIP graph and replayed: one launch instead of ~40 per level.
+// The cycle clears with fill kernels, never with memset nodes.
+static void graphs_clear(flgpu_mg *mg) {
+  for (auto &g : mg->graphs) (void)hipGraphExecDestroy(g.exec);
+  mg->graphs.clear();
+}
+
+void mg_apply_cached(flgpu_mg *mg, int vprec, void *x, void *b, void *r) {
+  if (!mg->use_graphs) {
+    mg_apply_ptr(mg, vprec, x, b, r);
+    return;
+  }
+  hipStream_t s = mg->ctx->stream;
+  for (auto &g : mg->graphs)
+    if (g.x == x && g.b == b && g.r == r && g.vprec == vprec) {
+      FLGPU_HIP(hipGraphLaunch(g.exec, s));
+      return;
+    }
+  if (mg->graphs.size() >= 8) {  // bounded cache: drop the oldest
+    FLGPU_HIP(hipStreamSynchronize(s));
+    (void)hipGraphExecDestroy(mg->graphs.front().exec);
+    mg->graphs.erase(mg->graphs.begin());
+  }
+  hipGraph_t graph = nullptr;
+  FLGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    mg_apply_ptr(mg, vprec, x, b, r);
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  FLGPU_HIP(hipStreamEndCapture(s, &graph));
+  hipGraphExec_t exec = nullptr;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  FLGPU_HIP(e);
+  mg->graphs.push_back({x, b, r, vprec, exec});
+  FLGPU_HIP(hipGraphLaunch(exec, s));
+}
+
 static void check_params(const flgpu_mg_params *p, int nlevels) {
   FLGPU_REQUIRE(p != nullptr, "params must not be NULL");
   FLGPU_REQUIRE(p->smoother == FLGPU_SMOOTH_JACOBI || p->smoother == FLGPU_SMOOTH_SOR,
@@ -283,6 +334,7 @@ void mg_unet(flgpu_mg *mg, flgpu_unet **u, const flgpu_vec **kappa, int *n) {
 
 void mg_free(flgpu_mg *mg) {
   if (!mg) return;
+  graphs_clear(mg);
   for (auto &L : mg->lv) {
     mat_free(L.A);
     mat_free(L.I);
@@ -452,6 +504,7 @@ extern "C" int flgpu_mg_reassemble(flgpu_mg *mg, const flgpu_vec *kappa, int nk)
         mat_refresh_values(L.A);
       }
     }
+    graphs_clear(mg);  // the coarse inverse is reallocated below
     MGLevel &B = mg->lv[mg->p.base_level];
     dev_free(B.cinv);
     dev_free(B.ctmp);
@@ -514,7 +567,18 @@ extern "C" int flgpu_mg_apply(flgpu_mg *mg, flgpu_vec *x, flgpu_vec *b, flgpu_ve
     FLGPU_REQUIRE(x->prec == b->prec && b->prec == r->prec, "vectors must share one precision");
     FLGPU_REQUIRE(x->d != r->d && x->d != b->d, "x must differ from b and r");
     CTX_LOCK(mg->ctx);
-    mg_apply_ptr(mg, x->prec, x->d, b->d, r->d);
+    mg_apply_cached(mg, x->prec, x->d, b->d, r->d);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mg_use_graphs(flgpu_mg *mg, int enable) {
+  return guard([&] {
+    FLGPU_REQUIRE(mg, "hierarchy must not be NULL");
+    CTX_LOCK(mg->ctx);
+    FLGPU_HIP(hipStreamSynchronize(mg->ctx->stream));
+    graphs_clear(mg);
+    mg->use_graphs = enable ? 1 : 0;
     return 0;
   });
 }
@@ -552,10 +616,10 @@ static void precondition(flgpu_mg *mg, const flgpu_solve_params *p, SolveWork &W
   vcopy(W.ctx, W.N, W.prec, r, W.prec, W.w);
   vfill(W.ctx, W.N, W.prec, q, 0.0);
   if (p->alias) {
-    mg_apply_ptr(mg, W.prec, q, W.w, W.w);
+    mg_apply_cached(mg, W.prec, q, W.w, W.w);
   } else {
     vcopy(W.ctx, W.N, W.prec, r, W.prec, W.w2);
-    mg_apply_ptr(mg, W.prec, q, W.w, W.w2);
+    mg_apply_cached(mg, W.prec, q, W.w, W.w2);
   }
   if (learned) learned->correct(r, q);
 }
@@ -646,7 +710,7 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
       // next-step
       switch (p->outer) {
         case FLGPU_OUTER_MG:
-          mg_apply_ptr(mg, prec, x->d, b->d, W.r);
+          mg_apply_cached(mg, prec, x->d, b->d, W.r);
           residual_p = false;
           break;
         case FLGPU_OUTER_CG:

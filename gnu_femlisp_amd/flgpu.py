@@ -99,6 +99,7 @@ SIGNATURES = {
     "flgpu_mg_top_matrix": (_ci, [_vp, _P(_vp)]),
     "flgpu_mg_level_matrix": (_ci, [_vp, _ci, _ci, _P(_vp)]),
     "flgpu_mg_apply": (_ci, [_vp, _vp, _vp, _vp]),
+    "flgpu_mg_use_graphs": (_ci, [_vp, _ci]),
     "flgpu_solve": (_ci, [_vp, _vp, _P(SolveParams), _vp, _vp, _P(SolveReport), _vp, _ci]),
     "flgpu_mg_reassemble": (_ci, [_vp, _vp, _ci]),
     "flgpu_unet_create": (_ci, [_vp, _ci, ctypes.c_ulong, _P(_vp)]),

@@ -145,6 +145,9 @@ class Hierarchy:
         check(lib().flgpu_mg_apply(self.h, x.h, b.h, r.h))
         return x
 
+    def use_graphs(self, enable=True):
+        check(lib().flgpu_mg_use_graphs(self.h, int(enable)))
+
     def __del__(self):
         try:
             if getattr(self, "h", None):

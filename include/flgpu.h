@@ -234,6 +234,10 @@ int flgpu_mg_level_matrix(flgpu_mg *mg, int level, int interp, flgpu_mat **A);
  * (iterate q w w), krylow.lisp:118-121).  Vectors may be FP64 with an FP32
  * hierarchy: then x <- x + V(r) with the cycle in FP32 (b must not be r). */
 int flgpu_mg_apply(flgpu_mg *mg, flgpu_vec *x, flgpu_vec *b, flgpu_vec *r);
+/* Replay cycles from HIP graphs (default 1): the first cycle on a vector
+ * triple is captured and later cycles on it are one graph launch.  0 launches
+ * every kernel separately. */
+int flgpu_mg_use_graphs(flgpu_mg *mg, int enable);
 
 /* Re-assemble every level of a lattice hierarchy for a new coefficient
  * (same pattern, transfers and colouring; coarse inverse recomputed). */
