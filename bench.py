@@ -110,27 +110,29 @@ def main():
         both storages and precisions"""
         out = {}
         for prec, vb, mode in ((F.F64, 8, F.ASM_F64), (F.F32, 4, F.ASM_MFMA)):
-            A, b = lat.assemble(3, 1, n, (kv, n), prec=prec, mode=mode)
-            C, _ = lat.assemble(3, 1, n, (kv, n), prec=prec, mode=mode)
-            C.cell_form(keep=False)
-            N, nnz = A.nrows, A.nnz
             tag = "f64" if prec == F.F64 else "f32"
-            y = GpuVector.from_array(np.random.default_rng(0).uniform(-1, 1, N), prec)
-            x, r = GpuVector(N, prec), GpuVector(N, prec)
-            for name, M, fn, byt in (
-                    (f"cell_spmv_{tag}", A, lambda M: gemm(1.0, M, y, 0.0, x),
-                     cell_spmv_bytes(n ** 3, N, vb)),
-                    (f"cell_sor_{tag}", A, lambda M: sor(M, x, b, r), cell_sor_bytes(n ** 3, N, vb)),
-                    (f"sell_spmv_{tag}", C, lambda M: gemm(1.0, M, y, 0.0, x),
-                     spmv_bytes(nnz, N, vb)),
-                    (f"sell_sor_{tag}", C, lambda M: sor(M, x, b, r), sor_sweep_bytes(nnz, N, vb))):
-                fn(M)
-                ctx.timer_start()
-                for _ in range(reps):
-                    fn(M)
-                ms = ctx.timer_stop() / reps
-                out[name] = {"ms": ms, "bytes": byt, "gbs": byt / (ms * 1e-3) / 1e9}
-            del A, C
+            for form, fname in ((F.STORAGE_CELL, "cell"), (F.STORAGE_STENCIL, "stencil"),
+                                (F.STORAGE_SELL, "sell")):
+                M, b = lat.assemble(3, 1, n, (kv, n), prec=prec, mode=mode)
+                assert M.storage(form) == form
+                N, nnz = M.nrows, M.nnz
+                y = GpuVector.from_array(np.random.default_rng(0).uniform(-1, 1, N), prec)
+                x, r = GpuVector(N, prec), GpuVector(N, prec)
+                if form == F.STORAGE_CELL:
+                    sb, wb = cell_spmv_bytes(n ** 3, N, vb), cell_sor_bytes(n ** 3, N, vb)
+                elif form == F.STORAGE_STENCIL:
+                    sb, wb = 27 * N * vb + 2 * N * vb, 27 * N * vb + 5 * N * vb
+                else:
+                    sb, wb = spmv_bytes(nnz, N, vb), sor_sweep_bytes(nnz, N, vb)
+                for name, fn, byt in ((f"{fname}_spmv_{tag}", lambda: gemm(1.0, M, y, 0.0, x), sb),
+                                      (f"{fname}_sor_{tag}", lambda: sor(M, x, b, r), wb)):
+                    fn()
+                    ctx.timer_start()
+                    for _ in range(reps):
+                        fn()
+                    ms = ctx.timer_stop() / reps
+                    out[name] = {"ms": ms, "bytes": byt, "gbs": byt / (ms * 1e-3) / 1e9}
+                del M
         return out
 
     if args.kernels_only:

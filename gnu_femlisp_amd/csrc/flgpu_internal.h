@@ -146,6 +146,9 @@ struct flgpu_mat {
   // by DOFs i and j), d = lattice offset; kernels apply A from alpha
   void *alpha = nullptr;  // n^3 cell factors (prec), nullptr if not available
   double kt[8] = {};      // Kref[0][b], b = bz*4 + by*2 + bx
+  // 3D Q1 lattice matrices otherwise: the 27 values of every row, SoA
+  // val27[k][row], k = (dz+1)*9 + (dy+1)*3 + (dx+1); columns are implied
+  void *val27 = nullptr;
 };
 
 namespace flgpu {
@@ -157,6 +160,9 @@ void mat_finish(flgpu_mat *A);  // SELL + diagonal from the device CSR
 void mat_refresh_values(flgpu_mat *A);  // SELL/diag/colour values after a CSR value change
 void mat_build_transpose(const flgpu_mat *A);
 void drop_alpha(flgpu_mat *A);
+bool is_q1_lattice(const flgpu_mat *A);
+void stencil_refresh(flgpu_mat *A);
+void drop_stencil(flgpu_mat *A);
 flgpu_mat *mat_convert(const flgpu_mat *A, int prec);
 void sell_free(Sell &s);
 Sell sell_build_rows(flgpu_mat *A, const std::vector<int> &rows);

@@ -473,7 +473,8 @@ extern "C" int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n
         else
           mg->lv[l].A = lattice_galerkin(mg->lv[l].I, mg->lv[l + 1].A, dim, order, n);
         // matrices in the per-cell form sweep the lattice colours directly
-        if (params->smoother == FLGPU_SMOOTH_SOR && l > params->base_level && !mg->lv[l].A->alpha)
+        if (params->smoother == FLGPU_SMOOTH_SOR && l > params->base_level &&
+            !mg->lv[l].A->alpha && !mg->lv[l].A->val27)
           lattice_colouring(mg->lv[l].A, dim, order, n);
       }
       mg_alloc_work(mg);

@@ -178,6 +178,10 @@ void mat_finish(flgpu_mat *A) {
     diag_refresh(A);
   }
   A->sell = sell_build(A, nullptr);
+  if (is_q1_lattice(A)) {
+    A->val27 = dev_alloc(prec_bytes(A->prec) * 27 * (size_t)A->nrows);
+    stencil_refresh(A);
+  }
 }
 
 void mat_free(flgpu_mat *A) {
@@ -191,6 +195,7 @@ void mat_free(flgpu_mat *A) {
   for (auto &s : A->colour_sell) sell_free(s);
   if (A->transpose) mat_free(A->transpose);
   dev_free(A->alpha);
+  dev_free(A->val27);
   delete A;
 }
 
@@ -202,6 +207,7 @@ void drop_alpha(flgpu_mat *A) {
 
 void mat_refresh_values(flgpu_mat *A) {
   diag_refresh(A);
+  stencil_refresh(A);
   if (A->prec == FLGPU_F64) {
     sell_fill<double>(A->ctx, A, A->sell, 0);
     for (auto &s : A->colour_sell) sell_fill<double>(A->ctx, A, s, 0);
@@ -476,6 +482,105 @@ __global__ __launch_bounds__(256) void k_q1_sor(int M, int n, int cz, int cy, in
   x[i] += omega * corr;
 }
 
+// ---- 3D Q1 lattice matrices in 27-point stencil storage ------------------------------
+
+// val27[k][i] = A(i, i + off_k) for the CSR entries of row i, 0 elsewhere
+template <class T>
+__global__ void k_st27_fill(int M, const int *__restrict__ rowptr, const int *__restrict__ colidx,
+                            const T *__restrict__ vals, T *val27) {
+  long N = (long)M * M * M;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int k = 0; k < 27; k++) val27[k * N + i] = (T)0;
+  int x = (int)(i % M), y = (int)((i / M) % M), z = (int)(i / ((long)M * M));
+  for (int e = rowptr[i]; e < rowptr[i + 1]; e++) {
+    long j = colidx[e];
+    int jx = (int)(j % M), jy = (int)((j / M) % M), jz = (int)(j / ((long)M * M));
+    int k = (jz - z + 1) * 9 + (jy - y + 1) * 3 + (jx - x + 1);
+    val27[k * N + i] = vals[e];
+  }
+}
+
+// acc + sum over the lattice neighbours j of row i (ascending j) of
+// (coef x_j) A_ij; the same entries in the same order as the CSR row
+template <class T>
+__device__ __forceinline__ T st27_row(int M, long N, long i, int z, int y, int x,
+                                      const T *__restrict__ val27, T acc, T coef,
+                                      const T *__restrict__ xv, T *dii) {
+  long M2 = (long)M * M;
+  bool inner = z > 0 && y > 0 && x > 0 && z < M - 1 && y < M - 1 && x < M - 1;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        if (!inner) {
+          int zz = z + dz, yy = y + dy, xx = x + dx;
+          if (zz < 0 || zz >= M || yy < 0 || yy >= M || xx < 0 || xx >= M) continue;
+        }
+        int k = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
+        T a = val27[k * N + i];
+        if (k == 13) *dii = a;
+        acc += (coef * xv[i + dz * M2 + dy * M + dx]) * a;
+      }
+  return acc;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_st27_spmv(int M, const T *__restrict__ val27, T coef,
+                                                   const T *__restrict__ x, T beta, const T *z,
+                                                   T *out) {
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  long i = (long)lb * blockDim.x + threadIdx.x;
+  long N = (long)M * M * M;
+  if (i >= N) return;
+  int xx = (int)(i % M), yy = (int)((i / M) % M), zz = (int)(i / ((long)M * M));
+  T acc = (beta == (T)0) ? (T)0 : beta * z[i];
+  T d;
+  out[i] = st27_row<T>(M, N, i, zz, yy, xx, val27, acc, coef, x, &d);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_st27_sor(int M, int cz, int cy, int cx,
+                                                  const T *__restrict__ val27, T omega, T *x,
+                                                  const T *b, T *r) {
+  int nx = (M - cx + 1) / 2, ny = (M - cy + 1) / 2, nz = (M - cz + 1) / 2;
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  long t = (long)lb * blockDim.x + threadIdx.x;
+  if (t >= (long)nx * ny * nz) return;
+  int xx = 2 * (int)(t % nx) + cx, yy = 2 * (int)((t / nx) % ny) + cy,
+      zz = 2 * (int)(t / ((long)nx * ny)) + cz;
+  long N = (long)M * M * M;
+  long i = ((long)zz * M + yy) * M + xx;
+  T d = (T)1;
+  T corr = st27_row<T>(M, N, i, zz, yy, xx, val27, b[i], (T)-1, x, &d);
+  corr = corr / d;
+  r[i] = corr;
+  x[i] += omega * corr;
+}
+
+bool is_q1_lattice(const flgpu_mat *A) {
+  return A->lat_dim == 3 && A->lat_order == 1 &&
+         (long)A->nrows == (long)(A->lat_n + 1) * (A->lat_n + 1) * (A->lat_n + 1) &&
+         A->nrows == A->ncols;
+}
+
+void stencil_refresh(flgpu_mat *A) {
+  if (!A->val27) return;
+  int M = A->lat_n + 1;
+  long N = A->nrows;
+  if (A->prec == FLGPU_F64)
+    k_st27_fill<double><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
+        M, A->rowptr, A->colidx, (const double *)A->vals, (double *)A->val27);
+  else
+    k_st27_fill<float><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
+        M, A->rowptr, A->colidx, (const float *)A->vals, (float *)A->val27);
+  FLGPU_HIP(hipGetLastError());
+}
+
+void drop_stencil(flgpu_mat *A) {
+  dev_free(A->val27);
+  A->val27 = nullptr;
+}
+
 static Q1K q1k(const flgpu_mat *A) {
   Q1K k;
   for (int j = 0; j < 8; j++) k.k[j] = A->kt[j];
@@ -496,6 +601,20 @@ void spmv(const flgpu_mat *A, double alpha, const void *x, double beta, const vo
       k_q1_spmv<float><<<g, 256, 0, A->ctx->stream>>>(M, A->lat_n, (const float *)A->alpha,
                                                       q1k(A), (float)alpha, (const float *)x,
                                                       (float)beta, (const float *)z, (float *)out);
+    FLGPU_HIP(hipGetLastError());
+    return;
+  }
+  if (A->val27) {
+    int M = A->lat_n + 1;
+    unsigned g = xcd_grid(grid_for(A->nrows, 256));
+    if (A->prec == FLGPU_F64)
+      k_st27_spmv<double><<<g, 256, 0, A->ctx->stream>>>(M, (const double *)A->val27, alpha,
+                                                         (const double *)x, beta,
+                                                         (const double *)z, (double *)out);
+    else
+      k_st27_spmv<float><<<g, 256, 0, A->ctx->stream>>>(M, (const float *)A->val27, (float)alpha,
+                                                        (const float *)x, (float)beta,
+                                                        (const float *)z, (float *)out);
     FLGPU_HIP(hipGetLastError());
     return;
   }
@@ -562,7 +681,7 @@ __global__ __launch_bounds__(256) void k_sor_colour(int nsub, int nslices, const
 }
 
 void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const void *b, void *r) {
-  if (A->alpha) {
+  if (A->alpha || A->val27) {
     // lattice colours in the order of the lattice colouring: cz*4 + cy*2 + cx
     int M = A->lat_n + 1;
     for (int k = 0; k < 8; k++) {
@@ -570,7 +689,17 @@ void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const v
       int cz = c >> 2, cy = (c >> 1) & 1, cx = c & 1;
       long cnt = (long)((M - cx + 1) / 2) * ((M - cy + 1) / 2) * ((M - cz + 1) / 2);
       unsigned g = xcd_grid(grid_for(cnt, 256));
-      if (A->prec == FLGPU_F64)
+      if (!A->alpha) {
+        if (A->prec == FLGPU_F64)
+          k_st27_sor<double><<<g, 256, 0, A->ctx->stream>>>(M, cz, cy, cx,
+                                                            (const double *)A->val27, omega,
+                                                            (double *)x, (const double *)b,
+                                                            (double *)r);
+        else
+          k_st27_sor<float><<<g, 256, 0, A->ctx->stream>>>(M, cz, cy, cx, (const float *)A->val27,
+                                                           (float)omega, (float *)x,
+                                                           (const float *)b, (float *)r);
+      } else if (A->prec == FLGPU_F64)
         k_q1_sor<double><<<g, 256, 0, A->ctx->stream>>>(M, A->lat_n, cz, cy, cx,
                                                         (const double *)A->alpha, q1k(A), omega,
                                                         (double *)x, (const double *)b,
@@ -786,16 +915,19 @@ extern "C" int flgpu_mat_convert(const flgpu_mat *A, int prec, flgpu_mat **out) 
   });
 }
 
-extern "C" int flgpu_mat_cell_form(flgpu_mat *A, int keep, int *active) {
+extern "C" int flgpu_mat_storage(flgpu_mat *A, int max_form, int *form) {
   return guard([&] {
     FLGPU_REQUIRE(A, "matrix must not be NULL");
     CTX_LOCK(A->ctx);
-    if (!keep && A->alpha) {
-      FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
-      drop_alpha(A);
-      if (A->lat_dim > 0 && A->ncolours == 0) lattice_colouring(A, A->lat_dim, A->lat_order, A->lat_n);
-    }
-    if (active) *active = A->alpha != nullptr;
+    FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+    if (max_form < FLGPU_STORAGE_CELL) drop_alpha(A);
+    if (max_form < FLGPU_STORAGE_STENCIL) drop_stencil(A);
+    long lat_rows = 1;
+    for (int k = 0; k < A->lat_dim; k++) lat_rows *= (long)A->lat_order * A->lat_n + 1;
+    if (!A->alpha && !A->val27 && A->lat_dim > 0 && A->ncolours == 0 && A->nrows == lat_rows)
+      lattice_colouring(A, A->lat_dim, A->lat_order, A->lat_n);
+    if (form)
+      *form = A->alpha ? FLGPU_STORAGE_CELL : A->val27 ? FLGPU_STORAGE_STENCIL : FLGPU_STORAGE_SELL;
     return 0;
   });
 }
@@ -863,7 +995,9 @@ extern "C" int flgpu_mat_set_colouring(flgpu_mat *A, int ncolours, const int *co
   return guard([&] {
     FLGPU_REQUIRE(A && colour, "arguments must not be NULL");
     CTX_LOCK(A->ctx);
-    drop_alpha(A);
+    FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+    drop_alpha(A);  // sweeps follow the caller's colours from now on
+    drop_stencil(A);
     set_colouring(A, ncolours, colour);
     return 0;
   });
@@ -891,7 +1025,9 @@ extern "C" int flgpu_mat_colour_greedy(flgpu_mat *A, int *ncolours) {
       colour[r] = k;
       if (k == nc) nc++;
     }
+    FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
     drop_alpha(A);
+    drop_stencil(A);
     set_colouring(A, nc, colour.data());
     if (ncolours) *ncolours = nc;
     return 0;

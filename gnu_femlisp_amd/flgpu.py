@@ -18,6 +18,7 @@ ASM_F64, ASM_MFMA = 0, 1
 SMOOTH_JACOBI, SMOOTH_SOR = 0, 1
 OUTER_MG, OUTER_CG, OUTER_CG_PLAIN, OUTER_FCG = 0, 1, 2, 3
 STOP_REDUCTION, STOP_DEFNORM = 0, 1
+STORAGE_SELL, STORAGE_STENCIL, STORAGE_CELL = 0, 1, 2
 
 
 class FlgpuError(RuntimeError):
@@ -78,7 +79,7 @@ SIGNATURES = {
     "flgpu_mat_info": (_ci, [_vp, _P(_ci), _P(_ci), _P(_cl), _P(_ci)]),
     "flgpu_mat_convert": (_ci, [_vp, _ci, _P(_vp)]),
     "flgpu_mat_destroy": (_ci, [_vp]),
-    "flgpu_mat_cell_form": (_ci, [_vp, _ci, _P(_ci)]),
+    "flgpu_mat_storage": (_ci, [_vp, _ci, _P(_ci)]),
     "flgpu_gemm_nn": (_ci, [_cd, _vp, _vp, _cd, _vp]),
     "flgpu_gemm_tn": (_ci, [_cd, _vp, _vp, _cd, _vp]),
     "flgpu_compute_residual": (_ci, [_vp, _vp, _vp, _vp]),

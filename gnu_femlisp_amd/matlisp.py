@@ -164,12 +164,12 @@ class GpuSparseMatrix:
         colour = np.ascontiguousarray(colour, dtype=np.int32)
         check(lib().flgpu_mat_set_colouring(self.h, int(colour.max()) + 1, ptr(colour)))
 
-    def cell_form(self, keep=True):
-        """Whether the per-cell form (3D Q1, coefficient constant per cell) is in
-        use; keep=False switches it off for good."""
-        act = ctypes.c_int()
-        check(lib().flgpu_mat_cell_form(self.h, int(keep), ctypes.byref(act)))
-        return bool(act.value)
+    def storage(self, max_form=_f.STORAGE_CELL):
+        """Storage the kernels use (STORAGE_CELL / STENCIL / SELL), capped at
+        max_form (richer forms are dropped for good)."""
+        form = ctypes.c_int()
+        check(lib().flgpu_mat_storage(self.h, int(max_form), ctypes.byref(form)))
+        return form.value
 
     def colour_greedy(self):
         n = ctypes.c_int()

@@ -100,13 +100,22 @@ int flgpu_mat_info(const flgpu_mat *A, int *nrows, int *ncols, long *nnz,
                    int *prec);
 /* copy of A stored in another precision */
 int flgpu_mat_convert(const flgpu_mat *A, int prec, flgpu_mat **out);
-/* Per-cell form of a 3D Q1 lattice matrix whose coefficient is constant per
- * cell: the kernels then apply A from one factor per cell instead of
- * streaming its values (same operator, values are sums of the same element
- * contributions).  keep = 0 switches it off (the CSR/SELL storage is used);
- * *active tells whether it is in use.  New values, or a colouring other than
- * the lattice one, switch it off as well. */
-int flgpu_mat_cell_form(flgpu_mat *A, int keep, int *active);
+/* Storage the kernels stream (the CSR is always kept):
+ *   FLGPU_STORAGE_CELL     3D Q1 lattice matrix with a coefficient constant
+ *                          per cell: A applied from one factor per cell
+ *                          (A_ij = Kref-entry x sum of the factors of the cells
+ *                          shared by i and j), no matrix values streamed;
+ *   FLGPU_STORAGE_STENCIL  3D Q1 lattice matrix: the 27 values of each row
+ *                          (SoA), columns implied by the lattice;
+ *   FLGPU_STORAGE_SELL     any matrix: SELL-64 values and column indices.
+ * Lattice matrices get the richest form that applies.  max_form caps it
+ * (forms are only ever dropped); *form receives the form in use.  New
+ * values drop the per-cell form; a caller's colouring drops both lattice
+ * forms (their sweeps follow the lattice colouring). */
+#define FLGPU_STORAGE_SELL 0
+#define FLGPU_STORAGE_STENCIL 1
+#define FLGPU_STORAGE_CELL 2
+int flgpu_mat_storage(flgpu_mat *A, int max_form, int *form);
 int flgpu_mat_destroy(flgpu_mat *A);
 
 /* gemm-nn! alpha A y beta x : x <- beta x + alpha A y

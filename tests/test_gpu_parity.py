@@ -143,19 +143,23 @@ def test_lattice_spmv_parity(dim, order, n):
     assert relerr(gr.numpy(), b.numpy() - po.spmv(rp, ci, v, y)) < 1e-13
 
 
+@pytest.mark.parametrize("form", [F.STORAGE_CELL, F.STORAGE_STENCIL])
 @pytest.mark.parametrize("prec", [F.F64, F.F32])
-def test_cell_form_matches_csr(prec):
-    """3D Q1 with a per-cell coefficient: the operator applied from the cell
-    factors equals the assembled CSR (SpMV, compute-residual and a <sor>
-    sweep), and the oracle's SpMV."""
-    say(f"cell form prec={prec}")
+def test_lattice_storage_matches_csr(prec, form):
+    """3D Q1 lattice matrices: the operator applied from the per-cell factors
+    or from the 27-point stencil storage equals the SELL-streamed CSR (SpMV,
+    compute-residual and <sor> sweeps both ways), and the oracle's SpMV.  The
+    stencil storage sums the same entries in the same order as the CSR row."""
+    say(f"storage form={form} prec={prec}")
     n = 12
     rng = np.random.default_rng(17)
     kap = np.exp(rng.normal(scale=1.5, size=(n, n, n)))
     A, b = lat.assemble(3, 1, n, kap, prec=prec)
     B, _ = lat.assemble(3, 1, n, kap, prec=prec)
-    assert A.cell_form() and not B.cell_form(keep=False)
+    assert A.storage(form) == form and B.storage(F.STORAGE_SELL) == F.STORAGE_SELL
     tol = 1e-13 if prec == F.F64 else 2e-6
+    if form == F.STORAGE_STENCIL:
+        tol = 1e-14 if prec == F.F64 else 1e-6
     y = rng.uniform(-1, 1, A.ncols)
     ya, yb = GpuVector.from_array(y, prec), GpuVector.from_array(y, prec)
     xa, xb = GpuVector(A.nrows, prec), GpuVector(A.nrows, prec)
