@@ -496,8 +496,10 @@ __global__ void k_st27_fill(int M, const int *__restrict__ rowptr, const int *__
   for (int e = rowptr[i]; e < rowptr[i + 1]; e++) {
     long j = colidx[e];
     int jx = (int)(j % M), jy = (int)((j / M) % M), jz = (int)(j / ((long)M * M));
-    int k = (jz - z + 1) * 9 + (jy - y + 1) * 3 + (jx - x + 1);
-    val27[k * N + i] = vals[e];
+    int ax = jx - x + 1, ay = jy - y + 1, az = jz - z + 1;
+    // lattice patterns only hold neighbours; anything else is never stored
+    if ((unsigned)ax > 2u || (unsigned)ay > 2u || (unsigned)az > 2u) continue;
+    val27[(long)(az * 9 + ay * 3 + ax) * N + i] = vals[e];
   }
 }
 
