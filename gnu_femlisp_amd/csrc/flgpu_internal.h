@@ -187,6 +187,8 @@ flgpu_mat *lattice_pattern(flgpu_ctx *ctx, int dim, int order, int n, int prec);
 flgpu_mat *lattice_galerkin(const flgpu_mat *P, const flgpu_mat *Afine, int dim, int order,
                             int nc);
 void lattice_dirichlet_identity(flgpu_mat *A, int dim, int order, int n);
+// operator-dependent interpolation values of P (3D Q1) from the fine matrix A
+void lattice_blackbox(const flgpu_mat *A, flgpu_mat *P);
 void lattice_reassemble(flgpu_mat *A, const flgpu_vec *kappa, int nk, int mode);
 
 // vectors (flgpu_core.hip)

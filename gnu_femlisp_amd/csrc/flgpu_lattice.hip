@@ -773,6 +773,120 @@ flgpu_mat *lattice_galerkin(const flgpu_mat *P, const flgpu_mat *Afine, int dim,
   return Ac;
 }
 
+// ---- operator-dependent ("black-box") interpolation, 3D Q1 ------------------------------
+// Not in femlisp: J. E. Dendy's black-box prolongation (J. Comput. Phys. 48 (1982)
+// 366-386) for 27-point stencils.  W[b][i] is the weight of fine node i on corner b
+// (bits z,y,x) of the coarse cell whose lowest corner is (z>>1, y>>1, x>>1).  A node with
+// s odd coordinates solves its row, collapsed onto its odd axes, for itself:
+//   e_i = sum_k (-a_k / a0) e_(i + k masked to the odd axes),
+//   a0 = sum of the a_k whose offset vanishes on the odd axes,
+// with the weights of the neighbours, which have fewer odd coordinates and come from an
+// earlier pass.  Boundary nodes keep W = 0: fine Dirichlet rows and coarse Dirichlet
+// columns of the interpolation are zero, as in femlisp's constrained interpolation
+// (constraints.lisp:58-72).  An interior row of a lattice matrix holds its 27 neighbours
+// in ascending order, so entry k of the row is offset k.
+template <class T>
+__global__ __launch_bounds__(256) void k_bbox_pass(int M, int pass, const int *__restrict__ rowptr,
+                                                   const T *__restrict__ vals, double *W) {
+  long N = (long)M * M * M;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int x = (int)(i % M), y = (int)((i / M) % M), z = (int)(i / ((long)M * M));
+  int oz = z & 1, oy = y & 1, ox = x & 1;
+  if (oz + oy + ox != pass) return;
+  if (z == 0 || y == 0 || x == 0 || z == M - 1 || y == M - 1 || x == M - 1) return;
+  if (pass == 0) {
+    W[i] = 1.0;
+    return;
+  }
+  const T *a = vals + rowptr[i];
+  double a0 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 27; k++) {
+    int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+    if ((oz ? dz : 0) == 0 && (oy ? dy : 0) == 0 && (ox ? dx : 0) == 0) a0 += (double)a[k];
+  }
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (a0 != 0.0) {
+    for (int k = 0; k < 27; k++) {
+      int kz = oz ? k / 9 - 1 : 0, ky = oy ? (k / 3) % 3 - 1 : 0, kx = ox ? k % 3 - 1 : 0;
+      if (kz == 0 && ky == 0 && kx == 0) continue;
+      double s = -(double)a[k] / a0;
+      int nz = z + kz, ny = y + ky, nx = x + kx;
+      long j = ((long)nz * M + ny) * M + nx;
+      int sz = (nz >> 1) - (z >> 1), sy = (ny >> 1) - (y >> 1), sx = (nx >> 1) - (x >> 1);
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        int tz = sz + (b >> 2), ty = sy + ((b >> 1) & 1), tx = sx + (b & 1);
+        if (tz > 1 || ty > 1 || tx > 1) continue;  // an even axis of j: no weight
+        double w = W[(long)b * N + j];
+        int tb = (tz << 2) | (ty << 1) | tx;
+#pragma unroll
+        for (int t = 0; t < 8; t++) acc[t] += (t == tb) ? s * w : 0.0;
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 8; t++) W[(long)t * N + i] = acc[t];
+}
+
+// P's values from W: entry (i, c) takes the weight of i on coarse node c, 0 for coarse
+// boundary nodes and for nodes outside the coarse cell around i
+template <class T>
+__global__ void k_bbox_write(int M, int Mc, const int *__restrict__ prowptr,
+                             const int *__restrict__ pcolidx, const double *__restrict__ W,
+                             T *pvals) {
+  long N = (long)M * M * M;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int x = (int)(i % M), y = (int)((i / M) % M), z = (int)(i / ((long)M * M));
+  for (int e = prowptr[i]; e < prowptr[i + 1]; e++) {
+    int c = pcolidx[e];
+    int cx = c % Mc, cy = (c / Mc) % Mc, cz = c / (Mc * Mc);
+    int bz = cz - (z >> 1), by = cy - (y >> 1), bx = cx - (x >> 1);
+    bool in = bz >= 0 && bz <= 1 && by >= 0 && by <= 1 && bx >= 0 && bx <= 1;
+    bool cb = cz == 0 || cy == 0 || cx == 0 || cz == Mc - 1 || cy == Mc - 1 || cx == Mc - 1;
+    pvals[e] = (in && !cb) ? (T)W[(long)(bz * 4 + by * 2 + bx) * N + i] : (T)0;
+  }
+}
+
+void lattice_blackbox(const flgpu_mat *A, flgpu_mat *P) {
+  FLGPU_REQUIRE(is_q1_lattice(A) && A->lat_n % 2 == 0,
+                "black-box interpolation needs a 3D Q1 lattice matrix with an even n");
+  int n = A->lat_n, M = n + 1, Mc = n / 2 + 1;
+  long N = A->nrows;
+  FLGPU_REQUIRE(A->nnz == ipow(3L * n + 1, 3), "the matrix does not hold the lattice pattern");
+  FLGPU_REQUIRE(P->nrows == A->nrows && P->ncols == (long)Mc * Mc * Mc,
+                "the interpolation does not map the coarse lattice to the matrix's lattice");
+  flgpu_ctx *ctx = A->ctx;
+  double *W = (double *)dev_alloc(sizeof(double) * 8 * (size_t)N);
+  try {
+    FLGPU_HIP(hipMemsetAsync(W, 0, sizeof(double) * 8 * (size_t)N, ctx->stream));
+    for (int pass = 0; pass <= 3; pass++) {
+      if (A->prec == FLGPU_F64)
+        k_bbox_pass<double><<<grid_for(N, 256), 256, 0, ctx->stream>>>(
+            M, pass, A->rowptr, (const double *)A->vals, W);
+      else
+        k_bbox_pass<float><<<grid_for(N, 256), 256, 0, ctx->stream>>>(
+            M, pass, A->rowptr, (const float *)A->vals, W);
+      FLGPU_HIP(hipGetLastError());
+    }
+    if (P->prec == FLGPU_F64)
+      k_bbox_write<double><<<grid_for(N, 256), 256, 0, ctx->stream>>>(
+          M, Mc, P->rowptr, P->colidx, W, (double *)P->vals);
+    else
+      k_bbox_write<float><<<grid_for(N, 256), 256, 0, ctx->stream>>>(
+          M, Mc, P->rowptr, P->colidx, W, (float *)P->vals);
+    FLGPU_HIP(hipGetLastError());
+    FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    dev_free(W);
+    throw;
+  }
+  dev_free(W);
+  mat_refresh_values(P);
+}
+
 // lattice colouring: colour = per-axis (coordinate mod (order+1)) digits
 void lattice_colouring(flgpu_mat *A, int dim, int order, int n) {
   int M = order * n + 1, m = order + 1;
@@ -878,6 +992,16 @@ extern "C" int flgpu_lattice_interpolation(flgpu_ctx *ctx, int dim, int order, i
     FLGPU_REQUIRE(ctx && P, "ctx and P must not be NULL");
     CTX_LOCK(ctx);
     *P = lattice_interpolation(ctx, dim, order, nc, prec);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_lattice_blackbox_interpolation(const flgpu_mat *A, flgpu_mat *P) {
+  return guard([&] {
+    FLGPU_REQUIRE(A && P, "arguments must not be NULL");
+    FLGPU_REQUIRE(A->ctx == P->ctx, "A and P belong to different contexts");
+    CTX_LOCK(A->ctx);
+    lattice_blackbox(A, P);
     return 0;
   });
 }

@@ -305,6 +305,8 @@ static void check_params(const flgpu_mg_params *p, int nlevels) {
   FLGPU_REQUIRE(p->pre_steps >= 0 && p->post_steps >= 0, "negative smoothing steps");
   FLGPU_REQUIRE(p->base_level >= 0 && p->base_level < nlevels, "base_level out of range");
   FLGPU_REQUIRE(p->prec == FLGPU_F64 || p->prec == FLGPU_F32, "unknown precision");
+  FLGPU_REQUIRE(p->transfer == FLGPU_TRANSFER_GEOMETRIC || p->transfer == FLGPU_TRANSFER_BLACKBOX,
+                "unknown transfer");
 }
 
 static void mg_alloc_work(flgpu_mg *mg) {
@@ -397,9 +399,9 @@ extern "C" int flgpu_mg_create(flgpu_ctx *ctx, int nlevels, flgpu_mat *const *A,
     FLGPU_REQUIRE(nlevels >= 1, "nlevels must be >= 1");
     FLGPU_REQUIRE(nlevels == 1 || I, "I must not be NULL");
     check_params(params, nlevels);
-    FLGPU_REQUIRE(!params->galerkin,
-                  "galerkin = 1 is for lattice hierarchies; build coarse matrices with "
-                  "flgpu_galerkin_product");
+    FLGPU_REQUIRE(!params->galerkin && params->transfer == FLGPU_TRANSFER_GEOMETRIC,
+                  "galerkin = 1 and black-box transfers are for lattice hierarchies; build "
+                  "coarse matrices with flgpu_galerkin_product");
     for (int l = 0; l < nlevels; l++) {
       FLGPU_REQUIRE(A[l] && A[l]->ctx == ctx, "level matrix missing or of another context");
       FLGPU_REQUIRE(A[l]->nrows == A[l]->ncols, "level matrices must be square");
@@ -447,6 +449,8 @@ extern "C" int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n
     FLGPU_REQUIRE(nlevels >= 1 && n0 >= 1, "nlevels and n0 must be >= 1");
     check_params(params, nlevels);
     FLGPU_REQUIRE(!kappa || kappa->ctx == ctx, "kappa belongs to another context");
+    FLGPU_REQUIRE(params->transfer == FLGPU_TRANSFER_GEOMETRIC || (dim == 3 && order == 1),
+                  "black-box transfers are defined for 3D Q1 lattices");
     CTX_LOCK(ctx);
     flgpu_mg *mg = new flgpu_mg;
     mg->ctx = ctx;
@@ -467,11 +471,14 @@ extern "C" int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n
       for (int l = nlevels - 1; l >= 0; l--) {
         int n = n0 << l;
         bool is_top = (l == nlevels - 1);
-        if (is_top || !params->galerkin)
+        bool bbox = params->transfer == FLGPU_TRANSFER_BLACKBOX;
+        if (is_top || (!params->galerkin && !bbox)) {
           mg->lv[l].A = lattice_assemble(ctx, dim, order, n, kappa, nk, f, params->prec, asm_mode,
                                          (is_top && rhs) ? &b : nullptr);
-        else
+        } else {
+          if (bbox) lattice_blackbox(mg->lv[l + 1].A, mg->lv[l].I);
           mg->lv[l].A = lattice_galerkin(mg->lv[l].I, mg->lv[l + 1].A, dim, order, n);
+        }
         // matrices in the per-cell form sweep the lattice colours directly
         if (params->smoother == FLGPU_SMOOTH_SOR && l > params->base_level &&
             !mg->lv[l].A->alpha && !mg->lv[l].A->val27)
@@ -497,9 +504,11 @@ extern "C" int flgpu_mg_reassemble(flgpu_mg *mg, const flgpu_vec *kappa, int nk)
     CTX_LOCK(mg->ctx);
     for (int l = mg->top; l >= 0; l--) {
       MGLevel &L = mg->lv[l];
-      if (l == mg->top || !mg->p.galerkin) {
+      bool bbox = mg->p.transfer == FLGPU_TRANSFER_BLACKBOX;
+      if (l == mg->top || (!mg->p.galerkin && !bbox)) {
         lattice_reassemble(L.A, kappa, nk, mg->asm_mode);
       } else {
+        if (bbox) lattice_blackbox(mg->lv[l + 1].A, L.I);
         galerkin_product(L.I, mg->lv[l + 1].A, L.A);
         lattice_dirichlet_identity(L.A, mg->lat_dim, mg->lat_order, mg->lat_n0 << l);
         mat_refresh_values(L.A);

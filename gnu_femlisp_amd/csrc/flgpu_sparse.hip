@@ -139,6 +139,26 @@ __global__ void k_diag(int n, const int *pos, const T *vals, T *diag) {
   diag[r] = p >= 0 ? vals[p] : (T)0;
 }
 
+// T = A^T with T's pattern already built: T_cf = A_fc (row c of T, column f)
+template <class T>
+__global__ void k_transpose_gather(int nrows_t, const int *__restrict__ trowptr,
+                                   const int *__restrict__ tcolidx, T *tvals,
+                                   const int *__restrict__ rowptr, const int *__restrict__ colidx,
+                                   const T *__restrict__ vals) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrows_t) return;
+  for (int p = trowptr[c]; p < trowptr[c + 1]; p++) {
+    int f = tcolidx[p];
+    T v = (T)0;
+    for (int e = rowptr[f]; e < rowptr[f + 1]; e++)
+      if (colidx[e] == c) {
+        v = vals[e];
+        break;
+      }
+    tvals[p] = v;
+  }
+}
+
 static void diag_refresh(flgpu_mat *A) {
   int n = std::min(A->nrows, A->ncols);
   if (n == 0) return;
@@ -216,8 +236,20 @@ void mat_refresh_values(flgpu_mat *A) {
     for (auto &s : A->colour_sell) sell_fill<float>(A->ctx, A, s, 0);
   }
   if (A->transpose) {
-    mat_free(A->transpose);
-    A->transpose = nullptr;
+    // same pattern: the transpose's values are gathered from the new ones
+    flgpu_mat *T = A->transpose;
+    if (T->nrows > 0) {
+      if (A->prec == FLGPU_F64)
+        k_transpose_gather<double><<<grid_for(T->nrows, 256), 256, 0, A->ctx->stream>>>(
+            T->nrows, T->rowptr, T->colidx, (double *)T->vals, A->rowptr, A->colidx,
+            (const double *)A->vals);
+      else
+        k_transpose_gather<float><<<grid_for(T->nrows, 256), 256, 0, A->ctx->stream>>>(
+            T->nrows, T->rowptr, T->colidx, (float *)T->vals, A->rowptr, A->colidx,
+            (const float *)A->vals);
+      FLGPU_HIP(hipGetLastError());
+    }
+    mat_refresh_values(T);
   }
 }
 

@@ -19,6 +19,7 @@ SMOOTH_JACOBI, SMOOTH_SOR = 0, 1
 OUTER_MG, OUTER_CG, OUTER_CG_PLAIN, OUTER_FCG = 0, 1, 2, 3
 STOP_REDUCTION, STOP_DEFNORM = 0, 1
 STORAGE_SELL, STORAGE_STENCIL, STORAGE_CELL = 0, 1, 2
+TRANSFER_GEOMETRIC, TRANSFER_BLACKBOX = 0, 1
 
 
 class FlgpuError(RuntimeError):
@@ -34,7 +35,8 @@ class MgParams(ctypes.Structure):
     _fields_ = [("smoother", ctypes.c_int), ("omega", ctypes.c_double),
                 ("pre_steps", ctypes.c_int), ("post_steps", ctypes.c_int),
                 ("base_level", ctypes.c_int), ("symmetric", ctypes.c_int),
-                ("prec", ctypes.c_int), ("galerkin", ctypes.c_int)]
+                ("prec", ctypes.c_int), ("galerkin", ctypes.c_int),
+                ("transfer", ctypes.c_int)]
 
 
 class SolveParams(ctypes.Structure):
@@ -92,6 +94,7 @@ SIGNATURES = {
     "flgpu_lattice_reassemble": (_ci, [_vp, _vp, _ci, _ci]),
     "flgpu_lattice_interpolation": (_ci, [_vp, _ci, _ci, _ci, _ci, _P(_vp)]),
     "flgpu_lattice_colouring": (_ci, [_vp, _ci, _ci, _ci]),
+    "flgpu_lattice_blackbox_interpolation": (_ci, [_vp, _vp]),
     "flgpu_lattice_element_matrices": (_ci, [_vp, _ci, _ci, _ci, _vp, _ci, _ci, _ci, _cl, _cl, _vp]),
     "flgpu_mg_create": (_ci, [_vp, _ci, _vp, _vp, _P(MgParams), _P(_vp)]),
     "flgpu_mg_create_lattice": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _ci, _cd, _ci, _P(MgParams),

@@ -61,7 +61,8 @@ class GeometricCS(LinearIteration):
     """
 
     def __init__(self, smoother=None, pre_steps=1, post_steps=1, base_level=0, gamma=1,
-                 symmetric=False, prec=F64, combination="multiplicative", galerkin=False):
+                 symmetric=False, prec=F64, combination="multiplicative", galerkin=False,
+                 transfer="geometric"):
         if gamma != 1 or combination != "multiplicative":
             raise NotImplementedError("only gamma = 1 multiplicative V-cycles run on the GPU")
         self.smoother = smoother or GaussSeidel()
@@ -70,13 +71,18 @@ class GeometricCS(LinearIteration):
         self.symmetric = symmetric
         self.prec = prec
         self.galerkin = galerkin
+        if transfer not in ("geometric", "blackbox"):
+            raise ValueError("transfer must be 'geometric' or 'blackbox'")
+        self.transfer = transfer
 
     def params(self):
         sm = self.smoother
         kind = _f.SMOOTH_JACOBI if isinstance(sm, Jacobi) else _f.SMOOTH_SOR
         omega = sm.damp if isinstance(sm, Jacobi) else sm.omega
         return MgParams(kind, omega, self.pre_steps, self.post_steps, self.base_level,
-                        int(self.symmetric), self.prec, int(self.galerkin))
+                        int(self.symmetric), self.prec, int(self.galerkin),
+                        _f.TRANSFER_BLACKBOX if self.transfer == "blackbox"
+                        else _f.TRANSFER_GEOMETRIC)
 
 
 class CG(LinearIteration):
@@ -144,6 +150,15 @@ class Hierarchy:
     def apply(self, x, b, r):
         check(lib().flgpu_mg_apply(self.h, x.h, b.h, r.h))
         return x
+
+    def reassemble(self, kappa):
+        """New coefficient on every level of a lattice hierarchy (same pattern;
+        transfers, Galerkin levels and the coarse inverse follow)."""
+        from .lattice import _kappa_vec
+        kv, nk = _kappa_vec(kappa, self.ctx)
+        check(lib().flgpu_mg_reassemble(self.h, kv.h if kv else None, nk))
+        self._keep = tuple(self._keep) + (kv,)
+        return self
 
     def use_graphs(self, enable=True):
         check(lib().flgpu_mg_use_graphs(self.h, int(enable)))

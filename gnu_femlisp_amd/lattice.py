@@ -58,6 +58,14 @@ def interpolation(dim, order, nc, prec=F64, ctx=None):
     return GpuSparseMatrix(h, ctx)
 
 
+def blackbox_interpolation(A, P):
+    """Overwrite the values of the lattice interpolation P (n/2 -> n cells per
+    axis, 3D Q1) with the operator-dependent weights computed from the fine
+    matrix A; P's transpose follows."""
+    check(lib().flgpu_lattice_blackbox_interpolation(A.h, P.h))
+    return P
+
+
 def colour(A, dim, order, n):
     check(lib().flgpu_lattice_colouring(A.h, dim, order, n))
     return A

@@ -185,6 +185,18 @@ int flgpu_lattice_reassemble(flgpu_mat *A, const flgpu_vec *kappa, int nk,
  * Dirichlet columns zeroed but kept. */
 int flgpu_lattice_interpolation(flgpu_ctx *ctx, int dim, int order, int nc,
                                 int prec, flgpu_mat **P);
+/* Operator-dependent ("black-box") interpolation for 3D Q1, no femlisp
+ * counterpart: it takes the place of flgpu_lattice_interpolation's values
+ * (femlisp's constrained :local interpolation, fetransfer.lisp:73-138) when
+ * the coefficient jumps.  P must be a lattice interpolation from n/2 to n
+ * cells per axis (its pattern), A the fine matrix on n cells (n even, lattice
+ * pattern).  Dendy's black-box prolongation (J. Comput. Phys. 48 (1982)
+ * 366-386) for 27-point stencils: a fine node with s odd coordinates solves
+ * its row of A, summed onto its odd axes, for itself, using the rows of nodes
+ * with fewer odd coordinates; fine Dirichlet rows and coarse Dirichlet
+ * columns are zero.  For a constant coefficient it equals femlisp's trilinear
+ * interpolation. */
+int flgpu_lattice_blackbox_interpolation(const flgpu_mat *A, flgpu_mat *P);
 /* lattice colouring for <sor>: coordinate parity (Q1) / coordinate mod 3 (Q2) */
 int flgpu_lattice_colouring(flgpu_mat *A, int dim, int order, int n);
 /* The element kernel alone (discretize-locally for a batch of cells,
@@ -212,7 +224,14 @@ typedef struct flgpu_mg_params {
   int prec;        /* storage/arithmetic of the V-cycle: FLGPU_F64 / FLGPU_F32 */
   int galerkin;    /* lattice hierarchies: 1 = coarse levels P^T A P
                       (geomg.lisp:122-123 :galerkin-p t), 0 = re-discretised */
+  int transfer;    /* lattice hierarchies: FLGPU_TRANSFER_GEOMETRIC (femlisp's
+                      interpolation) or FLGPU_TRANSFER_BLACKBOX (3D Q1:
+                      operator-dependent interpolation from each finer level's
+                      matrix, coarse levels P^T A P whatever galerkin says) */
 } flgpu_mg_params;
+
+#define FLGPU_TRANSFER_GEOMETRIC 0
+#define FLGPU_TRANSFER_BLACKBOX 1
 
 /* Hierarchy from caller matrices: A[0..nlevels-1] level matrices (coarsest
  * first), I[0..nlevels-2] interpolations level l -> l+1 (rows = DOFs of

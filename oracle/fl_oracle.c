@@ -417,6 +417,96 @@ long flo_interpolation(int dim, int order, int nc, int *rowptr, int *colidx,
   return pos;
 }
 
+/* Operator-dependent ("black-box") interpolation for 3D Q1 lattices: not in
+ * femlisp.  It restates J. E. Dendy's black-box multigrid prolongation
+ * (J. Comput. Phys. 48 (1982) 366-386) for 27-point stencils.  Fine node i
+ * with s odd coordinates is interpolated from the 2^s coarse nodes of the
+ * coarse cell around it.  Its row of A is collapsed onto the odd axes
+ * (coll[key] = sum of a_ij over the even-axis offsets), and the equation
+ * coll[0] e_i + sum_key coll[key] e_(i+key) = 0 is solved for e_i.  The
+ * neighbours' e are taken from their own interpolation rows: nodes with fewer
+ * odd coordinates are done first (1, then 2, then 3).  Boundary nodes carry
+ * no weight, which zeroes fine Dirichlet rows and coarse Dirichlet columns as
+ * femlisp's constrained interpolation does (constraints.lisp:58-72).  The
+ * values go into the pattern of flo_interpolation(3, 1, nf/2, ...). */
+int flo_blackbox_interpolation(int nf, const int *rowptr, const int *colidx,
+                               const double *vals, const int *prowptr,
+                               const int *pcolidx, double *pvals) {
+  if (nf < 2 || nf % 2) return -1;
+  int M = nf + 1, Mc = nf / 2 + 1;
+  long N = (long)M * M * M;
+  double *W = (double *)calloc((size_t)N * 8, sizeof(double));
+  if (!W) return -1;
+  for (int pass = 0; pass <= 3; pass++)
+    for (long i = 0; i < N; i++) {
+      int c[3];
+      unrank(i, 3, M, c);
+      int odd[3] = {c[0] & 1, c[1] & 1, c[2] & 1};
+      if (odd[0] + odd[1] + odd[2] != pass) continue;
+      int bd = 0;
+      for (int k = 0; k < 3; k++)
+        if (c[k] == 0 || c[k] == M - 1) bd = 1;
+      if (bd) continue; /* boundary nodes keep W = 0 */
+      if (pass == 0) {
+        W[i * 8] = 1.0;
+        continue;
+      }
+      double coll[27];
+      memset(coll, 0, sizeof(coll));
+      for (int e = rowptr[i]; e < rowptr[i + 1]; e++) {
+        int d[3], key = 0;
+        unrank(colidx[e], 3, M, d);
+        int ok = 1;
+        for (int k = 0; k < 3; k++) {
+          int o = d[k] - c[k];
+          if (o < -1 || o > 1) ok = 0;
+          key = key * 3 + (odd[k] ? o + 1 : 1);
+        }
+        if (ok) coll[key] += vals[e];
+      }
+      double a0 = coll[13];
+      if (a0 == 0.0) continue;
+      double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int key = 0; key < 27; key++) {
+        if (key == 13 || coll[key] == 0.0) continue;
+        int o[3] = {key / 9 - 1, (key / 3) % 3 - 1, key % 3 - 1}, nb[3];
+        for (int k = 0; k < 3; k++) nb[k] = c[k] + o[k];
+        long j = rank(nb, 3, M);
+        double s = -coll[key] / a0;
+        for (int b = 0; b < 8; b++) {
+          double w = W[j * 8 + b];
+          if (w == 0.0) continue;
+          /* corner b of j (bits z,y,x over j's own base corner) as a corner of i */
+          int tb = 0;
+          for (int k = 0; k < 3; k++) {
+            int bit = (b >> (2 - k)) & 1;
+            int cc = (nb[k] >> 1) + bit - (c[k] >> 1);
+            tb = tb * 2 + cc;
+          }
+          acc[tb] += s * w;
+        }
+      }
+      for (int b = 0; b < 8; b++) W[i * 8 + b] = acc[b];
+    }
+  long Nfr = N;
+  for (long i = 0; i < Nfr; i++) {
+    int c[3];
+    unrank(i, 3, M, c);
+    for (int e = prowptr[i]; e < prowptr[i + 1]; e++) {
+      int cc[3], b = 0, in = 1;
+      unrank(pcolidx[e], 3, Mc, cc);
+      for (int k = 0; k < 3; k++) {
+        int bit = cc[k] - (c[k] >> 1);
+        if (bit < 0 || bit > 1) in = 0;
+        b = b * 2 + bit;
+      }
+      pvals[e] = in ? W[i * 8 + b] : 0.0;
+    }
+  }
+  free(W);
+  return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* algebra                                                                   */
 /* ------------------------------------------------------------------------ */
@@ -610,12 +700,26 @@ flo_mg *flo_mg_create_ex(int dim, int order, int n0, int levels,
     L->rhs = (double *)calloc(L->N, sizeof(double));
     L->res = (double *)calloc(L->N, sizeof(double));
   }
+  if (galerkin_p == 2 && (dim != 3 || order != 1)) {
+    flo_mg_destroy(mg);
+    return NULL;
+  }
   if (galerkin_p)
-    for (int l = levels - 2; l >= 0; l--)
-      if (galerkin(&mg->lv[l], &mg->lv[l + 1]) != 0) {
+    for (int l = levels - 2; l >= 0; l--) {
+      level_t *C = &mg->lv[l];
+      const level_t *F = &mg->lv[l + 1];
+      if (galerkin_p == 2) {
+        /* operator-dependent transfer from the finer level's (Galerkin) matrix */
+        flo_blackbox_interpolation(F->n, F->A.rowptr, F->A.colidx, F->A.vals, C->I.rowptr,
+                                   C->I.colidx, C->I.vals);
+        flo_csr_transpose(C->I.nrows, C->I.ncols, C->I.rowptr, C->I.colidx, C->I.vals,
+                          C->IT.rowptr, C->IT.colidx, C->IT.vals);
+      }
+      if (galerkin(C, F) != 0) {
         flo_mg_destroy(mg);
         return NULL;
       }
+    }
   return mg;
 }
 

@@ -70,6 +70,17 @@ int flo_assemble(int dim, int order, int n, const double *kappa, int nk,
 long flo_interpolation(int dim, int order, int nc, int *rowptr, int *colidx,
                        double *vals);
 
+/* Operator-dependent interpolation for 3D Q1 (not in femlisp; Dendy's
+ * black-box prolongation for 27-point stencils): the values of the pattern of
+ * flo_interpolation(3, 1, nf/2, prowptr, pcolidx, .) computed from the fine
+ * matrix (nf cells per axis, lattice pattern).  A fine node with s odd
+ * coordinates solves its row, collapsed onto its odd axes, for itself, using
+ * the rows of nodes with fewer odd coordinates.  Fine Dirichlet rows and
+ * coarse Dirichlet columns are zero.  Returns 0, or -1 for odd nf. */
+int flo_blackbox_interpolation(int nf, const int *rowptr, const int *colidx,
+                               const double *vals, const int *prowptr,
+                               const int *pcolidx, double *pvals);
+
 /* ---- algebra (src/matlisp/sparse-matrix.lisp:378-442) */
 /* x <- beta*x + alpha*A*y  (gemm-nn!) */
 void flo_spmv(int nrows, const int *rowptr, const int *colidx,
@@ -94,7 +105,9 @@ flo_mg *flo_mg_create(int dim, int order, int n0, int levels,
                       const double *kappa, int nk, double f);
 /* as flo_mg_create; galerkin_p = 1 builds the coarse levels as P^T A P
  * (geomg.lisp:122-123 :galerkin-p t, amg.lisp:247-280) instead of
- * re-discretising them */
+ * re-discretising them; galerkin_p = 2 (3D Q1 only) does the same with the
+ * operator-dependent interpolation of flo_blackbox_interpolation, computed
+ * from each finer level's matrix, top level first */
 flo_mg *flo_mg_create_ex(int dim, int order, int n0, int levels,
                          const double *kappa, int nk, double f, int galerkin_p);
 void flo_mg_destroy(flo_mg *mg);

@@ -63,6 +63,8 @@ def lib():
         L.flo_assemble.argtypes = [ci, ci, ci, vp, ci, cd, vp, vp]
         L.flo_interpolation.restype = cl
         L.flo_interpolation.argtypes = [ci, ci, ci, vp, vp, vp]
+        L.flo_blackbox_interpolation.restype = ci
+        L.flo_blackbox_interpolation.argtypes = [ci, vp, vp, vp, vp, vp, vp]
         L.flo_spmv.argtypes = [ci, vp, vp, vp, cd, vp, cd, vp]
         L.flo_spmv_t.argtypes = [ci, ci, vp, vp, vp, cd, vp, cd, vp]
         L.flo_csr_transpose.argtypes = [ci, ci, vp, vp, vp, vp, vp, vp]
@@ -164,6 +166,19 @@ def interpolation(dim, order, nc):
     return rp, ci, v
 
 
+def blackbox_interpolation(nf, rp, ci, vals):
+    """Operator-dependent interpolation (3D Q1) from the fine matrix with nf
+    cells per axis, in the pattern of interpolation(3, 1, nf // 2)."""
+    prp, pci, _ = interpolation(3, 1, nf // 2)
+    pv = np.zeros(pci.size)
+    rp, ci = np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32)
+    vals = np.ascontiguousarray(vals, np.float64)
+    if lib().flo_blackbox_interpolation(nf, _p(rp), _p(ci), _p(vals), _p(prp), _p(pci),
+                                        _p(pv)) != 0:
+        raise ValueError("nf must be even")
+    return prp, pci, pv
+
+
 def spmv(rp, ci, vals, y, alpha=1.0, beta=0.0, x=None):
     n = len(rp) - 1
     x = np.zeros(n) if x is None else np.array(x, np.float64)
@@ -173,17 +188,20 @@ def spmv(rp, ci, vals, y, alpha=1.0, beta=0.0, x=None):
 
 
 class Hierarchy:
-    """Multigrid hierarchy of the oracle (re-discretised levels)."""
+    """Multigrid hierarchy of the oracle: re-discretised levels, or Galerkin
+    levels (galerkin=True) with femlisp's interpolation (transfer="geometric")
+    or the operator-dependent one (transfer="blackbox", 3D Q1)."""
 
-    def __init__(self, dim, order, n0, levels, kappa=None, f=1.0, galerkin=False):
+    def __init__(self, dim, order, n0, levels, kappa=None, f=1.0, galerkin=False,
+                 transfer="geometric"):
         self.dim, self.order, self.n0, self.levels = dim, order, n0, levels
         nk = 0
         self._kappa = None
         if kappa is not None:
             self._kappa = np.ascontiguousarray(kappa, np.float64)
             nk = int(round(self._kappa.size ** (1.0 / dim)))
-        self.h = lib().flo_mg_create_ex(dim, order, n0, levels, _p(self._kappa), nk, f,
-                                        int(galerkin))
+        mode = 2 if transfer == "blackbox" else int(galerkin)
+        self.h = lib().flo_mg_create_ex(dim, order, n0, levels, _p(self._kappa), nk, f, mode)
         if not self.h:
             raise ValueError("flo_mg_create failed")
 

@@ -372,6 +372,103 @@ def test_galerkin_hierarchy_parity(dim, order, levels):
         assert rowrel(v, ov, orp) < 1e-12
 
 
+@pytest.mark.parametrize("prec", [F.F64, F.F32])
+def test_blackbox_interpolation_parity(prec):
+    """Operator-dependent interpolation from a jumping coefficient: values
+    against the oracle's, and the transpose (restriction) refreshed with them."""
+    say(f"blackbox interpolation prec={prec}")
+    n = 16
+    rng = np.random.default_rng(31)
+    kap = np.exp(rng.normal(scale=2.0, size=(n, n, n)))
+    A, _ = lat.assemble(3, 1, n, kap, prec=prec)
+    P = lat.interpolation(3, 1, n // 2, prec=prec)
+    y = rng.uniform(-1, 1, P.nrows)
+    before = gemm(1.0, P, GpuVector.from_array(y, prec), 0.0, GpuVector(P.ncols, prec),
+                  job="tn").numpy()
+    lat.blackbox_interpolation(A, P)
+    prp, pci, pv = P.csr()
+    orp, oci, ov, _ = po.assemble(3, 1, n, kap)
+    if prec == F.F32:  # the oracle reads the same FP32-rounded matrix values
+        ov = ov.astype(np.float32).astype(np.float64)
+    brp, bci, bv = po.blackbox_interpolation(n, orp, oci, ov)
+    assert np.array_equal(prp, brp) and np.array_equal(pci, bci)
+    assert rowrel(pv, bv, brp) < (1e-13 if prec == F.F64 else 1e-6)
+    after = gemm(1.0, P, GpuVector.from_array(y, prec), 0.0, GpuVector(P.ncols, prec),
+                 job="tn").numpy()
+    Pt = sp.csr_matrix((bv, bci, brp), shape=(P.nrows, P.ncols)).T
+    assert relerr(after, Pt @ y) < (1e-13 if prec == F.F64 else 1e-5)
+    assert relerr(after, before) > 1e-3  # the restriction follows the new values
+
+
+def test_blackbox_hierarchy_solve_parity():
+    """Black-box transfers + Galerkin levels: level matrices against the
+    oracle's, then CG with the V-cycle: the oracle's iteration count, defect
+    history and solution."""
+    say("blackbox hierarchy solve")
+    dim, order, n0, levels = 3, 1, 2, 4
+    ntop = n0 << (levels - 1)
+    from gnu_femlisp_amd.fields import inclusions
+    kap = inclusions(ntop, seed=5, rmin=1.5, rmax=3.0)
+    oh = po.Hierarchy(dim, order, n0, levels, kappa=kap, transfer="blackbox")
+    cs = it.GeometricCS(it.SOR(1.0), 2, 2, 0, symmetric=True, transfer="blackbox")
+    H = it.Hierarchy.lattice(cs, dim, order, n0, levels, kappa=kap)
+    for l in range(levels):
+        rp, ci, v = H.level_matrix(l).csr()
+        orp, oci, ov = oh.matrix(l)
+        assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+        assert rowrel(v, ov, orp) < 1e-12
+    rng = np.random.default_rng(4)
+    b = rng.uniform(-1, 1, oh.ndofs())
+    b[po.boundary(dim, order, ntop)] = 0
+    prm = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_SOR, omega=1.0,
+                    ordering=po.ORDER_COLOUR, pre_steps=2, post_steps=2, base_level=0,
+                    symmetric=1, alias=0, tol=1e-10, maxsteps=200)
+    st, xo, so, dno = oh.solve(prm, b)
+    assert st == 0
+    bb = it.solve(it.LinearSolver(it.CG(cs), success_if=("<", "reduction", 1e-10),
+                                  failure_if=(">", "step", 200)),
+                  {"matrix": H.top_matrix(), "rhs": GpuVector.from_array(b), "hierarchy": H})
+    assert bb["status"] == "success" and bb["steps"] == so
+    assert relerr(bb["solution"].numpy(), xo) < 1e-10
+    assert np.allclose(bb["defnorms"], dno, rtol=1e-8)
+
+
+def test_blackbox_mixed_fcg_reassemble():
+    """FP32 black-box hierarchy inside FP64 FCG reaches a 1e-10 true residual
+    with fewer steps than the geometric transfers on a 1e4-contrast problem;
+    after re-assembly for another coefficient the transfers follow it (the
+    solve equals a freshly built hierarchy's)."""
+    say("blackbox mixed fcg")
+    from gnu_femlisp_amd.fields import inclusions
+    n0, levels = 2, 5
+    n = n0 << (levels - 1)
+    steps = {}
+    for tr in ("geometric", "blackbox"):
+        kap = inclusions(n, seed=11, rmin=2.0, rmax=4.0)
+        cs = it.GeometricCS(it.SOR(1.0), 2, 2, 0, symmetric=True, prec=F.F32, transfer=tr)
+        H = it.Hierarchy.lattice(cs, 3, 1, n0, levels, kappa=kap, mode=F.ASM_MFMA)
+        A, b = lat.assemble(3, 1, n, kap, prec=F.F64)
+        bb = it.solve(it.LinearSolver(it.FCG(cs), success_if=("<", "reduction", 1e-10),
+                                      failure_if=(">", "step", 300), true_residual=True),
+                      {"matrix": A, "rhs": b, "hierarchy": H})
+        assert bb["status"] == "success"
+        assert bb["true_residual"] / np.linalg.norm(b.numpy()) < 1e-10
+        steps[tr] = bb["steps"]
+    assert steps["blackbox"] < steps["geometric"]
+    kap2 = inclusions(n, seed=12, rmin=2.0, rmax=4.0)
+    H.reassemble(kap2)
+    A2, b2 = lat.assemble(3, 1, n, kap2, prec=F.F64)
+    H2 = it.Hierarchy.lattice(cs, 3, 1, n0, levels, kappa=kap2, mode=F.ASM_MFMA)
+    res = []
+    for h in (H, H2):
+        bb = it.solve(it.LinearSolver(it.FCG(cs), success_if=("<", "reduction", 1e-10),
+                                      failure_if=(">", "step", 300), true_residual=True),
+                      {"matrix": A2, "rhs": b2, "hierarchy": h})
+        res.append((bb["steps"], bb["defnorms"]))
+    assert res[0][0] == res[1][0]
+    assert np.allclose(res[0][1], res[1][1], rtol=1e-6)
+
+
 def test_known_answer_galerkin_rate_3d():
     """mg-cdr.lisp:100 on the GPU: Galerkin GS V(1,0), CR < 0.15."""
     say("known answer galerkin")

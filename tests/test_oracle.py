@@ -81,6 +81,78 @@ def test_galerkin_equals_rediscretisation_for_constant_coefficient():
     assert np.abs(v - vr).max() > 1e-3
 
 
+def _bbox(n, kap):
+    rp, ci, v, _ = po.assemble(3, 1, n, kap)
+    prp, pci, pv = po.blackbox_interpolation(n, rp, ci, v)
+    return csr(prp, pci, pv).tocsr(), n + 1, n // 2 + 1
+
+
+def test_blackbox_constant_coefficient_is_trilinear():
+    """For a constant coefficient the collapsed 27-point rows give weights 1/2
+    per odd axis: the operator-dependent interpolation equals femlisp's
+    trilinear one, pattern and values."""
+    for n in (4, 8, 16):
+        prp, pci, pv = po.blackbox_interpolation(n, *po.assemble(3, 1, n)[:3])
+        trp, tci, tv = po.interpolation(3, 1, n // 2)
+        assert np.array_equal(prp, trp) and np.array_equal(pci, tci)
+        assert np.abs(pv - tv).max() < 1e-15
+
+
+def test_blackbox_layered_coefficient_closed_form():
+    """kappa constant on layers across axis 2: a fine node with only that
+    coordinate odd sits between layers kappa_L and kappa_R, and flux
+    continuity gives the weight kappa_L / (kappa_L + kappa_R) on its lower
+    coarse neighbour (away from the boundary)."""
+    n = 16
+    lay = np.exp(np.random.default_rng(5).normal(scale=2.0, size=n))
+    kap = np.broadcast_to(lay, (n, n, n)).copy()
+    P, M, Mc = _bbox(n, kap)
+    checked = 0
+    for z in range(2, M - 2, 2):
+        for y in range(2, M - 2, 2):
+            for x in range(3, M - 3, 2):
+                i = (z * M + y) * M + x
+                lo = ((z // 2) * Mc + y // 2) * Mc + (x - 1) // 2
+                kl, kr = lay[x - 1], lay[x]
+                assert abs(P[i, lo] - kl / (kl + kr)) < 1e-13
+                assert abs(P[i, lo + 1] - kr / (kl + kr)) < 1e-13
+                checked += 1
+    assert checked > 100
+
+
+def test_blackbox_preserves_constants():
+    """A row of A away from Dirichlet nodes sums to zero, so every fine node
+    whose coordinates all lie in [2, M-3] interpolates constants exactly (its
+    weights sum to 1), whatever the coefficient; boundary rows are zero."""
+    n = 16
+    kap = np.exp(np.random.default_rng(6).normal(scale=2.5, size=(n, n, n)))
+    P, M, Mc = _bbox(n, kap)
+    s = np.asarray(P.sum(axis=1)).ravel()
+    c = np.indices((M, M, M)).reshape(3, -1)
+    inner = np.all((c >= 2) & (c <= M - 3), axis=0)
+    bnd = np.any((c == 0) | (c == M - 1), axis=0)
+    assert np.abs(s[inner] - 1.0).max() < 1e-13
+    assert np.abs(s[bnd]).max() == 0.0
+    assert P.data.min() > -1e-14  # M-matrix rows give non-negative weights
+
+
+def test_blackbox_hierarchy_beats_geometric_on_inclusions():
+    """1e4-contrast inclusions (the deploy coefficient) at 32^3: the Galerkin
+    hierarchy with black-box transfers needs fewer CG steps than femlisp's
+    re-discretised geometric hierarchy."""
+    from gnu_femlisp_amd.fields import inclusions
+    kap = inclusions(32)
+    prm = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_SOR, omega=1.0,
+                    ordering=po.ORDER_COLOUR, pre_steps=2, post_steps=2, base_level=0,
+                    symmetric=1, tol=1e-10, maxsteps=300)
+    steps = {}
+    for tr in ("geometric", "blackbox"):
+        h = po.Hierarchy(3, 1, 2, 5, kappa=kap, transfer=tr)
+        st, x, steps[tr], dn = h.solve(prm)
+        assert st == 0
+    assert steps["blackbox"] < steps["geometric"]
+
+
 def test_fe_value_1d():
     """ellsys-fe.lisp:249-256: 1D Q1 -u''=1 on level 2 gives u(0.5) = 0.125 exactly."""
     rp, ci, v, b = po.assemble(1, 1, 4)
