@@ -145,21 +145,24 @@ def main():
         return
 
     # ---- hierarchies ---------------------------------------------------------------
-    t = time.perf_counter()
-    cs64 = it.GeometricCS(it.SOR(1.0), 2, 2, args.base, symmetric=True, prec=F.F64)
-    H64 = it.Hierarchy.lattice(cs64, 3, 1, n0, levels, kappa=(kv, n), mode=F.ASM_F64)
-    ctx.synchronize()
-    setup_plain = time.perf_counter() - t
-    A = H64.top_matrix()
-    b = H64.rhs
+    # plain: femlisp's FP64 GMG-CG (geometric transfers, re-discretised levels);
+    # mixed: FP32 V-cycle (BF16-MFMA assembled) in FP64 FCG; "-bb": the same with
+    # operator-dependent (black-box) transfers and Galerkin coarse levels
+    spec = {"plain": (F.F64, F.ASM_F64, "geometric"), "mixed": (F.F32, F.ASM_MFMA, "geometric"),
+            "plain-bb": (F.F64, F.ASM_F64, "blackbox"), "mixed-bb": (F.F32, F.ASM_MFMA, "blackbox")}
+    hier, setup = {}, {}
+    for name, (prec, mode, tr) in spec.items():
+        t = time.perf_counter()
+        cs = it.GeometricCS(it.SOR(1.0), 2, 2, args.base, symmetric=True, prec=prec, transfer=tr)
+        hier[name] = (cs, it.Hierarchy.lattice(cs, 3, 1, n0, levels, kappa=(kv, n), mode=mode))
+        ctx.synchronize()
+        setup[name] = time.perf_counter() - t
+    A = hier["plain"][1].top_matrix()
+    b = hier["plain"][1].rhs
     N, nnz = A.nrows, A.nnz
     bnorm = norm(b)
-    t = time.perf_counter()
-    cs32 = it.GeometricCS(it.SOR(1.0), 2, 2, args.base, symmetric=True, prec=F.F32)
-    H32 = it.Hierarchy.lattice(cs32, 3, 1, n0, levels, kappa=(kv, n), mode=F.ASM_MFMA)
-    ctx.synchronize()
-    setup_mixed = time.perf_counter() - t
-    log(f"[bench] setup plain {setup_plain:.2f}s mixed {setup_mixed:.2f}s  N={N} nnz={nnz}")
+    log(f"[bench] setup " + " ".join(f"{k} {v:.2f}s" for k, v in setup.items()) +
+        f"  N={N} nnz={nnz}")
 
     net, train_info = None, None
     if not args.no_learned:
@@ -177,23 +180,22 @@ def main():
                           "train_s": time.perf_counter() - t - t_data,
                           "loss": hist, "holdout": vloss}
             del X, T
-        learned.attach(H32, net, kv, n)
+        learned.attach(hier["mixed"][1], net, kv, n)
 
     def run(path):
-        if path == "plain":
-            solver = it.LinearSolver(it.CG(cs64), failure_if=(">", "step", 600), true_residual=True)
-            H = H64
+        learned_p = path == "learned"
+        cs, H = hier["mixed" if learned_p else path]
+        if path.startswith("plain"):
+            solver = it.LinearSolver(it.CG(cs), failure_if=(">", "step", 600), true_residual=True)
         else:
-            solver = it.LinearSolver(it.FCG(cs32), failure_if=(">", "step", 600),
-                                     true_residual=True, learned=(path == "learned"),
-                                     stall_window=40)
-            H = H32
+            solver = it.LinearSolver(it.FCG(cs), failure_if=(">", "step", 600),
+                                     true_residual=True, learned=learned_p, stall_window=40)
         t0 = time.perf_counter()
         out = it.solve(solver, {"matrix": A, "rhs": b, "hierarchy": H})
         return time.perf_counter() - t0, out
 
     results = {}
-    for path in ("plain", "mixed") + (() if net is None else ("learned",)):
+    for path in ("plain", "mixed", "plain-bb", "mixed-bb") + (() if net is None else ("learned",)):
         for _ in range(args.warmup):
             run(path)
         times = []
@@ -215,7 +217,9 @@ def main():
 
     # ---- V-cycle GDOF/s --------------------------------------------------------------
     vcyc = {}
-    for name, H in (("f64", H64), ("f32", H32)):
+    for name, key in (("f64", "plain"), ("f32", "mixed"), ("f64-bb", "plain-bb"),
+                      ("f32-bb", "mixed-bb")):
+        H = hier[key][1]
         q, w1, w2 = GpuVector(N), GpuVector(N), GpuVector(N)
         vcopy(b, w1)
         vcopy(b, w2)
@@ -228,7 +232,8 @@ def main():
         vcyc[name] = {"ms": ms, "gdofs": N / (ms * 1e-3) / 1e9}
         log(f"[bench] V-cycle {name}: {ms:.3f} ms, {vcyc[name]['gdofs']:.3f} GDOF/s")
 
-    del H32, H64
+    b_host = b.numpy()
+    del hier, A, b, H
     kern = fine_kernels(args.kernel_reps)
     for k, v in kern.items():
         log(f"[bench] {k}: {v['ms'] * 1e3:.1f} us, {v['gbs']:.0f} GB/s "
@@ -237,7 +242,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle, 1 thread) ...")
-        cpu = cpu_baseline(kappa, n0, levels, args.base, b.numpy())
+        cpu = cpu_baseline(kappa, n0, levels, args.base, b_host)
         log(f"[bench] cpu: setup {cpu['setup_s']:.1f}s solve {cpu['solve_s']:.1f}s "
             f"steps {cpu['steps']}")
 
@@ -258,12 +263,13 @@ def main():
         "data": "synthetic: two-phase random inclusions (SplitMix64 seed 2024)",
         "config": {"workload": f"3D Q1 diffusion {n}^3 cells, N={N}, nnz={nnz}, contrast 1e4, "
                                f"30% inclusions r 3-8 cells, V(2,2) multicolour GS, "
-                               f"base level {args.base}; value = '{fast}' path",
+                               f"base level {args.base}; value = '{fast}' path "
+                               f"(plain = femlisp's FP64 GMG-CG)",
                    "global_batch": 1, "seq_len": N, "parallelism": "single GPU"},
         "paths": {p: {k: v for k, v in r.items()} for p, r in results.items()},
         "speedup_vs_plain": {p: results["plain"]["s"] / results[p]["s"] for p in results},
         "learned_training": train_info,
-        "setup_s": {"plain": setup_plain, "mixed": setup_mixed},
+        "setup_s": setup,
         "vcycle_gdofs": {k: v["gdofs"] for k, v in vcyc.items()},
         "vcycle_ms": {k: v["ms"] for k, v in vcyc.items()},
         "kernels": {k: {"us": v["ms"] * 1e3, "GBps": v["gbs"], "frac": v["gbs"] / 8000.0}
