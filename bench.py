@@ -73,9 +73,16 @@ def main():
     ap.add_argument("--train-problems", type=int, default=256)
     ap.add_argument("--train-epochs", type=int, default=4)
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--paths", default="plain,mixed,plain-bb,mixed-bb,learned",
+                    help="comma-separated solver paths to time")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="time the solves only (no V-cycle, kernel or CPU measurements; "
+                         "no JSON line), e.g. under a profiler")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the fine-level kernels and a copy of known size (counter runs)")
     args = ap.parse_args()
+    if not args.solve_only and "plain" not in args.paths.split(","):
+        ap.error("the JSON line compares with the plain path: include it or use --solve-only")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(--gpus must be 1)", file=sys.stderr)
@@ -151,21 +158,28 @@ def main():
     spec = {"plain": (F.F64, F.ASM_F64, "geometric"), "mixed": (F.F32, F.ASM_MFMA, "geometric"),
             "plain-bb": (F.F64, F.ASM_F64, "blackbox"), "mixed-bb": (F.F32, F.ASM_MFMA, "blackbox")}
     hier, setup = {}, {}
+    paths = args.paths.split(",")
     for name, (prec, mode, tr) in spec.items():
+        if name not in paths and not (name == "mixed" and "learned" in paths):
+            continue
         t = time.perf_counter()
         cs = it.GeometricCS(it.SOR(1.0), 2, 2, args.base, symmetric=True, prec=prec, transfer=tr)
         hier[name] = (cs, it.Hierarchy.lattice(cs, 3, 1, n0, levels, kappa=(kv, n), mode=mode))
         ctx.synchronize()
         setup[name] = time.perf_counter() - t
-    A = hier["plain"][1].top_matrix()
-    b = hier["plain"][1].rhs
+    # the outer iteration's FP64 system: an FP64 hierarchy's top level, else assembled
+    f64 = [hier[k][1] for k in ("plain", "plain-bb") if k in hier]
+    if f64:
+        A, b = f64[0].top_matrix(), f64[0].rhs
+    else:
+        A, b = lat.assemble(3, 1, n, (kv, n), prec=F.F64, mode=F.ASM_F64)
     N, nnz = A.nrows, A.nnz
     bnorm = norm(b)
     log(f"[bench] setup " + " ".join(f"{k} {v:.2f}s" for k, v in setup.items()) +
         f"  N={N} nnz={nnz}")
 
     net, train_info = None, None
-    if not args.no_learned:
+    if not args.no_learned and "learned" in args.paths.split(","):
         net = learned.UNet(2, 3)
         if os.path.exists(WEIGHTS):
             net.load(WEIGHTS)
@@ -195,7 +209,8 @@ def main():
         return time.perf_counter() - t0, out
 
     results = {}
-    for path in ("plain", "mixed", "plain-bb", "mixed-bb") + (() if net is None else ("learned",)):
+    wanted = [p for p in args.paths.split(",") if p != "learned" or net is not None]
+    for path in wanted:
         for _ in range(args.warmup):
             run(path)
         times = []
@@ -210,6 +225,9 @@ def main():
                          "x": out["solution"].numpy(), "fell_back": out["fell_back"]}
         log(f"[bench] {path}: {results[path]['s']:.4f}s median, {out['steps']} iterations, "
             f"true rel residual {rel:.2e}" + (" (fell back)" if out["fell_back"] else ""))
+    if args.solve_only:
+        log("[bench] solve-only done")
+        return
     x64 = results["plain"]["x"]
     for p in results:
         results[p]["rel_diff_vs_plain"] = float(np.linalg.norm(results[p].pop("x") - x64) /
@@ -219,6 +237,8 @@ def main():
     vcyc = {}
     for name, key in (("f64", "plain"), ("f32", "mixed"), ("f64-bb", "plain-bb"),
                       ("f32-bb", "mixed-bb")):
+        if key not in hier:
+            continue
         H = hier[key][1]
         q, w1, w2 = GpuVector(N), GpuVector(N), GpuVector(N)
         vcopy(b, w1)
