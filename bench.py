@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--paths", default="plain,mixed,plain-bb,mixed-bb,learned",
                     help="comma-separated solver paths to time")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every V-cycle kernel separately instead of replaying HIP graphs "
+                         "(for kernel tracing: graph replays are not followed by every tracer)")
     ap.add_argument("--solve-only", action="store_true",
                     help="time the solves only (no V-cycle, kernel or CPU measurements; "
                          "no JSON line), e.g. under a profiler")
@@ -165,6 +168,8 @@ def main():
         t = time.perf_counter()
         cs = it.GeometricCS(it.SOR(1.0), 2, 2, args.base, symmetric=True, prec=prec, transfer=tr)
         hier[name] = (cs, it.Hierarchy.lattice(cs, 3, 1, n0, levels, kappa=(kv, n), mode=mode))
+        if args.no_graphs:
+            hier[name][1].use_graphs(False)
         ctx.synchronize()
         setup[name] = time.perf_counter() - t
     # the outer iteration's FP64 system: an FP64 hierarchy's top level, else assembled

@@ -388,6 +388,45 @@ __device__ __forceinline__ T ld_stream(const T *p) {
   return __builtin_nontemporal_load(p);
 }
 
+// acc + sum_k (alpha x[col_k]) val_k over the w entries of one SELL row (stride 64), in
+// order.  Chunks of 8: the 8 values and columns, then the 8 gathers, then the 8
+// products, so a chunk costs two memory latencies rather than two per entry.
+template <class T>
+__device__ __forceinline__ T sell_row(const T *__restrict__ vp, const int *__restrict__ cp, int w,
+                                      T alpha, const T *__restrict__ x, T acc) {
+  constexpr int C = 8;
+  int k = 0;
+  for (; k + C <= w; k += C) {
+    T a[C], xv[C];
+    int c[C];
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      a[j] = ld_stream(vp + (long)(k + j) * 64);
+      c[j] = ld_stream(cp + (long)(k + j) * 64);
+    }
+#pragma unroll
+    for (int j = 0; j < C; j++) xv[j] = x[c[j]];
+#pragma unroll
+    for (int j = 0; j < C; j++) acc += (alpha * xv[j]) * a[j];
+  }
+  int rem = w - k;  // uniform over the slice
+  if (rem > 0) {
+    T a[C], xv[C];
+    int c[C];
+#pragma unroll
+    for (int j = 0; j < C; j++) {
+      a[j] = j < rem ? ld_stream(vp + (long)(k + j) * 64) : (T)0;
+      c[j] = j < rem ? ld_stream(cp + (long)(k + j) * 64) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < C; j++) xv[j] = j < rem ? x[c[j]] : (T)0;
+#pragma unroll
+    for (int j = 0; j < C; j++)
+      if (j < rem) acc += (alpha * xv[j]) * a[j];
+  }
+  return acc;
+}
+
 // out[row] = beta*z[row] + sum_k (alpha*x[col]) * val  over one SELL structure
 template <class T>
 __global__ __launch_bounds__(256) void k_sell_spmv(int nsub, int nslices, const int *__restrict__ rows,
@@ -406,14 +445,7 @@ __global__ __launch_bounds__(256) void k_sell_spmv(int nsub, int nslices, const 
   if (li >= nsub) return;
   int row = rows ? rows[li] : li;
   T acc = (beta == (T)0) ? (T)0 : beta * z[row];
-  const T *vp = val + base + lane;
-  const int *cp = col + base + lane;
-#pragma unroll 4
-  for (int k = 0; k < w; k++) {
-    T a = ld_stream(vp + (long)k * 64);
-    int c = ld_stream(cp + (long)k * 64);
-    acc += (alpha * x[c]) * a;
-  }
+  acc = sell_row<T>(val + base + lane, col + base + lane, w, alpha, x, acc);
   out[row] = acc;
 }
 
@@ -436,38 +468,49 @@ struct Q1K {
 
 // acc + sum over interior neighbours j of row i (ascending j) of (coef x_j) A_ij
 // with A_ij = kt[|dz|*4 + |dy|*2 + |dx|] * (sum of alpha over the cells shared
-// by i and j); *dii receives A_ii.  Row i = (z,y,x) must be interior.
+// by i and j); *dii receives A_ii.  Row i = (z,y,x) must be interior.  All
+// loads are issued before the first product: a Dirichlet column (eliminated,
+// A_ij = 0) reads x_i and adds a zero product, so the sum keeps its order and
+// the row waits for one memory latency instead of one per neighbour.
 template <class T>
 __device__ __forceinline__ T q1_row(int M, int n, long i, int z, int y, int x,
                                     const T *__restrict__ alpha, const Q1K &kt, T acc, T coef,
                                     const T *__restrict__ xv, T *dii) {
   T a[2][2][2];
+#pragma unroll
   for (int p = 0; p < 2; p++)
+#pragma unroll
     for (int q = 0; q < 2; q++)
+#pragma unroll
       for (int s = 0; s < 2; s++)
         a[p][q][s] = alpha[((long)(z - 1 + p) * n + (y - 1 + q)) * n + (x - 1 + s)];
   long M2 = (long)M * M;
-  for (int dz = -1; dz <= 1; dz++) {
-    int zz = z + dz;
-    if (zz == 0 || zz == M - 1) continue;  // Dirichlet column: eliminated
+  bool lz[3] = {z - 1 != 0, true, z + 1 != M - 1}, ly[3] = {y - 1 != 0, true, y + 1 != M - 1},
+       lx[3] = {x - 1 != 0, true, x + 1 != M - 1};
+  T xn[27];
+#pragma unroll
+  for (int k = 0; k < 27; k++) {
+    int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+    bool live = lz[dz + 1] && ly[dy + 1] && lx[dx + 1];
+    xn[k] = xv[live ? i + dz * M2 + dy * M + dx : i];
+  }
+#pragma unroll
+  for (int k = 0; k < 27; k++) {
+    int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
     int pz0 = dz > 0 ? 1 : 0, pz1 = dz < 0 ? 0 : 1;
-    for (int dy = -1; dy <= 1; dy++) {
-      int yy = y + dy;
-      if (yy == 0 || yy == M - 1) continue;
-      int py0 = dy > 0 ? 1 : 0, py1 = dy < 0 ? 0 : 1;
-      for (int dx = -1; dx <= 1; dx++) {
-        int xx = x + dx;
-        if (xx == 0 || xx == M - 1) continue;
-        int px0 = dx > 0 ? 1 : 0, px1 = dx < 0 ? 0 : 1;
-        T s = (T)0;
-        for (int p = pz0; p <= pz1; p++)
-          for (int q = py0; q <= py1; q++)
-            for (int r = px0; r <= px1; r++) s += a[p][q][r];
-        T aij = (T)kt.k[(dz != 0) * 4 + (dy != 0) * 2 + (dx != 0)] * s;
-        if (dz == 0 && dy == 0 && dx == 0) *dii = aij;
-        acc += (coef * xv[i + dz * M2 + dy * M + dx]) * aij;
-      }
-    }
+    int py0 = dy > 0 ? 1 : 0, py1 = dy < 0 ? 0 : 1;
+    int px0 = dx > 0 ? 1 : 0, px1 = dx < 0 ? 0 : 1;
+    T s = (T)0;
+#pragma unroll
+    for (int p = pz0; p <= pz1; p++)
+#pragma unroll
+      for (int q = py0; q <= py1; q++)
+#pragma unroll
+        for (int r = px0; r <= px1; r++) s += a[p][q][r];
+    T aij = (T)kt.k[(dz != 0) * 4 + (dy != 0) * 2 + (dx != 0)] * s;
+    if (k == 13) *dii = aij;
+    bool live = lz[dz + 1] && ly[dy + 1] && lx[dx + 1];
+    acc += (coef * xn[k]) * (live ? aij : (T)0);
   }
   return acc;
 }
@@ -536,25 +579,27 @@ __global__ void k_st27_fill(int M, const int *__restrict__ rowptr, const int *__
 }
 
 // acc + sum over the lattice neighbours j of row i (ascending j) of
-// (coef x_j) A_ij; the same entries in the same order as the CSR row
+// (coef x_j) A_ij; the same entries in the same order as the CSR row.  The 27
+// values and neighbour x are loaded first (a neighbour outside the lattice has
+// a stored 0 and reads x_i), then summed in order: one memory latency per row.
 template <class T>
 __device__ __forceinline__ T st27_row(int M, long N, long i, int z, int y, int x,
                                       const T *__restrict__ val27, T acc, T coef,
                                       const T *__restrict__ xv, T *dii) {
   long M2 = (long)M * M;
-  bool inner = z > 0 && y > 0 && x > 0 && z < M - 1 && y < M - 1 && x < M - 1;
-  for (int dz = -1; dz <= 1; dz++)
-    for (int dy = -1; dy <= 1; dy++)
-      for (int dx = -1; dx <= 1; dx++) {
-        if (!inner) {
-          int zz = z + dz, yy = y + dy, xx = x + dx;
-          if (zz < 0 || zz >= M || yy < 0 || yy >= M || xx < 0 || xx >= M) continue;
-        }
-        int k = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1);
-        T a = val27[k * N + i];
-        if (k == 13) *dii = a;
-        acc += (coef * xv[i + dz * M2 + dy * M + dx]) * a;
-      }
+  bool lz[3] = {z > 0, true, z < M - 1}, ly[3] = {y > 0, true, y < M - 1},
+       lx[3] = {x > 0, true, x < M - 1};
+  T av[27], xn[27];
+#pragma unroll
+  for (int k = 0; k < 27; k++) {
+    int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+    bool live = lz[dz + 1] && ly[dy + 1] && lx[dx + 1];
+    av[k] = val27[k * N + i];
+    xn[k] = xv[live ? i + dz * M2 + dy * M + dx : i];
+  }
+  *dii = av[13];
+#pragma unroll
+  for (int k = 0; k < 27; k++) acc += (coef * xn[k]) * av[k];
   return acc;
 }
 
@@ -700,15 +745,7 @@ __global__ __launch_bounds__(256) void k_sor_colour(int nsub, int nslices, const
   int w = (int)((slice_ptr[s + 1] - base) >> 6);
   if (li >= nsub) return;
   int row = rows[li];
-  T corr = b[row];
-  const T *vp = val + base + lane;
-  const int *cp = col + base + lane;
-#pragma unroll 4
-  for (int k = 0; k < w; k++) {
-    T a = ld_stream(vp + (long)k * 64);
-    int c = ld_stream(cp + (long)k * 64);
-    corr += (-x[c]) * a;
-  }
+  T corr = sell_row<T>(val + base + lane, col + base + lane, w, (T)-1, x, b[row]);
   corr = corr / diag[row];
   r[row] = corr;
   x[row] += omega * corr;
