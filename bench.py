@@ -257,6 +257,49 @@ def main():
         vcyc[name] = {"ms": ms, "gdofs": N / (ms * 1e-3) / 1e9}
         log(f"[bench] V-cycle {name}: {ms:.3f} ms, {vcyc[name]['gdofs']:.3f} GDOF/s")
 
+    # ---- roofline of the value path's dominant kernel -------------------------------
+    # k_st27_sor<float>: the colour launches of the stencil levels below the top, in the
+    # sequence one V-cycle makes (2 forward + 2 backward sweeps per level, 8 colours each).
+    # Events bracket the whole sequence, so the average includes the dispatch gap between
+    # back-to-back launches; large copies first let the host queue up ahead of the GPU.
+    roof = None
+    if "mixed-bb" in hier:
+        Hb = hier["mixed-bb"][1]
+        # levels whose largest colour exceeds one workgroup (smaller ones sweep in one launch)
+        mats = [M for M in (Hb.level_matrix(l) for l in range(args.base + 1, levels - 1))
+                if M.storage() == F.STORAGE_STENCIL and
+                ((round(M.nrows ** (1 / 3)) + 1) // 2) ** 3 > 1024]
+        vecs = [(GpuVector(M.nrows, F.F32), GpuVector.from_array(np.ones(M.nrows), F.F32),
+                 GpuVector(M.nrows, F.F32)) for M in mats]
+
+        def sweeps():
+            for M, (x, bv, r) in zip(mats, vecs):
+                for bw in (0, 0, 1, 1):
+                    check(lib().flgpu_sor_iterate(M.h, 1.0, bw, x.h, bv.h, r.h))
+        sweeps()
+        big_a, big_b = GpuVector(1 << 26), GpuVector(1 << 26)
+        for _ in range(24):
+            vcopy(big_a, big_b)
+        reps = 20
+        ctx.timer_start()
+        for _ in range(reps):
+            sweeps()
+        ms = ctx.timer_stop()
+        del big_a, big_b
+        launches = reps * 4 * 8 * len(mats)
+        total = reps * sum(4 * (27 + 5) * M.nrows * 4 for M in mats)
+        us = ms * 1e3 / launches
+        roof = {"bound": "hbm", "kernel": "k_st27_sor<float> (one colour of a <sor> sweep on the "
+                                          "stencil levels " + ", ".join(
+                                              f"{round(M.nrows ** (1 / 3)) - 1}^3" for M in mats)
+                                          + ", as in one V-cycle)",
+                "achieved": total / launches / (us * 1e-6) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                "frac": total / launches / (us * 1e-6) / 1e9 / 8000.0, "traffic": None,
+                "launch_us": us, "bytes_per_launch": total / launches}
+        log(f"[bench] roofline k_st27_sor<float>: {us:.2f} us per launch, "
+            f"{roof['achieved']:.0f} GB/s ({100 * roof['frac']:.1f}% of 8 TB/s)")
+        del mats, vecs, Hb
+
     b_host = b.numpy()
     del hier, A, b, H
     kern = fine_kernels(args.kernel_reps)
@@ -299,10 +342,10 @@ def main():
         "vcycle_ms": {k: v["ms"] for k, v in vcyc.items()},
         "kernels": {k: {"us": v["ms"] * 1e3, "GBps": v["gbs"], "frac": v["gbs"] / 8000.0}
                     for k, v in kern.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_q1_sor<float> (fine-level <sor> sweep, "
-                                               "per-cell form)",
-                     "achieved": dom["gbs"], "peak": 8000.0, "unit": "GB/s",
-                     "frac": dom["gbs"] / 8000.0, "traffic": None},
+        "roofline": roof if roof is not None else {
+            "bound": "hbm", "kernel": "k_q1_sor<float> (fine-level <sor> sweep, per-cell form)",
+            "achieved": dom["gbs"], "peak": 8000.0, "unit": "GB/s",
+            "frac": dom["gbs"] / 8000.0, "traffic": None},
         "cpu_baseline": None if cpu is None else {
             "value": cpu["solve_s"], "unit": "s", "cores": 1, "kind": "port",
             "sample": f"full plain FP64 GMG-CG solve of the same {n}^3 system "

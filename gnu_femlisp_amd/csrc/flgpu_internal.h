@@ -177,6 +177,9 @@ void spmv_t(const flgpu_mat *A, double alpha, const void *x, double beta,
 void jacobi_iterate(const flgpu_mat *A, double damp, void *x, void *r);
 void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x,
                const void *b, void *r);
+// `steps` sweeps (small stencil levels: one launch for all of them)
+void sor_sweeps(const flgpu_mat *A, double omega, int steps, int backwards, void *x,
+                const void *b, void *r);
 
 // lattices (flgpu_lattice.hip)
 flgpu_mat *lattice_assemble(flgpu_ctx *ctx, int dim, int order, int n, const flgpu_vec *kappa,

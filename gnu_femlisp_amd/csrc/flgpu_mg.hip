@@ -181,7 +181,7 @@ static void smooth(flgpu_mg *mg, int l, bool post) {
       if (steps > 1) spmv(L.A, -1.0, x, 1.0, b, r);  // residual-before and not residual-after
     }
   } else {
-    for (int k = 0; k < steps; k++) sor_sweep(L.A, p.omega, post && p.symmetric, x, b, r);
+    sor_sweeps(L.A, p.omega, steps, post && p.symmetric, x, b, r);
   }
   L.residual_p = false;  // residual-after nil
 }
