@@ -265,10 +265,8 @@ def main():
     roof = None
     if "mixed-bb" in hier:
         Hb = hier["mixed-bb"][1]
-        # levels whose largest colour exceeds one workgroup (smaller ones sweep in one launch)
         mats = [M for M in (Hb.level_matrix(l) for l in range(args.base + 1, levels - 1))
-                if M.storage() == F.STORAGE_STENCIL and
-                ((round(M.nrows ** (1 / 3)) + 1) // 2) ** 3 > 1024]
+                if M.storage() == F.STORAGE_STENCIL]
         vecs = [(GpuVector(M.nrows, F.F32), GpuVector.from_array(np.ones(M.nrows), F.F32),
                  GpuVector(M.nrows, F.F32)) for M in mats]
 

@@ -177,7 +177,7 @@ void spmv_t(const flgpu_mat *A, double alpha, const void *x, double beta,
 void jacobi_iterate(const flgpu_mat *A, double damp, void *x, void *r);
 void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x,
                const void *b, void *r);
-// `steps` sweeps (small stencil levels: one launch for all of them)
+// `steps` sweeps
 void sor_sweeps(const flgpu_mat *A, double omega, int steps, int backwards, void *x,
                 const void *b, void *r);
 

@@ -645,27 +645,6 @@ __global__ __launch_bounds__(256) void k_st27_sor(int M, int cz, int cy, int cx,
   st27_sor_row<T>(M, cz, cy, cx, nx, ny, t, val27, omega, x, b, r);
 }
 
-// `steps` whole <sor> sweeps of a small lattice level in one workgroup: the colours run in
-// the order of the lattice colouring (reversed when backwards), a barrier between colours
-// makes each colour see the previous one's x, exactly as separate launches do
-constexpr int kFusedThreads = 1024;
-template <class T>
-__global__ __launch_bounds__(kFusedThreads) void k_st27_sor_fused(int M, int steps, int backwards,
-                                                                  const T *__restrict__ val27,
-                                                                  T omega, T *x, const T *b,
-                                                                  T *r) {
-  for (int s = 0; s < steps; s++)
-    for (int k = 0; k < 8; k++) {
-      int c = backwards ? 7 - k : k;
-      int cz = c >> 2, cy = (c >> 1) & 1, cx = c & 1;
-      int nx = (M - cx + 1) / 2, ny = (M - cy + 1) / 2, nz = (M - cz + 1) / 2;
-      long cnt = (long)nx * ny * nz;
-      for (long t = threadIdx.x; t < cnt; t += kFusedThreads)
-        st27_sor_row<T>(M, cz, cy, cx, nx, ny, t, val27, omega, x, b, r);
-      __syncthreads();
-    }
-}
-
 bool is_q1_lattice(const flgpu_mat *A) {
   return A->lat_dim == 3 && A->lat_order == 1 &&
          (long)A->nrows == (long)(A->lat_n + 1) * (A->lat_n + 1) * (A->lat_n + 1) &&
@@ -831,24 +810,9 @@ void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const v
   }
 }
 
-// `steps` <sor> sweeps; a stencil level whose largest colour has at most one row per thread
-// of a workgroup runs them all in one launch instead of 8 launches per sweep
+// `steps` <sor> sweeps (the product-iterator of a smoother, linit.lisp:94-120)
 void sor_sweeps(const flgpu_mat *A, double omega, int steps, int backwards, void *x, const void *b,
                 void *r) {
-  if (steps <= 0) return;
-  int M = A->lat_n + 1;
-  if (A->val27 && !A->alpha && (long)((M + 1) / 2) * ((M + 1) / 2) * ((M + 1) / 2) <= kFusedThreads) {
-    if (A->prec == FLGPU_F64)
-      k_st27_sor_fused<double><<<1, kFusedThreads, 0, A->ctx->stream>>>(
-          M, steps, backwards, (const double *)A->val27, omega, (double *)x, (const double *)b,
-          (double *)r);
-    else
-      k_st27_sor_fused<float><<<1, kFusedThreads, 0, A->ctx->stream>>>(
-          M, steps, backwards, (const float *)A->val27, (float)omega, (float *)x,
-          (const float *)b, (float *)r);
-    FLGPU_HIP(hipGetLastError());
-    return;
-  }
   for (int k = 0; k < steps; k++) sor_sweep(A, omega, backwards, x, b, r);
 }
 

@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libflgpu.so")
+# FLGPU_LIB selects another build of the library (e.g. a variant for an A/B timing)
+LIB_PATH = os.environ.get("FLGPU_LIB") or os.path.join(HERE, "libflgpu.so")
 
 F64, F32 = 0, 1
 ASM_F64, ASM_MFMA = 0, 1
