@@ -73,7 +73,8 @@ def main():
     ap.add_argument("--train-problems", type=int, default=256)
     ap.add_argument("--train-epochs", type=int, default=4)
     ap.add_argument("--kernel-reps", type=int, default=20)
-    ap.add_argument("--paths", default="plain,mixed,plain-bb,mixed-bb,learned",
+    ap.add_argument("--paths", default="plain,mixed,plain-bb,mixed-bb,plain-bb-jac,mixed-bb-jac,"
+                                       "learned",
                     help="comma-separated solver paths to time")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every V-cycle kernel separately instead of replaying HIP graphs "
@@ -158,22 +159,32 @@ def main():
     # plain: femlisp's FP64 GMG-CG (geometric transfers, re-discretised levels);
     # mixed: FP32 V-cycle (BF16-MFMA assembled) in FP64 FCG; "-bb": the same with
     # operator-dependent (black-box) transfers and Galerkin coarse levels
-    spec = {"plain": (F.F64, F.ASM_F64, "geometric"), "mixed": (F.F32, F.ASM_MFMA, "geometric"),
-            "plain-bb": (F.F64, F.ASM_F64, "blackbox"), "mixed-bb": (F.F32, F.ASM_MFMA, "blackbox")}
+    # operator-dependent (black-box) transfers and Galerkin coarse levels; "-jac": damped
+    # <jacobi> V(4,4), omega 0.7, one launch per step instead of 8 colour launches per sweep
+    spec = {"plain": (F.F64, F.ASM_F64, "geometric", "gs"),
+            "mixed": (F.F32, F.ASM_MFMA, "geometric", "gs"),
+            "plain-bb": (F.F64, F.ASM_F64, "blackbox", "gs"),
+            "mixed-bb": (F.F32, F.ASM_MFMA, "blackbox", "gs"),
+            "plain-bb-jac": (F.F64, F.ASM_F64, "blackbox", "jac"),
+            "mixed-bb-jac": (F.F32, F.ASM_MFMA, "blackbox", "jac")}
     hier, setup = {}, {}
     paths = args.paths.split(",")
-    for name, (prec, mode, tr) in spec.items():
+    for name, (prec, mode, tr, sm) in spec.items():
         if name not in paths and not (name == "mixed" and "learned" in paths):
             continue
         t = time.perf_counter()
-        cs = it.GeometricCS(it.SOR(1.0), 2, 2, args.base, symmetric=True, prec=prec, transfer=tr)
+        if sm == "jac":
+            cs = it.GeometricCS(it.Jacobi(0.7), 4, 4, args.base, prec=prec, transfer=tr)
+        else:
+            cs = it.GeometricCS(it.SOR(1.0), 2, 2, args.base, symmetric=True, prec=prec,
+                                transfer=tr)
         hier[name] = (cs, it.Hierarchy.lattice(cs, 3, 1, n0, levels, kappa=(kv, n), mode=mode))
         if args.no_graphs:
             hier[name][1].use_graphs(False)
         ctx.synchronize()
         setup[name] = time.perf_counter() - t
     # the outer iteration's FP64 system: an FP64 hierarchy's top level, else assembled
-    f64 = [hier[k][1] for k in ("plain", "plain-bb") if k in hier]
+    f64 = [hier[k][1] for k in ("plain", "plain-bb", "plain-bb-jac") if k in hier]
     if f64:
         A, b = f64[0].top_matrix(), f64[0].rhs
     else:
@@ -241,7 +252,8 @@ def main():
     # ---- V-cycle GDOF/s --------------------------------------------------------------
     vcyc = {}
     for name, key in (("f64", "plain"), ("f32", "mixed"), ("f64-bb", "plain-bb"),
-                      ("f32-bb", "mixed-bb")):
+                      ("f32-bb", "mixed-bb"), ("f64-bb-jac", "plain-bb-jac"),
+                      ("f32-bb-jac", "mixed-bb-jac")):
         if key not in hier:
             continue
         H = hier[key][1]

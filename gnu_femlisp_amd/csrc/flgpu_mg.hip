@@ -29,6 +29,7 @@ struct MGLevel {
   flgpu_mat *A = nullptr;  // owned
   flgpu_mat *I = nullptr;  // owned: interpolation level -> level+1 (transpose attached)
   void *sol = nullptr, *rhs = nullptr, *res = nullptr;  // owned work vectors
+  void *tmp = nullptr;  // <jacobi> steps: the other half of x's ping-pong (smoother Jacobi)
   bool residual_p = false;
   double *cinv = nullptr;  // base level: dense inverse (FP64, row-major)
   double *ctmp = nullptr;
@@ -174,7 +175,11 @@ static void smooth(flgpu_mg *mg, int l, bool post) {
   if (steps <= 0) return;
   MGLevel &L = mg->lv[l];
   void *x = lsol(mg, l), *b = lrhs(mg, l), *r = lres(mg, l);
-  if (p.smoother == FLGPU_SMOOTH_JACOBI) {
+  if (p.smoother == FLGPU_SMOOTH_JACOBI && b != r && L.tmp) {
+    // residual recomputed inside each step; only femlisp's aliased rhs = residual (below)
+    // needs the in-place form that divides the residual vector itself
+    jacobi_steps(L.A, p.omega, steps, x, b, L.tmp);
+  } else if (p.smoother == FLGPU_SMOOTH_JACOBI) {
     ensure_residual(mg, l);  // <jacobi> has residual-before t
     for (int k = 0; k < steps; k++) {
       jacobi_iterate(L.A, p.omega, x, r);
@@ -319,6 +324,7 @@ static void mg_alloc_work(flgpu_mg *mg) {
     vfill(mg->ctx, L.N, mg->p.prec, L.sol, 0.0);
     vfill(mg->ctx, L.N, mg->p.prec, L.rhs, 0.0);
     vfill(mg->ctx, L.N, mg->p.prec, L.res, 0.0);
+    if (mg->p.smoother == FLGPU_SMOOTH_JACOBI) L.tmp = dev_alloc(b);
   }
   long N = mg->lv[mg->top].N;
   size_t b = prec_bytes(mg->p.prec) * N;
@@ -343,6 +349,7 @@ void mg_free(flgpu_mg *mg) {
     dev_free(L.sol);
     dev_free(L.rhs);
     dev_free(L.res);
+    dev_free(L.tmp);
     dev_free(L.cinv);
     dev_free(L.ctmp);
   }

@@ -724,6 +724,101 @@ __global__ void k_jacobi(int n, T damp, const T *__restrict__ diag, T *x, T *r) 
   x[i] += damp * ri;
 }
 
+// One <jacobi> step of a product-iterator (linit.lisp:94-120,220-250) with the residual it
+// needs recomputed in the same pass, out of place: r_i = b_i - (A x)_i (the sum of spmv with
+// alpha = -1, beta = 1), r_i /= A_ii, xo_i = x_i + damp r_i.  Per storage form:
+template <class T>
+__global__ __launch_bounds__(256) void k_q1_jacobi(int M, int n, const T *__restrict__ alpha,
+                                                   Q1K kt, const T *__restrict__ diag, T damp,
+                                                   const T *__restrict__ x,
+                                                   const T *__restrict__ b, T *__restrict__ xo) {
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  long i = (long)lb * blockDim.x + threadIdx.x;
+  long N = (long)M * M * M;
+  if (i >= N) return;
+  int xx = (int)(i % M), yy = (int)((i / M) % M), zz = (int)(i / ((long)M * M));
+  T acc = b[i];
+  if (xx == 0 || yy == 0 || zz == 0 || xx == M - 1 || yy == M - 1 || zz == M - 1) {
+    acc += (-x[i]) * (T)1;
+  } else {
+    T d;
+    acc = q1_row<T>(M, n, i, zz, yy, xx, alpha, kt, acc, (T)-1, x, &d);
+  }
+  xo[i] = x[i] + damp * (acc / diag[i]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_st27_jacobi(int M, const T *__restrict__ val27,
+                                                     const T *__restrict__ diag, T damp,
+                                                     const T *__restrict__ x,
+                                                     const T *__restrict__ b, T *__restrict__ xo) {
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  long i = (long)lb * blockDim.x + threadIdx.x;
+  long N = (long)M * M * M;
+  if (i >= N) return;
+  int xx = (int)(i % M), yy = (int)((i / M) % M), zz = (int)(i / ((long)M * M));
+  T d;
+  T acc = st27_row<T>(M, N, i, zz, yy, xx, val27, b[i], (T)-1, x, &d);
+  xo[i] = x[i] + damp * (acc / diag[i]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_sell_jacobi(int nsub, int nslices,
+                                                     const long *__restrict__ slice_ptr,
+                                                     const int *__restrict__ col,
+                                                     const T *__restrict__ val,
+                                                     const T *__restrict__ diag, T damp,
+                                                     const T *__restrict__ x,
+                                                     const T *__restrict__ b, T *__restrict__ xo) {
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  int s = (int)lb * 4 + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  int lane = threadIdx.x & 63;
+  int row = s * 64 + lane;
+  long base = slice_ptr[s];
+  int w = (int)((slice_ptr[s + 1] - base) >> 6);
+  if (row >= nsub) return;
+  T acc = sell_row<T>(val + base + lane, col + base + lane, w, (T)-1, x, b[row]);
+  xo[row] = x[row] + damp * (acc / diag[row]);
+}
+
+template <class T>
+static void jacobi_step(const flgpu_mat *A, T damp, const T *x, const T *b, T *xo) {
+  hipStream_t st = A->ctx->stream;
+  if (A->alpha) {
+    int M = A->lat_n + 1;
+    unsigned g = xcd_grid(grid_for((long)M * M * M, 256));
+    k_q1_jacobi<T><<<g, 256, 0, st>>>(M, A->lat_n, (const T *)A->alpha, q1k(A),
+                                      (const T *)A->diag, damp, x, b, xo);
+  } else if (A->val27) {
+    unsigned g = xcd_grid(grid_for(A->nrows, 256));
+    k_st27_jacobi<T><<<g, 256, 0, st>>>(A->lat_n + 1, (const T *)A->val27, (const T *)A->diag,
+                                        damp, x, b, xo);
+  } else {
+    const Sell &s = A->sell;
+    if (s.nrows == 0) return;
+    unsigned g = xcd_grid(grid_for(s.nslices, 4));
+    k_sell_jacobi<T><<<g, 256, 0, st>>>(s.nrows, s.nslices, s.slice_ptr, s.col, (const T *)s.val,
+                                        (const T *)A->diag, damp, x, b, xo);
+  }
+  FLGPU_HIP(hipGetLastError());
+}
+
+// `steps` <jacobi> steps, each from a freshly computed residual, x ping-ponged through tmp
+// (n values); the residual vector of the iteration is not touched (left stale)
+void jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp) {
+  if (steps <= 0 || A->nrows == 0) return;
+  FLGPU_REQUIRE(A->nrows == A->ncols, "<jacobi> needs a square matrix");
+  for (int k = 0; k < steps; k++) {
+    void *src = (k % 2 == 0) ? x : tmp, *dst = (k % 2 == 0) ? tmp : x;
+    if (A->prec == FLGPU_F64)
+      jacobi_step<double>(A, damp, (const double *)src, (const double *)b, (double *)dst);
+    else
+      jacobi_step<float>(A, (float)damp, (const float *)src, (const float *)b, (float *)dst);
+  }
+  if (steps % 2) vcopy(A->ctx, A->nrows, A->prec, tmp, A->prec, x);
+}
+
 void jacobi_iterate(const flgpu_mat *A, double damp, void *x, void *r) {
   int n = A->nrows;
   if (n == 0) return;
