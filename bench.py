@@ -270,45 +270,38 @@ def main():
         log(f"[bench] V-cycle {name}: {ms:.3f} ms, {vcyc[name]['gdofs']:.3f} GDOF/s")
 
     # ---- roofline of the value path's dominant kernel -------------------------------
-    # k_st27_sor<float>: the colour launches of the stencil levels below the top, in the
-    # sequence one V-cycle makes (2 forward + 2 backward sweeps per level, 8 colours each).
-    # Events bracket the whole sequence, so the average includes the dispatch gap between
-    # back-to-back launches; large copies first let the host queue up ahead of the GPU.
+    # k_q1_jacobi<float>: one damped <jacobi> step on the fine level (per-cell form) of the
+    # mixed-bb-jac hierarchy, 8 per V-cycle.  Events bracket 8 x reps back-to-back launches,
+    # so the average includes the dispatch gap between launches; large copies first let the
+    # host queue up ahead of the GPU.  Algorithmic bytes per launch: the cell factors once,
+    # x, b and the diagonal read, the new x written.
     roof = None
-    if "mixed-bb" in hier:
-        Hb = hier["mixed-bb"][1]
-        mats = [M for M in (Hb.level_matrix(l) for l in range(args.base + 1, levels - 1))
-                if M.storage() == F.STORAGE_STENCIL]
-        vecs = [(GpuVector(M.nrows, F.F32), GpuVector.from_array(np.ones(M.nrows), F.F32),
-                 GpuVector(M.nrows, F.F32)) for M in mats]
-
-        def sweeps():
-            for M, (x, bv, r) in zip(mats, vecs):
-                for bw in (0, 0, 1, 1):
-                    check(lib().flgpu_sor_iterate(M.h, 1.0, bw, x.h, bv.h, r.h))
-        sweeps()
+    if "mixed-bb-jac" in hier:
+        At = hier["mixed-bb-jac"][1].top_matrix()
+        assert At.storage() == F.STORAGE_CELL
+        xj = GpuVector(At.nrows, F.F32)
+        bj = GpuVector.from_array(b.numpy(), F.F32)
+        tj = GpuVector(At.nrows, F.F32)
+        check(lib().flgpu_jacobi_steps(At.h, 0.7, 8, xj.h, bj.h, tj.h))
         big_a, big_b = GpuVector(1 << 26), GpuVector(1 << 26)
         for _ in range(24):
             vcopy(big_a, big_b)
-        reps = 20
+        reps = 25
         ctx.timer_start()
         for _ in range(reps):
-            sweeps()
+            check(lib().flgpu_jacobi_steps(At.h, 0.7, 8, xj.h, bj.h, tj.h))
         ms = ctx.timer_stop()
         del big_a, big_b
-        launches = reps * 4 * 8 * len(mats)
-        total = reps * sum(4 * (27 + 5) * M.nrows * 4 for M in mats)
-        us = ms * 1e3 / launches
-        roof = {"bound": "hbm", "kernel": "k_st27_sor<float> (one colour of a <sor> sweep on the "
-                                          "stencil levels " + ", ".join(
-                                              f"{round(M.nrows ** (1 / 3)) - 1}^3" for M in mats)
-                                          + ", as in one V-cycle)",
-                "achieved": total / launches / (us * 1e-6) / 1e9, "peak": 8000.0, "unit": "GB/s",
-                "frac": total / launches / (us * 1e-6) / 1e9 / 8000.0, "traffic": None,
-                "launch_us": us, "bytes_per_launch": total / launches}
-        log(f"[bench] roofline k_st27_sor<float>: {us:.2f} us per launch, "
-            f"{roof['achieved']:.0f} GB/s ({100 * roof['frac']:.1f}% of 8 TB/s)")
-        del mats, vecs, Hb
+        us = ms * 1e3 / (8 * reps)
+        byt = cell_sor_bytes(n ** 3, At.nrows, 4)  # cells + 4 vector passes
+        gbs = byt / (us * 1e-6) / 1e9
+        roof = {"bound": "hbm", "kernel": f"k_q1_jacobi<float> (fine-level <jacobi> step, "
+                                          f"per-cell form, {n}^3)",
+                "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
+                "traffic": None, "launch_us": us, "bytes_per_launch": byt}
+        log(f"[bench] roofline k_q1_jacobi<float>: {us:.2f} us per launch, {gbs:.0f} GB/s "
+            f"({100 * gbs / 8000:.1f}% of 8 TB/s)")
+        del At, xj, bj, tj
 
     b_host = b.numpy()
     del hier, A, b, H

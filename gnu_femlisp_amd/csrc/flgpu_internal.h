@@ -177,8 +177,10 @@ void spmv_t(const flgpu_mat *A, double alpha, const void *x, double beta,
 void jacobi_iterate(const flgpu_mat *A, double damp, void *x, void *r);
 void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x,
                const void *b, void *r);
-// `steps` <jacobi> steps with fresh residuals, x ping-ponged through tmp
-void jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp);
+// `steps` <jacobi> steps with fresh residuals, x ping-ponged through tmp; x_zero: x is zero
+// and need not be read (the first step is x = damp b / diag)
+void jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp,
+                  bool x_zero = false);
 // `steps` sweeps
 void sor_sweeps(const flgpu_mat *A, double omega, int steps, int backwards, void *x,
                 const void *b, void *r);

@@ -31,6 +31,8 @@ struct MGLevel {
   void *sol = nullptr, *rhs = nullptr, *res = nullptr;  // owned work vectors
   void *tmp = nullptr;  // <jacobi> steps: the other half of x's ping-pong (smoother Jacobi)
   bool residual_p = false;
+  bool x_zero = false;  // sol is zero but not stored: the next smoother step or coarse solve
+                        // writes it instead of updating it
   double *cinv = nullptr;  // base level: dense inverse (FP64, row-major)
   double *ctmp = nullptr;
   long N = 0;
@@ -80,12 +82,12 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int n, const double *__restr
 }
 
 template <class T>
-__global__ void k_coarse_update(int n, const double *y, T *x, T *r) {
+__global__ void k_coarse_update(int n, const double *y, T *x, T *r, int x_zero) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   T v = (T)y[i];
   r[i] = v;
-  x[i] += v;
+  x[i] = x_zero ? v : x[i] + v;
 }
 
 // Dense LU without pivoting (the coarse matrices are SPD after Dirichlet
@@ -146,10 +148,12 @@ static void coarse_apply(flgpu_mg *mg, MGLevel &L, void *x, void *r) {
   hipStream_t s = mg->ctx->stream;
   if (mg->p.prec == FLGPU_F64) {
     k_dense_gemv<double><<<grid_for(n, 4), 256, 0, s>>>(n, L.cinv, (const double *)r, L.ctmp);
-    k_coarse_update<double><<<grid_for(n, 256), 256, 0, s>>>(n, L.ctmp, (double *)x, (double *)r);
+    k_coarse_update<double><<<grid_for(n, 256), 256, 0, s>>>(n, L.ctmp, (double *)x, (double *)r,
+                                                             L.x_zero);
   } else {
     k_dense_gemv<float><<<grid_for(n, 4), 256, 0, s>>>(n, L.cinv, (const float *)r, L.ctmp);
-    k_coarse_update<float><<<grid_for(n, 256), 256, 0, s>>>(n, L.ctmp, (float *)x, (float *)r);
+    k_coarse_update<float><<<grid_for(n, 256), 256, 0, s>>>(n, L.ctmp, (float *)x, (float *)r,
+                                                            L.x_zero);
   }
   FLGPU_HIP(hipGetLastError());
 }
@@ -178,7 +182,8 @@ static void smooth(flgpu_mg *mg, int l, bool post) {
   if (p.smoother == FLGPU_SMOOTH_JACOBI && b != r && L.tmp) {
     // residual recomputed inside each step; only femlisp's aliased rhs = residual (below)
     // needs the in-place form that divides the residual vector itself
-    jacobi_steps(L.A, p.omega, steps, x, b, L.tmp);
+    jacobi_steps(L.A, p.omega, steps, x, b, L.tmp, L.x_zero);
+    L.x_zero = false;
   } else if (p.smoother == FLGPU_SMOOTH_JACOBI) {
     ensure_residual(mg, l);  // <jacobi> has residual-before t
     for (int k = 0; k < steps; k++) {
@@ -195,6 +200,7 @@ static void lmgc(flgpu_mg *mg, int l) {
   MGLevel &L = mg->lv[l];
   if (l == mg->p.base_level) {
     coarse_apply(mg, L, lsol(mg, l), lres(mg, l));
+    L.x_zero = false;
     L.residual_p = false;
     return;
   }
@@ -202,11 +208,26 @@ static void lmgc(flgpu_mg *mg, int l) {
   // restrict (multigrid.lisp:202-224): res_{l-1} = I^T res_l
   ensure_residual(mg, l);
   MGLevel &C = mg->lv[l - 1];
-  spmv(C.I->transpose, 1.0, lres(mg, l), 0.0, C.res, C.res);
-  // :after methods: x_{l-1} = 0, rhs_{l-1} = res_{l-1}, residual-p t
-  vfill(mg->ctx, C.N, mg->p.prec, C.sol, 0.0);
-  vcopy(mg->ctx, C.N, mg->p.prec, C.res, mg->p.prec, C.rhs);
-  C.residual_p = true;
+  // :after methods: x_{l-1} = 0, rhs_{l-1} = res_{l-1}, residual-p t.  Two shortcuts keep
+  // the values: the coarse solve and fused <jacobi> steps write x instead of updating a
+  // stored zero, and fused steps recompute the residual from rhs, so the restriction goes
+  // straight into rhs there (the coarse solve reads only res)
+  bool base = (l - 1 == mg->p.base_level);
+  bool fused = mg->p.smoother == FLGPU_SMOOTH_JACOBI && C.tmp && mg->p.pre_steps > 0;
+  if (fused && !base) {
+    spmv(C.I->transpose, 1.0, lres(mg, l), 0.0, C.rhs, C.rhs);
+    C.x_zero = true;
+    C.residual_p = false;
+  } else {
+    spmv(C.I->transpose, 1.0, lres(mg, l), 0.0, C.res, C.res);
+    if (base) {
+      C.x_zero = true;
+    } else {
+      vfill(mg->ctx, C.N, mg->p.prec, C.sol, 0.0);
+      vcopy(mg->ctx, C.N, mg->p.prec, C.res, mg->p.prec, C.rhs);
+    }
+    C.residual_p = true;
+  }
   lmgc(mg, l - 1);
   // prolongate (multigrid.lisp:237-252): sol_l += I sol_{l-1}
   spmv(C.I, 1.0, C.sol, 1.0, lsol(mg, l), lsol(mg, l));

@@ -804,17 +804,37 @@ static void jacobi_step(const flgpu_mat *A, T damp, const T *x, const T *b, T *x
   FLGPU_HIP(hipGetLastError());
 }
 
+// the first step from x = 0: b - A 0 = b exactly, so xo = damp b / A_ii without reading A
+template <class T>
+__global__ void k_jacobi_zero(long n, const T *__restrict__ diag, T damp, const T *__restrict__ b,
+                              T *__restrict__ xo) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xo[i] = damp * (b[i] / diag[i]);
+}
+
 // `steps` <jacobi> steps, each from a freshly computed residual, x ping-ponged through tmp
-// (n values); the residual vector of the iteration is not touched (left stale)
-void jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp) {
+// (n values); the residual vector of the iteration is not touched (left stale).  x_zero:
+// x holds no values yet and stands for zero.
+void jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp,
+                  bool x_zero) {
   if (steps <= 0 || A->nrows == 0) return;
   FLGPU_REQUIRE(A->nrows == A->ncols, "<jacobi> needs a square matrix");
   for (int k = 0; k < steps; k++) {
     void *src = (k % 2 == 0) ? x : tmp, *dst = (k % 2 == 0) ? tmp : x;
-    if (A->prec == FLGPU_F64)
+    if (k == 0 && x_zero) {
+      unsigned g = grid_for(A->nrows, 256);
+      if (A->prec == FLGPU_F64)
+        k_jacobi_zero<double><<<g, 256, 0, A->ctx->stream>>>(
+            A->nrows, (const double *)A->diag, damp, (const double *)b, (double *)dst);
+      else
+        k_jacobi_zero<float><<<g, 256, 0, A->ctx->stream>>>(
+            A->nrows, (const float *)A->diag, (float)damp, (const float *)b, (float *)dst);
+      FLGPU_HIP(hipGetLastError());
+    } else if (A->prec == FLGPU_F64) {
       jacobi_step<double>(A, damp, (const double *)src, (const double *)b, (double *)dst);
-    else
+    } else {
       jacobi_step<float>(A, (float)damp, (const float *)src, (const float *)b, (float *)dst);
+    }
   }
   if (steps % 2) vcopy(A->ctx, A->nrows, A->prec, tmp, A->prec, x);
 }
@@ -1211,6 +1231,22 @@ extern "C" int flgpu_mat_colour_greedy(flgpu_mat *A, int *ncolours) {
     drop_stencil(A);
     set_colouring(A, nc, colour.data());
     if (ncolours) *ncolours = nc;
+    return 0;
+  });
+}
+
+extern "C" int flgpu_jacobi_steps(const flgpu_mat *A, double damp, int steps, flgpu_vec *x,
+                                  const flgpu_vec *b, flgpu_vec *tmp) {
+  return guard([&] {
+    FLGPU_REQUIRE(A, "matrix must not be NULL");
+    FLGPU_REQUIRE(A->nrows == A->ncols, "<jacobi> needs a square matrix");
+    FLGPU_REQUIRE(steps >= 0, "negative number of steps");
+    check_vec(A, x, A->nrows, "x");
+    check_vec(A, b, A->nrows, "b");
+    check_vec(A, tmp, A->nrows, "tmp");
+    FLGPU_REQUIRE(x != tmp && b != tmp && x != b, "x, b and tmp must be distinct vectors");
+    CTX_LOCK(A->ctx);
+    jacobi_steps(A, damp, steps, x->d, b->d, tmp->d);
     return 0;
   });
 }

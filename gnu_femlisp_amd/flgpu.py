@@ -90,6 +90,7 @@ SIGNATURES = {
     "flgpu_mat_set_colouring": (_ci, [_vp, _ci, _vp]),
     "flgpu_mat_colour_greedy": (_ci, [_vp, _P(_ci)]),
     "flgpu_jacobi_iterate": (_ci, [_vp, _cd, _vp, _vp]),
+    "flgpu_jacobi_steps": (_ci, [_vp, _cd, _ci, _vp, _vp, _vp]),
     "flgpu_sor_iterate": (_ci, [_vp, _cd, _ci, _vp, _vp, _vp]),
     "flgpu_lattice_assemble": (_ci, [_vp, _ci, _ci, _ci, _vp, _ci, _cd, _ci, _ci, _P(_vp), _P(_vp)]),
     "flgpu_lattice_reassemble": (_ci, [_vp, _vp, _ci, _ci]),

@@ -151,6 +151,13 @@ int flgpu_mat_colour_greedy(flgpu_mat *A, int *ncolours);
  * place), x_i <- x_i + damp r_i.  Needs an up-to-date residual. */
 int flgpu_jacobi_iterate(const flgpu_mat *A, double damp, flgpu_vec *x,
                          flgpu_vec *r);
+/* `steps` <jacobi> steps as femlisp's product-iterator runs them for a
+ * smoother (linit.lisp:94-120: residual-before t, residual-after nil): each
+ * step recomputes r = b - A x and sets x <- x + damp r_i / A_ii, in one pass
+ * over A, out of place through tmp (n values; x, b, tmp distinct).  The
+ * residual is not returned. */
+int flgpu_jacobi_steps(const flgpu_mat *A, double damp, int steps, flgpu_vec *x,
+                       const flgpu_vec *b, flgpu_vec *tmp);
 /* one <sor> sweep (linit.lisp:288-323) in colour order (backwards = 1
  * sweeps the colours last to first): per row corr = b_i - sum_j A_ij x_j,
  * corr /= A_ii, r_i <- corr, x_i += omega corr.  b may be r. */

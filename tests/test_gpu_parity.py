@@ -236,6 +236,39 @@ def test_jacobi_and_sor_iterates():
     assert relerr(gx.numpy(), xs) < 1e-13 and relerr(gr.numpy(), rs) < 1e-13
 
 
+@pytest.mark.parametrize("form", [F.STORAGE_CELL, F.STORAGE_STENCIL, F.STORAGE_SELL])
+@pytest.mark.parametrize("prec", [F.F64, F.F32])
+@pytest.mark.parametrize("steps", [1, 4])
+def test_jacobi_steps_match_product_iterator(form, prec, steps):
+    """flgpu_jacobi_steps (residual fused into each step, out of place) equals the
+    product-iterator of <jacobi> done the long way: r = b - A x, then the
+    in-place jacobi iterate, per step; and, in FP64, a host restatement."""
+    say(f"jacobi steps form={form} prec={prec} steps={steps}")
+    from gnu_femlisp_amd.flgpu import check, lib
+    n = 12
+    rng = np.random.default_rng(23)
+    kap = np.exp(rng.normal(scale=1.5, size=(n, n, n)))
+    A, b = lat.assemble(3, 1, n, kap, prec=prec)
+    assert A.storage(form) == form
+    x0 = rng.uniform(-1, 1, A.nrows)
+    x, tmp = GpuVector.from_array(x0, prec), GpuVector(A.nrows, prec)
+    check(lib().flgpu_jacobi_steps(A.h, 0.7, steps, x.h, b.h, tmp.h))
+    y, r = GpuVector.from_array(x0, prec), GpuVector(A.nrows, prec)
+    for _ in range(steps):
+        compute_residual(A, y, b, r)
+        check(lib().flgpu_jacobi_iterate(A.h, 0.7, y.h, r.h))
+    assert relerr(x.numpy(), y.numpy()) < (1e-14 if prec == F.F64 else 1e-6)
+    if prec == F.F64:
+        rp, ci, v = A.csr()
+        M = sp.csr_matrix((v, ci, rp))
+        xh, bh, d = x0.copy(), b.numpy(), M.diagonal()
+        for _ in range(steps):
+            xh = xh + 0.7 * (bh - M @ xh) / d
+        assert relerr(x.numpy(), xh) < 1e-13
+    with pytest.raises(F.FlgpuError):
+        check(lib().flgpu_jacobi_steps(A.h, 0.7, 1, x.h, b.h, x.h))
+
+
 def _gpu_hier(dim, order, n0, levels, smoother, omega, pre, post, base, symmetric=False,
               kappa=None, prec=F.F64, mode=F.ASM_F64):
     sm = it.Jacobi(omega) if smoother == po.SMOOTH_JACOBI else it.SOR(omega)
