@@ -270,7 +270,7 @@ def main():
         log(f"[bench] V-cycle {name}: {ms:.3f} ms, {vcyc[name]['gdofs']:.3f} GDOF/s")
 
     # ---- roofline of the value path's dominant kernel -------------------------------
-    # k_q1_tile<float, Q1_JACOBI>: one damped <jacobi> step on the fine level (per-cell form) of the
+    # k_q1_zblock<float, Q1_JACOBI>: one damped <jacobi> step on the fine level (per-cell form) of the
     # mixed-bb-jac hierarchy, 8 per V-cycle.  Events bracket 8 x reps back-to-back launches,
     # so the average includes the dispatch gap between launches; large copies first let the
     # host queue up ahead of the GPU.  Algorithmic bytes per launch: the cell factors once,
@@ -295,11 +295,11 @@ def main():
         us = ms * 1e3 / (8 * reps)
         byt = cell_sor_bytes(n ** 3, At.nrows, 4)  # cells + 4 vector passes
         gbs = byt / (us * 1e-6) / 1e9
-        roof = {"bound": "hbm", "kernel": f"k_q1_tile<float, Q1_JACOBI> (fine-level <jacobi> "
+        roof = {"bound": "hbm", "kernel": f"k_q1_zblock<float, Q1_JACOBI> (fine-level <jacobi> "
                                           f"step, per-cell form, {n}^3)",
                 "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
                 "traffic": None, "launch_us": us, "bytes_per_launch": byt}
-        log(f"[bench] roofline k_q1_tile<float, Q1_JACOBI>: {us:.2f} us per launch, {gbs:.0f} GB/s "
+        log(f"[bench] roofline k_q1_zblock<float, Q1_JACOBI>: {us:.2f} us per launch, {gbs:.0f} GB/s "
             f"({100 * gbs / 8000:.1f}% of 8 TB/s)")
         del At, xj, bj, tj
 

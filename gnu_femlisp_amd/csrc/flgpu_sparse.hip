@@ -515,116 +515,112 @@ __device__ __forceinline__ T q1_row(int M, int n, long i, int z, int y, int x,
   return acc;
 }
 
-// ---- LDS-staged passes over the whole lattice in the per-cell form -----------------------
-// A workgroup owns a QTX x QTY block of (x, y) rows and marches QTZ planes along z.  Per
-// plane it stages the next plane of x (with a one-node halo) and of the cell factors in LDS
-// with coalesced loads, then every thread forms its row from LDS with exactly q1_row's
-// arithmetic (same terms, same order).  Global traffic per row: about one x, one factor and
-// the row's own vectors, instead of 27 + 8 gathers.
-constexpr int QTX = 64, QTY = 4, QTZ = 8;
+// ---- whole-lattice passes in the per-cell form, blocked along z ---------------------------
+// A thread owns QZB consecutive rows along z at one (y, x).  It loads the 3x3 column of x over
+// the QZB + 2 planes and the 2x2 column of cell factors over QZB + 1 cell planes once (54 + 20
+// loads for 4 rows instead of 4 x 35), all issued before the first product, and forms each
+// row with exactly q1_row's arithmetic (same terms, same order).  Neighbouring threads are
+// neighbours in x, so every load instruction of a wave is contiguous.
+constexpr int QZB = 4;
 enum { Q1_SPMV = 0, Q1_JACOBI = 1 };
-
-__device__ __forceinline__ int mod3(int v) { return ((v % 3) + 3) % 3; }
 
 // OP = Q1_SPMV:   out_i = beta z_i + sum_j (coef x_j) A_ij          (v = z)
 // OP = Q1_JACOBI: out_i = x_i + coef (b_i - sum_j A_ij x_j) / A_ii  (v = b, coef = damp;
 //                 A_ii from the factors, 1 on Dirichlet rows)
 template <class T, int OP>
-__global__ __launch_bounds__(QTX *QTY) void k_q1_tile(int M, int n, const T *__restrict__ alpha,
-                                                      Q1K kt, T coef, T beta,
-                                                      const T *__restrict__ v,
-                                                      const T *__restrict__ x,
-                                                      T *__restrict__ out) {
-  __shared__ T xs[3][QTY + 2][QTX + 2];
-  __shared__ T as[2][QTY + 1][QTX + 1];
-  const int tid = threadIdx.x, tx = tid % QTX, ty = tid / QTX;
-  const int x0 = blockIdx.x * QTX, y0 = blockIdx.y * QTY, z0 = blockIdx.z * QTZ;
-  const long M2 = (long)M * M;
-  auto load_x = [&](int zz) {
-    T(*dst)[QTX + 2] = xs[mod3(zz)];
-    for (int e = tid; e < (QTY + 2) * (QTX + 2); e += QTX * QTY) {
-      int ly = e / (QTX + 2), lx = e % (QTX + 2);
-      int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
-      bool in = zz >= 0 && zz < M && gy >= 0 && gy < M && gx >= 0 && gx < M;
-      dst[ly][lx] = in ? x[zz * M2 + (long)gy * M + gx] : (T)0;
-    }
-  };
-  auto load_a = [&](int cz) {
-    T(*dst)[QTX + 1] = as[cz & 1];
-    for (int e = tid; e < (QTY + 1) * (QTX + 1); e += QTX * QTY) {
-      int ly = e / (QTX + 1), lx = e % (QTX + 1);
-      int cy = y0 - 1 + ly, cx = x0 - 1 + lx;
-      bool in = cz >= 0 && cz < n && cy >= 0 && cy < n && cx >= 0 && cx < n;
-      dst[ly][lx] = in ? alpha[((long)cz * n + cy) * n + cx] : (T)0;
-    }
-  };
-  load_x(z0 - 1);
-  load_x(z0);
-  load_a(z0 - 1);
-  const int zend = min(z0 + QTZ, M);
-  const int gx = x0 + tx, gy = y0 + ty;
-  for (int zz = z0; zz < zend; zz++) {
-    load_x(zz + 1);
-    load_a(zz);
-    __syncthreads();
-    if (gx < M && gy < M) {
-      long i = zz * M2 + (long)gy * M + gx;
-      T xi = xs[mod3(zz)][ty + 1][tx + 1];
-      bool bd = gx == 0 || gy == 0 || zz == 0 || gx == M - 1 || gy == M - 1 || zz == M - 1;
-      T acc, d = (T)1;
-      if (OP == Q1_SPMV) acc = (beta == (T)0) ? (T)0 : beta * v[i];
-      else acc = v[i];
-      T cf = (OP == Q1_SPMV) ? coef : (T)-1;
-      if (bd) {
-        acc += (cf * xi) * (T)1;
-      } else {
-        T a[2][2][2];
+__global__ __launch_bounds__(256) void k_q1_zblock(int M, int n, const T *__restrict__ alpha,
+                                                   Q1K kt, T coef, T beta,
+                                                   const T *__restrict__ v,
+                                                   const T *__restrict__ x,
+                                                   T *__restrict__ out) {
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  long t = (long)lb * blockDim.x + threadIdx.x;
+  long M2 = (long)M * M;
+  int nzb = (M + QZB - 1) / QZB;
+  if (t >= M2 * nzb) return;
+  int gx = (int)(t % M), gy = (int)((t / M) % M), z0 = (int)(t / M2) * QZB;
+  // neighbour coordinates clamped into the lattice: a clamped value only meets a zero
+  // coefficient (Dirichlet column) or a boundary row, which reads no neighbours
+  int xc[3], yc[3];
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+  for (int d = 0; d < 3; d++) {
+    xc[d] = min(max(gx + d - 1, 0), M - 1);
+    yc[d] = min(max(gy + d - 1, 0), M - 1);
+  }
+  T xv[QZB + 2][3][3];
 #pragma unroll
-          for (int q = 0; q < 2; q++)
+  for (int k = 0; k < QZB + 2; k++) {
+    long zo = (long)min(max(z0 - 1 + k, 0), M - 1) * M2;
 #pragma unroll
-            for (int s = 0; s < 2; s++) a[p][q][s] = as[(zz - 1 + p) & 1][ty + q][tx + s];
-        bool lz[3] = {zz - 1 != 0, true, zz + 1 != M - 1},
-             ly[3] = {gy - 1 != 0, true, gy + 1 != M - 1},
-             lx[3] = {gx - 1 != 0, true, gx + 1 != M - 1};
+    for (int dy = 0; dy < 3; dy++)
 #pragma unroll
-        for (int k = 0; k < 27; k++) {
-          int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
-          int pz0 = dz > 0 ? 1 : 0, pz1 = dz < 0 ? 0 : 1;
-          int py0 = dy > 0 ? 1 : 0, py1 = dy < 0 ? 0 : 1;
-          int px0 = dx > 0 ? 1 : 0, px1 = dx < 0 ? 0 : 1;
-          T s = (T)0;
+      for (int dx = 0; dx < 3; dx++) xv[k][dy][dx] = x[zo + (long)yc[dy] * M + xc[dx]];
+  }
+  int cxs[2], cys[2];
 #pragma unroll
-          for (int p = pz0; p <= pz1; p++)
+  for (int s = 0; s < 2; s++) {
+    cxs[s] = min(max(gx - 1 + s, 0), n - 1);
+    cys[s] = min(max(gy - 1 + s, 0), n - 1);
+  }
+  T av[QZB + 1][2][2];
 #pragma unroll
-            for (int q = py0; q <= py1; q++)
+  for (int k = 0; k < QZB + 1; k++) {
+    long co = (long)min(max(z0 - 1 + k, 0), n - 1) * n;
 #pragma unroll
-              for (int r = px0; r <= px1; r++) s += a[p][q][r];
-          T aij = (T)kt.k[(dz != 0) * 4 + (dy != 0) * 2 + (dx != 0)] * s;
-          if (k == 13) d = aij;
-          bool live = lz[dz + 1] && ly[dy + 1] && lx[dx + 1];
-          T xj = xs[mod3(zz + dz)][ty + 1 + dy][tx + 1 + dx];
-          acc += (cf * xj) * (live ? aij : (T)0);
-        }
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int s = 0; s < 2; s++) av[k][q][s] = alpha[(co + cys[q]) * n + cxs[s]];
+  }
+  bool ly[3] = {gy - 1 != 0, true, gy + 1 != M - 1}, lx[3] = {gx - 1 != 0, true, gx + 1 != M - 1};
+  bool side = gx == 0 || gy == 0 || gx == M - 1 || gy == M - 1;
+#pragma unroll
+  for (int r = 0; r < QZB; r++) {
+    int zz = z0 + r;
+    if (zz >= M) break;
+    long i = zz * M2 + (long)gy * M + gx;
+    T xi = xv[r + 1][1][1];
+    T acc, d = (T)1;
+    if (OP == Q1_SPMV) acc = (beta == (T)0) ? (T)0 : beta * v[i];
+    else acc = v[i];
+    T cf = (OP == Q1_SPMV) ? coef : (T)-1;
+    if (side || zz == 0 || zz == M - 1) {
+      acc += (cf * xi) * (T)1;
+    } else {
+      bool lz[3] = {zz - 1 != 0, true, zz + 1 != M - 1};
+#pragma unroll
+      for (int k = 0; k < 27; k++) {
+        int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+        int pz0 = dz > 0 ? 1 : 0, pz1 = dz < 0 ? 0 : 1;
+        int py0 = dy > 0 ? 1 : 0, py1 = dy < 0 ? 0 : 1;
+        int px0 = dx > 0 ? 1 : 0, px1 = dx < 0 ? 0 : 1;
+        T sum = (T)0;
+#pragma unroll
+        for (int p = pz0; p <= pz1; p++)
+#pragma unroll
+          for (int q = py0; q <= py1; q++)
+#pragma unroll
+            for (int s = px0; s <= px1; s++) sum += av[r + p][q][s];
+        T aij = (T)kt.k[(dz != 0) * 4 + (dy != 0) * 2 + (dx != 0)] * sum;
+        if (k == 13) d = aij;
+        bool live = lz[dz + 1] && ly[dy + 1] && lx[dx + 1];
+        acc += (cf * xv[r + 1 + dz][dy + 1][dx + 1]) * (live ? aij : (T)0);
       }
-      if (OP == Q1_SPMV) out[i] = acc;
-      else out[i] = xi + coef * (acc / d);
     }
-    __syncthreads();
+    if (OP == Q1_SPMV) out[i] = acc;
+    else out[i] = xi + coef * (acc / d);
   }
 }
 
 template <class T, int OP>
-static void launch_q1_tile(const flgpu_mat *A, T coef, T beta, const void *v, const void *x,
+static void launch_q1_pass(const flgpu_mat *A, T coef, T beta, const void *v, const void *x,
                            void *out) {
   int M = A->lat_n + 1;
-  dim3 grid((M + QTX - 1) / QTX, (M + QTY - 1) / QTY, (M + QTZ - 1) / QTZ);
+  long threads = (long)M * M * ((M + QZB - 1) / QZB);
+  unsigned g = xcd_grid(grid_for(threads, 256));
   Q1K kt;
   for (int b = 0; b < 8; b++) kt.k[b] = A->kt[b];
-  k_q1_tile<T, OP><<<grid, QTX * QTY, 0, A->ctx->stream>>>(M, A->lat_n, (const T *)A->alpha, kt,
-                                                           coef, beta, (const T *)v, (const T *)x,
-                                                           (T *)out);
+  k_q1_zblock<T, OP><<<g, 256, 0, A->ctx->stream>>>(M, A->lat_n, (const T *)A->alpha, kt, coef,
+                                                    beta, (const T *)v, (const T *)x, (T *)out);
   FLGPU_HIP(hipGetLastError());
 }
 
@@ -771,8 +767,8 @@ static Q1K q1k(const flgpu_mat *A) {
 void spmv(const flgpu_mat *A, double alpha, const void *x, double beta, const void *z,
           void *out) {
   if (A->alpha) {
-    if (A->prec == FLGPU_F64) launch_q1_tile<double, Q1_SPMV>(A, alpha, beta, z, x, out);
-    else launch_q1_tile<float, Q1_SPMV>(A, (float)alpha, (float)beta, z, x, out);
+    if (A->prec == FLGPU_F64) launch_q1_pass<double, Q1_SPMV>(A, alpha, beta, z, x, out);
+    else launch_q1_pass<float, Q1_SPMV>(A, (float)alpha, (float)beta, z, x, out);
     return;
   }
   if (A->val27) {
@@ -810,7 +806,7 @@ __global__ void k_jacobi(int n, T damp, const T *__restrict__ diag, T *x, T *r) 
 // One <jacobi> step of a product-iterator (linit.lisp:94-120,220-250) with the residual it
 // needs recomputed in the same pass, out of place: r_i = b_i - (A x)_i (the sum of spmv with
 // alpha = -1, beta = 1), r_i /= A_ii, xo_i = x_i + damp r_i.  The per-cell form runs
-// k_q1_tile (A_ii from the cell factors); the stencil and SELL forms read A_ii from diag:
+// k_q1_zblock (A_ii from the cell factors); the stencil and SELL forms read A_ii from diag:
 template <class T>
 __global__ __launch_bounds__(256) void k_st27_jacobi(int M, const T *__restrict__ val27,
                                                      const T *__restrict__ diag, T damp,
@@ -850,7 +846,7 @@ template <class T>
 static void jacobi_step(const flgpu_mat *A, T damp, const T *x, const T *b, T *xo) {
   hipStream_t st = A->ctx->stream;
   if (A->alpha) {
-    launch_q1_tile<T, Q1_JACOBI>(A, damp, (T)0, b, x, xo);
+    launch_q1_pass<T, Q1_JACOBI>(A, damp, (T)0, b, x, xo);
     return;
   } else if (A->val27) {
     unsigned g = xcd_grid(grid_for(A->nrows, 256));
