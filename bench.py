@@ -273,8 +273,8 @@ def main():
     # k_q1_zblock<float, Q1_JACOBI>: one damped <jacobi> step on the fine level (per-cell form) of the
     # mixed-bb-jac hierarchy, 8 per V-cycle.  Events bracket 8 x reps back-to-back launches,
     # so the average includes the dispatch gap between launches; large copies first let the
-    # host queue up ahead of the GPU.  Algorithmic bytes per launch: the cell factors once,
-    # x, b and the diagonal read, the new x written.
+    # host queue up ahead of the GPU.  Algorithmic bytes per launch: the cell factors, x and
+    # b read once, the new x written (the diagonal follows from the factors).
     roof = None
     if "mixed-bb-jac" in hier:
         At = hier["mixed-bb-jac"][1].top_matrix()
@@ -293,7 +293,7 @@ def main():
         ms = ctx.timer_stop()
         del big_a, big_b
         us = ms * 1e3 / (8 * reps)
-        byt = cell_sor_bytes(n ** 3, At.nrows, 4)  # cells + 4 vector passes
+        byt = n ** 3 * 4 + 3 * At.nrows * 4  # cell factors, x and b read, new x written
         gbs = byt / (us * 1e-6) / 1e9
         roof = {"bound": "hbm", "kernel": f"k_q1_zblock<float, Q1_JACOBI> (fine-level <jacobi> "
                                           f"step, per-cell form, {n}^3)",
