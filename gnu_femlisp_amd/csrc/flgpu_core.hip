@@ -87,6 +87,57 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partial(long n, const T *x,
   }
 }
 
+// up to three dots in one pass over the vectors: dot k's per-block partial is formed exactly
+// as k_dot_partial forms it (same grid, same order), into partial[k * gridDim.x + block]
+template <class T, int K>
+__global__ __launch_bounds__(kRedThreads) void k_dot_partial_multi(long n, const T *x0, const T *y0,
+                                                                    const T *x1, const T *y1,
+                                                                    const T *x2, const T *y2,
+                                                                    double *partial) {
+  __shared__ double s[K][kRedThreads / 64];
+  const T *xs[3] = {x0, x1, x2}, *ys[3] = {y0, y1, y2};
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride)
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] += (double)xs[k][i] * (double)ys[k][i];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double v = wave_sum(acc[k]);
+    if (lane == 0) s[k][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double t = 0.0;
+    for (int j = 0; j < kRedThreads / 64; j++) t += s[threadIdx.x][j];
+    partial[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// block k reduces dot k's partials exactly as k_dot_final does, into out[slot[k]]
+__global__ __launch_bounds__(kRedBlocks) void k_dot_final_multi(const double *partial, int np,
+                                                                 double *out, int s0, int s1,
+                                                                 int s2) {
+  __shared__ double s[kRedBlocks / 64];
+  const double *pk = partial + (long)blockIdx.x * np;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += pk[i];
+  acc = wave_sum(acc);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) s[w] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x / 64); k++) t += s[k];
+    int slot = blockIdx.x == 0 ? s0 : (blockIdx.x == 1 ? s1 : s2);
+    out[slot] = t;
+  }
+}
+
 __global__ __launch_bounds__(kRedBlocks) void k_dot_final(const double *partial, int np,
                                                            double *out) {
   __shared__ double s[kRedBlocks / 64];
@@ -166,6 +217,35 @@ void vdot_dev(flgpu_ctx *ctx, long n, int prec, const void *x, const void *y, in
   FLGPU_HIP(hipGetLastError());
 }
 
+// dots (x[k], y[k]) into slots[k], k < count <= 3, in one pass; each equals vdot_dev's result
+void vdot_multi_dev(flgpu_ctx *ctx, long n, int prec, int count, const void *const *x,
+                    const void *const *y, const int *slots) {
+  FLGPU_REQUIRE(count >= 1 && count <= 3, "vdot_multi_dev takes 1 to 3 dots");
+  for (int k = 0; k < count; k++)
+    FLGPU_REQUIRE(slots[k] >= 0 && slots[k] < kRedSlots, "reduction slot out of range");
+  const void *xx[3] = {x[0], count > 1 ? x[1] : x[0], count > 2 ? x[2] : x[0]};
+  const void *yy[3] = {y[0], count > 1 ? y[1] : y[0], count > 2 ? y[2] : y[0]};
+  hipStream_t st = ctx->stream;
+#define FLGPU_DOTS(T, K)                                                                      \
+  k_dot_partial_multi<T, K><<<kRedBlocks, kRedThreads, 0, st>>>(                              \
+      n, (const T *)xx[0], (const T *)yy[0], (const T *)xx[1], (const T *)yy[1],              \
+      (const T *)xx[2], (const T *)yy[2], ctx->red_partial)
+  if (prec == FLGPU_F64) {
+    if (count == 1) FLGPU_DOTS(double, 1);
+    else if (count == 2) FLGPU_DOTS(double, 2);
+    else FLGPU_DOTS(double, 3);
+  } else {
+    if (count == 1) FLGPU_DOTS(float, 1);
+    else if (count == 2) FLGPU_DOTS(float, 2);
+    else FLGPU_DOTS(float, 3);
+  }
+#undef FLGPU_DOTS
+  k_dot_final_multi<<<count, kRedBlocks, 0, st>>>(ctx->red_partial, kRedBlocks, ctx->red_out,
+                                                  slots[0], count > 1 ? slots[1] : slots[0],
+                                                  count > 2 ? slots[2] : slots[0]);
+  FLGPU_HIP(hipGetLastError());
+}
+
 void fetch_slots(flgpu_ctx *ctx, int nslots) {
   FLGPU_HIP(hipMemcpyAsync(ctx->host_out, ctx->red_out, sizeof(double) * nslots,
                            hipMemcpyDeviceToHost, ctx->stream));
@@ -222,7 +302,7 @@ extern "C" int flgpu_ctx_create(int device, flgpu_ctx **out) {
     c->cus = prop.multiProcessorCount;
     c->name = prop.gcnArchName;
     FLGPU_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    c->red_partial = (double *)dev_alloc(sizeof(double) * kRedBlocks);
+    c->red_partial = (double *)dev_alloc(sizeof(double) * 3 * kRedBlocks);
     c->red_out = (double *)dev_alloc(sizeof(double) * kRedSlots);
     FLGPU_HIP(hipHostMalloc((void **)&c->host_out, sizeof(double) * kRedSlots, hipHostMallocDefault));
     FLGPU_HIP(hipEventCreate(&c->ev_start));

@@ -209,6 +209,9 @@ void vaxpy(flgpu_ctx *ctx, long n, int prec, double a, const void *x, void *y);
 // result stays on device in ctx->red_out[slot]; dot_host syncs and returns it
 void vdot_dev(flgpu_ctx *ctx, long n, int prec, const void *x, const void *y,
               int slot);
+// 1 to 3 dots in one pass, each identical to vdot_dev's
+void vdot_multi_dev(flgpu_ctx *ctx, long n, int prec, int count, const void *const *x,
+                    const void *const *y, const int *slots);
 double vdot_host(flgpu_ctx *ctx, long n, int prec, const void *x,
                  const void *y);
 void fetch_slots(flgpu_ctx *ctx, int nslots);  // red_out[0..n) -> host_out

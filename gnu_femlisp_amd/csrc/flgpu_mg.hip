@@ -58,7 +58,7 @@ struct flgpu_mg {
   // captured cycles: one HIP graph per (x, b, r, precision) of the caller
   struct Graph {
     void *x, *b, *r;
-    int vprec;
+    int vprec, mode;  // mode 0: x += V(r); mode 1: x = V(b) from zero
     hipGraphExec_t exec;
   };
   std::vector<Graph> graphs;
@@ -290,14 +290,49 @@ static void graphs_clear(flgpu_mg *mg) {
   mg->graphs.clear();
 }
 
-void mg_apply_cached(flgpu_mg *mg, int vprec, void *x, void *b, void *r) {
+// <cg>'s preconditioner call (copy! r w) (x<-0 q) (iterate q w w2) without its copies: the
+// cycle never writes its top-level rhs, so it reads r in place (converted once for a cycle of
+// another precision), and q is written instead of cleared and updated.  work (n values of
+// vprec) is the top-level residual; it is seeded from r only when the first smoothing step
+// would read it (an in-place <jacobi> iterate, or no pre-smoothing).
+static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work) {
+  const flgpu_mg_params &p = mg->p;
+  MGLevel &T = mg->lv[mg->top];
+  long N = T.N;
+  bool base_top = mg->top == p.base_level;  // one level: the coarse solve reads r
+  bool jac = p.smoother == FLGPU_SMOOTH_JACOBI && T.tmp && p.pre_steps > 0 && !base_top;
+  bool reads_r = base_top || !(jac || (p.smoother == FLGPU_SMOOTH_SOR && p.pre_steps > 0));
+  bool zero = jac || base_top;  // the first step writes x
+  if (vprec == p.prec) {
+    if (zero) T.x_zero = true;
+    else vfill(mg->ctx, N, vprec, q, 0.0);
+    if (reads_r) vcopy(mg->ctx, N, vprec, r, vprec, work);
+    mg_cycle(mg, q, r, work);
+    return;
+  }
+  vcopy(mg->ctx, N, vprec, r, p.prec, mg->bs);
+  if (zero) T.x_zero = true;
+  else vfill(mg->ctx, N, p.prec, mg->xs, 0.0);
+  if (reads_r) vcopy(mg->ctx, N, p.prec, mg->bs, p.prec, mg->rs);
+  mg_cycle(mg, mg->xs, mg->bs, mg->rs);
+  vcopy(mg->ctx, N, p.prec, mg->xs, vprec, q);
+}
+
+static void mg_run(flgpu_mg *mg, int mode, int vprec, void *x, void *b, void *r) {
+  if (mode == 1) mg_apply_zero(mg, vprec, x, b, r);
+  else mg_apply_ptr(mg, vprec, x, b, r);
+}
+
+// mode 0: x <- x + V(r) on (x, b, r); mode 1: x <- V(b) from zero, b read only, r work.
+// With graphs on, the first call on a (mode, x, b, r, precision) is captured and replayed.
+static void mg_run_cached(flgpu_mg *mg, int mode, int vprec, void *x, void *b, void *r) {
   if (!mg->use_graphs) {
-    mg_apply_ptr(mg, vprec, x, b, r);
+    mg_run(mg, mode, vprec, x, b, r);
     return;
   }
   hipStream_t s = mg->ctx->stream;
   for (auto &g : mg->graphs)
-    if (g.x == x && g.b == b && g.r == r && g.vprec == vprec) {
+    if (g.x == x && g.b == b && g.r == r && g.vprec == vprec && g.mode == mode) {
       FLGPU_HIP(hipGraphLaunch(g.exec, s));
       return;
     }
@@ -309,7 +344,7 @@ void mg_apply_cached(flgpu_mg *mg, int vprec, void *x, void *b, void *r) {
   hipGraph_t graph = nullptr;
   FLGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   try {
-    mg_apply_ptr(mg, vprec, x, b, r);
+    mg_run(mg, mode, vprec, x, b, r);
   } catch (...) {
     (void)hipStreamEndCapture(s, &graph);
     if (graph) (void)hipGraphDestroy(graph);
@@ -320,8 +355,12 @@ void mg_apply_cached(flgpu_mg *mg, int vprec, void *x, void *b, void *r) {
   hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   FLGPU_HIP(e);
-  mg->graphs.push_back({x, b, r, vprec, exec});
+  mg->graphs.push_back({x, b, r, vprec, mode, exec});
   FLGPU_HIP(hipGraphLaunch(exec, s));
+}
+
+void mg_apply_cached(flgpu_mg *mg, int vprec, void *x, void *b, void *r) {
+  mg_run_cached(mg, 0, vprec, x, b, r);
 }
 
 static void check_params(const flgpu_mg_params *p, int nlevels) {
@@ -382,16 +421,6 @@ void mg_free(flgpu_mg *mg) {
 
 // ---- Krylov kernels with device scalars ----------------------------------------------------
 
-// y += sign * (num/den) * x
-template <class T>
-__global__ void k_axpy_ratio(long n, const double *num, const double *den, double sign,
-                             const T *__restrict__ x, T *y) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  T lam = (T)(sign * (*num / *den));
-  y[i] = lam * x[i] + y[i];
-}
-
 // p <- (a/b) p + q   (<cg>: scal! (/ new-alpha alpha) p, m+! q p)
 template <class T>
 __global__ void k_cg_p(long n, const double *a, const double *b, T *p, const T *__restrict__ q) {
@@ -412,6 +441,22 @@ __global__ void k_fcg_p(long n, const double *zr, const double *zro, const doubl
 }
 
 __global__ void k_slot_copy(double *s, int from, int to) { s[to] = s[from]; }
+
+// <cg>'s update in one pass: rold <- r (FCG, when given), x <- x + lam p, r <- r - lam a with
+// lam = num / den rounded to T once per sign (femlisp's axpy! with (/ alpha beta))
+template <class T>
+__global__ void k_cg_update(long n, const double *num, const double *den, const T *__restrict__ p,
+                            const T *__restrict__ a, T *__restrict__ x, T *__restrict__ r,
+                            T *__restrict__ rold) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double q = *num / *den;
+  T lp = (T)(1.0 * q), lm = (T)(-1.0 * q);
+  T ri = r[i];
+  if (rold) rold[i] = ri;
+  x[i] = lp * p[i] + x[i];
+  r[i] = lm * a[i] + ri;
+}
 
 }  // namespace flgpu
 
@@ -651,14 +696,15 @@ namespace flgpu {
 // (funcall iterate q w w); alias = 0 passes a second copy as the residual
 static void precondition(flgpu_mg *mg, const flgpu_solve_params *p, SolveWork &W, void *r,
                          void *q, LearnedHook *learned) {
+  if (!p->alias) {
+    mg_run_cached(mg, 1, W.prec, q, r, W.w2);
+    if (learned) learned->correct(r, q);
+    return;
+  }
+  // femlisp's literal (iterate q w w): rhs and residual are one vector
   vcopy(W.ctx, W.N, W.prec, r, W.prec, W.w);
   vfill(W.ctx, W.N, W.prec, q, 0.0);
-  if (p->alias) {
-    mg_apply_cached(mg, W.prec, q, W.w, W.w);
-  } else {
-    vcopy(W.ctx, W.N, W.prec, r, W.prec, W.w2);
-    mg_apply_cached(mg, W.prec, q, W.w, W.w2);
-  }
+  mg_apply_cached(mg, W.prec, q, W.w, W.w);
   if (learned) learned->correct(r, q);
 }
 
@@ -767,25 +813,25 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
           if (alpha != 0.0) {
             spmv(A, 1.0, W.p, 0.0, W.a, W.a);         // gemm! 1 A p 0 a
             vdot_dev(ctx, N, prec, W.a, W.p, 1);       // beta = a.p
-            if (fcg) vcopy(ctx, N, prec, W.r, prec, W.rold);
-            if (prec == FLGPU_F64) {
-              k_axpy_ratio<double><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, 1.0, (const double *)W.p,
-                                                     (double *)x->d);
-              k_axpy_ratio<double><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, -1.0,
-                                                     (const double *)W.a, (double *)W.r);
-            } else {
-              k_axpy_ratio<float><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, 1.0, (const float *)W.p,
-                                                    (float *)x->d);
-              k_axpy_ratio<float><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, -1.0, (const float *)W.a,
-                                                    (float *)W.r);
-            }
+            // x += (alpha/beta) p, r -= (alpha/beta) a (FCG keeps the old r first)
+            if (prec == FLGPU_F64)
+              k_cg_update<double><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, (const double *)W.p,
+                                                    (const double *)W.a, (double *)x->d,
+                                                    (double *)W.r,
+                                                    fcg ? (double *)W.rold : nullptr);
+            else
+              k_cg_update<float><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, (const float *)W.p,
+                                                   (const float *)W.a, (float *)x->d,
+                                                   (float *)W.r, fcg ? (float *)W.rold : nullptr);
             FLGPU_HIP(hipGetLastError());
             if (pc) precondition(mg, p, W, W.r, W.q, learned);
             else vcopy(ctx, N, prec, W.r, prec, W.q);
-            vdot_dev(ctx, N, prec, W.q, W.r, 2);  // new alpha = q.r
-            vdot_dev(ctx, N, prec, W.r, W.r, 3);  // defect norm^2
+            {  // new alpha = q.r, defect norm^2 = r.r, FCG's q.rold: one pass
+              const void *dx[3] = {W.q, W.r, W.q}, *dy[3] = {W.r, W.r, W.rold};
+              const int slots[3] = {2, 3, 4};
+              vdot_multi_dev(ctx, N, prec, fcg ? 3 : 2, dx, dy, slots);
+            }
             if (fcg) {
-              vdot_dev(ctx, N, prec, W.q, W.rold, 4);
               if (prec == FLGPU_F64)
                 k_fcg_p<double><<<g, 256, 0, s>>>(N, slot + 2, slot + 4, slot + 0, (double *)W.p,
                                                   (const double *)W.q);
