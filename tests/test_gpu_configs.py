@@ -1,0 +1,101 @@
+"""The reference configurations at their full sizes (GPU, through the C-ABI).
+
+- 2D Poisson, Q1, 32x32 cells, V(2,2) damped Jacobi (omega 0.6) PCG to 1e-10:
+  the oracle's iteration count, defect history and solution.
+- 3D Poisson, Q1, 64^3 cells, rhs U[-1,1] per interior node from a seed,
+  V(2,2) multicolour GS PCG to 1e-10: the oracle's iteration count and
+  solution.
+- 3D variable coefficient, Q2, 48^3 cells, kappa = exp(g), g ~ N(0, 1.5^2)
+  per cell, f = 1: FP32 V-cycle with BF16/FP32-accumulate MFMA assembly inside
+  FP64 flexible CG with the true residual to 1e-10.  The oracle takes minutes at
+  this size, so the result is checked through size-independent properties: the
+  FP64 true residual, and agreement with the GPU's plain FP64 GMG-CG (whose
+  parity with the oracle the smaller cases pin) to 1e-9.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from gnu_femlisp_amd import flgpu as F
+from gnu_femlisp_amd import iteration as it
+from gnu_femlisp_amd import lattice as lat
+from gnu_femlisp_amd.matlisp import GpuVector
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(x, y):
+    d = np.linalg.norm(y)
+    return np.linalg.norm(x - y) / (d if d > 0 else 1.0)
+
+
+def test_config_2d_poisson_jacobi_cg():
+    say("config: 2D Q1 32^2 Poisson, V(2,2) Jacobi 0.6, PCG")
+    oh = po.Hierarchy(2, 1, 1, 6)
+    prm = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_JACOBI, omega=0.6, pre_steps=2,
+                    post_steps=2, base_level=0, alias=0, tol=1e-10, maxsteps=100)
+    st, xo, so, dno = oh.solve(prm)
+    assert st == 0 and oh.ndofs() == 1089
+    cs = it.GeometricCS(it.Jacobi(0.6), 2, 2, 0)
+    H = it.Hierarchy.lattice(cs, 2, 1, 1, 6)
+    bb = it.solve(it.LinearSolver(it.CG(cs), success_if=("<", "reduction", 1e-10),
+                                  failure_if=(">", "step", 100)),
+                  {"matrix": H.top_matrix(), "rhs": H.rhs, "hierarchy": H})
+    assert bb["status"] == "success" and bb["steps"] == so
+    assert np.allclose(bb["defnorms"], dno, rtol=1e-8)
+    assert relerr(bb["solution"].numpy(), xo) < 1e-10
+
+
+def test_config_3d_poisson_64_gs_cg():
+    say("config: 3D Q1 64^3 Poisson, random rhs, V(2,2) multicolour GS, PCG")
+    n, levels = 64, 7
+    oh = po.Hierarchy(3, 1, 1, levels)
+    N = oh.ndofs()
+    assert N == 274625
+    b = np.random.default_rng(2024).uniform(-1, 1, N)
+    b[po.boundary(3, 1, n)] = 0
+    prm = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_SOR, omega=1.0,
+                    ordering=po.ORDER_COLOUR, pre_steps=2, post_steps=2, base_level=0,
+                    symmetric=1, alias=0, tol=1e-10, maxsteps=100)
+    st, xo, so, dno = oh.solve(prm, b)
+    assert st == 0
+    cs = it.GeometricCS(it.SOR(1.0), 2, 2, 0, symmetric=True)
+    H = it.Hierarchy.lattice(cs, 3, 1, 1, levels)
+    bb = it.solve(it.LinearSolver(it.CG(cs), success_if=("<", "reduction", 1e-10),
+                                  failure_if=(">", "step", 100)),
+                  {"matrix": H.top_matrix(), "rhs": GpuVector.from_array(b), "hierarchy": H})
+    assert bb["status"] == "success" and bb["steps"] == so
+    assert np.allclose(bb["defnorms"], dno, rtol=1e-8)
+    assert relerr(bb["solution"].numpy(), xo) < 1e-10
+
+
+def test_config_q2_48_mixed_fcg():
+    say("config: 3D Q2 48^3, kappa = exp(N(0, 1.5^2)), mixed FCG")
+    n0, levels = 3, 5
+    n = n0 << (levels - 1)
+    kap = np.exp(np.random.default_rng(48).normal(scale=1.5, size=(n, n, n)))
+    A, b = lat.assemble(3, 2, n, kap, prec=F.F64)
+    assert A.nrows == 912673 and A.nnz == 57066625
+    cs64 = it.GeometricCS(it.SOR(1.0), 2, 2, 0, symmetric=True)
+    H64 = it.Hierarchy.lattice(cs64, 3, 2, n0, levels, kappa=kap)
+    plain = it.solve(it.LinearSolver(it.CG(cs64), success_if=("<", "reduction", 1e-12),
+                                     failure_if=(">", "step", 300), true_residual=True),
+                     {"matrix": A, "rhs": b, "hierarchy": H64})
+    assert plain["status"] == "success"
+    del H64
+    cs32 = it.GeometricCS(it.SOR(1.0), 2, 2, 0, symmetric=True, prec=F.F32)
+    H32 = it.Hierarchy.lattice(cs32, 3, 2, n0, levels, kappa=kap, mode=F.ASM_MFMA)
+    mixed = it.solve(it.LinearSolver(it.FCG(cs32), success_if=("<", "reduction", 1e-10),
+                                     failure_if=(">", "step", 300), true_residual=True),
+                     {"matrix": A, "rhs": b, "hierarchy": H32})
+    say(f"  plain {plain['steps']} steps, mixed {mixed['steps']} steps")
+    assert mixed["status"] == "success"
+    assert mixed["true_residual"] / np.linalg.norm(b.numpy()) < 1e-10
+    assert relerr(mixed["solution"].numpy(), plain["solution"].numpy()) < 1e-9
