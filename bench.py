@@ -129,19 +129,28 @@ def main():
                 N, nnz = M.nrows, M.nnz
                 y = GpuVector.from_array(np.random.default_rng(0).uniform(-1, 1, N), prec)
                 x, r = GpuVector(N, prec), GpuVector(N, prec)
+                # algorithmic bytes: SpMV, one <sor> sweep, one <jacobi> step (x, b read, new x
+                # written; the diagonal read unless the per-cell factors give it)
                 if form == F.STORAGE_CELL:
                     sb, wb = cell_spmv_bytes(n ** 3, N, vb), cell_sor_bytes(n ** 3, N, vb)
+                    jb = n ** 3 * vb + 3 * N * vb
                 elif form == F.STORAGE_STENCIL:
                     sb, wb = 27 * N * vb + 2 * N * vb, 27 * N * vb + 5 * N * vb
+                    jb = 27 * N * vb + 4 * N * vb
                 else:
                     sb, wb = spmv_bytes(nnz, N, vb), sor_sweep_bytes(nnz, N, vb)
-                for name, fn, byt in ((f"{fname}_spmv_{tag}", lambda: gemm(1.0, M, y, 0.0, x), sb),
-                                      (f"{fname}_sor_{tag}", lambda: sor(M, x, b, r), wb)):
+                    jb = nnz * (vb + 4) + 4 * (N + 1) + 4 * N * vb
+                # 8 <jacobi> steps per call (an even count ends in x: no copy)
+                jac = (lambda: check(lib().flgpu_jacobi_steps(M.h, 0.7, 8, x.h, b.h, r.h)))
+                for name, fn, byt, per in (
+                        (f"{fname}_spmv_{tag}", lambda: gemm(1.0, M, y, 0.0, x), sb, 1),
+                        (f"{fname}_sor_{tag}", lambda: sor(M, x, b, r), wb, 1),
+                        (f"{fname}_jacobi_{tag}", jac, jb, 8)):
                     fn()
                     ctx.timer_start()
                     for _ in range(reps):
                         fn()
-                    ms = ctx.timer_stop() / reps
+                    ms = ctx.timer_stop() / (reps * per)
                     out[name] = {"ms": ms, "bytes": byt, "gbs": byt / (ms * 1e-3) / 1e9}
                 del M
         return out
