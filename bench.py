@@ -304,10 +304,14 @@ def main():
         us = ms * 1e3 / (8 * reps)
         byt = n ** 3 * 4 + 3 * At.nrows * 4  # cell factors, x and b read, new x written
         gbs = byt / (us * 1e-6) / 1e9
+        # HBM bytes per dispatch of this kernel at 128^3 from rocprofv3 FETCH_SIZE (x 2.0) and
+        # WRITE_SIZE (x 1.0), the factors a copy of known size gives; the command and the
+        # table are in profiles/hbm_traffic_q1_128_kernels.md
+        traffic = 35.8e6 if n == 128 else None
         roof = {"bound": "hbm", "kernel": f"k_q1_zblock<float, Q1_JACOBI> (fine-level <jacobi> "
                                           f"step, per-cell form, {n}^3)",
                 "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
-                "traffic": None, "launch_us": us, "bytes_per_launch": byt}
+                "traffic": traffic, "launch_us": us, "bytes_per_launch": byt}
         log(f"[bench] roofline k_q1_zblock<float, Q1_JACOBI>: {us:.2f} us per launch, {gbs:.0f} GB/s "
             f"({100 * gbs / 8000:.1f}% of 8 TB/s)")
         del At, xj, bj, tj
