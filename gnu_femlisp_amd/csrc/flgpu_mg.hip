@@ -228,7 +228,16 @@ static void lmgc(flgpu_mg *mg, int l) {
     }
     C.residual_p = true;
   }
-  lmgc(mg, l - 1);
+  // gamma coarse cycles, one when the coarse level is the base level, none for gamma 0
+  // (multigrid.lisp:268-273).  femlisp brings the coarse residual up to date between them;
+  // here every path that reads it recomputes it first (the smoothers' residual-before, the
+  // restriction's ensure-mg-residual), so that step is left out.
+  int reps = mg->p.gamma == 0 ? 0 : (base ? 1 : mg->p.gamma);
+  for (int i = 0; i < reps; i++) lmgc(mg, l - 1);
+  if (reps == 0) {  // the coarse correction is zero: sol_{l-1} must hold it
+    if (C.x_zero) vfill(mg->ctx, C.N, mg->p.prec, C.sol, 0.0);
+    C.x_zero = false;
+  }
   // prolongate (multigrid.lisp:237-252): sol_l += I sol_{l-1}
   spmv(C.I, 1.0, C.sol, 1.0, lsol(mg, l), lsol(mg, l));
   L.residual_p = false;
@@ -372,6 +381,7 @@ static void check_params(const flgpu_mg_params *p, int nlevels) {
   FLGPU_REQUIRE(p->prec == FLGPU_F64 || p->prec == FLGPU_F32, "unknown precision");
   FLGPU_REQUIRE(p->transfer == FLGPU_TRANSFER_GEOMETRIC || p->transfer == FLGPU_TRANSFER_BLACKBOX,
                 "unknown transfer");
+  FLGPU_REQUIRE(p->gamma >= 0 && p->gamma <= 16, "gamma must be in 0..16");
 }
 
 static void mg_alloc_work(flgpu_mg *mg) {

@@ -53,7 +53,8 @@ class LU(LinearIteration):
 
 
 class GeometricCS(LinearIteration):
-    """<geometric-cs>: correction scheme, multiplicative, gamma = 1.
+    """<geometric-cs>: correction scheme, multiplicative; gamma coarse cycles per
+    visit (1 V-cycle, 2 W-cycle, multigrid.lisp:268-273).
 
     levels: either a lattice description dict(dim=, order=, n0=, nlevels=,
     kappa=, mode=) assembled on the GPU, or explicit (A_list, I_list) of
@@ -63,8 +64,11 @@ class GeometricCS(LinearIteration):
     def __init__(self, smoother=None, pre_steps=1, post_steps=1, base_level=0, gamma=1,
                  symmetric=False, prec=F64, combination="multiplicative", galerkin=False,
                  transfer="geometric"):
-        if gamma != 1 or combination != "multiplicative":
-            raise NotImplementedError("only gamma = 1 multiplicative V-cycles run on the GPU")
+        if combination != "multiplicative":
+            raise NotImplementedError("only the multiplicative combination runs on the GPU")
+        if gamma < 0:
+            raise ValueError("gamma must be >= 0")
+        self.gamma = gamma
         self.smoother = smoother or GaussSeidel()
         self.pre_steps, self.post_steps = pre_steps, post_steps
         self.base_level = base_level
@@ -82,7 +86,7 @@ class GeometricCS(LinearIteration):
         return MgParams(kind, omega, self.pre_steps, self.post_steps, self.base_level,
                         int(self.symmetric), self.prec, int(self.galerkin),
                         _f.TRANSFER_BLACKBOX if self.transfer == "blackbox"
-                        else _f.TRANSFER_GEOMETRIC)
+                        else _f.TRANSFER_GEOMETRIC, int(self.gamma))
 
 
 class CG(LinearIteration):

@@ -235,6 +235,9 @@ typedef struct flgpu_mg_params {
                       interpolation) or FLGPU_TRANSFER_BLACKBOX (3D Q1:
                       operator-dependent interpolation from each finer level's
                       matrix, coarse levels P^T A P whatever galerkin says) */
+  int gamma;       /* coarse cycles per visit of a level (<mg-iteration> gamma,
+                      multigrid.lisp:60,268-273): 1 V-cycle, 2 W-cycle, 0 smoothing
+                      only; one when the coarse level is the base level */
 } flgpu_mg_params;
 
 #define FLGPU_TRANSFER_GEOMETRIC 0

@@ -940,7 +940,16 @@ static void lmgc(mgrun_t *R, int l) {
   memset(C->sol, 0, sizeof(double) * C->N);
   memcpy(C->rhs, C->res, sizeof(double) * C->N);
   C->residual_p = 1;
-  lmgc(R, l - 1);
+  /* gamma coarse cycles, one when the coarse level is the base level, none for gamma 0; the
+   * coarse residual is brought up to date between them (multigrid.lisp:268-273) */
+  int reps = R->p->gamma == 0 ? 0 : (l - 1 == R->p->base_level ? 1 : R->p->gamma);
+  for (int i = reps; i >= 1; i--) {
+    lmgc(R, l - 1);
+    if (i != 1 && !C->residual_p) {
+      compute_residual(&C->A, C->sol, C->rhs, C->res);
+      C->residual_p = 1;
+    }
+  }
   /* prolongate: sol_l += I sol_{l-1}; residual-p nil */
   flo_spmv(C->I.nrows, C->I.rowptr, C->I.colidx, C->I.vals, 1.0, C->sol, 1.0, L->sol);
   L->residual_p = 0;

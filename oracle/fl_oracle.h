@@ -139,6 +139,8 @@ typedef struct flo_params {
   int stop;         /* FLO_STOP_* */
   double tol;
   int maxsteps;     /* failure-if (> :step maxsteps) */
+  int gamma;        /* coarse cycles per visit: 1 V-cycle, 2 W-cycle, 0 smoothing only
+                       (<mg-iteration> gamma, multigrid.lisp:60,268-273) */
 } flo_params;
 
 /* Solve A x = b from x = 0 with femlisp's <linear-solver> loop

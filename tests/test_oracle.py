@@ -153,6 +153,41 @@ def test_blackbox_hierarchy_beats_geometric_on_inclusions():
     assert steps["blackbox"] < steps["geometric"]
 
 
+def _cycle_matrix(h, prm):
+    N = h.ndofs()
+    return np.stack([h.apply(prm, e) for e in np.eye(N)], axis=1)
+
+
+def test_gamma_cycles():
+    """femlisp's gamma (multigrid.lisp:268-273) pinned by what it must reduce to:
+    gamma = 0 is smoothing only (pre + post damped Jacobi from zero, restated
+    with scipy); on two levels every gamma is the V-cycle (the coarse level is
+    the base); and with many coarse cycles the three-level W-cycle approaches
+    the two-grid method with an exact coarse solve."""
+    h = po.Hierarchy(1, 1, 2, 4)
+    rp, ci, v = h.matrix(3)
+    A = csr(rp, ci, v).toarray()
+    d = np.diag(A)
+    base = dict(outer=po.OUTER_MG, smoother=po.SMOOTH_JACOBI, omega=0.6, pre_steps=2,
+                post_steps=1)
+    # gamma 0: x = post(pre(0)) with 3 Jacobi steps on b
+    M0 = _cycle_matrix(h, po.params(**base, base_level=0, gamma=0))
+    S = np.eye(len(d)) - 0.6 * (A / d[:, None])
+    Mj = np.zeros_like(A)
+    for _ in range(3):
+        Mj = S @ Mj + 0.6 * np.diag(1 / d)
+    assert np.abs(M0 - Mj).max() < 1e-14
+    # two levels (base 2 of levels 2, 3): gamma does not matter
+    M1 = _cycle_matrix(h, po.params(**base, base_level=2, gamma=1))
+    M2 = _cycle_matrix(h, po.params(**base, base_level=2, gamma=2))
+    assert np.array_equal(M1, M2)
+    # W-cycle with many coarse cycles on levels 3, 2, 1 -> exact solve on level 2
+    Mw = _cycle_matrix(h, po.params(**base, base_level=1, gamma=60))
+    Mw1 = _cycle_matrix(h, po.params(**base, base_level=1, gamma=1))
+    assert np.abs(Mw - M1).max() < 1e-10 * np.abs(M1).max()
+    assert np.abs(Mw1 - M1).max() > 1e-6 * np.abs(M1).max()
+
+
 def test_fe_value_1d():
     """ellsys-fe.lisp:249-256: 1D Q1 -u''=1 on level 2 gives u(0.5) = 0.125 exactly."""
     rp, ci, v, b = po.assemble(1, 1, 4)
