@@ -204,7 +204,10 @@ static void lmgc(flgpu_mg *mg, int l) {
     L.residual_p = false;
     return;
   }
-  smooth(mg, l, false);
+  // :multiplicative smooths before restricting; :additive restricts the residual first and
+  // smooths the same level from it (multigrid.lisp:262-269)
+  const bool add = mg->p.additive != 0;
+  if (!add) smooth(mg, l, false);
   // restrict (multigrid.lisp:202-224): res_{l-1} = I^T res_l
   ensure_residual(mg, l);
   MGLevel &C = mg->lv[l - 1];
@@ -213,7 +216,7 @@ static void lmgc(flgpu_mg *mg, int l) {
   // stored zero, and fused steps recompute the residual from rhs, so the restriction goes
   // straight into rhs there (the coarse solve reads only res)
   bool base = (l - 1 == mg->p.base_level);
-  bool fused = mg->p.smoother == FLGPU_SMOOTH_JACOBI && C.tmp && mg->p.pre_steps > 0;
+  bool fused = !add && mg->p.smoother == FLGPU_SMOOTH_JACOBI && C.tmp && mg->p.pre_steps > 0;
   if (fused && !base) {
     spmv(C.I->transpose, 1.0, lres(mg, l), 0.0, C.rhs, C.rhs);
     C.x_zero = true;
@@ -228,6 +231,7 @@ static void lmgc(flgpu_mg *mg, int l) {
     }
     C.residual_p = true;
   }
+  if (add) smooth(mg, l, false);
   // gamma coarse cycles, one when the coarse level is the base level, none for gamma 0
   // (multigrid.lisp:268-273).  femlisp brings the coarse residual up to date between them;
   // here every path that reads it recomputes it first (the smoothers' residual-before, the
@@ -238,10 +242,11 @@ static void lmgc(flgpu_mg *mg, int l) {
     if (C.x_zero) vfill(mg->ctx, C.N, mg->p.prec, C.sol, 0.0);
     C.x_zero = false;
   }
+  if (add) smooth(mg, l, true);  // :additive post-smooths before the prolongation
   // prolongate (multigrid.lisp:237-252): sol_l += I sol_{l-1}
   spmv(C.I, 1.0, C.sol, 1.0, lsol(mg, l), lsol(mg, l));
   L.residual_p = false;
-  smooth(mg, l, true);
+  if (!add) smooth(mg, l, true);
 }
 
 // the <mg-iteration> iterator on device pointers of the hierarchy's precision
@@ -310,8 +315,10 @@ static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work)
   long N = T.N;
   bool base_top = mg->top == p.base_level;  // one level: the coarse solve reads r
   bool jac = p.smoother == FLGPU_SMOOTH_JACOBI && T.tmp && p.pre_steps > 0 && !base_top;
-  bool reads_r = base_top || !(jac || (p.smoother == FLGPU_SMOOTH_SOR && p.pre_steps > 0));
-  bool zero = jac || base_top;  // the first step writes x
+  // :additive restricts before smoothing: the top residual is read and x must be stored
+  bool add = p.additive != 0;
+  bool reads_r = base_top || add || !(jac || (p.smoother == FLGPU_SMOOTH_SOR && p.pre_steps > 0));
+  bool zero = (jac && !add) || base_top;  // the first step writes x
   if (vprec == p.prec) {
     if (zero) T.x_zero = true;
     else vfill(mg->ctx, N, vprec, q, 0.0);
@@ -382,6 +389,7 @@ static void check_params(const flgpu_mg_params *p, int nlevels) {
   FLGPU_REQUIRE(p->transfer == FLGPU_TRANSFER_GEOMETRIC || p->transfer == FLGPU_TRANSFER_BLACKBOX,
                 "unknown transfer");
   FLGPU_REQUIRE(p->gamma >= 0 && p->gamma <= 16, "gamma must be in 0..16");
+  FLGPU_REQUIRE(p->additive == 0 || p->additive == 1, "additive must be 0 or 1");
 }
 
 static void mg_alloc_work(flgpu_mg *mg) {

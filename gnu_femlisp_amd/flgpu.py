@@ -37,7 +37,8 @@ class MgParams(ctypes.Structure):
                 ("pre_steps", ctypes.c_int), ("post_steps", ctypes.c_int),
                 ("base_level", ctypes.c_int), ("symmetric", ctypes.c_int),
                 ("prec", ctypes.c_int), ("galerkin", ctypes.c_int),
-                ("transfer", ctypes.c_int), ("gamma", ctypes.c_int)]
+                ("transfer", ctypes.c_int), ("gamma", ctypes.c_int),
+                ("additive", ctypes.c_int)]
 
 
 class SolveParams(ctypes.Structure):

@@ -53,8 +53,9 @@ class LU(LinearIteration):
 
 
 class GeometricCS(LinearIteration):
-    """<geometric-cs>: correction scheme, multiplicative; gamma coarse cycles per
-    visit (1 V-cycle, 2 W-cycle, multigrid.lisp:268-273).
+    """<geometric-cs>: correction scheme, :multiplicative or :additive
+    combination (multigrid.lisp:67-70,262-287); gamma coarse cycles per visit
+    (1 V-cycle, 2 W-cycle, multigrid.lisp:268-273).
 
     levels: either a lattice description dict(dim=, order=, n0=, nlevels=,
     kappa=, mode=) assembled on the GPU, or explicit (A_list, I_list) of
@@ -64,8 +65,9 @@ class GeometricCS(LinearIteration):
     def __init__(self, smoother=None, pre_steps=1, post_steps=1, base_level=0, gamma=1,
                  symmetric=False, prec=F64, combination="multiplicative", galerkin=False,
                  transfer="geometric"):
-        if combination != "multiplicative":
-            raise NotImplementedError("only the multiplicative combination runs on the GPU")
+        if combination not in ("multiplicative", "additive"):
+            raise ValueError("combination must be 'multiplicative' or 'additive'")
+        self.combination = combination
         if gamma < 0:
             raise ValueError("gamma must be >= 0")
         self.gamma = gamma
@@ -86,7 +88,8 @@ class GeometricCS(LinearIteration):
         return MgParams(kind, omega, self.pre_steps, self.post_steps, self.base_level,
                         int(self.symmetric), self.prec, int(self.galerkin),
                         _f.TRANSFER_BLACKBOX if self.transfer == "blackbox"
-                        else _f.TRANSFER_GEOMETRIC, int(self.gamma))
+                        else _f.TRANSFER_GEOMETRIC, int(self.gamma),
+                        int(self.combination == "additive"))
 
 
 class CG(LinearIteration):

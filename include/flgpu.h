@@ -238,6 +238,9 @@ typedef struct flgpu_mg_params {
   int gamma;       /* coarse cycles per visit of a level (<mg-iteration> gamma,
                       multigrid.lisp:60,268-273): 1 V-cycle, 2 W-cycle, 0 smoothing
                       only; one when the coarse level is the base level */
+  int additive;    /* 0 :multiplicative, 1 :additive combination (multigrid.lisp:
+                      67-70,262-287): restrict first, then smooth the level; post-smooth,
+                      then prolongate */
 } flgpu_mg_params;
 
 #define FLGPU_TRANSFER_GEOMETRIC 0

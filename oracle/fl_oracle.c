@@ -916,7 +916,7 @@ static void smoother_run(mgrun_t *R, int l, int post) {
   L->residual_p = 0;
 }
 
-/* lmgc with the correction scheme, multiplicative, gamma = 1
+/* lmgc with the correction scheme, :multiplicative or :additive, gamma coarse cycles
  * (multigrid.lisp:202-252,254-288). */
 static void lmgc(mgrun_t *R, int l) {
   flo_mg *mg = R->mg;
@@ -928,7 +928,7 @@ static void lmgc(mgrun_t *R, int l) {
     L->residual_p = 0;
     return;
   }
-  smoother_run(R, l, 0);
+  if (!R->p->additive) smoother_run(R, l, 0);
   /* restrict: ensure-mg-residual, res_{l-1} = I^T res_l */
   if (!L->residual_p) {
     compute_residual(&L->A, L->sol, L->rhs, L->res);
@@ -940,6 +940,8 @@ static void lmgc(mgrun_t *R, int l) {
   memset(C->sol, 0, sizeof(double) * C->N);
   memcpy(C->rhs, C->res, sizeof(double) * C->N);
   C->residual_p = 1;
+  /* :additive smooths the fine level after the restriction, from the same residual */
+  if (R->p->additive) smoother_run(R, l, 0);
   /* gamma coarse cycles, one when the coarse level is the base level, none for gamma 0; the
    * coarse residual is brought up to date between them (multigrid.lisp:268-273) */
   int reps = R->p->gamma == 0 ? 0 : (l - 1 == R->p->base_level ? 1 : R->p->gamma);
@@ -950,10 +952,12 @@ static void lmgc(mgrun_t *R, int l) {
       C->residual_p = 1;
     }
   }
+  /* :additive post-smooths before the prolongation */
+  if (R->p->additive) smoother_run(R, l, 1);
   /* prolongate: sol_l += I sol_{l-1}; residual-p nil */
   flo_spmv(C->I.nrows, C->I.rowptr, C->I.colidx, C->I.vals, 1.0, C->sol, 1.0, L->sol);
   L->residual_p = 0;
-  smoother_run(R, l, 1);
+  if (!R->p->additive) smoother_run(R, l, 1);
 }
 
 static void mgrun_init(mgrun_t *R, flo_mg *mg, const flo_params *p) {

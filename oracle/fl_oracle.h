@@ -141,6 +141,8 @@ typedef struct flo_params {
   int maxsteps;     /* failure-if (> :step maxsteps) */
   int gamma;        /* coarse cycles per visit: 1 V-cycle, 2 W-cycle, 0 smoothing only
                        (<mg-iteration> gamma, multigrid.lisp:60,268-273) */
+  int additive;     /* 0 :multiplicative, 1 :additive combination (multigrid.lisp:67-70,
+                       262-287): restrict first, then smooth; post-smooth, then prolongate */
 } flo_params;
 
 /* Solve A x = b from x = 0 with femlisp's <linear-solver> loop

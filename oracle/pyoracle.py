@@ -33,14 +33,15 @@ class Params(ctypes.Structure):
         ("tol", ctypes.c_double),
         ("maxsteps", ctypes.c_int),
         ("gamma", ctypes.c_int),
+        ("additive", ctypes.c_int),
     ]
 
 
 def params(outer=OUTER_MG, smoother=SMOOTH_JACOBI, omega=1.0, ordering=ORDER_INDEX,
            pre_steps=1, post_steps=1, base_level=0, symmetric=0, alias=0,
-           stop=STOP_REDUCTION, tol=1e-10, maxsteps=100, gamma=1):
+           stop=STOP_REDUCTION, tol=1e-10, maxsteps=100, gamma=1, additive=0):
     return Params(outer, smoother, omega, ordering, pre_steps, post_steps,
-                  base_level, symmetric, alias, stop, tol, maxsteps, gamma)
+                  base_level, symmetric, alias, stop, tol, maxsteps, gamma, additive)
 
 
 def lib():

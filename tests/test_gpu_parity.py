@@ -270,10 +270,12 @@ def test_jacobi_steps_match_product_iterator(form, prec, steps):
 
 
 def _gpu_hier(dim, order, n0, levels, smoother, omega, pre, post, base, symmetric=False,
-              kappa=None, prec=F.F64, mode=F.ASM_F64, gamma=1, transfer="geometric"):
+              kappa=None, prec=F.F64, mode=F.ASM_F64, gamma=1, transfer="geometric",
+              additive=False):
     sm = it.Jacobi(omega) if smoother == po.SMOOTH_JACOBI else it.SOR(omega)
     cs = it.GeometricCS(sm, pre, post, base, gamma=gamma, symmetric=symmetric, prec=prec,
-                        transfer=transfer)
+                        transfer=transfer,
+                        combination="additive" if additive else "multiplicative")
     return cs, it.Hierarchy.lattice(cs, dim, order, n0, levels, kappa=kappa, mode=mode)
 
 
@@ -391,16 +393,18 @@ def test_solve_parity(case):
     assert np.allclose(bb["defnorms"], dno, rtol=1e-8)
 
 
-@pytest.mark.parametrize("smoother,omega,steps,base,outer,gamma", [
-    (po.SMOOTH_SOR, 1.0, 2, 0, po.OUTER_CG, 2),
-    (po.SMOOTH_JACOBI, 0.7, 4, 1, po.OUTER_FCG, 2),
-    (po.SMOOTH_JACOBI, 0.7, 2, 1, po.OUTER_MG, 0),
-    (po.SMOOTH_SOR, 1.0, 1, 1, po.OUTER_MG, 3),
+@pytest.mark.parametrize("smoother,omega,steps,base,outer,gamma,additive", [
+    (po.SMOOTH_SOR, 1.0, 2, 0, po.OUTER_CG, 2, 0),
+    (po.SMOOTH_JACOBI, 0.7, 4, 1, po.OUTER_FCG, 2, 0),
+    (po.SMOOTH_JACOBI, 0.7, 2, 1, po.OUTER_MG, 0, 0),
+    (po.SMOOTH_SOR, 1.0, 1, 1, po.OUTER_MG, 3, 0),
+    (po.SMOOTH_JACOBI, 0.5, 2, 0, po.OUTER_FCG, 1, 1),
+    (po.SMOOTH_SOR, 1.0, 1, 1, po.OUTER_FCG, 2, 1),
 ])
-def test_gamma_solve_parity(smoother, omega, steps, base, outer, gamma):
-    """W-cycles (gamma 2, 3) and smoothing only (gamma 0): the oracle's steps,
-    defect history and solution."""
-    say(f"gamma solve smoother={smoother} gamma={gamma} outer={outer}")
+def test_gamma_solve_parity(smoother, omega, steps, base, outer, gamma, additive):
+    """W-cycles (gamma 2, 3), smoothing only (gamma 0) and the :additive
+    combination: the oracle's steps, defect history and solution."""
+    say(f"gamma solve smoother={smoother} gamma={gamma} outer={outer} additive={additive}")
     dim, order, n0, levels = 3, 1, 1, 5
     rng = np.random.default_rng(29)
     ntop = n0 << (levels - 1)
@@ -412,10 +416,11 @@ def test_gamma_solve_parity(smoother, omega, steps, base, outer, gamma):
     maxsteps = 400 if gamma == 0 else 100
     prm = po.params(outer=outer, smoother=smoother, omega=omega, ordering=po.ORDER_COLOUR,
                     pre_steps=steps, post_steps=steps, base_level=base, symmetric=sym, alias=0,
-                    tol=1e-8 if gamma == 0 else 1e-10, maxsteps=maxsteps, gamma=gamma)
+                    tol=1e-8 if gamma == 0 else 1e-10, maxsteps=maxsteps, gamma=gamma,
+                    additive=additive)
     st, xo, so, dno = oh.solve(prm, b)
     cs, H = _gpu_hier(dim, order, n0, levels, smoother, omega, steps, steps, base,
-                      symmetric=bool(sym), kappa=kap, gamma=gamma)
+                      symmetric=bool(sym), kappa=kap, gamma=gamma, additive=bool(additive))
     itn = {po.OUTER_MG: cs, po.OUTER_CG: it.CG(cs), po.OUTER_FCG: it.FCG(cs)}[outer]
     bb = it.solve(it.LinearSolver(itn, success_if=("<", "reduction", prm.tol),
                                   failure_if=(">", "step", maxsteps)),

@@ -188,6 +188,26 @@ def test_gamma_cycles():
     assert np.abs(Mw1 - M1).max() > 1e-6 * np.abs(M1).max()
 
 
+def test_additive_combination():
+    """:additive (multigrid.lisp:262-287) restricts the residual before smoothing, so on
+    two levels its cycle is the smoothing-only cycle plus P A_c^-1 P^T exactly, with
+    P and A_c read back from the hierarchy; the multiplicative cycle is not."""
+    for dim, order, n0 in ((1, 1, 4), (2, 1, 2), (2, 2, 1)):
+        h = po.Hierarchy(dim, order, n0, 2)
+        base = dict(outer=po.OUTER_MG, smoother=po.SMOOTH_JACOBI, omega=0.6, pre_steps=2,
+                    post_steps=1, base_level=0)
+        Madd = _cycle_matrix(h, po.params(**base, additive=1))
+        Msm = _cycle_matrix(h, po.params(**base, gamma=0))
+        Mmul = _cycle_matrix(h, po.params(**base))
+        prp, pci, pv = h.interp(0)
+        P = sp.csr_matrix((pv, pci, prp), shape=(h.ndofs(1), h.ndofs(0))).toarray()
+        crp, cci, cv = h.matrix(0)
+        Ac = csr(crp, cci, cv).toarray()
+        C = P @ np.linalg.inv(Ac) @ P.T
+        assert np.abs(Madd - (Msm + C)).max() < 1e-12 * np.abs(Madd).max()
+        assert np.abs(Mmul - (Msm + C)).max() > 1e-6 * np.abs(Madd).max()
+
+
 def test_fe_value_1d():
     """ellsys-fe.lisp:249-256: 1D Q1 -u''=1 on level 2 gives u(0.5) = 0.125 exactly."""
     rp, ci, v, b = po.assemble(1, 1, 4)
