@@ -120,6 +120,8 @@ SIGNATURES = {
     "flgpu_unet_pack": (_ci, [_vp, _vp, _ci, _vp, _cl, _ci]),
     "flgpu_vec_gather_rows": (_ci, [_vp, _cl, _vp, _ci, _vp]),
     "flgpu_mg_attach_unet": (_ci, [_vp, _vp, _vp, _ci]),
+    "flgpu_system_read": (_ci, [_vp, ctypes.c_char_p, _ci, _P(_vp), _P(_vp)]),
+    "flgpu_system_write": (_ci, [ctypes.c_char_p, _vp, _vp]),
 }
 
 _LIB = None

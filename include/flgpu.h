@@ -215,6 +215,23 @@ int flgpu_lattice_element_matrices(flgpu_ctx *ctx, int dim, int order, int n,
                                    int mode, long first_cell, long ncells,
                                    double *out);
 
+/* ---- system files (next to femlisp's export sparse-matrix->ccs,
+ *      src/matlisp/sparselu.lisp:235-333) --------------------------------- */
+
+/* Read a system file: magic "FLSYS01\0", int32 orientation (0 rows / 1 columns
+ * compressed), int32 flags (1: rhs follows, 2: key permutation follows), int64
+ * nrows, ncols, nnz, int32 starts[major + 1], int32 indices[nnz], float64
+ * values[nnz], [float64 rhs[nrows]], [int32 perm[nrows]] (little endian).
+ * perm[k] is the canonical index of the file's row and column k (the rank of
+ * its DOF midpoint in femlisp's sort-lexicographically order,
+ * src/mesh/meshgen.lisp:214-217).  The matrix is returned in canonical
+ * numbering as CSR with ascending columns; *rhs (if rhs is not NULL) receives
+ * the permuted right-hand side or NULL when the file has none. */
+int flgpu_system_read(flgpu_ctx *ctx, const char *path, int prec, flgpu_mat **A,
+                      flgpu_vec **rhs);
+/* Write A (rows compressed, no permutation) and, if not NULL, rhs. */
+int flgpu_system_write(const char *path, const flgpu_mat *A, const flgpu_vec *rhs);
+
 /* ---- geometric multigrid (src/iteration/multigrid.lisp:55-393,
  *      geomg.lisp:109-142) ------------------------------------------------- */
 
