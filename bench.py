@@ -156,7 +156,8 @@ def main():
         return out
 
     if args.kernels_only:
-        fine_kernels(args.kernel_reps)
+        for k, v in fine_kernels(args.kernel_reps).items():
+            log(f"[bench] {k}: {v['ms'] * 1e3:.1f} us, {v['gbs']:.0f} GB/s")
         src, dst = GpuVector(1 << 26), GpuVector(1 << 26)  # 512 MiB read + 512 MiB written
         for _ in range(args.kernel_reps):
             vcopy(src, dst)

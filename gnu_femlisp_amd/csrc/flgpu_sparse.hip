@@ -516,111 +516,222 @@ __device__ __forceinline__ T q1_row(int M, int n, long i, int z, int y, int x,
 }
 
 // ---- whole-lattice passes in the per-cell form, blocked along z ---------------------------
-// A thread owns QZB consecutive rows along z at one (y, x).  It loads the 3x3 column of x over
-// the QZB + 2 planes and the 2x2 column of cell factors over QZB + 1 cell planes once (54 + 20
-// loads for 4 rows instead of 4 x 35), all issued before the first product, and forms each
-// row with exactly q1_row's arithmetic (same terms, same order).  Neighbouring threads are
-// neighbours in x, so every load instruction of a wave is contiguous.
-constexpr int QZB = 4;
+// A thread owns QZB consecutive rows along z at one (y, x).  It loads its own x column of the
+// 3 lines y-1..y+1 over the QZB + 2 planes and its own cell-factor column of the lines y-1, y
+// over QZB + 1 cell planes (18 + 10 loads for 4 rows), and takes the x-1 and x+1 columns from
+// the neighbouring lanes of its wave (DPP whole-wave shifts).  Each wave computes 62
+// consecutive positions; its two edge lanes only load the columns its first and last position
+// need.  Dirichlet values are zeroed as they are loaded, which is what the explicit zeros of
+// the eliminated columns contribute.  Each row is formed with exactly q1_row's arithmetic (same
+// terms, same order; the cell sums start from their first term, as alpha > 0).
 enum { Q1_SPMV = 0, Q1_JACOBI = 1 };
+
+template <class T> struct Q1KT {
+  T k[8];
+};
+
+// value of lane - 1 (DPP wave_shr:1) and of lane + 1 (DPP wave_shl:1); 0 on the end lanes
+__device__ __forceinline__ int lane_up_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true);
+}
+__device__ __forceinline__ int lane_down_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true);
+}
+__device__ __forceinline__ float lane_up(float v) { return __int_as_float(lane_up_i(__float_as_int(v))); }
+__device__ __forceinline__ float lane_down(float v) {
+  return __int_as_float(lane_down_i(__float_as_int(v)));
+}
+__device__ __forceinline__ double lane_up(double v) {
+  return __hiloint2double(lane_up_i(__double2hiint(v)), lane_up_i(__double2loint(v)));
+}
+__device__ __forceinline__ double lane_down(double v) {
+  return __hiloint2double(lane_down_i(__double2hiint(v)), lane_down_i(__double2loint(v)));
+}
+
+template <class T, class I>
+__device__ __forceinline__ T ld_at(const T *__restrict__ p, I byte_offset) {
+  return *(const T *)((const char *)p + byte_offset);
+}
 
 // OP = Q1_SPMV:   out_i = beta z_i + sum_j (coef x_j) A_ij          (v = z)
 // OP = Q1_JACOBI: out_i = x_i + coef (b_i - sum_j A_ij x_j) / A_ii  (v = b, coef = damp;
 //                 A_ii from the factors, 1 on Dirichlet rows)
-template <class T, int OP>
+// I is the offset type: unsigned when the byte size of a lattice vector fits 32 bits and the
+// position count 31 bits.  QZB rows per thread.
+template <class T, int OP, class I, int QZB>
 __global__ __launch_bounds__(256) void k_q1_zblock(int M, int n, const T *__restrict__ alpha,
-                                                   Q1K kt, T coef, T beta,
+                                                   Q1KT<T> kt, T coef, T beta,
                                                    const T *__restrict__ v,
                                                    const T *__restrict__ x,
                                                    T *__restrict__ out) {
   unsigned lb = xcd_block(blockIdx.x, gridDim.x);
-  long t = (long)lb * blockDim.x + threadIdx.x;
-  long M2 = (long)M * M;
-  int nzb = (M + QZB - 1) / QZB;
-  if (t >= M2 * nzb) return;
-  int gx = (int)(t % M), gy = (int)((t / M) % M), z0 = (int)(t / M2) * QZB;
-  // neighbour coordinates clamped into the lattice: a clamped value only meets a zero
-  // coefficient (Dirichlet column) or a boundary row, which reads no neighbours
-  int xc[3], yc[3];
+  const int lane = threadIdx.x & 63;
+  const I Mi = (I)M, M2 = Mi * Mi;
+  const I total = M2 * (I)((M + QZB - 1) / QZB);
+  const I tw = ((I)lb * (blockDim.x / 64) + (threadIdx.x >> 6)) * 62 + (I)lane;  // position + 1
+  const bool compute = lane >= 1 && lane <= 62 && tw <= total;  // lanes 0 and 63 are halo
+  const I t = tw == 0 ? (I)0 : (tw - 1 < total ? tw - 1 : total - 1);  // every lane loads
+  const I line = t / Mi, zb = line / Mi;
+  const int gx = (int)(t - line * Mi), gy = (int)(line - zb * Mi), z0 = (int)zb * QZB;
+  const bool xbd = gx == 0 || gx == M - 1;
+  // byte offsets (the narrow I keeps them in 32 bits: loads then take one VGPR offset from
+  // the SGPR base)
+  constexpr I B = (I)sizeof(T);
+  const I M2B = M2 * B;
+  I yo[3];
+  bool ybd[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    xc[d] = min(max(gx + d - 1, 0), M - 1);
-    yc[d] = min(max(gy + d - 1, 0), M - 1);
+    int y = min(max(gy + d - 1, 0), M - 1);
+    yo[d] = (I)(y * M + gx) * B;
+    ybd[d] = y == 0 || y == M - 1;
   }
-  T xv[QZB + 2][3][3];
+  // plane k (z = z0 - 1 + k, clamped to the lattice) starts at zpl[k]; the stride is uniform,
+  // so only plane 0 needs a product (zbase wraps around for z0 = 0, where plane 0 clamps to 0)
+  I zpl[QZB + 2];
+  const I zbase = (I)z0 * M2B - M2B, ztop = (I)(M - 1) * M2B;
 #pragma unroll
-  for (int k = 0; k < QZB + 2; k++) {
-    long zo = (long)min(max(z0 - 1 + k, 0), M - 1) * M2;
+  for (int k = 0; k < QZB + 2; k++)
+    zpl[k] = k == 0 ? (z0 == 0 ? (I)0 : zbase) : min(zbase + (I)k * M2B, ztop);
+  const I nnB = (I)n * (I)n * B;
+  I co[2];
 #pragma unroll
-    for (int dy = 0; dy < 3; dy++)
+  for (int q = 0; q < 2; q++) co[q] = (I)(min(max(gy - 1 + q, 0), n - 1) * n + min(gx, n - 1)) * B;
+  const I cb = (I)z0 * nnB - nnB, ctop = (I)(n - 1) * nnB;
+
+  // loads, issued in the order the rows consume them (row r needs x planes r..r+2, cell
+  // planes r, r+1 and v at row r): the in-order memory counter then lets row r start while
+  // the later planes are in flight
+  T xv[QZB + 2][3][3], xi[QZB], av[QZB + 1][2][2], vr[QZB];
+  auto load_x = [&](int k) {
 #pragma unroll
-      for (int dx = 0; dx < 3; dx++) xv[k][dy][dx] = x[zo + (long)yc[dy] * M + xc[dx]];
-  }
-  int cxs[2], cys[2];
+    for (int dy = 0; dy < 3; dy++) xv[k][dy][1] = ld_at(x, zpl[k] + yo[dy]);
+  };
+  auto load_a = [&](int k) {
+    I zo = k == 0 ? (z0 == 0 ? (I)0 : cb) : min(cb + (I)k * nnB, ctop);
 #pragma unroll
-  for (int s = 0; s < 2; s++) {
-    cxs[s] = min(max(gx - 1 + s, 0), n - 1);
-    cys[s] = min(max(gy - 1 + s, 0), n - 1);
-  }
-  T av[QZB + 1][2][2];
-#pragma unroll
-  for (int k = 0; k < QZB + 1; k++) {
-    long co = (long)min(max(z0 - 1 + k, 0), n - 1) * n;
-#pragma unroll
-    for (int q = 0; q < 2; q++)
-#pragma unroll
-      for (int s = 0; s < 2; s++) av[k][q][s] = alpha[(co + cys[q]) * n + cxs[s]];
-  }
-  bool ly[3] = {gy - 1 != 0, true, gy + 1 != M - 1}, lx[3] = {gx - 1 != 0, true, gx + 1 != M - 1};
-  bool side = gx == 0 || gy == 0 || gx == M - 1 || gy == M - 1;
+    for (int q = 0; q < 2; q++) av[k][q][1] = ld_at(alpha, zo + co[q]);
+  };
+  load_x(0);
+  load_x(1);
+  load_a(0);
 #pragma unroll
   for (int r = 0; r < QZB; r++) {
-    int zz = z0 + r;
-    if (zz >= M) break;
-    long i = zz * M2 + (long)gy * M + gx;
-    T xi = xv[r + 1][1][1];
-    T acc, d = (T)1;
-    if (OP == Q1_SPMV) acc = (beta == (T)0) ? (T)0 : beta * v[i];
-    else acc = v[i];
-    T cf = (OP == Q1_SPMV) ? coef : (T)-1;
-    if (side || zz == 0 || zz == M - 1) {
-      acc += (cf * xi) * (T)1;
-    } else {
-      bool lz[3] = {zz - 1 != 0, true, zz + 1 != M - 1};
+    load_x(r + 2);
+    load_a(r + 1);
+    if (OP == Q1_SPMV) vr[r] = (beta == (T)0) ? (T)0 : ld_at(v, zpl[r + 1] + yo[1]);
+    else vr[r] = ld_at(v, zpl[r + 1] + yo[1]);
+  }
+
+  // Dirichlet values zeroed (the row's own value kept in xi); the x-1 / x+1 columns and the x-1
+  // cell column from the neighbouring lanes: the same line and z-block for every interior row
+  // (a line ends only at a Dirichlet row, which reads none).  Every lane runs to the end (the
+  // shifts read all lanes); only computing lanes store.
+  auto prep_x = [&](int k) {
+    int z = min(max(z0 - 1 + k, 0), M - 1);
+    bool zbd = z == 0 || z == M - 1;
 #pragma unroll
-      for (int k = 0; k < 27; k++) {
-        int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
-        int pz0 = dz > 0 ? 1 : 0, pz1 = dz < 0 ? 0 : 1;
-        int py0 = dy > 0 ? 1 : 0, py1 = dy < 0 ? 0 : 1;
-        int px0 = dx > 0 ? 1 : 0, px1 = dx < 0 ? 0 : 1;
-        T sum = (T)0;
-#pragma unroll
-        for (int p = pz0; p <= pz1; p++)
-#pragma unroll
-          for (int q = py0; q <= py1; q++)
-#pragma unroll
-            for (int s = px0; s <= px1; s++) sum += av[r + p][q][s];
-        T aij = (T)kt.k[(dz != 0) * 4 + (dy != 0) * 2 + (dx != 0)] * sum;
-        if (k == 13) d = aij;
-        bool live = lz[dz + 1] && ly[dy + 1] && lx[dx + 1];
-        acc += (cf * xv[r + 1 + dz][dy + 1][dx + 1]) * (live ? aij : (T)0);
-      }
+    for (int dy = 0; dy < 3; dy++) {
+      T val = xv[k][dy][1];
+      if (dy == 1 && k >= 1 && k <= QZB) xi[k - 1] = val;
+      val = (zbd || ybd[dy] || xbd) ? (T)0 : val;
+      xv[k][dy][1] = val;
+      xv[k][dy][0] = lane_up(val);
+      xv[k][dy][2] = lane_down(val);
     }
-    if (OP == Q1_SPMV) out[i] = acc;
-    else out[i] = xi + coef * (acc / d);
+  };
+  // cs[k][dy][dx]: the ordered sum over cell plane k of the cells an entry with in-plane
+  // offset (dy, dx) shares; pl = kt * cs is the entry itself when the offset crosses planes
+  // (dz = -1 of row k, dz = +1 of row k - 1).  An entry within a plane (dz = 0) continues
+  // row r's plane-r sum with the plane-(r + 1) cells, which is q1_row's order.
+  T cs[QZB + 1][3][3], pl[QZB + 1][3][3];
+  auto prep_a = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) av[k][q][0] = lane_up(av[k][q][1]);
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+      int dy = e / 3 - 1, dx = e % 3 - 1;
+      T sum = (T)-0.0;
+#pragma unroll
+      for (int q = (dy > 0); q <= (dy >= 0); q++)
+#pragma unroll
+        for (int c = (dx > 0); c <= (dx >= 0); c++) sum += av[k][q][c];
+      cs[k][dy + 1][dx + 1] = sum;
+      pl[k][dy + 1][dx + 1] = kt.k[4 + (dy != 0) * 2 + (dx != 0)] * sum;
+    }
+  };
+  const bool side = xbd || gy == 0 || gy == M - 1;
+  const T cf = (OP == Q1_SPMV) ? coef : (T)-1;
+  prep_x(0);
+  prep_x(1);
+  prep_a(0);
+#pragma unroll
+  for (int r = 0; r < QZB; r++) {
+    prep_x(r + 2);
+    prep_a(r + 1);
+    const int zz = z0 + r;
+    const I i = zpl[r + 1] + yo[1];  // row zz, clamped to the lattice
+    const T acc0 = vr[r];
+    T acc = acc0, d = (T)1;
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+      int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+      T aij;
+      if (dz != 0) {
+        aij = pl[r + (dz > 0)][dy + 1][dx + 1];
+      } else {
+        T sum = cs[r][dy + 1][dx + 1];
+#pragma unroll
+        for (int q = (dy > 0); q <= (dy >= 0); q++)
+#pragma unroll
+          for (int c = (dx > 0); c <= (dx >= 0); c++) sum += av[r + 1][q][c];
+        aij = kt.k[(dy != 0) * 2 + (dx != 0)] * sum;
+      }
+      if (k == 13) d = aij;
+      acc += (cf * xv[r + 1 + dz][dy + 1][dx + 1]) * aij;
+    }
+    const bool dir = side || zz == 0 || zz == M - 1;
+    T res;
+    if (OP == Q1_SPMV) {
+      res = dir ? acc0 + (cf * xi[r]) * (T)1 : acc;
+    } else {
+      T accd = acc0 + (cf * xi[r]) * (T)1;
+      res = xi[r] + coef * (dir ? accd / (T)1 : acc / d);
+    }
+    if (compute && zz < M) *(T *)((char *)out + i) = res;
   }
 }
+
+// rows per thread of the per-cell passes (measured on 129^3 lattices: FP32 5 rows; FP64 3 for
+// the product, 4 for the <jacobi> step)
+#ifndef FLGPU_QZB_F32
+#define FLGPU_QZB_F32 5
+#endif
+#ifndef FLGPU_QZB_F64_SPMV
+#define FLGPU_QZB_F64_SPMV 3
+#endif
+#ifndef FLGPU_QZB_F64_JACOBI
+#define FLGPU_QZB_F64_JACOBI 4
+#endif
 
 template <class T, int OP>
 static void launch_q1_pass(const flgpu_mat *A, T coef, T beta, const void *v, const void *x,
                            void *out) {
+  constexpr int QZB = sizeof(T) == 4 ? FLGPU_QZB_F32
+                      : (OP == Q1_SPMV ? FLGPU_QZB_F64_SPMV : FLGPU_QZB_F64_JACOBI);
   int M = A->lat_n + 1;
-  long threads = (long)M * M * ((M + QZB - 1) / QZB);
-  unsigned g = xcd_grid(grid_for(threads, 256));
-  Q1K kt;
-  for (int b = 0; b < 8; b++) kt.k[b] = A->kt[b];
-  k_q1_zblock<T, OP><<<g, 256, 0, A->ctx->stream>>>(M, A->lat_n, (const T *)A->alpha, kt, coef,
-                                                    beta, (const T *)v, (const T *)x, (T *)out);
+  long total = (long)M * M * ((M + QZB - 1) / QZB);
+  long waves = (total + 61) / 62;
+  unsigned g = xcd_grid(grid_for(waves * 64, 256));
+  Q1KT<T> kt;
+  for (int b = 0; b < 8; b++) kt.k[b] = (T)A->kt[b];
+  bool narrow = (long)M * M * M * (long)sizeof(T) < (1L << 32) && (long)g * 256 < (1L << 31);
+  if (narrow)
+    k_q1_zblock<T, OP, unsigned, QZB><<<g, 256, 0, A->ctx->stream>>>(
+        M, A->lat_n, (const T *)A->alpha, kt, coef, beta, (const T *)v, (const T *)x, (T *)out);
+  else
+    k_q1_zblock<T, OP, unsigned long, QZB><<<g, 256, 0, A->ctx->stream>>>(
+        M, A->lat_n, (const T *)A->alpha, kt, coef, beta, (const T *)v, (const T *)x, (T *)out);
   FLGPU_HIP(hipGetLastError());
 }
 
