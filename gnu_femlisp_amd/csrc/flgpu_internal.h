@@ -86,6 +86,21 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G8) {
 }
 inline unsigned xcd_grid(unsigned g) { return (g + 7u) & ~7u; }
 
+// n / d for 0 <= n < 2^31 and d >= 2 by one multiply (Granlund & Montgomery 1994, theorem
+// 4.2 with N = 31): l = ceil(log2 d), m = ceil(2^(31+l) / d) < 2^32, n / d = umulhi(n, m) >>
+// (l - 1).  m d - 2^(31+l) lies in [0, d) and d <= 2^l, which is the theorem's condition.
+struct FastDiv {
+  unsigned d, m;
+  int s;
+};
+inline FastDiv fast_div(unsigned d) {
+  int l = 0;
+  while ((1ull << l) < d) l++;
+  unsigned long long m = ((1ull << (31 + l)) + d - 1) / d;
+  return FastDiv{d, (unsigned)m, l - 1};
+}
+__device__ __forceinline__ unsigned fdiv(unsigned n, FastDiv f) { return __umulhi(n, f.m) >> f.s; }
+
 }  // namespace flgpu
 
 // ---- handles ------------------------------------------------------------------
@@ -149,6 +164,15 @@ struct flgpu_mat {
   // 3D Q1 lattice matrices otherwise: the 27 values of every row, SoA
   // val27[k][row], k = (dz+1)*9 + (dy+1)*3 + (dx+1); columns are implied
   void *val27 = nullptr;
+  // order-1 3D lattice interpolation from a (tp_nc+1)^3 to a (2 tp_nc+1)^3 lattice with the
+  // full trilinear pattern (1: the interpolation, 2: its transpose): products read the CSR
+  // values with the columns and row starts implied by the pattern (0: not such a matrix)
+  int tp_kind = 0, tp_nc = 0;
+  // the values again as 27 planes over the coarse nodes c: for the transpose (tp_kind 2)
+  // tp27[k][c], k = (dz+1)*9 + (dy+1)*3 + (dx+1), the entry of fine neighbour 2c + d; for the
+  // interpolation (tp_kind 1) the entries of the fine nodes of c's block by parity class
+  // (k_tp8_fill); 0 where there is no such entry
+  void *tp27 = nullptr;
 };
 
 namespace flgpu {
@@ -160,6 +184,8 @@ void mat_finish(flgpu_mat *A);  // SELL + diagonal from the device CSR
 void mat_refresh_values(flgpu_mat *A);  // SELL/diag/colour values after a CSR value change
 void mat_build_transpose(const flgpu_mat *A);
 void drop_alpha(flgpu_mat *A);
+void tp_prepare(flgpu_mat *T);  // (re)fill tp27 of an interpolation or its transpose
+void tp_drop(flgpu_mat *A);     // products of A (and its transpose) go back to SELL
 bool is_q1_lattice(const flgpu_mat *A);
 void stencil_refresh(flgpu_mat *A);
 void drop_stencil(flgpu_mat *A);

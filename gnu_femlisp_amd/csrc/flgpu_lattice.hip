@@ -948,6 +948,13 @@ flgpu_mat *lattice_interpolation(flgpu_ctx *ctx, int dim, int order, int nc, int
     mat_finish(P);
     mat_finish(PT);
     P->transpose = PT;
+    if (dim == 3 && order == 1) {  // the trilinear pattern: products with implied columns
+      P->tp_kind = 1;
+      PT->tp_kind = 2;
+      P->tp_nc = PT->tp_nc = nc;
+      tp_prepare(P);
+      tp_prepare(PT);
+    }
   } catch (...) {
     mat_free(P);
     mat_free(PT);

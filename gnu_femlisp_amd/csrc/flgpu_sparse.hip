@@ -216,6 +216,7 @@ void mat_free(flgpu_mat *A) {
   if (A->transpose) mat_free(A->transpose);
   dev_free(A->alpha);
   dev_free(A->val27);
+  dev_free(A->tp27);
   delete A;
 }
 
@@ -228,6 +229,7 @@ void drop_alpha(flgpu_mat *A) {
 void mat_refresh_values(flgpu_mat *A) {
   diag_refresh(A);
   stencil_refresh(A);
+  if (A->tp_kind) tp_prepare(A);
   if (A->prec == FLGPU_F64) {
     sell_fill<double>(A->ctx, A, A->sell, 0);
     for (auto &s : A->colour_sell) sell_fill<double>(A->ctx, A, s, 0);
@@ -336,6 +338,8 @@ flgpu_mat *mat_convert(const flgpu_mat *A, int prec) {
   B->lat_order = A->lat_order;
   B->lat_n = A->lat_n;
   B->lat_f = A->lat_f;
+  B->tp_kind = A->tp_kind;
+  B->tp_nc = A->tp_nc;
   if (A->alpha) {
     long nc = (long)A->lat_n * A->lat_n * A->lat_n;
     B->alpha = dev_alloc(prec_bytes(prec) * nc);
@@ -344,6 +348,7 @@ flgpu_mat *mat_convert(const flgpu_mat *A, int prec) {
   }
   mat_finish(B);
   if (A->transpose) B->transpose = mat_convert(A->transpose, prec);
+  if (B->tp_kind) tp_prepare(B);
   if (A->ncolours > 0) {
     // rebuild the colour subsets from A's row lists
     std::vector<int> colour(A->nrows, 0);
@@ -875,8 +880,231 @@ static Q1K q1k(const flgpu_mat *A) {
   return k;
 }
 
+// ---- 3D order-1 lattice interpolation with the pattern implied ------------------------------
+// 1-D: fine node i (0..2nc) has the coarse neighbours i/2 (even i) or (i-1)/2, (i+1)/2 (odd
+// i), c(i) = 1 + (i & 1) of them, and its entries start at o(i) = i + i/2 of the S = 3nc + 1
+// entries of the 1-D table.  Coarse node c has the fine neighbours 2c-1 (c > 0), 2c, 2c+1
+// (c < nc), ct(c) of them, starting at ot(c) = 3c - 1 (c > 0), 0.  A 3-D row (z, y, x) of the
+// tensor product starts at o(z) S^2 + c(z) (o(y) S + c(y) o(x)) and lists its entries in
+// ascending column order, z-major.  The products below form every row exactly as the SELL
+// product does: acc = beta z_i, then acc += (alpha x_j) a_ij in ascending j.
+
+// start of fine row (fz, fy, fx) in the interpolation's values
+__device__ __forceinline__ long tp_row_start(int nc, int fz, int fy, int fx) {
+  const long S = 3L * nc + 1;
+  return (long)(fz + fz / 2) * S * S +
+         (long)(1 + (fz & 1)) * ((long)(fy + fy / 2) * S + (long)(1 + (fy & 1)) * (fx + fx / 2));
+}
+
+// (z, y, x) of lattice index i < M^3 < 2^31, M = dM.d
+__device__ __forceinline__ void lat_decode(unsigned i, FastDiv dM, int &z, int &y, int &x) {
+  const unsigned line = fdiv(i, dM), zz = fdiv(line, dM);
+  x = (int)(i - line * dM.d);
+  y = (int)(line - zz * dM.d);
+  z = (int)zz;
+}
+
+// The interpolation's values as 27 planes over the coarse nodes c, for the fine nodes
+// 2c + (a, b, e) of c's block, a, b, e in {0, 1} (parity classes in lexicographic order): class
+// (a, b, e) owns 2^(a+b+e) planes from tp_class_base, one per coarse corner c + (pz, py, px),
+// pz <= a, py <= b, px <= e, in the row's own (ascending column) order.  Classes whose fine
+// node lies past the lattice (a = 1 at cz = nc, ...) hold zeros there.
+__host__ __device__ constexpr int tp_class_base(int cls) {
+  return cls == 0 ? 0 : cls == 1 ? 1 : cls == 2 ? 3 : cls == 3 ? 5 : cls == 4 ? 9
+       : cls == 5 ? 11 : cls == 6 ? 15 : 19;
+}
+
+template <class T>
+__global__ void k_tp8_fill(int nc, const T *__restrict__ vals, T *__restrict__ p27) {
+  const int Mc = nc + 1, Mf = 2 * nc + 1;
+  const long Nc = (long)Mc * Mc * Mc;
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= Nc) return;
+  const int cx = (int)(c % Mc), cy = (int)((c / Mc) % Mc), cz = (int)(c / ((long)Mc * Mc));
+  for (int cls = 0; cls < 8; cls++) {
+    const int a = cls >> 2, b = (cls >> 1) & 1, e = cls & 1;
+    const int fz = 2 * cz + a, fy = 2 * cy + b, fx = 2 * cx + e;
+    const bool in = fz < Mf && fy < Mf && fx < Mf;
+    const long rs = in ? tp_row_start(nc, fz, fy, fx) : 0;
+    for (int k = 0; k < (1 << (a + b + e)); k++)
+      p27[(long)(tp_class_base(cls) + k) * Nc + c] = in ? vals[rs + k] : (T)0;
+  }
+}
+
+// out = beta z + alpha P x: one thread per coarse node c writes the fine nodes of c's block
+// from the 8 coarse values of the cell [c, c + 1]^3 and the 27 planes
+template <class T>
+__global__ __launch_bounds__(256) void k_tp_prolong(int nc, FastDiv dMc,
+                                                    const T *__restrict__ p27, T alpha,
+                                                    const T *__restrict__ x, T beta,
+                                                    const T *__restrict__ z, T *__restrict__ out) {
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned Mc = dMc.d, Mf = 2 * nc + 1, Nc = Mc * Mc * Mc;
+  const unsigned c = lb * blockDim.x + threadIdx.x;
+  if (c >= Nc) return;
+  int cz, cy, cx;
+  lat_decode(c, dMc, cz, cy, cx);
+  const bool uz = cz < nc, uy = cy < nc, ux = cx < nc;  // the upper neighbours exist
+  T xc[8], w[27], zf[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int pz = q >> 2, py = (q >> 1) & 1, px = q & 1;
+    if ((!pz || uz) && (!py || uy) && (!px || ux))
+      xc[q] = x[c + ((unsigned)pz * Mc + (unsigned)py) * Mc + (unsigned)px];
+  }
+#pragma unroll
+  for (int k = 0; k < 27; k++) w[k] = p27[(unsigned long)k * Nc + c];
+  const unsigned f0 = ((unsigned)(2 * cz) * Mf + (unsigned)(2 * cy)) * Mf + (unsigned)(2 * cx);
+#pragma unroll
+  for (int cls = 0; cls < 8; cls++) {
+    const int a = cls >> 2, b = (cls >> 1) & 1, e = cls & 1;
+    if ((!a || uz) && (!b || uy) && (!e || ux) && beta != (T)0)
+      zf[cls] = z[f0 + ((unsigned)a * Mf + (unsigned)b) * Mf + (unsigned)e];
+  }
+#pragma unroll
+  for (int cls = 0; cls < 8; cls++) {
+    const int a = cls >> 2, b = (cls >> 1) & 1, e = cls & 1;
+    if ((!a || uz) && (!b || uy) && (!e || ux)) {
+      T acc = (beta == (T)0) ? (T)0 : beta * zf[cls];
+#pragma unroll
+      for (int pz = 0; pz <= a; pz++)
+#pragma unroll
+        for (int py = 0; py <= b; py++)
+#pragma unroll
+          for (int px = 0; px <= e; px++)
+            acc += (alpha * xc[(pz * 2 + py) * 2 + px]) *
+                   w[tp_class_base(cls) + (pz * (b + 1) + py) * (e + 1) + px];
+      out[f0 + ((unsigned)a * Mf + (unsigned)b) * Mf + (unsigned)e] = acc;
+    }
+  }
+}
+
+// coarse row c of the transpose: its fine neighbours per axis and where offset 0 sits
+struct TpRow {
+  int tz, ty, tx, lz, ly, lx;
+  long off;
+};
+__device__ __forceinline__ TpRow tp_trow(int nc, int cz, int cy, int cx) {
+  const long S = 3L * nc + 1;
+  auto cnt = [&](int q) { return 3 - (q == 0) - (q == nc); };
+  auto ot = [&](int q) { return q == 0 ? 0 : 3 * q - 1; };
+  TpRow t;
+  t.tz = cnt(cz);
+  t.ty = cnt(cy);
+  t.tx = cnt(cx);
+  t.lz = cz > 0;
+  t.ly = cy > 0;
+  t.lx = cx > 0;
+  t.off = (long)ot(cz) * S * S + (long)t.tz * ((long)ot(cy) * S + (long)t.ty * ot(cx));
+  return t;
+}
+
+// tp27[k][c] from the transpose's CSR values
+template <class T>
+__global__ void k_tp27_fill(int nc, const T *__restrict__ tvals, T *__restrict__ t27) {
+  const unsigned Mc = nc + 1;
+  const unsigned Nc = Mc * Mc * Mc;
+  const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= Nc) return;
+  const unsigned line = c / Mc;
+  const int cx = (int)(c - line * Mc), cy = (int)(line % Mc), cz = (int)(line / Mc);
+  const TpRow t = tp_trow(nc, cz, cy, cx);
+  for (int e = 0; e < 27; e++) {
+    const int iz = e / 9 - 1 + t.lz, iy = (e / 3) % 3 - 1 + t.ly, ix = e % 3 - 1 + t.lx;
+    const bool live = iz >= 0 && iz < t.tz && iy >= 0 && iy < t.ty && ix >= 0 && ix < t.tx;
+    t27[(long)e * Nc + c] = live ? tvals[t.off + (iz * t.ty + iy) * t.tx + ix] : (T)0;
+  }
+}
+
+// out = beta z + alpha P^T r (coarse rows; t27 = the transpose's values as 27 planes)
+template <class T>
+__global__ __launch_bounds__(256) void k_tp_restrict(int nc, FastDiv dMc,
+                                                     const T *__restrict__ t27, T alpha,
+                                                     const T *__restrict__ r, T beta,
+                                                     const T *__restrict__ z,
+                                                     T *__restrict__ out) {
+  unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned Mf = 2 * nc + 1, Mc = dMc.d;
+  const unsigned Nc = Mc * Mc * Mc;
+  const unsigned c = lb * blockDim.x + threadIdx.x;
+  if (c >= Nc) return;
+  int cz, cy, cx;
+  lat_decode(c, dMc, cz, cy, cx);
+  const bool vz[3] = {cz > 0, true, cz < nc}, vy[3] = {cy > 0, true, cy < nc},
+             vx[3] = {cx > 0, true, cx < nc};
+  const unsigned f0 = ((unsigned)(2 * cz) * Mf + (unsigned)(2 * cy)) * Mf + (unsigned)(2 * cx);
+  T a[27], rv[27];
+#pragma unroll
+  for (int e = 0; e < 27; e++) {
+    const int dz = e / 9 - 1, dy = (e / 3) % 3 - 1, dx = e % 3 - 1;
+    const bool live = vz[dz + 1] && vy[dy + 1] && vx[dx + 1];
+    a[e] = t27[(long)e * Nc + c];
+    if (live) rv[e] = r[f0 + (unsigned)((dz * (int)Mf + dy) * (int)Mf + dx)];
+  }
+  T acc = (beta == (T)0) ? (T)0 : beta * z[c];
+#pragma unroll
+  for (int e = 0; e < 27; e++) {
+    const int dz = e / 9 - 1, dy = (e / 3) % 3 - 1, dx = e % 3 - 1;
+    if (vz[dz + 1] && vy[dy + 1] && vx[dx + 1]) acc += (alpha * rv[e]) * a[e];
+  }
+  out[c] = acc;
+}
+
+void tp_prepare(flgpu_mat *T) {
+  if (T->tp_kind == 0) return;
+  const int nc = T->tp_nc;
+  const long Nc = (long)(nc + 1) * (nc + 1) * (nc + 1);
+  if (!T->tp27) T->tp27 = dev_alloc(27 * Nc * prec_bytes(T->prec));
+  const unsigned g = grid_for(Nc, 256);
+  if (T->tp_kind == 1) {
+    if (T->prec == FLGPU_F64)
+      k_tp8_fill<double><<<g, 256, 0, T->ctx->stream>>>(nc, (const double *)T->vals,
+                                                        (double *)T->tp27);
+    else
+      k_tp8_fill<float><<<g, 256, 0, T->ctx->stream>>>(nc, (const float *)T->vals,
+                                                       (float *)T->tp27);
+  } else {
+    if (T->prec == FLGPU_F64)
+      k_tp27_fill<double><<<g, 256, 0, T->ctx->stream>>>(nc, (const double *)T->vals,
+                                                         (double *)T->tp27);
+    else
+      k_tp27_fill<float><<<g, 256, 0, T->ctx->stream>>>(nc, (const float *)T->vals,
+                                                        (float *)T->tp27);
+  }
+  FLGPU_HIP(hipGetLastError());
+}
+
+void tp_drop(flgpu_mat *A) {
+  if (A->tp_kind == 1 && A->transpose) tp_drop(A->transpose);
+  dev_free(A->tp27);
+  A->tp27 = nullptr;
+  A->tp_kind = 0;
+}
+
+template <class T>
+static void launch_tp(const flgpu_mat *A, T alpha, const void *x, T beta, const void *z,
+                      void *out) {
+  const int nc = A->tp_nc;
+  const long Nc = (long)(nc + 1) * (nc + 1) * (nc + 1);
+  unsigned g = xcd_grid(grid_for(Nc, 256));
+  if (A->tp_kind == 1)
+    k_tp_prolong<T><<<g, 256, 0, A->ctx->stream>>>(nc, fast_div(nc + 1), (const T *)A->tp27,
+                                                   alpha, (const T *)x, beta, (const T *)z,
+                                                   (T *)out);
+  else
+    k_tp_restrict<T><<<g, 256, 0, A->ctx->stream>>>(nc, fast_div(nc + 1), (const T *)A->tp27,
+                                                    alpha, (const T *)x, beta, (const T *)z,
+                                                    (T *)out);
+  FLGPU_HIP(hipGetLastError());
+}
+
 void spmv(const flgpu_mat *A, double alpha, const void *x, double beta, const void *z,
           void *out) {
+  if (A->tp_kind) {
+    if (A->prec == FLGPU_F64) launch_tp<double>(A, alpha, x, beta, z, out);
+    else launch_tp<float>(A, (float)alpha, x, (float)beta, z, out);
+    return;
+  }
   if (A->alpha) {
     if (A->prec == FLGPU_F64) launch_q1_pass<double, Q1_SPMV>(A, alpha, beta, z, x, out);
     else launch_q1_pass<float, Q1_SPMV>(A, (float)alpha, (float)beta, z, x, out);
@@ -1291,13 +1519,18 @@ extern "C" int flgpu_mat_storage(flgpu_mat *A, int max_form, int *form) {
     CTX_LOCK(A->ctx);
     FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
     if (max_form < FLGPU_STORAGE_CELL) drop_alpha(A);
-    if (max_form < FLGPU_STORAGE_STENCIL) drop_stencil(A);
+    if (max_form < FLGPU_STORAGE_STENCIL) {
+      drop_stencil(A);
+      tp_drop(A);
+    }
     long lat_rows = 1;
     for (int k = 0; k < A->lat_dim; k++) lat_rows *= (long)A->lat_order * A->lat_n + 1;
     if (!A->alpha && !A->val27 && A->lat_dim > 0 && A->ncolours == 0 && A->nrows == lat_rows)
       lattice_colouring(A, A->lat_dim, A->lat_order, A->lat_n);
     if (form)
-      *form = A->alpha ? FLGPU_STORAGE_CELL : A->val27 ? FLGPU_STORAGE_STENCIL : FLGPU_STORAGE_SELL;
+      *form = A->alpha                   ? FLGPU_STORAGE_CELL
+              : (A->val27 || A->tp_kind) ? FLGPU_STORAGE_STENCIL
+                                         : FLGPU_STORAGE_SELL;
     return 0;
   });
 }

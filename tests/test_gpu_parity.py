@@ -473,6 +473,46 @@ def test_blackbox_interpolation_parity(prec):
     assert relerr(after, before) > 1e-3  # the restriction follows the new values
 
 
+@pytest.mark.parametrize("prec", [F.F64, F.F32])
+@pytest.mark.parametrize("kind", ["geometric", "blackbox"])
+@pytest.mark.parametrize("nc", [1, 2, 5, 8])
+def test_structured_transfers_match_sell(prec, kind, nc):
+    """3D Q1 lattice interpolation: the products with columns and row starts implied by the
+    trilinear pattern (prolongation P y, restriction P^T r) equal the SELL products of the
+    same matrix bit for bit, for several alpha / beta, and the CSR product to rounding."""
+    say(f"structured transfers prec={prec} kind={kind} nc={nc}")
+    rng = np.random.default_rng(40 + nc)
+    P = lat.interpolation(3, 1, nc, prec=prec)
+    if kind == "blackbox":
+        n = 2 * nc
+        A, _ = lat.assemble(3, 1, n, np.exp(rng.normal(scale=2.0, size=(n, n, n))), prec=prec)
+        lat.blackbox_interpolation(A, P)
+    assert P.storage() == F.STORAGE_STENCIL
+    xc, zf = rng.uniform(-1, 1, P.ncols), rng.uniform(-1, 1, P.nrows)
+    yf, zc = rng.uniform(-1, 1, P.nrows), rng.uniform(-1, 1, P.ncols)
+    cases = [(1.0, 0.0), (1.0, 1.0), (-0.5, 2.0)]
+
+    def products():
+        out = []
+        for alpha, beta in cases:
+            out.append(gemm(alpha, P, GpuVector.from_array(xc, prec), beta,
+                            GpuVector.from_array(zf, prec)).numpy())
+            out.append(gemm(alpha, P, GpuVector.from_array(yf, prec), beta,
+                            GpuVector.from_array(zc, prec), job="tn").numpy())
+        return out
+    implied = products()
+    assert P.storage(F.STORAGE_SELL) == F.STORAGE_SELL
+    sell = products()
+    for a, b in zip(implied, sell):
+        assert np.array_equal(a, b)
+    rp, ci, v = P.csr()
+    Ph = sp.csr_matrix((v, ci, rp), shape=(P.nrows, P.ncols))
+    tol = 1e-14 if prec == F.F64 else 1e-6
+    for k, (alpha, beta) in enumerate(cases):
+        assert relerr(implied[2 * k], alpha * (Ph @ xc) + beta * zf) < tol
+        assert relerr(implied[2 * k + 1], alpha * (Ph.T @ yf) + beta * zc) < tol
+
+
 def test_blackbox_hierarchy_solve_parity():
     """Black-box transfers + Galerkin levels: level matrices against the
     oracle's, then CG with the V-cycle: the oracle's iteration count, defect
