@@ -206,7 +206,16 @@ void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x,
 // `steps` <jacobi> steps with fresh residuals, x ping-ponged through tmp; x_zero: x is zero
 // and need not be read (the first step is x = damp b / diag)
 void jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp,
-                  bool x_zero = false);
+                  bool x_zero = false, int done = 0);
+// rhs = P^T r (beta 0) and, in the same pass, xo = damp (rhs / diag(Ac)): the first <jacobi>
+// step from zero on the coarse level; false (nothing launched) unless PT is a structured
+// transpose (tp_kind 2)
+bool restrict_first_step(const flgpu_mat *PT, const void *r, void *rhs, const flgpu_mat *Ac,
+                         double damp, void *xo);
+// b = r converted to A's precision and xo = damp (b / diag(A)), the first <jacobi> step from
+// zero, in one pass
+void stage_first_step(const flgpu_mat *A, int rprec, const void *r, void *b, double damp,
+                      void *xo);
 // `steps` sweeps
 void sor_sweeps(const flgpu_mat *A, double omega, int steps, int backwards, void *x,
                 const void *b, void *r);

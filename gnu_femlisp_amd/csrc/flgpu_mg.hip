@@ -33,6 +33,8 @@ struct MGLevel {
   bool residual_p = false;
   bool x_zero = false;  // sol is zero but not stored: the next smoother step or coarse solve
                         // writes it instead of updating it
+  bool step1 = false;   // with x_zero: tmp already holds the first <jacobi> step from zero
+                        // (written by the pass that produced rhs)
   double *cinv = nullptr;  // base level: dense inverse (FP64, row-major)
   double *ctmp = nullptr;
   long N = 0;
@@ -182,8 +184,9 @@ static void smooth(flgpu_mg *mg, int l, bool post) {
   if (p.smoother == FLGPU_SMOOTH_JACOBI && b != r && L.tmp) {
     // residual recomputed inside each step; only femlisp's aliased rhs = residual (below)
     // needs the in-place form that divides the residual vector itself
-    jacobi_steps(L.A, p.omega, steps, x, b, L.tmp, L.x_zero);
+    jacobi_steps(L.A, p.omega, steps, x, b, L.tmp, L.x_zero, L.step1 ? 1 : 0);
     L.x_zero = false;
+    L.step1 = false;
   } else if (p.smoother == FLGPU_SMOOTH_JACOBI) {
     ensure_residual(mg, l);  // <jacobi> has residual-before t
     for (int k = 0; k < steps; k++) {
@@ -218,7 +221,10 @@ static void lmgc(flgpu_mg *mg, int l) {
   bool base = (l - 1 == mg->p.base_level);
   bool fused = !add && mg->p.smoother == FLGPU_SMOOTH_JACOBI && C.tmp && mg->p.pre_steps > 0;
   if (fused && !base) {
-    spmv(C.I->transpose, 1.0, lres(mg, l), 0.0, C.rhs, C.rhs);
+    // the first coarse <jacobi> step from zero is pointwise in rhs: taken in the same pass
+    // where the transfer allows it
+    C.step1 = restrict_first_step(C.I->transpose, lres(mg, l), C.rhs, C.A, mg->p.omega, C.tmp);
+    if (!C.step1) spmv(C.I->transpose, 1.0, lres(mg, l), 0.0, C.rhs, C.rhs);
     C.x_zero = true;
     C.residual_p = false;
   } else {
@@ -241,6 +247,7 @@ static void lmgc(flgpu_mg *mg, int l) {
   if (reps == 0) {  // the coarse correction is zero: sol_{l-1} must hold it
     if (C.x_zero) vfill(mg->ctx, C.N, mg->p.prec, C.sol, 0.0);
     C.x_zero = false;
+    C.step1 = false;
   }
   if (add) smooth(mg, l, true);  // :additive post-smooths before the prolongation
   // prolongate (multigrid.lisp:237-252): sol_l += I sol_{l-1}
@@ -326,7 +333,13 @@ static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work)
     mg_cycle(mg, q, r, work);
     return;
   }
-  vcopy(mg->ctx, N, vprec, r, p.prec, mg->bs);
+  if (zero && jac && !reads_r) {
+    // the staging pass also takes the first top-level <jacobi> step from zero
+    stage_first_step(T.A, vprec, r, mg->bs, p.omega, T.tmp);
+    T.step1 = true;
+  } else {
+    vcopy(mg->ctx, N, vprec, r, p.prec, mg->bs);
+  }
   if (zero) T.x_zero = true;
   else vfill(mg->ctx, N, p.prec, mg->xs, 0.0);
   if (reads_r) vcopy(mg->ctx, N, p.prec, mg->bs, p.prec, mg->rs);

@@ -1016,13 +1016,17 @@ __global__ void k_tp27_fill(int nc, const T *__restrict__ tvals, T *__restrict__
   }
 }
 
-// out = beta z + alpha P^T r (coarse rows; t27 = the transpose's values as 27 planes)
-template <class T>
+// out = beta z + alpha P^T r (coarse rows; t27 = the transpose's values as 27 planes).  FIRST:
+// also the first <jacobi> step from zero on the coarse level, xo = damp (out / diag), as
+// k_jacobi_zero forms it
+template <class T, bool FIRST = false>
 __global__ __launch_bounds__(256) void k_tp_restrict(int nc, FastDiv dMc,
                                                      const T *__restrict__ t27, T alpha,
                                                      const T *__restrict__ r, T beta,
                                                      const T *__restrict__ z,
-                                                     T *__restrict__ out) {
+                                                     T *__restrict__ out,
+                                                     const T *__restrict__ diag = nullptr,
+                                                     T damp = (T)0, T *__restrict__ xo = nullptr) {
   unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const unsigned Mf = 2 * nc + 1, Mc = dMc.d;
   const unsigned Nc = Mc * Mc * Mc;
@@ -1048,6 +1052,7 @@ __global__ __launch_bounds__(256) void k_tp_restrict(int nc, FastDiv dMc,
     if (vz[dz + 1] && vy[dy + 1] && vx[dx + 1]) acc += (alpha * rv[e]) * a[e];
   }
   out[c] = acc;
+  if (FIRST) xo[c] = damp * (acc / diag[c]);
 }
 
 void tp_prepare(flgpu_mat *T) {
@@ -1095,6 +1100,53 @@ static void launch_tp(const flgpu_mat *A, T alpha, const void *x, T beta, const 
     k_tp_restrict<T><<<g, 256, 0, A->ctx->stream>>>(nc, fast_div(nc + 1), (const T *)A->tp27,
                                                     alpha, (const T *)x, beta, (const T *)z,
                                                     (T *)out);
+  FLGPU_HIP(hipGetLastError());
+}
+
+bool restrict_first_step(const flgpu_mat *PT, const void *r, void *rhs, const flgpu_mat *Ac,
+                         double damp, void *xo) {
+  if (PT->tp_kind != 2 || !Ac->diag || Ac->nrows != PT->nrows) return false;
+  const int nc = PT->tp_nc;
+  const long Nc = (long)(nc + 1) * (nc + 1) * (nc + 1);
+  unsigned g = xcd_grid(grid_for(Nc, 256));
+  if (PT->prec == FLGPU_F64)
+    k_tp_restrict<double, true><<<g, 256, 0, PT->ctx->stream>>>(
+        nc, fast_div(nc + 1), (const double *)PT->tp27, 1.0, (const double *)r, 0.0,
+        (const double *)rhs, (double *)rhs, (const double *)Ac->diag, damp, (double *)xo);
+  else
+    k_tp_restrict<float, true><<<g, 256, 0, PT->ctx->stream>>>(
+        nc, fast_div(nc + 1), (const float *)PT->tp27, 1.0f, (const float *)r, 0.0f,
+        (const float *)rhs, (float *)rhs, (const float *)Ac->diag, (float)damp, (float *)xo);
+  FLGPU_HIP(hipGetLastError());
+  return true;
+}
+
+template <class S, class T>
+__global__ void k_stage_first(long n, const S *__restrict__ r, T *__restrict__ b,
+                              const T *__restrict__ diag, T damp, T *__restrict__ xo) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T bi = (T)r[i];
+  b[i] = bi;
+  xo[i] = damp * (bi / diag[i]);
+}
+
+void stage_first_step(const flgpu_mat *A, int rprec, const void *r, void *b, double damp,
+                      void *xo) {
+  long n = A->nrows;
+  unsigned g = grid_for(n, 256);
+  if (rprec == FLGPU_F64 && A->prec == FLGPU_F32)
+    k_stage_first<double, float><<<g, 256, 0, A->ctx->stream>>>(
+        n, (const double *)r, (float *)b, (const float *)A->diag, (float)damp, (float *)xo);
+  else if (rprec == FLGPU_F32 && A->prec == FLGPU_F64)
+    k_stage_first<float, double><<<g, 256, 0, A->ctx->stream>>>(
+        n, (const float *)r, (double *)b, (const double *)A->diag, damp, (double *)xo);
+  else if (A->prec == FLGPU_F64)
+    k_stage_first<double, double><<<g, 256, 0, A->ctx->stream>>>(
+        n, (const double *)r, (double *)b, (const double *)A->diag, damp, (double *)xo);
+  else
+    k_stage_first<float, float><<<g, 256, 0, A->ctx->stream>>>(
+        n, (const float *)r, (float *)b, (const float *)A->diag, (float)damp, (float *)xo);
   FLGPU_HIP(hipGetLastError());
 }
 
@@ -1211,12 +1263,14 @@ __global__ void k_jacobi_zero(long n, const T *__restrict__ diag, T damp, const 
 
 // `steps` <jacobi> steps, each from a freshly computed residual, x ping-ponged through tmp
 // (n values); the residual vector of the iteration is not touched (left stale).  x_zero:
-// x holds no values yet and stands for zero.
+// x holds no values yet and stands for zero.  done = 1: the first step from zero has been
+// taken already and its result is in tmp.
 void jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp,
-                  bool x_zero) {
+                  bool x_zero, int done) {
   if (steps <= 0 || A->nrows == 0) return;
   FLGPU_REQUIRE(A->nrows == A->ncols, "<jacobi> needs a square matrix");
-  for (int k = 0; k < steps; k++) {
+  FLGPU_REQUIRE(done == 0 || (done == 1 && x_zero), "only the first step from zero comes done");
+  for (int k = done; k < steps; k++) {
     void *src = (k % 2 == 0) ? x : tmp, *dst = (k % 2 == 0) ? tmp : x;
     if (k == 0 && x_zero) {
       unsigned g = grid_for(A->nrows, 256);
