@@ -252,6 +252,14 @@ void fetch_slots(flgpu_ctx *ctx, int nslots) {
   FLGPU_HIP(hipStreamSynchronize(ctx->stream));
 }
 
+void fetch_slots_async(flgpu_ctx *ctx, int nslots) {
+  FLGPU_HIP(hipMemcpyAsync(ctx->host_out, ctx->red_out, sizeof(double) * nslots,
+                           hipMemcpyDeviceToHost, ctx->stream));
+  FLGPU_HIP(hipEventRecord(ctx->ev_fetch, ctx->stream));
+}
+
+void fetch_wait(flgpu_ctx *ctx) { FLGPU_HIP(hipEventSynchronize(ctx->ev_fetch)); }
+
 double vdot_host(flgpu_ctx *ctx, long n, int prec, const void *x, const void *y) {
   vdot_dev(ctx, n, prec, x, y, 0);
   fetch_slots(ctx, 1);
@@ -307,6 +315,7 @@ extern "C" int flgpu_ctx_create(int device, flgpu_ctx **out) {
     FLGPU_HIP(hipHostMalloc((void **)&c->host_out, sizeof(double) * kRedSlots, hipHostMallocDefault));
     FLGPU_HIP(hipEventCreate(&c->ev_start));
     FLGPU_HIP(hipEventCreate(&c->ev_stop));
+    FLGPU_HIP(hipEventCreateWithFlags(&c->ev_fetch, hipEventDisableTiming));
     *out = c;
     return 0;
   });
@@ -322,6 +331,7 @@ extern "C" int flgpu_ctx_destroy(flgpu_ctx *ctx) {
     if (ctx->host_out) (void)hipHostFree(ctx->host_out);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->ev_fetch) (void)hipEventDestroy(ctx->ev_fetch);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;

@@ -114,6 +114,7 @@ struct flgpu_ctx {
   double *red_out = nullptr;      // device, reduction results (64 slots)
   double *host_out = nullptr;     // pinned host copy of red_out
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  hipEvent_t ev_fetch = nullptr;  // marks the end of an asynchronous slot fetch
   std::string name;
 };
 
@@ -250,5 +251,9 @@ void vdot_multi_dev(flgpu_ctx *ctx, long n, int prec, int count, const void *con
 double vdot_host(flgpu_ctx *ctx, long n, int prec, const void *x,
                  const void *y);
 void fetch_slots(flgpu_ctx *ctx, int nslots);  // red_out[0..n) -> host_out
+// the same copy queued on the stream; host_out is valid after fetch_wait, while work queued
+// in between runs on
+void fetch_slots_async(flgpu_ctx *ctx, int nslots);
+void fetch_wait(flgpu_ctx *ctx);
 
 }  // namespace flgpu

@@ -771,6 +771,10 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
     bool residual_p = false, initialized = false;
     int step = 0, status = 1;
     double initial = -1.0, defnorm = 0.0, alpha = 0.0;
+    // a = A p and its dot with p for the current p are already queued: they modify nothing
+    // the iteration keeps, so they are queued ahead of the host's wait for the stopping
+    // test and simply dropped when the iteration stops or restarts
+    bool ap_ready = false;
     int fell_back = 0;
     double best_true = -1.0;
     int best_step = 0;
@@ -798,6 +802,7 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
         if (!ok) {
           vcopy(ctx, N, prec, W.rold, prec, W.r);
           initialized = false;  // restart the recurrence from the true residual
+          ap_ready = false;
         }
       }
       if (ok) {
@@ -818,6 +823,7 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
           learned = nullptr;
           fell_back = 1;
           initialized = false;
+          ap_ready = false;
           best_true = defnorm;
           best_step = step;
         }
@@ -842,8 +848,11 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             initialized = true;
           }
           if (alpha != 0.0) {
-            spmv(A, 1.0, W.p, 0.0, W.a, W.a);         // gemm! 1 A p 0 a
-            vdot_dev(ctx, N, prec, W.a, W.p, 1);       // beta = a.p
+            if (!ap_ready) {
+              spmv(A, 1.0, W.p, 0.0, W.a, W.a);         // gemm! 1 A p 0 a
+              vdot_dev(ctx, N, prec, W.a, W.p, 1);       // beta = a.p
+            }
+            ap_ready = false;
             // x += (alpha/beta) p, r -= (alpha/beta) a (FCG keeps the old r first)
             if (prec == FLGPU_F64)
               k_cg_update<double><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, (const double *)W.p,
@@ -879,7 +888,11 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             }
             k_slot_copy<<<1, 1, 0, s>>>(slot, 2, 0);
             FLGPU_HIP(hipGetLastError());
-            fetch_slots(ctx, 4);
+            fetch_slots_async(ctx, 4);
+            spmv(A, 1.0, W.p, 0.0, W.a, W.a);  // the next step's product, ahead of the wait
+            vdot_dev(ctx, N, prec, W.a, W.p, 1);
+            ap_ready = true;
+            fetch_wait(ctx);
             alpha = ctx->host_out[0];
           } else {
             vdot_dev(ctx, N, prec, W.r, W.r, 3);
