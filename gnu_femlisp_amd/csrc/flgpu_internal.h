@@ -206,8 +206,9 @@ void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x,
                const void *b, void *r);
 // `steps` <jacobi> steps with fresh residuals, x ping-ponged through tmp; x_zero: x is zero
 // and need not be read (the first step is x = damp b / diag)
-void jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp,
-                  bool x_zero = false, int done = 0);
+// x64 (FP64, n values) may receive a copy of the final x: true when the last step wrote it
+bool jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp,
+                  bool x_zero = false, int done = 0, double *x64 = nullptr);
 // rhs = P^T r (beta 0) and, in the same pass, xo = damp (rhs / diag(Ac)): the first <jacobi>
 // step from zero on the coarse level; false (nothing launched) unless PT is a structured
 // transpose (tp_kind 2)

@@ -36,7 +36,6 @@ struct MGLevel {
   bool step1 = false;   // with x_zero: tmp already holds the first <jacobi> step from zero
                         // (written by the pass that produced rhs)
   double *cinv = nullptr;  // base level: dense inverse (FP64, row-major)
-  double *ctmp = nullptr;
   long N = 0;
 };
 
@@ -57,6 +56,10 @@ struct flgpu_mg {
   flgpu_unet *unet = nullptr;
   const flgpu_vec *unet_kappa = nullptr;
   int unet_n = 0;
+  // an FP32 cycle for FP64 vectors: where the last top-level step may also store its result
+  // in FP64 (set for one cycle; x64_done tells whether it did)
+  double *x64 = nullptr;
+  bool x64_done = false;
   // captured cycles: one HIP graph per (x, b, r, precision) of the caller
   struct Graph {
     void *x, *b, *r;
@@ -69,27 +72,37 @@ struct flgpu_mg {
 
 namespace flgpu {
 
-// ---- coarse solve: x += A^-1 r, r <- A^-1 r (FP64 dense inverse) ----------------------
-
+// ---- coarse solve: x += A^-1 r (FP64 dense inverse) -----------------------------------------
+// One wave per row, its loads issued in batches of 8 per lane before the products are summed
+// (lane l sums j = l, l + 64, ... in order), then a wave reduction; lane 0 updates x.
+// femlisp's <lu> iterate also leaves A^-1 r in r; nothing reads r afterwards (residual-p is
+// cleared), and r is read by every row of this pass, so it keeps the restricted residual.
 template <class T>
-__global__ __launch_bounds__(256) void k_dense_gemv(int n, const double *__restrict__ Ainv,
-                                                    const T *__restrict__ r, double *y) {
+__global__ __launch_bounds__(256) void k_coarse_solve(int n, const double *__restrict__ Ainv,
+                                                      const T *__restrict__ r, T *__restrict__ x,
+                                                      int x_zero) {
   int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= n) return;
   int lane = threadIdx.x & 63;
+  constexpr int U = 8;
   double acc = 0.0;
-  for (int j = lane; j < n; j += 64) acc += Ainv[(long)row * n + j] * (double)r[j];
+  for (int j0 = 0; j0 < n; j0 += 64 * U) {
+    double a[U], v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      int j = j0 + lane + 64 * u;
+      a[u] = j < n ? Ainv[(long)row * n + j] : 0.0;
+      v[u] = j < n ? (double)r[j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (j0 + lane + 64 * u < n) acc += a[u] * v[u];
+  }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if (lane == 0) y[row] = acc;
-}
-
-template <class T>
-__global__ void k_coarse_update(int n, const double *y, T *x, T *r, int x_zero) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  T v = (T)y[i];
-  r[i] = v;
-  x[i] = x_zero ? v : x[i] + v;
+  if (lane == 0) {
+    T v = (T)acc;
+    x[row] = x_zero ? v : x[row] + v;
+  }
 }
 
 // Dense LU without pivoting (the coarse matrices are SPD after Dirichlet
@@ -139,7 +152,6 @@ static void coarse_setup(flgpu_mg *mg, MGLevel &L) {
                 "coarse matrix download failed");
   std::vector<double> inv = dense_inverse(n, rp, ci, v);
   L.cinv = (double *)dev_alloc(sizeof(double) * (size_t)n * n);
-  L.ctmp = (double *)dev_alloc(sizeof(double) * n);
   FLGPU_HIP(hipMemcpyAsync(L.cinv, inv.data(), sizeof(double) * (size_t)n * n,
                            hipMemcpyHostToDevice, mg->ctx->stream));
   FLGPU_HIP(hipStreamSynchronize(mg->ctx->stream));
@@ -148,15 +160,12 @@ static void coarse_setup(flgpu_mg *mg, MGLevel &L) {
 static void coarse_apply(flgpu_mg *mg, MGLevel &L, void *x, void *r) {
   int n = (int)L.N;
   hipStream_t s = mg->ctx->stream;
-  if (mg->p.prec == FLGPU_F64) {
-    k_dense_gemv<double><<<grid_for(n, 4), 256, 0, s>>>(n, L.cinv, (const double *)r, L.ctmp);
-    k_coarse_update<double><<<grid_for(n, 256), 256, 0, s>>>(n, L.ctmp, (double *)x, (double *)r,
-                                                             L.x_zero);
-  } else {
-    k_dense_gemv<float><<<grid_for(n, 4), 256, 0, s>>>(n, L.cinv, (const float *)r, L.ctmp);
-    k_coarse_update<float><<<grid_for(n, 256), 256, 0, s>>>(n, L.ctmp, (float *)x, (float *)r,
-                                                            L.x_zero);
-  }
+  if (mg->p.prec == FLGPU_F64)
+    k_coarse_solve<double><<<grid_for(n, 4), 256, 0, s>>>(n, L.cinv, (const double *)r,
+                                                          (double *)x, L.x_zero);
+  else
+    k_coarse_solve<float><<<grid_for(n, 4), 256, 0, s>>>(n, L.cinv, (const float *)r, (float *)x,
+                                                         L.x_zero);
   FLGPU_HIP(hipGetLastError());
 }
 
@@ -174,8 +183,9 @@ static void ensure_residual(flgpu_mg *mg, int l) {
   L.residual_p = true;
 }
 
-// smooth (multigrid.lisp:179-189) with the product-iterator of the smoother
-static void smooth(flgpu_mg *mg, int l, bool post) {
+// smooth (multigrid.lisp:179-189) with the product-iterator of the smoother.  last: the
+// cycle's final operation (top level, post-smoothing, multiplicative)
+static void smooth(flgpu_mg *mg, int l, bool post, bool last = false) {
   const flgpu_mg_params &p = mg->p;
   int steps = post ? p.post_steps : p.pre_steps;
   if (steps <= 0) return;
@@ -184,7 +194,9 @@ static void smooth(flgpu_mg *mg, int l, bool post) {
   if (p.smoother == FLGPU_SMOOTH_JACOBI && b != r && L.tmp) {
     // residual recomputed inside each step; only femlisp's aliased rhs = residual (below)
     // needs the in-place form that divides the residual vector itself
-    jacobi_steps(L.A, p.omega, steps, x, b, L.tmp, L.x_zero, L.step1 ? 1 : 0);
+    bool w = jacobi_steps(L.A, p.omega, steps, x, b, L.tmp, L.x_zero, L.step1 ? 1 : 0,
+                          last ? mg->x64 : nullptr);
+    if (last) mg->x64_done = w;
     L.x_zero = false;
     L.step1 = false;
   } else if (p.smoother == FLGPU_SMOOTH_JACOBI) {
@@ -253,7 +265,7 @@ static void lmgc(flgpu_mg *mg, int l) {
   // prolongate (multigrid.lisp:237-252): sol_l += I sol_{l-1}
   spmv(C.I, 1.0, C.sol, 1.0, lsol(mg, l), lsol(mg, l));
   L.residual_p = false;
-  if (!add) smooth(mg, l, true);
+  if (!add) smooth(mg, l, true, l == mg->top);
 }
 
 // the <mg-iteration> iterator on device pointers of the hierarchy's precision
@@ -343,8 +355,12 @@ static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work)
   if (zero) T.x_zero = true;
   else vfill(mg->ctx, N, p.prec, mg->xs, 0.0);
   if (reads_r) vcopy(mg->ctx, N, p.prec, mg->bs, p.prec, mg->rs);
+  mg->x64 = (vprec == FLGPU_F64) ? (double *)q : nullptr;
+  mg->x64_done = false;
   mg_cycle(mg, mg->xs, mg->bs, mg->rs);
-  vcopy(mg->ctx, N, p.prec, mg->xs, vprec, q);
+  if (!mg->x64_done) vcopy(mg->ctx, N, p.prec, mg->xs, vprec, q);
+  mg->x64 = nullptr;
+  mg->x64_done = false;
 }
 
 static void mg_run(flgpu_mg *mg, int mode, int vprec, void *x, void *b, void *r) {
@@ -442,7 +458,6 @@ void mg_free(flgpu_mg *mg) {
     dev_free(L.res);
     dev_free(L.tmp);
     dev_free(L.cinv);
-    dev_free(L.ctmp);
   }
   dev_free(mg->xs);
   dev_free(mg->bs);
@@ -621,9 +636,7 @@ extern "C" int flgpu_mg_reassemble(flgpu_mg *mg, const flgpu_vec *kappa, int nk)
     graphs_clear(mg);  // the coarse inverse is reallocated below
     MGLevel &B = mg->lv[mg->p.base_level];
     dev_free(B.cinv);
-    dev_free(B.ctmp);
     B.cinv = nullptr;
-    B.ctmp = nullptr;
     coarse_setup(mg, B);
     FLGPU_HIP(hipStreamSynchronize(mg->ctx->stream));
     return 0;

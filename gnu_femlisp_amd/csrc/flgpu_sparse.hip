@@ -563,12 +563,15 @@ __device__ __forceinline__ T ld_at(const T *__restrict__ p, I byte_offset) {
 //                 A_ii from the factors, 1 on Dirichlet rows)
 // I is the offset type: unsigned when the byte size of a lattice vector fits 32 bits and the
 // position count 31 bits.  QZB rows per thread.
-template <class T, int OP, class I, int QZB>
+// W2: every result is also stored to out2 in FP64 (the last step of a cycle run in FP32 for
+// an FP64 caller).
+template <class T, int OP, class I, int QZB, bool W2 = false>
 __global__ __launch_bounds__(256) void k_q1_zblock(int M, int n, const T *__restrict__ alpha,
                                                    Q1KT<T> kt, T coef, T beta,
                                                    const T *__restrict__ v,
                                                    const T *__restrict__ x,
-                                                   T *__restrict__ out) {
+                                                   T *__restrict__ out,
+                                                   double *__restrict__ out2 = nullptr) {
   unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x & 63;
   const I Mi = (I)M, M2 = Mi * Mi;
@@ -703,7 +706,10 @@ __global__ __launch_bounds__(256) void k_q1_zblock(int M, int n, const T *__rest
       T accd = acc0 + (cf * xi[r]) * (T)1;
       res = xi[r] + coef * (dir ? accd / (T)1 : acc / d);
     }
-    if (compute && zz < M) *(T *)((char *)out + i) = res;
+    if (compute && zz < M) {
+      *(T *)((char *)out + i) = res;
+      if (W2) *(double *)((char *)out2 + i * (I)(sizeof(double) / sizeof(T))) = (double)res;
+    }
   }
 }
 
@@ -721,7 +727,7 @@ __global__ __launch_bounds__(256) void k_q1_zblock(int M, int n, const T *__rest
 
 template <class T, int OP>
 static void launch_q1_pass(const flgpu_mat *A, T coef, T beta, const void *v, const void *x,
-                           void *out) {
+                           void *out, double *out2 = nullptr) {
   constexpr int QZB = sizeof(T) == 4 ? FLGPU_QZB_F32
                       : (OP == Q1_SPMV ? FLGPU_QZB_F64_SPMV : FLGPU_QZB_F64_JACOBI);
   int M = A->lat_n + 1;
@@ -731,6 +737,21 @@ static void launch_q1_pass(const flgpu_mat *A, T coef, T beta, const void *v, co
   Q1KT<T> kt;
   for (int b = 0; b < 8; b++) kt.k[b] = (T)A->kt[b];
   bool narrow = (long)M * M * M * (long)sizeof(T) < (1L << 32) && (long)g * 256 < (1L << 31);
+  if constexpr (OP == Q1_JACOBI && sizeof(T) == 4) {
+    if (out2) {  // the FP64 byte offsets must fit too
+      if (narrow && (long)M * M * M * (long)sizeof(double) < (1L << 32))
+        k_q1_zblock<T, OP, unsigned, QZB, true><<<g, 256, 0, A->ctx->stream>>>(
+            M, A->lat_n, (const T *)A->alpha, kt, coef, beta, (const T *)v, (const T *)x,
+            (T *)out, out2);
+      else
+        k_q1_zblock<T, OP, unsigned long, QZB, true><<<g, 256, 0, A->ctx->stream>>>(
+            M, A->lat_n, (const T *)A->alpha, kt, coef, beta, (const T *)v, (const T *)x,
+            (T *)out, out2);
+      FLGPU_HIP(hipGetLastError());
+      return;
+    }
+  }
+  FLGPU_REQUIRE(out2 == nullptr, "an FP64 copy is written only by the FP32 <jacobi> pass");
   if (narrow)
     k_q1_zblock<T, OP, unsigned, QZB><<<g, 256, 0, A->ctx->stream>>>(
         M, A->lat_n, (const T *)A->alpha, kt, coef, beta, (const T *)v, (const T *)x, (T *)out);
@@ -1233,6 +1254,122 @@ __global__ __launch_bounds__(256) void k_sell_jacobi(int nsub, int nslices,
   xo[row] = x[row] + damp * (acc / diag[row]);
 }
 
+// S <jacobi> steps of a small stencil-form level in one pass.  A block owns a region of R^3
+// rows: a tile of TE^3 rows with a halo of S - 1 (R = TE + 2 (S - 1)).  x is staged in LDS over
+// the region and one more layer; step s (0-based) updates the rows of the region shrunk by s,
+// from the previous step's values in LDS, so the last step has exact values on the tile, which
+// alone is stored.  Each row keeps its 27 coefficients in registers across the steps and is
+// formed as k_st27_jacobi forms it (st27_row's order; a neighbour outside the lattice reads the
+// row's own value against its zero coefficient).  All global loads of a thread are issued
+// before its first LDS store.  Halo rows are recomputed by neighbouring blocks; this pays only
+// where a level is small enough that a launch costs more than its work.
+constexpr int MJ_MAXLOAD = 8;  // x values a thread stages: (R + 2)^3 <= MJ_MAXLOAD * block
+
+template <class T, int S>
+__global__ __launch_bounds__(1024) void k_st27_jacobi_multi(int M, int TE, const T *__restrict__ val27,
+                                                           const T *__restrict__ diag, T damp,
+                                                           const T *__restrict__ x,
+                                                           const T *__restrict__ b,
+                                                           T *__restrict__ xo) {
+  extern __shared__ unsigned char mj_lds[];
+  const int R = TE + 2 * (S - 1), L = R + 2, L3 = L * L * L;
+  T *xs0 = (T *)mj_lds, *xs1 = xs0 + L3;
+  const int nt = (M + TE - 1) / TE;
+  const int tb = blockIdx.x;
+  const int oz = (tb / (nt * nt)) * TE - (S - 1), oy = ((tb / nt) % nt) * TE - (S - 1),
+            ox = (tb % nt) * TE - (S - 1);  // region origin
+  const long M2 = (long)M * M, N = M2 * M;
+  const int t = threadIdx.x;
+  const int rz = t / (R * R), ry = (t / R) % R, rx = t % R;
+  const int gz = oz + rz, gy = oy + ry, gx = ox + rx;
+  const bool row = t < R * R * R && gz >= 0 && gz < M && gy >= 0 && gy < M && gx >= 0 && gx < M;
+  const long i = row ? gz * M2 + (long)gy * M + gx : 0;
+  T av[27], bi = (T)0, di = (T)1;
+  if (row) {
+#pragma unroll
+    for (int k = 0; k < 27; k++) av[k] = val27[k * N + i];
+    bi = b[i];
+    di = diag[i];
+  }
+  // x over the region and one more layer (lattice values only; the rest is never read)
+  T xl[MJ_MAXLOAD];
+#pragma unroll
+  for (int j = 0; j < MJ_MAXLOAD; j++) {
+    const int k = t + j * blockDim.x;
+    const int gz2 = oz - 1 + k / (L * L), gy2 = oy - 1 + (k / L) % L, gx2 = ox - 1 + k % L;
+    const bool in = k < L3 && gz2 >= 0 && gz2 < M && gy2 >= 0 && gy2 < M && gx2 >= 0 && gx2 < M;
+    xl[j] = in ? x[gz2 * M2 + (long)gy2 * M + gx2] : (T)0;
+  }
+#pragma unroll
+  for (int j = 0; j < MJ_MAXLOAD; j++) {
+    const int k = t + j * blockDim.x;
+    if (k < L3) xs0[k] = xl[j];
+  }
+  const bool lz[3] = {gz > 0, true, gz < M - 1}, ly[3] = {gy > 0, true, gy < M - 1},
+             lx[3] = {gx > 0, true, gx < M - 1};
+  const int own = ((rz + 1) * L + (ry + 1)) * L + (rx + 1);
+  __syncthreads();
+#pragma unroll
+  for (int st = 0; st < S; st++) {
+    const T *cur = (st & 1) ? xs1 : xs0;
+    T *nxt = (st & 1) ? xs0 : xs1;
+    const bool act = row && rz >= st && rz < R - st && ry >= st && ry < R - st && rx >= st &&
+                     rx < R - st;
+    if (act) {
+      T acc = bi;
+#pragma unroll
+      for (int k = 0; k < 27; k++) {
+        int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+        bool live = lz[dz + 1] && ly[dy + 1] && lx[dx + 1];
+        T xn = cur[live ? own + (dz * L + dy) * L + dx : own];
+        acc += ((T)-1 * xn) * av[k];
+      }
+      T xn = cur[own] + damp * (acc / di);
+      if (st == S - 1) xo[i] = xn;  // the tile (the region shrunk by S - 1)
+      else nxt[own] = xn;
+    }
+    if (st < S - 1) __syncthreads();
+  }
+}
+
+// tile edge of a multi-step pass over a lattice of size M: about 200 blocks, a region of at
+// most 1024 rows and an x box a block can stage; 0 when S steps do not fit
+static int mj_tile(int M, int S) {
+  int te = std::max(2, (M + 5) / 6);
+  for (; te >= 2; te--) {
+    int R = te + 2 * (S - 1), L = R + 2;
+    int threads = (R * R * R + 63) / 64 * 64;
+    if (threads <= 1024 && L * L * L <= MJ_MAXLOAD * threads) return te;
+  }
+  return 0;
+}
+
+template <class T>
+static void launch_jacobi_multi(const flgpu_mat *A, int S, int te, T damp, const T *x,
+                                const T *b, T *xo) {
+  const int M = A->lat_n + 1;
+  const int nt = (M + te - 1) / te, R = te + 2 * (S - 1), L = R + 2;
+  const unsigned g = (unsigned)nt * nt * nt;
+  const unsigned threads = (R * R * R + 63) / 64 * 64;
+  const size_t lds = 2 * sizeof(T) * (size_t)L * L * L;
+  hipStream_t st = A->ctx->stream;
+  const T *v = (const T *)A->val27, *d = (const T *)A->diag;
+  switch (S) {
+    case 2: k_st27_jacobi_multi<T, 2><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo); break;
+    case 3: k_st27_jacobi_multi<T, 3><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo); break;
+    case 4: k_st27_jacobi_multi<T, 4><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo); break;
+    default: FLGPU_REQUIRE(false, "2 to 4 steps per pass");
+  }
+  FLGPU_HIP(hipGetLastError());
+}
+
+// stencil-form levels up to this lattice size take several <jacobi> steps per launch
+// (measured, FP32: 17^3 lattice, 3 steps 7.7 us against 3 x 3.7 us; 33^3, 3 steps 13.6 us
+// against 3 x 3.85 us, where the recomputed halos cost more than the launches)
+#ifndef FLGPU_MULTI_MAX_M
+#define FLGPU_MULTI_MAX_M 20
+#endif
+
 template <class T>
 static void jacobi_step(const flgpu_mat *A, T damp, const T *x, const T *b, T *xo) {
   hipStream_t st = A->ctx->stream;
@@ -1265,13 +1402,21 @@ __global__ void k_jacobi_zero(long n, const T *__restrict__ diag, T damp, const 
 // (n values); the residual vector of the iteration is not touched (left stale).  x_zero:
 // x holds no values yet and stands for zero.  done = 1: the first step from zero has been
 // taken already and its result is in tmp.
-void jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp,
-                  bool x_zero, int done) {
-  if (steps <= 0 || A->nrows == 0) return;
+bool jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const void *b, void *tmp,
+                  bool x_zero, int done, double *x64) {
+  if (steps <= 0 || A->nrows == 0) return false;
   FLGPU_REQUIRE(A->nrows == A->ncols, "<jacobi> needs a square matrix");
   FLGPU_REQUIRE(done == 0 || (done == 1 && x_zero), "only the first step from zero comes done");
-  for (int k = done; k < steps; k++) {
-    void *src = (k % 2 == 0) ? x : tmp, *dst = (k % 2 == 0) ? tmp : x;
+  // x64: the last step also stores x in FP64 when it is a per-cell FP32 pass that ends in x
+  const bool w2 = x64 && A->alpha && A->prec == FLGPU_F32 && steps % 2 == 0 && steps - 1 >= done;
+  // small stencil-form levels: several steps per launch (each launch writes the buffer it
+  // does not read, so the remaining steps are split to end in x)
+  const bool multi = A->val27 && !A->alpha && A->lat_n + 1 <= FLGPU_MULTI_MAX_M;
+  void *cur = done ? tmp : x;  // where the iterate is (x stands for zero when x_zero, !done)
+  int k = done;
+  while (k < steps) {
+    void *dst = (cur == x) ? tmp : x;
+    const int rem = steps - k;
     if (k == 0 && x_zero) {
       unsigned g = grid_for(A->nrows, 256);
       if (A->prec == FLGPU_F64)
@@ -1281,13 +1426,42 @@ void jacobi_steps(const flgpu_mat *A, double damp, int steps, void *x, const voi
         k_jacobi_zero<float><<<g, 256, 0, A->ctx->stream>>>(
             A->nrows, (const float *)A->diag, (float)damp, (const float *)b, (float *)dst);
       FLGPU_HIP(hipGetLastError());
-    } else if (A->prec == FLGPU_F64) {
-      jacobi_step<double>(A, damp, (const double *)src, (const double *)b, (double *)dst);
+      k++;
+    } else if (multi && rem >= 2) {
+      // from tmp one launch lands in x; from x two are needed
+      int S = (cur == tmp) ? rem : rem - rem / 2;
+      S = S < 4 ? S : 4;
+      int te = 0;
+      while (S >= 2 && (te = mj_tile(A->lat_n + 1, S)) == 0) S--;
+      if (S >= 2) {
+        if (A->prec == FLGPU_F64)
+          launch_jacobi_multi<double>(A, S, te, damp, (const double *)cur, (const double *)b,
+                                      (double *)dst);
+        else
+          launch_jacobi_multi<float>(A, S, te, (float)damp, (const float *)cur,
+                                     (const float *)b, (float *)dst);
+      } else {
+        S = 1;
+        if (A->prec == FLGPU_F64)
+          jacobi_step<double>(A, damp, (const double *)cur, (const double *)b, (double *)dst);
+        else
+          jacobi_step<float>(A, (float)damp, (const float *)cur, (const float *)b, (float *)dst);
+      }
+      k += S;
+    } else if (w2 && k == steps - 1) {
+      launch_q1_pass<float, Q1_JACOBI>(A, (float)damp, 0.0f, b, cur, dst, x64);
+      k++;
     } else {
-      jacobi_step<float>(A, (float)damp, (const float *)src, (const float *)b, (float *)dst);
+      if (A->prec == FLGPU_F64)
+        jacobi_step<double>(A, damp, (const double *)cur, (const double *)b, (double *)dst);
+      else
+        jacobi_step<float>(A, (float)damp, (const float *)cur, (const float *)b, (float *)dst);
+      k++;
     }
+    cur = dst;
   }
-  if (steps % 2) vcopy(A->ctx, A->nrows, A->prec, tmp, A->prec, x);
+  if (cur != x) vcopy(A->ctx, A->nrows, A->prec, cur, A->prec, x);
+  return w2 && cur == x;
 }
 
 void jacobi_iterate(const flgpu_mat *A, double damp, void *x, void *r) {

@@ -269,6 +269,28 @@ def test_jacobi_steps_match_product_iterator(form, prec, steps):
         check(lib().flgpu_jacobi_steps(A.h, 0.7, 1, x.h, b.h, x.h))
 
 
+@pytest.mark.parametrize("prec", [F.F64, F.F32])
+@pytest.mark.parametrize("n", [6, 16, 32])
+def test_jacobi_multi_step_passes_match_single_steps(prec, n):
+    """Small stencil-form levels take several <jacobi> steps per launch (the iterate staged in
+    LDS, a halo recomputed per block); every step count equals the same steps taken one
+    launch at a time, bit for bit, and the vectors' roles (x in, x out) are kept."""
+    say(f"jacobi multi-step n={n} prec={prec}")
+    from gnu_femlisp_amd.flgpu import check, lib
+    rng = np.random.default_rng(60 + n)
+    kap = np.exp(rng.normal(scale=1.5, size=(n, n, n)))
+    A, b = lat.assemble(3, 1, n, kap, prec=prec)
+    assert A.storage(F.STORAGE_STENCIL) == F.STORAGE_STENCIL
+    x0 = rng.uniform(-1, 1, A.nrows)
+    for steps in range(1, 7):
+        x, tmp = GpuVector.from_array(x0, prec), GpuVector(A.nrows, prec)
+        check(lib().flgpu_jacobi_steps(A.h, 0.7, steps, x.h, b.h, tmp.h))
+        y, t2 = GpuVector.from_array(x0, prec), GpuVector(A.nrows, prec)
+        for _ in range(steps):
+            check(lib().flgpu_jacobi_steps(A.h, 0.7, 1, y.h, b.h, t2.h))
+        assert np.array_equal(x.numpy(), y.numpy()), steps
+
+
 def _gpu_hier(dim, order, n0, levels, smoother, omega, pre, post, base, symmetric=False,
               kappa=None, prec=F.F64, mode=F.ASM_F64, gamma=1, transfer="geometric",
               additive=False):
