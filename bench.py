@@ -24,6 +24,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 WEIGHTS = os.path.join(ROOT, "gnu_femlisp_amd", "unet_weights.npz")
+PATH_DESC = {
+    "mixed-bb-jac": "FP32 damped-Jacobi V(4,4) V-cycle, black-box transfers, Galerkin levels, "
+                    "in FP64 flexible CG",
+    "plain-bb-jac": "FP64 damped-Jacobi V(4,4) V-cycle, black-box transfers, Galerkin levels, in CG",
+    "mixed-bb": "FP32 multicolour GS V(2,2) V-cycle, black-box transfers, in FP64 flexible CG",
+    "plain-bb": "FP64 multicolour GS V(2,2) V-cycle, black-box transfers, in CG",
+    "mixed": "FP32 multicolour GS V(2,2) V-cycle in FP64 flexible CG",
+}
+# the value path's dominant kernel as the profilers name it: the FP32 per-cell <jacobi> pass
+ROOF_KERNEL = "k_q1_zblock<float, 1, unsigned int, 5, false>"
 
 
 def spmv_bytes(nnz, N, vb):
@@ -46,19 +56,20 @@ def cell_sor_bytes(ncells, N, vb):
     return ncells * vb + 4 * N * vb
 
 
-def cpu_baseline(kappa, n0, levels, base, b):
-    """The oracle (C restatement of femlisp's algorithm, one thread) solving
-    the same system with the same plain FP64 GMG-CG."""
+def cpu_baseline(kappa, n0, levels, base, b, iters=24):
+    """The oracle (C restatement of femlisp's algorithm, one thread) running the
+    same plain FP64 GMG-CG on the same system for a bounded sample of `iters`
+    iterations (about 13 s at 128^3); the caller scales the time per iteration."""
     from oracle import pyoracle as po
     t0 = time.perf_counter()
     h = po.Hierarchy(3, 1, n0, levels, kappa=kappa)
     t1 = time.perf_counter()
     p = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_SOR, omega=1.0, ordering=po.ORDER_COLOUR,
                   pre_steps=2, post_steps=2, base_level=base, symmetric=1, tol=1e-10,
-                  maxsteps=500)
+                  maxsteps=iters - 1)
     st, x, steps, dn = h.solve(p, b)
     t2 = time.perf_counter()
-    return {"setup_s": t1 - t0, "solve_s": t2 - t1, "steps": steps, "status": st}
+    return {"setup_s": t1 - t0, "sample_s": t2 - t1, "steps": steps, "status": st}
 
 
 def main():
@@ -153,6 +164,26 @@ def main():
                     ms = ctx.timer_stop() / (reps * per)
                     out[name] = {"ms": ms, "bytes": byt, "gbs": byt / (ms * 1e-3) / 1e9}
                 del M
+            # the structured transfers between this level and the one below: prolongation with
+            # beta 1, as the V-cycle adds the correction; restriction into its own vector.
+            # Algorithmic bytes: the 27 value planes over the coarse nodes, the coarse vector,
+            # the fine vector read (and, for the prolongation, written)
+            P = lat.interpolation(3, 1, n // 2, prec=prec)
+            Nc, Nf = P.ncols, P.nrows
+            xc = GpuVector.from_array(np.random.default_rng(1).uniform(-1, 1, Nc), prec)
+            yf = GpuVector.from_array(np.random.default_rng(2).uniform(-1, 1, Nf), prec)
+            rc = GpuVector(Nc, prec)
+            for name, fn, byt in (
+                    (f"tp_prolong_{tag}", lambda: gemm(1.0, P, xc, 1.0, yf), (28 * Nc + 2 * Nf) * vb),
+                    (f"tp_restrict_{tag}", lambda: gemm(1.0, P, yf, 0.0, rc, job="tn"),
+                     (28 * Nc + Nf) * vb)):
+                fn()
+                ctx.timer_start()
+                for _ in range(reps):
+                    fn()
+                ms = ctx.timer_stop() / reps
+                out[name] = {"ms": ms, "bytes": byt, "gbs": byt / (ms * 1e-3) / 1e9}
+            del P
         return out
 
     if args.kernels_only:
@@ -280,8 +311,9 @@ def main():
         log(f"[bench] V-cycle {name}: {ms:.3f} ms, {vcyc[name]['gdofs']:.3f} GDOF/s")
 
     # ---- roofline of the value path's dominant kernel -------------------------------
-    # k_q1_zblock<float, Q1_JACOBI>: one damped <jacobi> step on the fine level (per-cell form) of the
-    # mixed-bb-jac hierarchy, 8 per V-cycle.  Events bracket 8 x reps back-to-back launches,
+    # k_q1_zblock<float, 1, unsigned int, 5, false> (the FP32 per-cell <jacobi> pass, 5 rows per
+    # thread): one damped <jacobi> step on the fine level of the mixed-bb-jac hierarchy, 8 per
+    # V-cycle.  Events bracket 8 x reps back-to-back launches,
     # so the average includes the dispatch gap between launches; large copies first let the
     # host queue up ahead of the GPU.  Algorithmic bytes per launch: the cell factors, x and
     # b read once, the new x written (the diagonal follows from the factors).
@@ -308,12 +340,12 @@ def main():
         # HBM bytes per dispatch of this kernel at 128^3 from rocprofv3 FETCH_SIZE (x 2.0) and
         # WRITE_SIZE (x 1.0), the factors a copy of known size gives; the command and the
         # table are in profiles/hbm_traffic_q1_128_kernels.md
-        traffic = 35.8e6 if n == 128 else None
-        roof = {"bound": "hbm", "kernel": f"k_q1_zblock<float, Q1_JACOBI> (fine-level <jacobi> "
-                                          f"step, per-cell form, {n}^3)",
+        traffic = 35.7e6 if n == 128 else None
+        roof = {"bound": "hbm", "kernel": f"{ROOF_KERNEL} (fine-level <jacobi> step, per-cell "
+                                          f"form, {n}^3)",
                 "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
                 "traffic": traffic, "launch_us": us, "bytes_per_launch": byt}
-        log(f"[bench] roofline k_q1_zblock<float, Q1_JACOBI>: {us:.2f} us per launch, {gbs:.0f} GB/s "
+        log(f"[bench] roofline {ROOF_KERNEL}: {us:.2f} us per launch, {gbs:.0f} GB/s "
             f"({100 * gbs / 8000:.1f}% of 8 TB/s)")
         del At, xj, bj, tj
 
@@ -328,8 +360,11 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle, 1 thread) ...")
         cpu = cpu_baseline(kappa, n0, levels, args.base, b_host)
-        log(f"[bench] cpu: setup {cpu['setup_s']:.1f}s solve {cpu['solve_s']:.1f}s "
-            f"steps {cpu['steps']}")
+        # time to 1e-10: the time per iteration of the sample times the iteration count the
+        # same plain GMG-CG needs on the GPU to the same (true) residual
+        cpu["solve_s"] = cpu["sample_s"] / max(cpu["steps"], 1) * results["plain"]["steps"]
+        log(f"[bench] cpu: setup {cpu['setup_s']:.1f}s, {cpu['steps']} iterations in "
+            f"{cpu['sample_s']:.1f}s -> {cpu['solve_s']:.1f}s for {results['plain']['steps']}")
 
     fast = min((p for p in results if p != "plain"), key=lambda p: results[p]["s"])
     dom = kern["cell_sor_f32"]
@@ -347,9 +382,9 @@ def main():
         "dtype": "f64 outer / f32 V-cycle / bf16-mfma coarse assembly",
         "data": "synthetic: two-phase random inclusions (SplitMix64 seed 2024)",
         "config": {"workload": f"3D Q1 diffusion {n}^3 cells, N={N}, nnz={nnz}, contrast 1e4, "
-                               f"30% inclusions r 3-8 cells, V(2,2) multicolour GS, "
-                               f"base level {args.base}; value = '{fast}' path "
-                               f"(plain = femlisp's FP64 GMG-CG)",
+                               f"30% inclusions r 3-8 cells, base level {args.base}; value = "
+                               f"'{fast}' path ({PATH_DESC.get(fast, fast)}); plain = femlisp's "
+                               f"FP64 GMG-CG, V(2,2) multicolour GS",
                    "global_batch": 1, "seq_len": N, "parallelism": "single GPU"},
         "paths": {p: {k: v for k, v in r.items()} for p, r in results.items()},
         "speedup_vs_plain": {p: results["plain"]["s"] / results[p]["s"] for p in results},
@@ -365,8 +400,9 @@ def main():
             "frac": dom["gbs"] / 8000.0, "traffic": None},
         "cpu_baseline": None if cpu is None else {
             "value": cpu["solve_s"], "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"full plain FP64 GMG-CG solve of the same {n}^3 system "
-                      f"({cpu['steps']} iterations; setup {cpu['setup_s']:.1f}s excluded)"},
+            "sample": f"plain FP64 GMG-CG on the same {n}^3 system, {cpu['steps']} iterations "
+                      f"in {cpu['sample_s']:.1f}s, scaled to the {results['plain']['steps']} "
+                      f"iterations of the plain path (setup {cpu['setup_s']:.1f}s excluded)"},
     }
     print(json.dumps(line), flush=True)
 
