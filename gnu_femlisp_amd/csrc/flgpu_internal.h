@@ -165,6 +165,7 @@ struct flgpu_mat {
   // 3D Q1 lattice matrices otherwise: the 27 values of every row, SoA
   // val27[k][row], k = (dz+1)*9 + (dy+1)*3 + (dx+1); columns are implied
   void *val27 = nullptr;
+  bool st_sym = false;  // val27 exactly symmetric: kernels stream 14 of the 27 planes
   // order-1 3D lattice interpolation from a (tp_nc+1)^3 to a (2 tp_nc+1)^3 lattice with the
   // full trilinear pattern (1: the interpolation, 2: its transpose): products read the CSR
   // values with the columns and row starts implied by the pattern (0: not such a matrix)

@@ -73,7 +73,7 @@ struct flgpu_mg {
 namespace flgpu {
 
 // ---- coarse solve: x += A^-1 r (FP64 dense inverse) -----------------------------------------
-// One wave per row, its loads issued in batches of 8 per lane before the products are summed
+// One wave per row, its loads issued in batches of 16 per lane before the products are summed
 // (lane l sums j = l, l + 64, ... in order), then a wave reduction; lane 0 updates x.
 // femlisp's <lu> iterate also leaves A^-1 r in r; nothing reads r afterwards (residual-p is
 // cleared), and r is read by every row of this pass, so it keeps the restricted residual.
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int n, const double *__res
   int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= n) return;
   int lane = threadIdx.x & 63;
-  constexpr int U = 8;
+  constexpr int U = 16;  // one batch up to 1024 columns (the 8^3 base level: 729)
   double acc = 0.0;
   for (int j0 = 0; j0 < n; j0 += 64 * U) {
     double a[U], v[U];

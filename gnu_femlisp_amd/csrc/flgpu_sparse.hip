@@ -808,7 +808,18 @@ __global__ void k_st27_fill(int M, const int *__restrict__ rowptr, const int *__
 // (coef x_j) A_ij; the same entries in the same order as the CSR row.  The 27
 // values and neighbour x are loaded first (a neighbour outside the lattice has
 // a stored 0 and reads x_i), then summed in order: one memory latency per row.
-template <class T>
+// coefficient k of row i, whose neighbour i + off is in the lattice when live.  SYM: the values
+// are exactly symmetric (A(i, i + d) == A(i + d, i), flgpu_mat::st_sym), so the lower half
+// (k < 13) is read from the upper plane 26 - k of the neighbour row: 14 planes are streamed
+// instead of 27, and the values read are the same
+template <class T, bool SYM>
+__device__ __forceinline__ T st27_coef(const T *__restrict__ val27, long N, int k, long i, long off,
+                                       bool live) {
+  if (SYM && k < 13) return live ? val27[(26 - k) * N + i + off] : (T)0;
+  return val27[k * N + i];
+}
+
+template <class T, bool SYM = false>
 __device__ __forceinline__ T st27_row(int M, long N, long i, int z, int y, int x,
                                       const T *__restrict__ val27, T acc, T coef,
                                       const T *__restrict__ xv, T *dii) {
@@ -820,8 +831,9 @@ __device__ __forceinline__ T st27_row(int M, long N, long i, int z, int y, int x
   for (int k = 0; k < 27; k++) {
     int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
     bool live = lz[dz + 1] && ly[dy + 1] && lx[dx + 1];
-    av[k] = val27[k * N + i];
-    xn[k] = xv[live ? i + dz * M2 + dy * M + dx : i];
+    long off = dz * M2 + dy * M + dx;
+    av[k] = st27_coef<T, SYM>(val27, N, k, i, off, live);
+    xn[k] = xv[live ? i + off : i];
   }
   *dii = av[13];
 #pragma unroll
@@ -829,7 +841,7 @@ __device__ __forceinline__ T st27_row(int M, long N, long i, int z, int y, int x
   return acc;
 }
 
-template <class T>
+template <class T, bool SYM>
 __global__ __launch_bounds__(256) void k_st27_spmv(int M, const T *__restrict__ val27, T coef,
                                                    const T *__restrict__ x, T beta, const T *z,
                                                    T *out) {
@@ -840,12 +852,12 @@ __global__ __launch_bounds__(256) void k_st27_spmv(int M, const T *__restrict__ 
   int xx = (int)(i % M), yy = (int)((i / M) % M), zz = (int)(i / ((long)M * M));
   T acc = (beta == (T)0) ? (T)0 : beta * z[i];
   T d;
-  out[i] = st27_row<T>(M, N, i, zz, yy, xx, val27, acc, coef, x, &d);
+  out[i] = st27_row<T, SYM>(M, N, i, zz, yy, xx, val27, acc, coef, x, &d);
 }
 
 // <sor> update of the t-th row of colour (cz, cy, cx): corr = (b_i - sum_j A_ij x_j) / A_ii,
 // r_i = corr, x_i += omega corr
-template <class T>
+template <class T, bool SYM>
 __device__ __forceinline__ void st27_sor_row(int M, int cz, int cy, int cx, int nx, int ny,
                                              long t, const T *__restrict__ val27, T omega, T *x,
                                              const T *b, T *r) {
@@ -854,13 +866,13 @@ __device__ __forceinline__ void st27_sor_row(int M, int cz, int cy, int cx, int 
   long N = (long)M * M * M;
   long i = ((long)zz * M + yy) * M + xx;
   T d = (T)1;
-  T corr = st27_row<T>(M, N, i, zz, yy, xx, val27, b[i], (T)-1, x, &d);
+  T corr = st27_row<T, SYM>(M, N, i, zz, yy, xx, val27, b[i], (T)-1, x, &d);
   corr = corr / d;
   r[i] = corr;
   x[i] += omega * corr;
 }
 
-template <class T>
+template <class T, bool SYM>
 __global__ __launch_bounds__(256) void k_st27_sor(int M, int cz, int cy, int cx,
                                                   const T *__restrict__ val27, T omega, T *x,
                                                   const T *b, T *r) {
@@ -868,7 +880,7 @@ __global__ __launch_bounds__(256) void k_st27_sor(int M, int cz, int cy, int cx,
   unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   long t = (long)lb * blockDim.x + threadIdx.x;
   if (t >= (long)nx * ny * nz) return;
-  st27_sor_row<T>(M, cz, cy, cx, nx, ny, t, val27, omega, x, b, r);
+  st27_sor_row<T, SYM>(M, cz, cy, cx, nx, ny, t, val27, omega, x, b, r);
 }
 
 bool is_q1_lattice(const flgpu_mat *A) {
@@ -877,22 +889,57 @@ bool is_q1_lattice(const flgpu_mat *A) {
          A->nrows == A->ncols;
 }
 
+// asym[0] = 1 unless every stored A(i, i + d), d in the lower half, equals A(i + d, i) and the
+// coefficients of neighbours outside the lattice are zero
+template <class T>
+__global__ void k_st27_symcheck(int M, const T *__restrict__ val27, int *asym) {
+  const long N = (long)M * M * M, M2 = (long)M * M;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int x = (int)(i % M), y = (int)((i / M) % M), z = (int)(i / M2);
+  const bool lz[3] = {z > 0, true, z < M - 1}, ly[3] = {y > 0, true, y < M - 1},
+             lx[3] = {x > 0, true, x < M - 1};
+  bool bad = false;
+  for (int k = 0; k < 13; k++) {
+    int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+    T a = val27[k * N + i];
+    if (lz[dz + 1] && ly[dy + 1] && lx[dx + 1])
+      bad |= !(a == val27[(26 - k) * N + i + dz * M2 + dy * M + dx]);
+    else
+      bad |= !(a == (T)0);
+  }
+  if (bad) asym[0] = 1;
+}
+
 void stencil_refresh(flgpu_mat *A) {
   if (!A->val27) return;
   int M = A->lat_n + 1;
   long N = A->nrows;
-  if (A->prec == FLGPU_F64)
+  int *asym = (int *)dev_alloc(sizeof(int));
+  FLGPU_HIP(hipMemsetAsync(asym, 0, sizeof(int), A->ctx->stream));
+  if (A->prec == FLGPU_F64) {
     k_st27_fill<double><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
         M, A->rowptr, A->colidx, (const double *)A->vals, (double *)A->val27);
-  else
+    k_st27_symcheck<double><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
+        M, (const double *)A->val27, asym);
+  } else {
     k_st27_fill<float><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
         M, A->rowptr, A->colidx, (const float *)A->vals, (float *)A->val27);
+    k_st27_symcheck<float><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
+        M, (const float *)A->val27, asym);
+  }
   FLGPU_HIP(hipGetLastError());
+  int h = 1;
+  FLGPU_HIP(hipMemcpyAsync(&h, asym, sizeof(int), hipMemcpyDeviceToHost, A->ctx->stream));
+  FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+  dev_free(asym);
+  A->st_sym = (h == 0);
 }
 
 void drop_stencil(flgpu_mat *A) {
   dev_free(A->val27);
   A->val27 = nullptr;
+  A->st_sym = false;
 }
 
 static Q1K q1k(const flgpu_mat *A) {
@@ -1124,6 +1171,19 @@ static void launch_tp(const flgpu_mat *A, T alpha, const void *x, T beta, const 
   FLGPU_HIP(hipGetLastError());
 }
 
+template <class T>
+static void launch_st27_spmv(const flgpu_mat *A, T alpha, const T *x, T beta, const T *z, T *out) {
+  int M = A->lat_n + 1;
+  unsigned g = xcd_grid(grid_for(A->nrows, 256));
+  if (A->st_sym)
+    k_st27_spmv<T, true><<<g, 256, 0, A->ctx->stream>>>(M, (const T *)A->val27, alpha, x, beta,
+                                                        z, out);
+  else
+    k_st27_spmv<T, false><<<g, 256, 0, A->ctx->stream>>>(M, (const T *)A->val27, alpha, x, beta,
+                                                         z, out);
+  FLGPU_HIP(hipGetLastError());
+}
+
 bool restrict_first_step(const flgpu_mat *PT, const void *r, void *rhs, const flgpu_mat *Ac,
                          double damp, void *xo) {
   if (PT->tp_kind != 2 || !Ac->diag || Ac->nrows != PT->nrows) return false;
@@ -1184,17 +1244,12 @@ void spmv(const flgpu_mat *A, double alpha, const void *x, double beta, const vo
     return;
   }
   if (A->val27) {
-    int M = A->lat_n + 1;
-    unsigned g = xcd_grid(grid_for(A->nrows, 256));
     if (A->prec == FLGPU_F64)
-      k_st27_spmv<double><<<g, 256, 0, A->ctx->stream>>>(M, (const double *)A->val27, alpha,
-                                                         (const double *)x, beta,
-                                                         (const double *)z, (double *)out);
+      launch_st27_spmv<double>(A, alpha, (const double *)x, beta, (const double *)z,
+                               (double *)out);
     else
-      k_st27_spmv<float><<<g, 256, 0, A->ctx->stream>>>(M, (const float *)A->val27, (float)alpha,
-                                                        (const float *)x, (float)beta,
-                                                        (const float *)z, (float *)out);
-    FLGPU_HIP(hipGetLastError());
+      launch_st27_spmv<float>(A, (float)alpha, (const float *)x, (float)beta, (const float *)z,
+                              (float *)out);
     return;
   }
   if (A->prec == FLGPU_F64) launch_spmv<double>(A->ctx, A->sell, alpha, x, beta, z, out);
@@ -1219,7 +1274,7 @@ __global__ void k_jacobi(int n, T damp, const T *__restrict__ diag, T *x, T *r) 
 // needs recomputed in the same pass, out of place: r_i = b_i - (A x)_i (the sum of spmv with
 // alpha = -1, beta = 1), r_i /= A_ii, xo_i = x_i + damp r_i.  The per-cell form runs
 // k_q1_zblock (A_ii from the cell factors); the stencil and SELL forms read A_ii from diag:
-template <class T>
+template <class T, bool SYM>
 __global__ __launch_bounds__(256) void k_st27_jacobi(int M, const T *__restrict__ val27,
                                                      const T *__restrict__ diag, T damp,
                                                      const T *__restrict__ x,
@@ -1230,7 +1285,7 @@ __global__ __launch_bounds__(256) void k_st27_jacobi(int M, const T *__restrict_
   if (i >= N) return;
   int xx = (int)(i % M), yy = (int)((i / M) % M), zz = (int)(i / ((long)M * M));
   T d;
-  T acc = st27_row<T>(M, N, i, zz, yy, xx, val27, b[i], (T)-1, x, &d);
+  T acc = st27_row<T, SYM>(M, N, i, zz, yy, xx, val27, b[i], (T)-1, x, &d);
   xo[i] = x[i] + damp * (acc / diag[i]);
 }
 
@@ -1265,7 +1320,7 @@ __global__ __launch_bounds__(256) void k_sell_jacobi(int nsub, int nslices,
 // where a level is small enough that a launch costs more than its work.
 constexpr int MJ_MAXLOAD = 8;  // x values a thread stages: (R + 2)^3 <= MJ_MAXLOAD * block
 
-template <class T, int S>
+template <class T, int S, bool SYM>
 __global__ __launch_bounds__(1024) void k_st27_jacobi_multi(int M, int TE, const T *__restrict__ val27,
                                                            const T *__restrict__ diag, T damp,
                                                            const T *__restrict__ x,
@@ -1284,10 +1339,16 @@ __global__ __launch_bounds__(1024) void k_st27_jacobi_multi(int M, int TE, const
   const int gz = oz + rz, gy = oy + ry, gx = ox + rx;
   const bool row = t < R * R * R && gz >= 0 && gz < M && gy >= 0 && gy < M && gx >= 0 && gx < M;
   const long i = row ? gz * M2 + (long)gy * M + gx : 0;
+  const bool lz[3] = {gz > 0, true, gz < M - 1}, ly[3] = {gy > 0, true, gy < M - 1},
+             lx[3] = {gx > 0, true, gx < M - 1};
   T av[27], bi = (T)0, di = (T)1;
   if (row) {
 #pragma unroll
-    for (int k = 0; k < 27; k++) av[k] = val27[k * N + i];
+    for (int k = 0; k < 27; k++) {
+      int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+      av[k] = st27_coef<T, SYM>(val27, N, k, i, dz * M2 + dy * M + dx,
+                                lz[dz + 1] && ly[dy + 1] && lx[dx + 1]);
+    }
     bi = b[i];
     di = diag[i];
   }
@@ -1305,8 +1366,6 @@ __global__ __launch_bounds__(1024) void k_st27_jacobi_multi(int M, int TE, const
     const int k = t + j * blockDim.x;
     if (k < L3) xs0[k] = xl[j];
   }
-  const bool lz[3] = {gz > 0, true, gz < M - 1}, ly[3] = {gy > 0, true, gy < M - 1},
-             lx[3] = {gx > 0, true, gx < M - 1};
   const int own = ((rz + 1) * L + (ry + 1)) * L + (rx + 1);
   __syncthreads();
 #pragma unroll
@@ -1354,11 +1413,15 @@ static void launch_jacobi_multi(const flgpu_mat *A, int S, int te, T damp, const
   const size_t lds = 2 * sizeof(T) * (size_t)L * L * L;
   hipStream_t st = A->ctx->stream;
   const T *v = (const T *)A->val27, *d = (const T *)A->diag;
-  switch (S) {
-    case 2: k_st27_jacobi_multi<T, 2><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo); break;
-    case 3: k_st27_jacobi_multi<T, 3><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo); break;
-    case 4: k_st27_jacobi_multi<T, 4><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo); break;
-    default: FLGPU_REQUIRE(false, "2 to 4 steps per pass");
+  FLGPU_REQUIRE(S >= 2 && S <= 4, "2 to 4 steps per pass");
+  if (A->st_sym) {
+    if (S == 2) k_st27_jacobi_multi<T, 2, true><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo);
+    if (S == 3) k_st27_jacobi_multi<T, 3, true><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo);
+    if (S == 4) k_st27_jacobi_multi<T, 4, true><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo);
+  } else {
+    if (S == 2) k_st27_jacobi_multi<T, 2, false><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo);
+    if (S == 3) k_st27_jacobi_multi<T, 3, false><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo);
+    if (S == 4) k_st27_jacobi_multi<T, 4, false><<<g, threads, lds, st>>>(M, te, v, d, damp, x, b, xo);
   }
   FLGPU_HIP(hipGetLastError());
 }
@@ -1378,8 +1441,12 @@ static void jacobi_step(const flgpu_mat *A, T damp, const T *x, const T *b, T *x
     return;
   } else if (A->val27) {
     unsigned g = xcd_grid(grid_for(A->nrows, 256));
-    k_st27_jacobi<T><<<g, 256, 0, st>>>(A->lat_n + 1, (const T *)A->val27, (const T *)A->diag,
-                                        damp, x, b, xo);
+    if (A->st_sym)
+      k_st27_jacobi<T, true><<<g, 256, 0, st>>>(A->lat_n + 1, (const T *)A->val27,
+                                                (const T *)A->diag, damp, x, b, xo);
+    else
+      k_st27_jacobi<T, false><<<g, 256, 0, st>>>(A->lat_n + 1, (const T *)A->val27,
+                                                 (const T *)A->diag, damp, x, b, xo);
   } else {
     const Sell &s = A->sell;
     if (s.nrows == 0) return;
@@ -1510,15 +1577,22 @@ void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const v
       long cnt = (long)((M - cx + 1) / 2) * ((M - cy + 1) / 2) * ((M - cz + 1) / 2);
       unsigned g = xcd_grid(grid_for(cnt, 256));
       if (!A->alpha) {
-        if (A->prec == FLGPU_F64)
-          k_st27_sor<double><<<g, 256, 0, A->ctx->stream>>>(M, cz, cy, cx,
-                                                            (const double *)A->val27, omega,
-                                                            (double *)x, (const double *)b,
-                                                            (double *)r);
+        if (A->prec == FLGPU_F64 && A->st_sym)
+          k_st27_sor<double, true><<<g, 256, 0, A->ctx->stream>>>(
+              M, cz, cy, cx, (const double *)A->val27, omega, (double *)x, (const double *)b,
+              (double *)r);
+        else if (A->prec == FLGPU_F64)
+          k_st27_sor<double, false><<<g, 256, 0, A->ctx->stream>>>(
+              M, cz, cy, cx, (const double *)A->val27, omega, (double *)x, (const double *)b,
+              (double *)r);
+        else if (A->st_sym)
+          k_st27_sor<float, true><<<g, 256, 0, A->ctx->stream>>>(
+              M, cz, cy, cx, (const float *)A->val27, (float)omega, (float *)x, (const float *)b,
+              (float *)r);
         else
-          k_st27_sor<float><<<g, 256, 0, A->ctx->stream>>>(M, cz, cy, cx, (const float *)A->val27,
-                                                           (float)omega, (float *)x,
-                                                           (const float *)b, (float *)r);
+          k_st27_sor<float, false><<<g, 256, 0, A->ctx->stream>>>(
+              M, cz, cy, cx, (const float *)A->val27, (float)omega, (float *)x, (const float *)b,
+              (float *)r);
       } else if (A->prec == FLGPU_F64)
         k_q1_sor<double><<<g, 256, 0, A->ctx->stream>>>(M, A->lat_n, cz, cy, cx,
                                                         (const double *)A->alpha, q1k(A), omega,
@@ -1569,7 +1643,7 @@ __global__ void k_rap(int nc, const int *__restrict__ ptrp, const int *__restric
                       const int *__restrict__ aci, const T *__restrict__ av,
                       const int *__restrict__ prp, const int *__restrict__ pci,
                       const T *__restrict__ pv, const int *__restrict__ crp,
-                      const int *__restrict__ cci, double *acc, int *bad) {
+                      const int *__restrict__ cci, double *acc, int *bad, int upper) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nc) return;
   int c0 = crp[k], c1 = crp[k + 1];
@@ -1582,6 +1656,7 @@ __global__ void k_rap(int nc, const int *__restrict__ ptrp, const int *__restric
       double w = plk * (double)av[b];
       for (int c = prp[m]; c < prp[m + 1]; c++) {
         int n = pci[c];
+        if (upper && n < k) continue;  // the lower triangle is mirrored afterwards
         int lo = c0, hi = c1 - 1;
         while (lo < hi) {
           int mid = (lo + hi) >> 1;
@@ -1604,7 +1679,70 @@ __global__ void k_store(long n, const double *acc, T *v) {
   if (i < n) v[i] = (T)acc[i];
 }
 
-// Ac's values <- P^T A P (pattern of Ac given); all three share one precision
+// asym[0] = 1 unless A's values are exactly symmetric (every A(i, j) has an equal A(j, i))
+template <class T>
+__global__ void k_csr_symcheck(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                               const T *__restrict__ v, int *asym) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool bad = false;
+  for (int e = rp[i]; e < rp[i + 1]; e++) {
+    int j = ci[e];
+    if (j == i) continue;
+    int lo = rp[j], hi = rp[j + 1] - 1;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (ci[mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    bad |= lo > hi || ci[lo] != i || !(v[lo] == v[e]);
+  }
+  if (bad) asym[0] = 1;
+}
+
+// the lower triangle of the coarse accumulators from the upper one (missing[0] = 1 when the
+// pattern has no transposed position)
+__global__ void k_mirror(int nc, const int *__restrict__ crp, const int *__restrict__ cci,
+                         double *acc, int *missing) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nc) return;
+  for (int e = crp[k]; e < crp[k + 1]; e++) {
+    int m = cci[e];
+    if (m >= k) break;
+    int lo = crp[m], hi = crp[m + 1] - 1;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (cci[mid] < k) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo > hi || cci[lo] != k) missing[0] = 1;
+    else acc[e] = acc[lo];
+  }
+}
+
+static bool csr_symmetric(const flgpu_mat *A) {
+  if (A->nrows != A->ncols || A->nrows == 0) return false;
+  flgpu_ctx *ctx = A->ctx;
+  int *asym = (int *)dev_alloc(sizeof(int));
+  FLGPU_HIP(hipMemsetAsync(asym, 0, sizeof(int), ctx->stream));
+  if (A->prec == FLGPU_F64)
+    k_csr_symcheck<double><<<grid_for(A->nrows, 256), 256, 0, ctx->stream>>>(
+        A->nrows, A->rowptr, A->colidx, (const double *)A->vals, asym);
+  else
+    k_csr_symcheck<float><<<grid_for(A->nrows, 256), 256, 0, ctx->stream>>>(
+        A->nrows, A->rowptr, A->colidx, (const float *)A->vals, asym);
+  FLGPU_HIP(hipGetLastError());
+  int h = 1;
+  FLGPU_HIP(hipMemcpyAsync(&h, asym, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+  dev_free(asym);
+  return h == 0;
+}
+
+// Ac's values <- P^T A P (pattern of Ac given); all three share one precision.  When A is
+// exactly symmetric, only the upper triangle is accumulated and mirrored, so Ac is exactly
+// symmetric too (the two triangles of P^T A P otherwise round differently), which lets the
+// stencil kernels stream half the coefficients.
 void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac) {
   FLGPU_REQUIRE(P->nrows == A->nrows && A->nrows == A->ncols, "galerkin: P and A do not match");
   FLGPU_REQUIRE(Ac->nrows == P->ncols && Ac->ncols == P->ncols, "galerkin: Ac does not match P");
@@ -1616,27 +1754,44 @@ void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac) {
   int *bad = (int *)dev_alloc(sizeof(int));
   FLGPU_HIP(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
   int nc = Ac->nrows;
-  if (A->prec == FLGPU_F64) {
-    k_rap<double><<<grid_for(nc, 128), 128, 0, ctx->stream>>>(
-        nc, PT->rowptr, PT->colidx, (const double *)PT->vals, A->rowptr, A->colidx,
-        (const double *)A->vals, P->rowptr, P->colidx, (const double *)P->vals, Ac->rowptr,
-        Ac->colidx, acc, bad);
+  int *missing = (int *)dev_alloc(sizeof(int));
+  auto rap = [&](int upper) {
+    if (A->prec == FLGPU_F64)
+      k_rap<double><<<grid_for(nc, 128), 128, 0, ctx->stream>>>(
+          nc, PT->rowptr, PT->colidx, (const double *)PT->vals, A->rowptr, A->colidx,
+          (const double *)A->vals, P->rowptr, P->colidx, (const double *)P->vals, Ac->rowptr,
+          Ac->colidx, acc, bad, upper);
+    else
+      k_rap<float><<<grid_for(nc, 128), 128, 0, ctx->stream>>>(
+          nc, PT->rowptr, PT->colidx, (const float *)PT->vals, A->rowptr, A->colidx,
+          (const float *)A->vals, P->rowptr, P->colidx, (const float *)P->vals, Ac->rowptr,
+          Ac->colidx, acc, bad, upper);
+    FLGPU_HIP(hipGetLastError());
+  };
+  int hmiss = 1;
+  if (csr_symmetric(A)) {
+    FLGPU_HIP(hipMemsetAsync(missing, 0, sizeof(int), ctx->stream));
+    rap(1);
+    k_mirror<<<grid_for(nc, 128), 128, 0, ctx->stream>>>(nc, Ac->rowptr, Ac->colidx, acc,
+                                                          missing);
+    FLGPU_HIP(hipGetLastError());
+    FLGPU_HIP(hipMemcpyAsync(&hmiss, missing, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  if (hmiss) rap(0);  // not symmetric, or the coarse pattern is not: the full product
+  if (A->prec == FLGPU_F64)
     k_store<double><<<grid_for(Ac->nnz, 256), 256, 0, ctx->stream>>>(Ac->nnz, acc,
                                                                      (double *)Ac->vals);
-  } else {
-    k_rap<float><<<grid_for(nc, 128), 128, 0, ctx->stream>>>(
-        nc, PT->rowptr, PT->colidx, (const float *)PT->vals, A->rowptr, A->colidx,
-        (const float *)A->vals, P->rowptr, P->colidx, (const float *)P->vals, Ac->rowptr,
-        Ac->colidx, acc, bad);
+  else
     k_store<float><<<grid_for(Ac->nnz, 256), 256, 0, ctx->stream>>>(Ac->nnz, acc,
                                                                     (float *)Ac->vals);
-  }
   FLGPU_HIP(hipGetLastError());
   int hbad = 0;
   FLGPU_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   FLGPU_HIP(hipStreamSynchronize(ctx->stream));
   dev_free(acc);
   dev_free(bad);
+  dev_free(missing);
   FLGPU_REQUIRE(!hbad, "galerkin: P^T A P has entries outside the coarse pattern");
 }
 
