@@ -25,9 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 WEIGHTS = os.path.join(ROOT, "gnu_femlisp_amd", "unet_weights.npz")
 PATH_DESC = {
-    "mixed-bb-jac": "FP32 damped-Jacobi V(4,4) V-cycle, black-box transfers, Galerkin levels, "
+    "mixed-bb-jac": "FP32 damped-Jacobi V(2,2) V-cycle, omega 0.9, black-box transfers, Galerkin levels, "
                     "in FP64 flexible CG",
-    "plain-bb-jac": "FP64 damped-Jacobi V(4,4) V-cycle, black-box transfers, Galerkin levels, in CG",
+    "plain-bb-jac": "FP64 damped-Jacobi V(2,2) V-cycle, omega 0.9, black-box transfers, Galerkin levels, "
+                    "in CG",
     "mixed-bb": "FP32 multicolour GS V(2,2) V-cycle, black-box transfers, in FP64 flexible CG",
     "plain-bb": "FP64 multicolour GS V(2,2) V-cycle, black-box transfers, in CG",
     "mixed": "FP32 multicolour GS V(2,2) V-cycle in FP64 flexible CG",
@@ -152,7 +153,7 @@ def main():
                     sb, wb = spmv_bytes(nnz, N, vb), sor_sweep_bytes(nnz, N, vb)
                     jb = nnz * (vb + 4) + 4 * (N + 1) + 4 * N * vb
                 # 8 <jacobi> steps per call (an even count ends in x: no copy)
-                jac = (lambda: check(lib().flgpu_jacobi_steps(M.h, 0.7, 8, x.h, b.h, r.h)))
+                jac = (lambda: check(lib().flgpu_jacobi_steps(M.h, 0.9, 8, x.h, b.h, r.h)))
                 for name, fn, byt, per in (
                         (f"{fname}_spmv_{tag}", lambda: gemm(1.0, M, y, 0.0, x), sb, 1),
                         (f"{fname}_sor_{tag}", lambda: sor(M, x, b, r), wb, 1),
@@ -201,7 +202,7 @@ def main():
     # mixed: FP32 V-cycle (BF16-MFMA assembled) in FP64 FCG; "-bb": the same with
     # operator-dependent (black-box) transfers and Galerkin coarse levels
     # operator-dependent (black-box) transfers and Galerkin coarse levels; "-jac": damped
-    # <jacobi> V(4,4), omega 0.7, one launch per step instead of 8 colour launches per sweep
+    # <jacobi> V(2,2), damping 0.9, one launch per step instead of 8 colour launches per sweep
     spec = {"plain": (F.F64, F.ASM_F64, "geometric", "gs"),
             "mixed": (F.F32, F.ASM_MFMA, "geometric", "gs"),
             "plain-bb": (F.F64, F.ASM_F64, "blackbox", "gs"),
@@ -215,7 +216,7 @@ def main():
             continue
         t = time.perf_counter()
         if sm == "jac":
-            cs = it.GeometricCS(it.Jacobi(0.7), 4, 4, args.base, prec=prec, transfer=tr)
+            cs = it.GeometricCS(it.Jacobi(0.9), 2, 2, args.base, prec=prec, transfer=tr)
         else:
             cs = it.GeometricCS(it.SOR(1.0), 2, 2, args.base, symmetric=True, prec=prec,
                                 transfer=tr)
@@ -312,7 +313,7 @@ def main():
 
     # ---- roofline of the value path's dominant kernel -------------------------------
     # k_q1_zblock<float, 1, unsigned int, 5, false> (the FP32 per-cell <jacobi> pass, 5 rows per
-    # thread): one damped <jacobi> step on the fine level of the mixed-bb-jac hierarchy, 8 per
+    # thread): one damped <jacobi> step on the fine level of the mixed-bb-jac hierarchy, 4 per
     # V-cycle.  Events bracket 8 x reps back-to-back launches,
     # so the average includes the dispatch gap between launches; large copies first let the
     # host queue up ahead of the GPU.  Algorithmic bytes per launch: the cell factors, x and
@@ -324,14 +325,14 @@ def main():
         xj = GpuVector(At.nrows, F.F32)
         bj = GpuVector.from_array(b.numpy(), F.F32)
         tj = GpuVector(At.nrows, F.F32)
-        check(lib().flgpu_jacobi_steps(At.h, 0.7, 8, xj.h, bj.h, tj.h))
+        check(lib().flgpu_jacobi_steps(At.h, 0.9, 8, xj.h, bj.h, tj.h))
         big_a, big_b = GpuVector(1 << 26), GpuVector(1 << 26)
         for _ in range(24):
             vcopy(big_a, big_b)
         reps = 25
         ctx.timer_start()
         for _ in range(reps):
-            check(lib().flgpu_jacobi_steps(At.h, 0.7, 8, xj.h, bj.h, tj.h))
+            check(lib().flgpu_jacobi_steps(At.h, 0.9, 8, xj.h, bj.h, tj.h))
         ms = ctx.timer_stop()
         del big_a, big_b
         us = ms * 1e3 / (8 * reps)
