@@ -328,7 +328,21 @@ static void graphs_clear(flgpu_mg *mg) {
 // another precision), and q is written instead of cleared and updated.  work (n values of
 // vprec) is the top-level residual; it is seeded from r only when the first smoothing step
 // would read it (an in-place <jacobi> iterate, or no pre-smoothing).
-static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work) {
+// true when an FP32 (or FP64) cycle for vectors of the other precision stages r by the pass
+// that also takes the first top-level <jacobi> step (stage_first_step): then a caller that
+// produces r anyway may write that pass's two outputs itself (mg_run mode 2)
+static bool mg_stages_first(const flgpu_mg *mg, int vprec) {
+  const flgpu_mg_params &p = mg->p;
+  const MGLevel &T = mg->lv[mg->top];
+  bool base_top = mg->top == p.base_level;
+  bool jac = p.smoother == FLGPU_SMOOTH_JACOBI && T.tmp && p.pre_steps > 0 && !base_top;
+  return vprec != p.prec && jac && p.additive == 0;
+}
+
+// staged: the caller has already written bs (r converted) and the top level's tmp (the first
+// <jacobi> step from zero) as stage_first_step would (only when mg_stages_first)
+static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work,
+                          bool staged = false) {
   const flgpu_mg_params &p = mg->p;
   MGLevel &T = mg->lv[mg->top];
   long N = T.N;
@@ -338,6 +352,7 @@ static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work)
   bool add = p.additive != 0;
   bool reads_r = base_top || add || !(jac || (p.smoother == FLGPU_SMOOTH_SOR && p.pre_steps > 0));
   bool zero = (jac && !add) || base_top;  // the first step writes x
+  FLGPU_REQUIRE(!staged || mg_stages_first(mg, vprec), "this cycle does not take a staged rhs");
   if (vprec == p.prec) {
     if (zero) T.x_zero = true;
     else vfill(mg->ctx, N, vprec, q, 0.0);
@@ -347,7 +362,7 @@ static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work)
   }
   if (zero && jac && !reads_r) {
     // the staging pass also takes the first top-level <jacobi> step from zero
-    stage_first_step(T.A, vprec, r, mg->bs, p.omega, T.tmp);
+    if (!staged) stage_first_step(T.A, vprec, r, mg->bs, p.omega, T.tmp);
     T.step1 = true;
   } else {
     vcopy(mg->ctx, N, vprec, r, p.prec, mg->bs);
@@ -364,11 +379,12 @@ static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work)
 }
 
 static void mg_run(flgpu_mg *mg, int mode, int vprec, void *x, void *b, void *r) {
-  if (mode == 1) mg_apply_zero(mg, vprec, x, b, r);
+  if (mode == 1 || mode == 2) mg_apply_zero(mg, vprec, x, b, r, mode == 2);
   else mg_apply_ptr(mg, vprec, x, b, r);
 }
 
-// mode 0: x <- x + V(r) on (x, b, r); mode 1: x <- V(b) from zero, b read only, r work.
+// mode 0: x <- x + V(r) on (x, b, r); mode 1: x <- V(b) from zero, b read only, r work; mode
+// 2: mode 1 with b already staged (mg_apply_zero's `staged`).
 // With graphs on, the first call on a (mode, x, b, r, precision) is captured and replayed.
 static void mg_run_cached(flgpu_mg *mg, int mode, int vprec, void *x, void *b, void *r) {
   if (!mg->use_graphs) {
@@ -486,14 +502,17 @@ __global__ void k_fcg_p(long n, const double *zr, const double *zro, const doubl
   p[i] = q[i] + beta * p[i];
 }
 
-__global__ void k_slot_copy(double *s, int from, int to) { s[to] = s[from]; }
-
 // <cg>'s update in one pass: rold <- r (FCG, when given), x <- x + lam p, r <- r - lam a with
 // lam = num / den rounded to T once per sign (femlisp's axpy! with (/ alpha beta))
-template <class T>
+// STAGE: also the next preconditioner call's staging pass on the new r (stage_first_step, for
+// an FP32 hierarchy under FP64 vectors): bs = (float)r, xo = damp (bs / diag), the values that
+// pass would write
+template <class T, bool STAGE = false>
 __global__ void k_cg_update(long n, const double *num, const double *den, const T *__restrict__ p,
                             const T *__restrict__ a, T *__restrict__ x, T *__restrict__ r,
-                            T *__restrict__ rold) {
+                            T *__restrict__ rold, float *__restrict__ bs = nullptr,
+                            const float *__restrict__ diag = nullptr, float damp = 0.0f,
+                            float *__restrict__ xo = nullptr) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double q = *num / *den;
@@ -501,7 +520,13 @@ __global__ void k_cg_update(long n, const double *num, const double *den, const 
   T ri = r[i];
   if (rold) rold[i] = ri;
   x[i] = lp * p[i] + x[i];
-  r[i] = lm * a[i] + ri;
+  T rn = lm * a[i] + ri;
+  r[i] = rn;
+  if (STAGE) {
+    float bi = (float)rn;
+    bs[i] = bi;
+    xo[i] = damp * (bi / diag[i]);
+  }
 }
 
 }  // namespace flgpu
@@ -739,9 +764,9 @@ namespace flgpu {
 // preconditioner call of <cg> (krylow.lisp:118-121): (copy! r w) (x<-0 q)
 // (funcall iterate q w w); alias = 0 passes a second copy as the residual
 static void precondition(flgpu_mg *mg, const flgpu_solve_params *p, SolveWork &W, void *r,
-                         void *q, LearnedHook *learned) {
+                         void *q, LearnedHook *learned, bool staged = false) {
   if (!p->alias) {
-    mg_run_cached(mg, 1, W.prec, q, r, W.w2);
+    mg_run_cached(mg, staged ? 2 : 1, W.prec, q, r, W.w2);
     if (learned) learned->correct(r, q);
     return;
   }
@@ -788,6 +813,9 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
     // the iteration keeps, so they are queued ahead of the host's wait for the stopping
     // test and simply dropped when the iteration stops or restarts
     bool ap_ready = false;
+    // the current alpha's slot: 0 after a restart, then alternating 2 / 0 (the new alpha goes
+    // to the other one, so no copy is needed between steps)
+    int ca = 0;
     int fell_back = 0;
     double best_true = -1.0;
     int best_step = 0;
@@ -858,6 +886,7 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             vdot_dev(ctx, N, prec, W.p, W.r, 0);
             fetch_slots(ctx, 1);
             alpha = ctx->host_out[0];
+            ca = 0;
             initialized = true;
           }
           if (alpha != 0.0) {
@@ -866,47 +895,57 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
               vdot_dev(ctx, N, prec, W.a, W.p, 1);       // beta = a.p
             }
             ap_ready = false;
-            // x += (alpha/beta) p, r -= (alpha/beta) a (FCG keeps the old r first)
-            if (prec == FLGPU_F64)
-              k_cg_update<double><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, (const double *)W.p,
+            // x += (alpha/beta) p, r -= (alpha/beta) a (FCG keeps the old r first); for an
+            // FP32 hierarchy under FP64 vectors the same pass stages the new r for the cycle
+            const bool staged = pc && !p->alias && prec == FLGPU_F64 &&
+                                mg->p.prec == FLGPU_F32 && mg_stages_first(mg, prec);
+            if (staged) {
+              MGLevel &T = mg->lv[mg->top];
+              k_cg_update<double, true><<<g, 256, 0, s>>>(
+                  N, slot + ca, slot + 1, (const double *)W.p, (const double *)W.a,
+                  (double *)x->d, (double *)W.r, fcg ? (double *)W.rold : nullptr,
+                  (float *)mg->bs, (const float *)T.A->diag, (float)mg->p.omega, (float *)T.tmp);
+            } else if (prec == FLGPU_F64)
+              k_cg_update<double><<<g, 256, 0, s>>>(N, slot + ca, slot + 1, (const double *)W.p,
                                                     (const double *)W.a, (double *)x->d,
                                                     (double *)W.r,
                                                     fcg ? (double *)W.rold : nullptr);
             else
-              k_cg_update<float><<<g, 256, 0, s>>>(N, slot + 0, slot + 1, (const float *)W.p,
+              k_cg_update<float><<<g, 256, 0, s>>>(N, slot + ca, slot + 1, (const float *)W.p,
                                                    (const float *)W.a, (float *)x->d,
                                                    (float *)W.r, fcg ? (float *)W.rold : nullptr);
             FLGPU_HIP(hipGetLastError());
-            if (pc) precondition(mg, p, W, W.r, W.q, learned);
+            if (pc) precondition(mg, p, W, W.r, W.q, learned, staged);
             else vcopy(ctx, N, prec, W.r, prec, W.q);
+            const int na = 2 - ca;  // the new alpha's slot
             {  // new alpha = q.r, defect norm^2 = r.r, FCG's q.rold: one pass
               const void *dx[3] = {W.q, W.r, W.q}, *dy[3] = {W.r, W.r, W.rold};
-              const int slots[3] = {2, 3, 4};
+              const int slots[3] = {na, 3, 4};
               vdot_multi_dev(ctx, N, prec, fcg ? 3 : 2, dx, dy, slots);
             }
             if (fcg) {
               if (prec == FLGPU_F64)
-                k_fcg_p<double><<<g, 256, 0, s>>>(N, slot + 2, slot + 4, slot + 0, (double *)W.p,
-                                                  (const double *)W.q);
+                k_fcg_p<double><<<g, 256, 0, s>>>(N, slot + na, slot + 4, slot + ca,
+                                                  (double *)W.p, (const double *)W.q);
               else
-                k_fcg_p<float><<<g, 256, 0, s>>>(N, slot + 2, slot + 4, slot + 0, (float *)W.p,
+                k_fcg_p<float><<<g, 256, 0, s>>>(N, slot + na, slot + 4, slot + ca, (float *)W.p,
                                                  (const float *)W.q);
             } else {
               if (prec == FLGPU_F64)
-                k_cg_p<double><<<g, 256, 0, s>>>(N, slot + 2, slot + 0, (double *)W.p,
+                k_cg_p<double><<<g, 256, 0, s>>>(N, slot + na, slot + ca, (double *)W.p,
                                                  (const double *)W.q);
               else
-                k_cg_p<float><<<g, 256, 0, s>>>(N, slot + 2, slot + 0, (float *)W.p,
+                k_cg_p<float><<<g, 256, 0, s>>>(N, slot + na, slot + ca, (float *)W.p,
                                                 (const float *)W.q);
             }
-            k_slot_copy<<<1, 1, 0, s>>>(slot, 2, 0);
             FLGPU_HIP(hipGetLastError());
+            ca = na;
             fetch_slots_async(ctx, 4);
             spmv(A, 1.0, W.p, 0.0, W.a, W.a);  // the next step's product, ahead of the wait
             vdot_dev(ctx, N, prec, W.a, W.p, 1);
             ap_ready = true;
             fetch_wait(ctx);
-            alpha = ctx->host_out[0];
+            alpha = ctx->host_out[ca];
           } else {
             vdot_dev(ctx, N, prec, W.r, W.r, 3);
             fetch_slots(ctx, 4);
