@@ -552,6 +552,10 @@ def test_blackbox_hierarchy_solve_parity():
         orp, oci, ov = oh.matrix(l)
         assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
         assert rowrel(v, ov, orp) < 1e-12
+        # the fine matrix is exactly symmetric, so the Galerkin levels are made exactly
+        # symmetric too (upper triangle mirrored): the stencil kernels rely on it
+        Ml = sp.csr_matrix((v, ci, rp))
+        assert (Ml != Ml.T).nnz == 0, l
     rng = np.random.default_rng(4)
     b = rng.uniform(-1, 1, oh.ndofs())
     b[po.boundary(dim, order, ntop)] = 0
