@@ -372,6 +372,24 @@ static int interp_1d(int order, int nc, int F, int *cols, double *w) {
   return cnt;
 }
 
+long flo_projection(int dim, int order, int nc, int *rowptr, int *colidx, double *vals) {
+  int Mc = order * nc + 1, Mf = order * 2 * nc + 1;
+  long Nc = ipow(Mc, dim);
+  for (long c = 0; c < Nc; c++) {
+    long q = c, f = 0, stride = 1;
+    for (int k = 0; k < dim; k++) { /* the coarse node's lattice coordinates, doubled */
+      f += 2 * (q % Mc) * stride;
+      q /= Mc;
+      stride *= Mf;
+    }
+    if (rowptr) rowptr[c] = (int)c;
+    if (colidx) colidx[c] = (int)f;
+    if (vals) vals[c] = 1.0;
+  }
+  if (rowptr) rowptr[Nc] = (int)Nc;
+  return Nc;
+}
+
 long flo_interpolation(int dim, int order, int nc, int *rowptr, int *colidx,
                        double *vals) {
   int Mf = 2 * order * nc + 1, Mc = order * nc + 1;
@@ -574,6 +592,7 @@ typedef struct {
   csr_t A;
   csr_t I;  /* interpolation level -> level+1 (rows: level+1 DOFs) */
   csr_t IT; /* its transpose */
+  csr_t R;  /* FAS projection level+1 -> level (flo_projection) */
   double *diag;
   unsigned char *bdry;
   double *lu; /* dense LU of A (base level), row-major */
@@ -695,6 +714,12 @@ flo_mg *flo_mg_create_ex(int dim, int order, int n0, int levels,
       L->IT.vals = (double *)malloc(sizeof(double) * inz);
       flo_csr_transpose((int)Nf, (int)L->N, L->I.rowptr, L->I.colidx, L->I.vals,
                         L->IT.rowptr, L->IT.colidx, L->IT.vals);
+      L->R.nrows = (int)L->N;
+      L->R.ncols = (int)Nf;
+      L->R.rowptr = (int *)malloc(sizeof(int) * (L->N + 1));
+      L->R.colidx = (int *)malloc(sizeof(int) * L->N);
+      L->R.vals = (double *)malloc(sizeof(double) * L->N);
+      flo_projection(dim, order, L->n, L->R.rowptr, L->R.colidx, L->R.vals);
     }
     L->sol = (double *)calloc(L->N, sizeof(double));
     L->rhs = (double *)calloc(L->N, sizeof(double));
@@ -730,6 +755,7 @@ void flo_mg_destroy(flo_mg *mg) {
     csr_free(&L->A);
     csr_free(&L->I);
     csr_free(&L->IT);
+    csr_free(&L->R);
     free(L->diag);
     free(L->bdry);
     free(L->lu);
@@ -772,6 +798,15 @@ int flo_mg_get_interp(const flo_mg *mg, int level, int *rowptr, int *colidx,
   if (rowptr) memcpy(rowptr, I->rowptr, sizeof(int) * (I->nrows + 1));
   if (colidx) memcpy(colidx, I->colidx, sizeof(int) * nnz);
   if (vals) memcpy(vals, I->vals, sizeof(double) * nnz);
+  return 0;
+}
+int flo_mg_get_projection(const flo_mg *mg, int level, int *rowptr, int *colidx,
+                          double *vals) {
+  if (level < 0 || level >= mg->levels - 1) return -1;
+  const csr_t *R = &mg->lv[level].R;
+  if (rowptr) memcpy(rowptr, R->rowptr, sizeof(int) * (R->nrows + 1));
+  if (colidx) memcpy(colidx, R->colidx, sizeof(int) * R->nrows);
+  if (vals) memcpy(vals, R->vals, sizeof(double) * R->nrows);
   return 0;
 }
 int flo_mg_get_rhs(const flo_mg *mg, double *rhs) {
@@ -916,7 +951,14 @@ static void smoother_run(mgrun_t *R, int l, int post) {
   L->residual_p = 0;
 }
 
-/* lmgc with the correction scheme, :multiplicative or :additive, gamma coarse cycles
+/* FAS-R_{l-1} applied to level l's vector v (gemm! with the projection matrix, multigrid.lisp:
+ * 230, 246): sign = 0: out = R v; sign = -1: out = out - R v */
+static void fas_project(const flo_mg *mg, int l, const double *v, double *out, int sign) {
+  const csr_t *R = &mg->lv[l - 1].R;
+  flo_spmv(R->nrows, R->rowptr, R->colidx, R->vals, sign ? -1.0 : 1.0, v, sign ? 1.0 : 0.0, out);
+}
+
+/* lmgc with the correction scheme or FAS, :multiplicative or :additive, gamma coarse cycles
  * (multigrid.lisp:202-252,254-288). */
 static void lmgc(mgrun_t *R, int l) {
   flo_mg *mg = R->mg;
@@ -936,9 +978,17 @@ static void lmgc(mgrun_t *R, int l) {
   }
   level_t *C = &mg->lv[l - 1];
   flo_spmv(C->IT.nrows, C->IT.rowptr, C->IT.colidx, C->IT.vals, 1.0, L->res, 0.0, C->res);
-  /* :after methods: x_c = 0, rhs_c = res_c, residual-p t */
-  memset(C->sol, 0, sizeof(double) * C->N);
-  memcpy(C->rhs, C->res, sizeof(double) * C->N);
+  if (R->p->fas) {
+    /* <fas> :after (multigrid.lisp:226-232): rhs_c = res_c, sol_c = FAS-R sol_l,
+     * rhs_c += A_c sol_c, residual-p t */
+    memcpy(C->rhs, C->res, sizeof(double) * C->N);
+    fas_project(mg, l, L->sol, C->sol, 0);
+    flo_spmv(C->A.nrows, C->A.rowptr, C->A.colidx, C->A.vals, 1.0, C->sol, 1.0, C->rhs);
+  } else {
+    /* <correction-scheme> :after: x_c = 0, rhs_c = res_c, residual-p t */
+    memset(C->sol, 0, sizeof(double) * C->N);
+    memcpy(C->rhs, C->res, sizeof(double) * C->N);
+  }
   C->residual_p = 1;
   /* :additive smooths the fine level after the restriction, from the same residual */
   if (R->p->additive) smoother_run(R, l, 0);
@@ -954,9 +1004,17 @@ static void lmgc(mgrun_t *R, int l) {
   }
   /* :additive post-smooths before the prolongation */
   if (R->p->additive) smoother_run(R, l, 1);
-  /* prolongate: sol_l += I sol_{l-1}; residual-p nil */
-  flo_spmv(C->I.nrows, C->I.rowptr, C->I.colidx, C->I.vals, 1.0, C->sol, 1.0, L->sol);
-  L->residual_p = 0;
+  if (R->p->fas) {
+    /* <fas> prolongate (multigrid.lisp:241-247): res_c = sol_c - FAS-R sol_l,
+     * sol_l += I res_c */
+    memcpy(C->res, C->sol, sizeof(double) * C->N);
+    fas_project(mg, l, L->sol, C->res, -1);
+    flo_spmv(C->I.nrows, C->I.rowptr, C->I.colidx, C->I.vals, 1.0, C->res, 1.0, L->sol);
+  } else {
+    /* prolongate: sol_l += I sol_{l-1} */
+    flo_spmv(C->I.nrows, C->I.rowptr, C->I.colidx, C->I.vals, 1.0, C->sol, 1.0, L->sol);
+  }
+  L->residual_p = 0; /* prolongate :after, residual-p nil */
   if (!R->p->additive) smoother_run(R, l, 1);
 }
 

@@ -70,6 +70,13 @@ int flo_assemble(int dim, int order, int n, const double *kappa, int nk,
 long flo_interpolation(int dim, int order, int nc, int *rowptr, int *colidx,
                        double *vals);
 
+/* FAS projection (femlisp's projection-matrix, src/discretization/sparseas.lisp:278-297, with
+ * compute-local-pmatrix, fetransfer.lisp:201-213: the injection of the dof functionals)
+ * from the level with 2*nc cells per axis to the level with nc: rows coarse DOFs, one entry
+ * 1.0 each, at the fine DOF of the same point (lattice index 2c per axis).  Returns nnz
+ * (the coarse DOF count); rowptr, colidx, vals may be NULL. */
+long flo_projection(int dim, int order, int nc, int *rowptr, int *colidx, double *vals);
+
 /* Operator-dependent interpolation for 3D Q1 (not in femlisp; Dendy's
  * black-box prolongation for 27-point stencils): the values of the pattern of
  * flo_interpolation(3, 1, nf/2, prowptr, pcolidx, .) computed from the fine
@@ -120,6 +127,9 @@ long flo_mg_interp_nnz(const flo_mg *mg, int level);
 int flo_mg_get_interp(const flo_mg *mg, int level, int *rowptr, int *colidx,
                       double *vals);
 int flo_mg_get_rhs(const flo_mg *mg, double *rhs);
+/* copy of the FAS projection R_level (level+1 -> level), flo_projection's layout */
+int flo_mg_get_projection(const flo_mg *mg, int level, int *rowptr, int *colidx,
+                          double *vals);
 
 enum { FLO_OUTER_MG = 0, FLO_OUTER_CG = 1, FLO_OUTER_CG_PLAIN = 2,
        FLO_OUTER_FCG = 3 };
@@ -143,6 +153,8 @@ typedef struct flo_params {
                        (<mg-iteration> gamma, multigrid.lisp:60,268-273) */
   int additive;     /* 0 :multiplicative, 1 :additive combination (multigrid.lisp:67-70,
                        262-287): restrict first, then smooth; post-smooth, then prolongate */
+  int fas;          /* 0 correction scheme, 1 Brandt's FAS (<geometric-fas>, geomg.lisp:174-194;
+                       restrict/prolongate methods multigrid.lisp:226-247) */
 } flo_params;
 
 /* Solve A x = b from x = 0 with femlisp's <linear-solver> loop

@@ -34,14 +34,15 @@ class Params(ctypes.Structure):
         ("maxsteps", ctypes.c_int),
         ("gamma", ctypes.c_int),
         ("additive", ctypes.c_int),
+        ("fas", ctypes.c_int),
     ]
 
 
 def params(outer=OUTER_MG, smoother=SMOOTH_JACOBI, omega=1.0, ordering=ORDER_INDEX,
            pre_steps=1, post_steps=1, base_level=0, symmetric=0, alias=0,
-           stop=STOP_REDUCTION, tol=1e-10, maxsteps=100, gamma=1, additive=0):
+           stop=STOP_REDUCTION, tol=1e-10, maxsteps=100, gamma=1, additive=0, fas=0):
     return Params(outer, smoother, omega, ordering, pre_steps, post_steps,
-                  base_level, symmetric, alias, stop, tol, maxsteps, gamma, additive)
+                  base_level, symmetric, alias, stop, tol, maxsteps, gamma, additive, fas)
 
 
 def lib():
@@ -65,6 +66,9 @@ def lib():
         L.flo_assemble.argtypes = [ci, ci, ci, vp, ci, cd, vp, vp]
         L.flo_interpolation.restype = cl
         L.flo_interpolation.argtypes = [ci, ci, ci, vp, vp, vp]
+        L.flo_projection.restype = cl
+        L.flo_projection.argtypes = [ci, ci, ci, vp, vp, vp]
+        L.flo_mg_get_projection.argtypes = [vp, ci, vp, vp, vp]
         L.flo_blackbox_interpolation.restype = ci
         L.flo_blackbox_interpolation.argtypes = [ci, vp, vp, vp, vp, vp, vp]
         L.flo_spmv.argtypes = [ci, vp, vp, vp, cd, vp, cd, vp]
@@ -168,6 +172,14 @@ def interpolation(dim, order, nc):
     return rp, ci, v
 
 
+def projection(dim, order, nc):
+    """FAS projection from 2 nc to nc cells per axis (flo_projection)."""
+    Nc = ndofs(dim, order, nc)
+    rp, ci, v = np.zeros(Nc + 1, np.int32), np.zeros(Nc, np.int32), np.zeros(Nc)
+    lib().flo_projection(dim, order, nc, _p(rp), _p(ci), _p(v))
+    return rp, ci, v
+
+
 def blackbox_interpolation(nf, rp, ci, vals):
     """Operator-dependent interpolation (3D Q1) from the fine matrix with nf
     cells per axis, in the pattern of interpolation(3, 1, nf // 2)."""
@@ -232,6 +244,13 @@ class Hierarchy:
         ci = np.zeros(nnz, np.int32)
         v = np.zeros(nnz)
         lib().flo_mg_get_interp(self.h, level, _p(rp), _p(ci), _p(v))
+        return rp, ci, v
+
+    def projection(self, level):
+        """the FAS projection level + 1 -> level"""
+        Nc = self.ndofs(level)
+        rp, ci, v = np.zeros(Nc + 1, np.int32), np.zeros(Nc, np.int32), np.zeros(Nc)
+        assert lib().flo_mg_get_projection(self.h, level, _p(rp), _p(ci), _p(v)) == 0
         return rp, ci, v
 
     def rhs(self):

@@ -208,6 +208,64 @@ def test_additive_combination():
         assert np.abs(Mmul - (Msm + C)).max() > 1e-6 * np.abs(Madd).max()
 
 
+def test_fas_equals_correction_scheme_for_linear_problems():
+    """Brandt's FAS (<geometric-fas>, geomg.lisp:174-194; restrict/prolongate at
+    multigrid.lisp:226-247) approximates the solution on every level.  For a linear
+    operator its multiplicative cycle is the correction scheme's up to rounding, whatever the
+    smoother, the number of coarse cycles, the dimension or the order: the coarse solution
+    starts from the projected fine one with rhs_c = I^T r + A_c R x, and the correction
+    prolongated is x_c - R x.  Dropping the A_c R x term, or projecting anything but the
+    coincident fine node, breaks the equality."""
+    rng = np.random.default_rng(12)
+    for dim, order, n0, levels in ((1, 1, 2, 4), (2, 1, 2, 3), (2, 2, 1, 3), (3, 1, 1, 3)):
+        h = po.Hierarchy(dim, order, n0, levels)
+        b = rng.uniform(-1, 1, h.ndofs())
+        for smoother, omega in ((po.SMOOTH_JACOBI, 0.6), (po.SMOOTH_SOR, 1.0)):
+            for gamma in (1, 2):
+                base = dict(outer=po.OUTER_MG, smoother=smoother, omega=omega,
+                            ordering=po.ORDER_COLOUR, pre_steps=2, post_steps=1, base_level=0,
+                            gamma=gamma)
+                x_cs = h.apply(po.params(**base), b)
+                x_fas = h.apply(po.params(**base, fas=1), b)
+                assert np.linalg.norm(x_fas - x_cs) < 1e-12 * np.linalg.norm(x_cs), \
+                    (dim, order, smoother, gamma)
+        # as a preconditioner of <cg>: the same steps and solution
+        prm = dict(outer=po.OUTER_CG, smoother=po.SMOOTH_SOR, omega=1.0,
+                   ordering=po.ORDER_COLOUR, pre_steps=2, post_steps=2, base_level=0,
+                   symmetric=1, tol=1e-10, maxsteps=200)
+        st0, x0, s0, _ = h.solve(po.params(**prm), b)
+        st1, x1, s1, _ = h.solve(po.params(**prm, fas=1), b)
+        assert st0 == st1 == 0 and s0 == s1
+        assert np.linalg.norm(x1 - x0) < 1e-9 * np.linalg.norm(x0)
+
+
+def test_fas_projection_is_nodal_injection():
+    """FAS-R is femlisp's projection-matrix: the injection of the dof functionals
+    (fetransfer.lisp:201-213).  Pinned by the reference's own identity R I = Id
+    (sparseas.lisp:436-448 checks P I = Id) and by what injection means: R applied to a
+    function sampled on the fine lattice gives its samples at the coarse nodes exactly."""
+    for dim, order, nc in ((1, 1, 4), (2, 1, 3), (2, 2, 2), (3, 1, 2), (3, 2, 1)):
+        rp, ci, v = po.projection(dim, order, nc)
+        Nc, Nf = po.ndofs(dim, order, nc), po.ndofs(dim, order, 2 * nc)
+        R = sp.csr_matrix((v, ci, rp), shape=(Nc, Nf))
+        irp, ici, iv = po.interpolation(dim, order, nc)
+        I = sp.csr_matrix((iv, ici, irp), shape=(Nf, Nc))
+        RI = (R @ I).toarray()
+        # interior identity exactly; Dirichlet rows of I are zeroed (constraints.lisp:58-72)
+        inner = np.flatnonzero(~po.boundary(dim, order, nc))
+        assert np.array_equal(RI[np.ix_(inner, inner)], np.eye(len(inner)))
+
+        def sample(n):
+            M = order * n + 1
+            g = np.meshgrid(*([np.arange(M) / (M - 1)] * dim), indexing="ij")
+            return np.ravel(np.sin(3 * g[0]) + sum((k + 2) * g[k] ** 2 for k in range(1, dim)))
+        assert np.array_equal(R @ sample(2 * nc), sample(nc))
+    h = po.Hierarchy(3, 1, 1, 3)
+    for l in range(2):
+        a, b = h.projection(l), po.projection(3, 1, 1 << l)
+        assert all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
 def test_fe_value_1d():
     """ellsys-fe.lisp:249-256: 1D Q1 -u''=1 on level 2 gives u(0.5) = 0.125 exactly."""
     rp, ci, v, b = po.assemble(1, 1, 4)
