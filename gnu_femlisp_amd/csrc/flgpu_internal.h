@@ -227,6 +227,7 @@ void sor_sweeps(const flgpu_mat *A, double omega, int steps, int backwards, void
 flgpu_mat *lattice_assemble(flgpu_ctx *ctx, int dim, int order, int n, const flgpu_vec *kappa,
                             int nk, double f, int prec, int mode, flgpu_vec **rhs);
 flgpu_mat *lattice_interpolation(flgpu_ctx *ctx, int dim, int order, int nc, int prec);
+flgpu_mat *lattice_projection(flgpu_ctx *ctx, int dim, int order, int nc, int prec);
 void lattice_colouring(flgpu_mat *A, int dim, int order, int n);
 flgpu_mat *lattice_pattern(flgpu_ctx *ctx, int dim, int order, int n, int prec);
 flgpu_mat *lattice_galerkin(const flgpu_mat *P, const flgpu_mat *Afine, int dim, int order,

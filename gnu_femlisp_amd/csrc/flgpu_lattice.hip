@@ -912,6 +912,48 @@ void lattice_colouring(flgpu_mat *A, int dim, int order, int n) {
   A->ncolours = ncol;
 }
 
+// FAS projection, nc -> 2 nc cells per axis: coarse row c has one entry 1 at the fine DOF of
+// the same point, which doubles every lattice coordinate (the injection of the dof
+// functionals, fetransfer.lisp:201-213)
+template <class T>
+__global__ void k_projection_fill(long Nc, int dim, int Mc, int Mf, int *rowptr, int *colidx,
+                                  T *vals) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > Nc) return;
+  rowptr[c] = (int)c;
+  if (c == Nc) return;
+  long q = c, f = 0, stride = 1;
+  for (int k = 0; k < dim; k++) {
+    f += 2 * (q % Mc) * stride;
+    q /= Mc;
+    stride *= Mf;
+  }
+  colidx[c] = (int)f;
+  vals[c] = (T)1;
+}
+
+flgpu_mat *lattice_projection(flgpu_ctx *ctx, int dim, int order, int nc, int prec) {
+  check_lattice_args(dim, order, 2 * nc);
+  const int Mc = order * nc + 1, Mf = order * 2 * nc + 1;
+  const long Nc = ipow(Mc, dim), Nf = ipow(Mf, dim);
+  flgpu_mat *R = mat_new(ctx, (int)Nc, (int)Nf, Nc, prec);
+  try {
+    if (prec == FLGPU_F64)
+      k_projection_fill<double><<<grid_for(Nc + 1, 256), 256, 0, ctx->stream>>>(
+          Nc, dim, Mc, Mf, R->rowptr, R->colidx, (double *)R->vals);
+    else
+      k_projection_fill<float><<<grid_for(Nc + 1, 256), 256, 0, ctx->stream>>>(
+          Nc, dim, Mc, Mf, R->rowptr, R->colidx, (float *)R->vals);
+    FLGPU_HIP(hipGetLastError());
+    mat_finish(R);
+    FLGPU_HIP(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    mat_free(R);
+    throw;
+  }
+  return R;
+}
+
 flgpu_mat *lattice_interpolation(flgpu_ctx *ctx, int dim, int order, int nc, int prec) {
   check_lattice_args(dim, order, 2 * nc);
   Table1D t = table_interp(order, nc);
@@ -999,6 +1041,16 @@ extern "C" int flgpu_lattice_interpolation(flgpu_ctx *ctx, int dim, int order, i
     FLGPU_REQUIRE(ctx && P, "ctx and P must not be NULL");
     CTX_LOCK(ctx);
     *P = lattice_interpolation(ctx, dim, order, nc, prec);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_lattice_projection(flgpu_ctx *ctx, int dim, int order, int nc, int prec,
+                                        flgpu_mat **R) {
+  return guard([&] {
+    FLGPU_REQUIRE(ctx && R, "ctx and R must not be NULL");
+    CTX_LOCK(ctx);
+    *R = lattice_projection(ctx, dim, order, nc, prec);
     return 0;
   });
 }

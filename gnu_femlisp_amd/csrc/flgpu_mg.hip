@@ -28,6 +28,7 @@ namespace flgpu {
 struct MGLevel {
   flgpu_mat *A = nullptr;  // owned
   flgpu_mat *I = nullptr;  // owned: interpolation level -> level+1 (transpose attached)
+  flgpu_mat *R = nullptr;  // owned, FAS only: projection level+1 -> level (:fas-r-vec)
   void *sol = nullptr, *rhs = nullptr, *res = nullptr;  // owned work vectors
   void *tmp = nullptr;  // <jacobi> steps: the other half of x's ping-pong (smoother Jacobi)
   bool residual_p = false;
@@ -232,7 +233,17 @@ static void lmgc(flgpu_mg *mg, int l) {
   // straight into rhs there (the coarse solve reads only res)
   bool base = (l - 1 == mg->p.base_level);
   bool fused = !add && mg->p.smoother == FLGPU_SMOOTH_JACOBI && C.tmp && mg->p.pre_steps > 0;
-  if (fused && !base) {
+  if (mg->p.fas) {
+    // <fas> :after (multigrid.lisp:226-232): rhs_{l-1} = res_{l-1}, sol_{l-1} = R sol_l,
+    // rhs_{l-1} += A_{l-1} sol_{l-1}, residual-p t
+    spmv(C.I->transpose, 1.0, lres(mg, l), 0.0, C.res, C.res);
+    vcopy(mg->ctx, C.N, mg->p.prec, C.res, mg->p.prec, C.rhs);
+    spmv(C.R, 1.0, lsol(mg, l), 0.0, C.sol, C.sol);
+    spmv(C.A, 1.0, C.sol, 1.0, C.rhs, C.rhs);
+    C.x_zero = false;
+    C.step1 = false;
+    C.residual_p = true;
+  } else if (fused && !base) {
     // the first coarse <jacobi> step from zero is pointwise in rhs: taken in the same pass
     // where the transfer allows it
     C.step1 = restrict_first_step(C.I->transpose, lres(mg, l), C.rhs, C.A, mg->p.omega, C.tmp);
@@ -262,8 +273,15 @@ static void lmgc(flgpu_mg *mg, int l) {
     C.step1 = false;
   }
   if (add) smooth(mg, l, true);  // :additive post-smooths before the prolongation
-  // prolongate (multigrid.lisp:237-252): sol_l += I sol_{l-1}
-  spmv(C.I, 1.0, C.sol, 1.0, lsol(mg, l), lsol(mg, l));
+  if (mg->p.fas) {
+    // <fas> prolongate (multigrid.lisp:241-247): res_{l-1} = sol_{l-1} - R sol_l,
+    // sol_l += I res_{l-1}
+    spmv(C.R, -1.0, lsol(mg, l), 1.0, C.sol, C.res);
+    spmv(C.I, 1.0, C.res, 1.0, lsol(mg, l), lsol(mg, l));
+  } else {
+    // prolongate (multigrid.lisp:237-252): sol_l += I sol_{l-1}
+    spmv(C.I, 1.0, C.sol, 1.0, lsol(mg, l), lsol(mg, l));
+  }
   L.residual_p = false;
   if (!add) smooth(mg, l, true, l == mg->top);
 }
@@ -435,6 +453,7 @@ static void check_params(const flgpu_mg_params *p, int nlevels) {
                 "unknown transfer");
   FLGPU_REQUIRE(p->gamma >= 0 && p->gamma <= 16, "gamma must be in 0..16");
   FLGPU_REQUIRE(p->additive == 0 || p->additive == 1, "additive must be 0 or 1");
+  FLGPU_REQUIRE(p->fas == 0 || p->fas == 1, "fas must be 0 or 1");
 }
 
 static void mg_alloc_work(flgpu_mg *mg) {
@@ -469,6 +488,7 @@ void mg_free(flgpu_mg *mg) {
   for (auto &L : mg->lv) {
     mat_free(L.A);
     mat_free(L.I);
+    mat_free(L.R);
     dev_free(L.sol);
     dev_free(L.rhs);
     dev_free(L.res);
@@ -555,6 +575,9 @@ extern "C" int flgpu_mg_create(flgpu_ctx *ctx, int nlevels, flgpu_mat *const *A,
                       "interpolation dimensions do not match the level matrices");
       }
     }
+    FLGPU_REQUIRE(!params->fas,
+                  "FAS needs the projection matrices of a lattice hierarchy "
+                  "(flgpu_mg_create_lattice)");
     CTX_LOCK(ctx);
     flgpu_mg *mg = new flgpu_mg;
     mg->ctx = ctx;
@@ -611,6 +634,8 @@ extern "C" int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n
         int n = n0 << l;
         bool is_top = (l == nlevels - 1);
         if (!is_top) mg->lv[l].I = lattice_interpolation(ctx, dim, order, n, params->prec);
+        if (!is_top && params->fas)
+          mg->lv[l].R = lattice_projection(ctx, dim, order, n, params->prec);
       }
       for (int l = nlevels - 1; l >= 0; l--) {
         int n = n0 << l;
@@ -705,8 +730,10 @@ extern "C" int flgpu_mg_level_matrix(flgpu_mg *mg, int level, int interp, flgpu_
   return guard([&] {
     FLGPU_REQUIRE(mg && A, "arguments must not be NULL");
     FLGPU_REQUIRE(level >= 0 && level <= mg->top, "level out of range");
+    FLGPU_REQUIRE(interp >= 0 && interp <= 2, "interp must be 0, 1 or 2");
     FLGPU_REQUIRE(!interp || level < mg->top, "the top level has no interpolation");
-    *A = interp ? mg->lv[level].I : mg->lv[level].A;
+    FLGPU_REQUIRE(interp != 2 || mg->lv[level].R, "no projection: the hierarchy is not FAS");
+    *A = interp == 2 ? mg->lv[level].R : interp ? mg->lv[level].I : mg->lv[level].A;
     return 0;
   });
 }

@@ -38,7 +38,7 @@ class MgParams(ctypes.Structure):
                 ("base_level", ctypes.c_int), ("symmetric", ctypes.c_int),
                 ("prec", ctypes.c_int), ("galerkin", ctypes.c_int),
                 ("transfer", ctypes.c_int), ("gamma", ctypes.c_int),
-                ("additive", ctypes.c_int)]
+                ("additive", ctypes.c_int), ("fas", ctypes.c_int)]
 
 
 class SolveParams(ctypes.Structure):
@@ -96,6 +96,7 @@ SIGNATURES = {
     "flgpu_lattice_assemble": (_ci, [_vp, _ci, _ci, _ci, _vp, _ci, _cd, _ci, _ci, _P(_vp), _P(_vp)]),
     "flgpu_lattice_reassemble": (_ci, [_vp, _vp, _ci, _ci]),
     "flgpu_lattice_interpolation": (_ci, [_vp, _ci, _ci, _ci, _ci, _P(_vp)]),
+    "flgpu_lattice_projection": (_ci, [_vp, _ci, _ci, _ci, _ci, _P(_vp)]),
     "flgpu_lattice_colouring": (_ci, [_vp, _ci, _ci, _ci]),
     "flgpu_lattice_blackbox_interpolation": (_ci, [_vp, _vp]),
     "flgpu_lattice_element_matrices": (_ci, [_vp, _ci, _ci, _ci, _vp, _ci, _ci, _ci, _cl, _cl, _vp]),

@@ -6,6 +6,7 @@ Classes and their femlisp counterparts:
   SOR(omega), GaussSeidel()   <sor>/<gauss-seidel> with a colour :compare
                                                 src/iteration/linit.lisp:258-385
   GeometricCS(...)            <geometric-cs>    src/iteration/geomg.lisp:109-155
+  GeometricFAS(...)           <geometric-fas>   src/iteration/geomg.lisp:174-194
   CG(preconditioner)          <cg>              src/iteration/krylow.lisp:67-132
   FCG(preconditioner)         flexible CG (no femlisp counterpart; equals <cg>
                               for a fixed SPD preconditioner)
@@ -89,7 +90,28 @@ class GeometricCS(LinearIteration):
                         int(self.symmetric), self.prec, int(self.galerkin),
                         _f.TRANSFER_BLACKBOX if self.transfer == "blackbox"
                         else _f.TRANSFER_GEOMETRIC, int(self.gamma),
-                        int(self.combination == "additive"))
+                        int(self.combination == "additive"), int(self.fas))
+
+    fas = False
+
+
+class GeometricFAS(GeometricCS):
+    """<geometric-fas> (geomg.lisp:174-194): Brandt's FAS, the same cycle as GeometricCS
+    except that every coarse level approximates the solution: it starts from R x (the
+    projection-matrix, injection of the nodal functionals) with rhs I^T r + A_c R x and
+    prolongates x_c - R x (multigrid.lisp:226-247).  Lattice hierarchies only."""
+
+    fas = True
+
+
+def geometric_cs(**kw):
+    """femlisp's (geometric-cs ...) constructor (geomg.lisp:152-155)."""
+    return GeometricCS(**kw)
+
+
+def fas(**kw):
+    """femlisp's (fas ...) constructor (geomg.lisp:180-182)."""
+    return GeometricFAS(**kw)
 
 
 class CG(LinearIteration):
@@ -148,6 +170,8 @@ class Hierarchy:
         return m
 
     def level_matrix(self, level, interp=False):
+        """the level's matrix (interp False / 0), its interpolation to the next level
+        (True / 1) or, in a FAS hierarchy, its projection from the next level (2)"""
         h = ctypes.c_void_p()
         check(lib().flgpu_mg_level_matrix(self.h, level, int(interp), ctypes.byref(h)))
         m = GpuSparseMatrix(h, self.ctx, owner=False)

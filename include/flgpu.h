@@ -192,6 +192,13 @@ int flgpu_lattice_reassemble(flgpu_mat *A, const flgpu_vec *kappa, int nk,
  * Dirichlet columns zeroed but kept. */
 int flgpu_lattice_interpolation(flgpu_ctx *ctx, int dim, int order, int nc,
                                 int prec, flgpu_mat **P);
+/* FAS projection from the lattice with 2 nc cells per axis to the one with nc
+ * cells (femlisp's projection-matrix, sparseas.lisp:278-297, whose local matrix
+ * for Lagrange elements is the injection of the dof functionals,
+ * fetransfer.lisp:201-213): rows coarse DOFs, one entry 1 each, at the fine DOF
+ * of the same point. */
+int flgpu_lattice_projection(flgpu_ctx *ctx, int dim, int order, int nc, int prec,
+                             flgpu_mat **R);
 /* Operator-dependent ("black-box") interpolation for 3D Q1, no femlisp
  * counterpart: it takes the place of flgpu_lattice_interpolation's values
  * (femlisp's constrained :local interpolation, fetransfer.lisp:73-138) when
@@ -258,6 +265,11 @@ typedef struct flgpu_mg_params {
   int additive;    /* 0 :multiplicative, 1 :additive combination (multigrid.lisp:
                       67-70,262-287): restrict first, then smooth the level; post-smooth,
                       then prolongate */
+  int fas;         /* 0 correction scheme (<geometric-cs>), 1 Brandt's FAS
+                      (<geometric-fas>, geomg.lisp:174-194): the coarse level starts
+                      from R x with rhs I^T r + A_c R x and prolongates x_c - R x
+                      (multigrid.lisp:226-247); R = flgpu_lattice_projection, so lattice
+                      hierarchies only */
 } flgpu_mg_params;
 
 #define FLGPU_TRANSFER_GEOMETRIC 0
@@ -283,8 +295,9 @@ int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n0,
 int flgpu_mg_destroy(flgpu_mg *mg);
 /* the top-level matrix of the hierarchy (owned by mg) */
 int flgpu_mg_top_matrix(flgpu_mg *mg, flgpu_mat **A);
-/* the matrix (I = 0) or the interpolation to the next level (I = 1) of one
- * level, owned by mg (the :a-vec / :i-vec entries, multigrid.lisp:108-109) */
+/* the matrix (I = 0), the interpolation to the next level (I = 1) or, with FAS,
+ * the projection from the next level (I = 2) of one level, owned by mg (the
+ * :a-vec / :i-vec / :fas-r-vec entries, multigrid.lisp:108-109,121) */
 int flgpu_mg_level_matrix(flgpu_mg *mg, int level, int interp, flgpu_mat **A);
 /* The <mg-iteration> iterator (multigrid.lisp:368-391): one V-cycle on
  * (x, b, r) with r = b - A x up to date; x is updated, r and b are used as

@@ -4,8 +4,9 @@
 ;;; (src/basic/port.lisp:442-455) in the manner of femlisp's SuperLU binding
 ;;; (src/alien/superlu.lisp:57-75), defines <gpu-sparse-matrix> and <gpu-vector>
 ;;; with the matlisp generics the solver path calls, make-iterator methods for
-;;; <jacobi>, <sor> and <geometric-cs> on GPU matrices, and EXPORT-SYSTEM, which
-;;; writes a femlisp system in the layout flgpu_system_read reads.
+;;; <jacobi> and <sor> on GPU matrices, the flgpu_mg_create binding (the hierarchy
+;;; and its parameters are built by the caller), and EXPORT-SYSTEM, which writes a
+;;; femlisp system in the layout flgpu_system_read reads.
 ;;;
 ;;; No Lisp implementation is installed where this library is built, so this file
 ;;; has not been executed; the same C-ABI is exercised by the Python host side and

@@ -293,12 +293,94 @@ def test_jacobi_multi_step_passes_match_single_steps(prec, n):
 
 def _gpu_hier(dim, order, n0, levels, smoother, omega, pre, post, base, symmetric=False,
               kappa=None, prec=F.F64, mode=F.ASM_F64, gamma=1, transfer="geometric",
-              additive=False):
+              additive=False, fas=False):
     sm = it.Jacobi(omega) if smoother == po.SMOOTH_JACOBI else it.SOR(omega)
-    cs = it.GeometricCS(sm, pre, post, base, gamma=gamma, symmetric=symmetric, prec=prec,
-                        transfer=transfer,
-                        combination="additive" if additive else "multiplicative")
+    cls = it.GeometricFAS if fas else it.GeometricCS
+    cs = cls(sm, pre, post, base, gamma=gamma, symmetric=symmetric, prec=prec,
+             transfer=transfer, combination="additive" if additive else "multiplicative")
     return cs, it.Hierarchy.lattice(cs, dim, order, n0, levels, kappa=kappa, mode=mode)
+
+
+@pytest.mark.parametrize("prec", [F.F64, F.F32])
+def test_lattice_projection_matches_oracle(prec):
+    """flgpu_lattice_projection (femlisp's projection-matrix: injection of the nodal
+    functionals) equals the oracle's matrix bit for bit, and R I = Id on the GPU."""
+    say(f"lattice projection prec={prec}")
+    from gnu_femlisp_amd.flgpu import check, lib
+    import ctypes
+    for dim, order, nc in ((1, 1, 4), (2, 1, 3), (2, 2, 2), (3, 1, 4), (3, 2, 2)):
+        h = ctypes.c_void_p()
+        check(lib().flgpu_lattice_projection(default_context().h, dim, order, nc, prec,
+                                             ctypes.byref(h)))
+        R = GpuSparseMatrix(h, default_context())
+        rp, ci, v = R.csr()
+        orp, oci, ov = po.projection(dim, order, nc)
+        assert np.array_equal(rp, orp) and np.array_equal(ci, oci) and np.array_equal(v, ov)
+        P = lat.interpolation(dim, order, nc, prec=prec)
+        xc = np.random.default_rng(nc).uniform(-1, 1, P.ncols)
+        xc[po.boundary(dim, order, nc)] = 0  # Dirichlet columns of P are zero
+        y = gemm(1.0, P, GpuVector.from_array(xc, prec), 0.0, GpuVector(P.nrows, prec))
+        back = gemm(1.0, R, y, 0.0, GpuVector(P.ncols, prec)).numpy()
+        assert np.array_equal(back, xc.astype(np.float32) if prec == F.F32 else xc)
+
+
+@pytest.mark.parametrize("smoother", [po.SMOOTH_JACOBI, po.SMOOTH_SOR])
+@pytest.mark.parametrize("gamma,additive", [(1, 0), (2, 0), (1, 1)])
+@pytest.mark.parametrize("dim,order,n0,levels", [(3, 1, 1, 4), (2, 2, 1, 3)])
+def test_fas_cycle_parity(smoother, gamma, additive, dim, order, n0, levels):
+    """<geometric-fas> on the GPU: one cycle from zero against the oracle's FAS cycle; for the
+    multiplicative combination also against the GPU's own correction scheme, which FAS equals
+    up to rounding for a linear operator; the hierarchy's projections are the oracle's."""
+    say(f"fas cycle smoother={smoother} gamma={gamma} additive={additive} dim={dim} "
+        f"order={order}")
+    omega = 0.6 if smoother == po.SMOOTH_JACOBI else 1.0
+    base = 1 if levels > 3 else 0
+    kw = dict(gamma=gamma, additive=bool(additive))
+    cs, H = _gpu_hier(dim, order, n0, levels, smoother, omega, 2, 2, base, fas=True, **kw)
+    oh = po.Hierarchy(dim, order, n0, levels)
+    for l in range(levels - 1):
+        rp, ci, v = H.level_matrix(l, 2).csr()
+        orp, oci, ov = oh.projection(l)
+        assert np.array_equal(ci, oci) and np.array_equal(v, ov)
+    prm = po.params(smoother=smoother, omega=omega, ordering=po.ORDER_COLOUR, pre_steps=2,
+                    post_steps=2, base_level=base, gamma=gamma, additive=additive, fas=1)
+    rng = np.random.default_rng(21)
+    ntop = n0 << (levels - 1)
+    r = rng.uniform(-1, 1, oh.ndofs())
+    r[po.boundary(dim, order, ntop)] = 0
+    ref = oh.apply(prm, r)
+    q = GpuVector(len(r))
+    H.apply(q, GpuVector.from_array(r), GpuVector.from_array(r))
+    assert relerr(q.numpy(), ref) < 1e-12
+    if not additive:
+        _, Hc = _gpu_hier(dim, order, n0, levels, smoother, omega, 2, 2, base, **kw)
+        qc = GpuVector(len(r))
+        Hc.apply(qc, GpuVector.from_array(r), GpuVector.from_array(r))
+        assert relerr(q.numpy(), qc.numpy()) < 1e-12
+
+
+def test_fas_preconditioned_cg_parity():
+    """FAS as <cg>'s preconditioner: the oracle's iteration count, defects and solution."""
+    say("fas cg")
+    dim, order, n0, levels = 3, 1, 1, 4
+    rng = np.random.default_rng(5)
+    kap = np.exp(rng.normal(scale=1.0, size=(8, 8, 8)))
+    cs, H = _gpu_hier(dim, order, n0, levels, po.SMOOTH_SOR, 1.0, 2, 2, 0, symmetric=True,
+                      kappa=kap, fas=True)
+    oh = po.Hierarchy(dim, order, n0, levels, kappa=kap)
+    b = rng.uniform(-1, 1, oh.ndofs())
+    b[po.boundary(dim, order, 8)] = 0
+    prm = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_SOR, omega=1.0,
+                    ordering=po.ORDER_COLOUR, pre_steps=2, post_steps=2, base_level=0,
+                    symmetric=1, tol=1e-10, maxsteps=200, fas=1)
+    st, xo, so, dno = oh.solve(prm, b)
+    assert st == 0
+    bb = it.solve(it.LinearSolver(it.CG(cs), success_if=("<", "reduction", 1e-10),
+                                  failure_if=(">", "step", 200)),
+                  {"matrix": H.top_matrix(), "rhs": GpuVector.from_array(b), "hierarchy": H})
+    assert bb["status"] == "success" and bb["steps"] == so
+    assert relerr(bb["solution"].numpy(), xo) < 1e-10
+    assert np.allclose(bb["defnorms"], dno, rtol=1e-8)
 
 
 @pytest.mark.parametrize("alias", [0, 1])
