@@ -884,6 +884,146 @@ static long *make_ordering(int dim, int order, const level_t *L, int mode,
   return ord;
 }
 
+/* setup-blocks of <geometric-blocking-mixin> with :vertex-centered blocks (geoblock.lisp:
+ * 101-157) and find-vertex-centered-block (:57-86).  One block per mesh vertex v (a lattice
+ * point whose coordinates are multiples of order): v itself and every DOF of a cell containing
+ * v that is not Dirichlet -- the lattice points v + d with |d_k| <= order - 1 (a point with
+ * |d_k| = order lies on a cell without v).  Uniform meshes have no hanging nodes, so the
+ * constraints-Q loop adds nothing.  keys[b * W + s], W = (2 order - 1)^dim, slot s = d in
+ * lexicographic order (axis 0 most significant), -1 where v + d lies outside the lattice or is
+ * Dirichlet (v itself is kept either way, as the reference keeps the vertex key).  Blocks come
+ * sorted by colour -- the parity of v / order per axis, axis 0 most significant -- then by
+ * index: blocks of one colour neither share a key nor read one another's keys through A, so any
+ * order within a colour gives the same iterate.  Returns the number of blocks (n + 1)^dim;
+ * keys may be NULL. */
+long flo_vertex_blocks(int dim, int order, int n, int *keys) {
+  if (dim < 1 || dim > FLO_MAXDIM || order < 1 || n < 1) return -1;
+  int M = order * n + 1, h = order - 1, w1 = 2 * order - 1;
+  int W = (int)ipow(w1, dim);
+  long nv = ipow(n + 1, dim);
+  if (!keys) return nv;
+  unsigned char *bd = (unsigned char *)malloc((size_t)ipow(M, dim));
+  flo_lattice_boundary(dim, order, n, bd);
+  long pos = 0;
+  for (int col = 0; col < (1 << dim); col++)
+    for (long iv = 0; iv < nv; iv++) {
+      int a[FLO_MAXDIM];
+      unrank(iv, dim, n + 1, a);
+      int cc = 0;
+      for (int k = 0; k < dim; k++) cc = 2 * cc + (a[k] & 1);
+      if (cc != col) continue;
+      int *slot = keys + pos * W;
+      for (int s = 0; s < W; s++) {
+        int d[FLO_MAXDIM], c[FLO_MAXDIM], inside = 1, centre = 1;
+        unrank(s, dim, w1, d);
+        for (int k = 0; k < dim; k++) {
+          c[k] = a[k] * order + d[k] - h;
+          if (c[k] < 0 || c[k] >= M) inside = 0;
+          if (d[k] != h) centre = 0;
+        }
+        long key = inside ? rank(c, dim, M) : -1;
+        if (key >= 0 && !centre && bd[key]) key = -1;
+        slot[s] = (int)key;
+      }
+      pos++;
+    }
+  free(bd);
+  return nv;
+}
+
+/* A_ij of a CSR matrix with ascending columns (0 outside the pattern) */
+static double csr_entry(const csr_t *A, int i, int j) {
+  int lo = A->rowptr[i], hi = A->rowptr[i + 1] - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) / 2;
+    if (A->colidx[mid] == j) return A->vals[mid];
+    if (A->colidx[mid] < j) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return 0.0;
+}
+
+/* gesv! (matrix.lisp:428-436, LAPACK dgesv): LU with partial pivoting of the row-major m x m
+ * matrix a (destroyed), b overwritten by the solution */
+static void dense_gesv(int m, double *a, double *b) {
+  for (int k = 0; k < m; k++) {
+    int p = k;
+    for (int i = k + 1; i < m; i++)
+      if (fabs(a[(size_t)i * m + k]) > fabs(a[(size_t)p * m + k])) p = i;
+    if (p != k) {
+      for (int j = 0; j < m; j++) {
+        double t = a[(size_t)k * m + j];
+        a[(size_t)k * m + j] = a[(size_t)p * m + j];
+        a[(size_t)p * m + j] = t;
+      }
+      double t = b[k];
+      b[k] = b[p];
+      b[p] = t;
+    }
+    for (int i = k + 1; i < m; i++) {
+      double l = a[(size_t)i * m + k] / a[(size_t)k * m + k];
+      for (int j = k + 1; j < m; j++) a[(size_t)i * m + j] -= l * a[(size_t)k * m + j];
+      b[i] -= l * b[k];
+    }
+  }
+  for (int i = m - 1; i >= 0; i--) {
+    double s = b[i];
+    for (int j = i + 1; j < m; j++) s -= a[(size_t)i * m + j] * b[j];
+    b[i] = s / a[(size_t)i * m + i];
+  }
+}
+
+/* One <ssc> or <psc> iterate on a <sparse-matrix> (make-iterator of <block-iteration>,
+ * blockit.lisp:113-203; store-p nil, no inner iteration), blocks as flo_vertex_blocks gives
+ * them, first to last (backwards: last to first).  Per block B:
+ *  <ssc> first updates the residual on B, r_k := b_k, r_k -= sum_j A_kj x_j (:156-161);
+ *  both then solve A_BB y = r_B by gesv! on a copy (:179-185) and set x_B += omega y, where
+ *  omega is <ssc>'s omega and <psc>'s damp (:139-142).
+ * <psc> reads r as the caller left it (residual-before t) and never writes it. */
+static void block_iterate(const csr_t *A, const int *keys, long nb, int W, int ssc,
+                          double omega, int backwards, double *x, const double *b, double *r) {
+  double *loc = (double *)malloc(sizeof(double) * W * W);
+  double *y = (double *)malloc(sizeof(double) * W);
+  int *ks = (int *)malloc(sizeof(int) * W);
+  for (long q = 0; q < nb; q++) {
+    const int *blk = keys + (backwards ? nb - 1 - q : q) * W;
+    int m = 0;
+    for (int s = 0; s < W; s++)
+      if (blk[s] >= 0) ks[m++] = blk[s];
+    if (ssc)
+      for (int i = 0; i < m; i++) {
+        int k = ks[i];
+        double rk = b[k];
+        for (int e = A->rowptr[k]; e < A->rowptr[k + 1]; e++)
+          rk += -1.0 * (A->vals[e] * x[A->colidx[e]]);
+        r[k] = rk;
+      }
+    for (int i = 0; i < m; i++) {
+      y[i] = r[ks[i]];
+      for (int j = 0; j < m; j++) loc[i * m + j] = csr_entry(A, ks[i], ks[j]);
+    }
+    dense_gesv(m, loc, y);
+    for (int i = 0; i < m; i++) x[ks[i]] += omega * y[i];
+  }
+  free(loc);
+  free(y);
+  free(ks);
+}
+
+int flo_block_iterate(int nrows, const int *rowptr, const int *colidx, const double *vals,
+                      int dim, int order, int n, int ssc, double omega, int backwards,
+                      double *x, const double *b, double *r) {
+  long nb = flo_vertex_blocks(dim, order, n, NULL);
+  if (nb < 0 || flo_lattice_ndofs(dim, order, n) != nrows) return -1;
+  int W = (int)ipow(2 * order - 1, dim);
+  int *keys = (int *)malloc(sizeof(int) * nb * W);
+  flo_vertex_blocks(dim, order, n, keys);
+  csr_t A = {nrows, nrows, (int *)rowptr, (int *)colidx, (double *)vals};
+  block_iterate(&A, keys, nb, W, ssc, omega, backwards, x, b, r);
+  free(keys);
+  return 0;
+}
+
 /* dense LU without pivoting of the base-level matrix: femlisp factors the
  * coarse matrix once (<lu> store-p, linit.lisp:132-162; sparse-ldu,
  * src/matlisp/sparselu.lisp:61-126) and applies getrs!. */
@@ -923,6 +1063,9 @@ typedef struct {
   flo_mg *mg;
   const flo_params *p;
   long **pre_ord, **post_ord;
+  int **blk;   /* vertex-centred blocks per level (block smoothers) */
+  long *nblk;
+  int blk_w;
   int top;
 } mgrun_t;
 
@@ -943,11 +1086,22 @@ static void smoother_run(mgrun_t *R, int l, int post) {
       jacobi_iterate(L, p->omega, L->sol, L->res);
       if (steps > 1) compute_residual(&L->A, L->sol, L->rhs, L->res);
     }
+  } else if (p->smoother == FLO_SMOOTH_SSC || p->smoother == FLO_SMOOTH_PSC) {
+    /* <block-iteration> iterators have residual-before t (blockit.lisp:125) */
+    if (!L->residual_p) {
+      compute_residual(&L->A, L->sol, L->rhs, L->res);
+      L->residual_p = 1;
+    }
+    for (int k = 0; k < steps; k++) {
+      block_iterate(&L->A, R->blk[l], R->nblk[l], R->blk_w, p->smoother == FLO_SMOOTH_SSC,
+                    p->omega, post && p->symmetric, L->sol, L->rhs, L->res);
+      if (steps > 1) compute_residual(&L->A, L->sol, L->rhs, L->res);
+    }
   } else {
     long *ord = post ? R->post_ord[l] : R->pre_ord[l];
     for (int k = 0; k < steps; k++) sor_iterate(L, p->omega, ord, L->sol, L->rhs, L->res);
   }
-  /* residual-after nil for both smoothers */
+  /* residual-after nil for every smoother */
   L->residual_p = 0;
 }
 
@@ -1031,6 +1185,16 @@ static void mgrun_init(mgrun_t *R, flo_mg *mg, const flo_params *p) {
                            ? make_ordering(mg->dim, mg->order, &mg->lv[l], p->ordering, 1)
                            : R->pre_ord[l];
     }
+  R->blk = (int **)calloc(mg->levels, sizeof(int *));
+  R->nblk = (long *)calloc(mg->levels, sizeof(long));
+  R->blk_w = (int)ipow(2 * mg->order - 1, mg->dim);
+  if (p->smoother == FLO_SMOOTH_SSC || p->smoother == FLO_SMOOTH_PSC)
+    for (int l = p->base_level + 1; l < mg->levels; l++) {
+      int n = mg->lv[l].n;
+      R->nblk[l] = flo_vertex_blocks(mg->dim, mg->order, n, NULL);
+      R->blk[l] = (int *)malloc(sizeof(int) * R->nblk[l] * R->blk_w);
+      flo_vertex_blocks(mg->dim, mg->order, n, R->blk[l]);
+    }
   level_t *B = &mg->lv[p->base_level];
   if (!B->lu) coarse_factor(B);
 }
@@ -1039,9 +1203,12 @@ static void mgrun_free(mgrun_t *R) {
   for (int l = 0; l < R->mg->levels; l++) {
     if (R->post_ord[l] != R->pre_ord[l]) free(R->post_ord[l]);
     free(R->pre_ord[l]);
+    free(R->blk[l]);
   }
   free(R->pre_ord);
   free(R->post_ord);
+  free(R->blk);
+  free(R->nblk);
 }
 
 /* the <mg-iteration> iterator (multigrid.lisp:368-391): top-level sol/rhs/res

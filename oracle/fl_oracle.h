@@ -102,6 +102,21 @@ void flo_csr_transpose(int nrows, int ncols, const int *rowptr,
                        const int *colidx, const double *vals, int *trowptr,
                        int *tcolidx, double *tvals);
 
+/* ---- block smoothers (src/iteration/blockit.lisp:41-203, geoblock.lisp:37-157) */
+
+/* vertex-centred blocks of the lattice with n cells per axis: (n+1)^dim blocks of
+ * W = (2 order - 1)^dim slots, keys[b*W + s] the DOF of vertex b's slot s (lattice offset
+ * d - (order - 1), d in lexicographic order) or -1 (outside, or Dirichlet other than the
+ * vertex); blocks in colour order (parity of the vertex coordinates).  keys may be NULL;
+ * returns the number of blocks, -1 on bad arguments. */
+long flo_vertex_blocks(int dim, int order, int n, int *keys);
+/* one <geometric-ssc> (ssc = 1, omega) or <geometric-psc> (ssc = 0, damp = omega) iterate on
+ * the lattice matrix (x, b, r as femlisp's iterator closure; <psc> needs r = b - A x),
+ * blocks in flo_vertex_blocks order or reversed (backwards = 1). */
+int flo_block_iterate(int nrows, const int *rowptr, const int *colidx, const double *vals,
+                      int dim, int order, int n, int ssc, double omega, int backwards,
+                      double *x, const double *b, double *r);
+
 /* ---- multigrid hierarchy + solvers (src/iteration/ multigrid, linit, krylow) */
 typedef struct flo_mg flo_mg;
 
@@ -133,14 +148,15 @@ int flo_mg_get_projection(const flo_mg *mg, int level, int *rowptr, int *colidx,
 
 enum { FLO_OUTER_MG = 0, FLO_OUTER_CG = 1, FLO_OUTER_CG_PLAIN = 2,
        FLO_OUTER_FCG = 3 };
-enum { FLO_SMOOTH_JACOBI = 0, FLO_SMOOTH_SOR = 1 };
+enum { FLO_SMOOTH_JACOBI = 0, FLO_SMOOTH_SOR = 1, FLO_SMOOTH_SSC = 2, FLO_SMOOTH_PSC = 3 };
 enum { FLO_ORDER_INDEX = 0, FLO_ORDER_COLOUR = 1 };
 enum { FLO_STOP_REDUCTION = 0, FLO_STOP_DEFNORM = 1 };
 
 typedef struct flo_params {
   int outer;        /* FLO_OUTER_* */
   int smoother;     /* FLO_SMOOTH_* */
-  double omega;     /* <jacobi> damp / <sor> omega */
+  double omega;     /* <jacobi> damp / <sor> omega / <geometric-ssc> omega /
+                       <geometric-psc> damp */
   int ordering;     /* FLO_ORDER_* for <sor> */
   int pre_steps, post_steps;
   int base_level;

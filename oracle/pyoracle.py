@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 OUTER_MG, OUTER_CG, OUTER_CG_PLAIN, OUTER_FCG = 0, 1, 2, 3
-SMOOTH_JACOBI, SMOOTH_SOR = 0, 1
+SMOOTH_JACOBI, SMOOTH_SOR, SMOOTH_SSC, SMOOTH_PSC = 0, 1, 2, 3
 ORDER_INDEX, ORDER_COLOUR = 0, 1
 STOP_REDUCTION, STOP_DEFNORM = 0, 1
 
@@ -71,6 +71,9 @@ def lib():
         L.flo_mg_get_projection.argtypes = [vp, ci, vp, vp, vp]
         L.flo_blackbox_interpolation.restype = ci
         L.flo_blackbox_interpolation.argtypes = [ci, vp, vp, vp, vp, vp, vp]
+        L.flo_vertex_blocks.restype = cl
+        L.flo_vertex_blocks.argtypes = [ci, ci, ci, vp]
+        L.flo_block_iterate.argtypes = [ci, vp, vp, vp, ci, ci, ci, ci, cd, ci, vp, vp, vp]
         L.flo_spmv.argtypes = [ci, vp, vp, vp, cd, vp, cd, vp]
         L.flo_spmv_t.argtypes = [ci, ci, vp, vp, vp, cd, vp, cd, vp]
         L.flo_csr_transpose.argtypes = [ci, ci, vp, vp, vp, vp, vp, vp]
@@ -191,6 +194,28 @@ def blackbox_interpolation(nf, rp, ci, vals):
                                         _p(pv)) != 0:
         raise ValueError("nf must be even")
     return prp, pci, pv
+
+
+def vertex_blocks(dim, order, n):
+    """Vertex-centred blocks (flo_vertex_blocks): int32 array (blocks, (2 order - 1)^dim),
+    -1 for absent slots, blocks in colour order."""
+    nb = lib().flo_vertex_blocks(dim, order, n, None)
+    keys = np.zeros((nb, (2 * order - 1) ** dim), np.int32)
+    lib().flo_vertex_blocks(dim, order, n, _p(keys))
+    return keys
+
+
+def block_iterate(rp, ci, vals, dim, order, n, ssc, omega, x, b, r, backwards=False):
+    """One <geometric-ssc> (ssc=True) or <geometric-psc> iterate; returns new (x, r)."""
+    x = np.array(x, np.float64)
+    r = np.array(r, np.float64)
+    b = r if b is None else np.ascontiguousarray(b, np.float64)
+    rp, ci = np.ascontiguousarray(rp, np.int32), np.ascontiguousarray(ci, np.int32)
+    vals = np.ascontiguousarray(vals, np.float64)
+    rc = lib().flo_block_iterate(len(rp) - 1, _p(rp), _p(ci), _p(vals), dim, order, n,
+                                 int(ssc), omega, int(backwards), _p(x), _p(b), _p(r))
+    assert rc == 0, rc
+    return x, r
 
 
 def spmv(rp, ci, vals, y, alpha=1.0, beta=0.0, x=None):

@@ -509,3 +509,137 @@ def test_solution_matches_direct_solve():
     u = sla.spsolve(csr(rp, ci, v).tocsc(), h.rhs())
     assert st == 0
     assert np.linalg.norm(x - u) / np.linalg.norm(u) < 1e-10
+
+
+# ---- vertex-centred block smoothers (blockit.lisp:113-203, geoblock.lisp:57-157) ----------
+
+def _geometric_blocks(dim, order, n):
+    """Blocks from the definition: vertex v and every non-Dirichlet DOF whose carrying cell
+    has v in its closure.  A DOF at lattice point m is carried by the cell that extends along
+    the axes where m is not on a vertex plane; that cell contains v when m_k = v_k on those
+    planes and |m_k - v_k| < order elsewhere.  Returned as {vertex: set of keys} and colours."""
+    M = order * n + 1
+    bd = po.boundary(dim, order, n)
+    pts = np.array(np.unravel_index(np.arange(M ** dim), (M,) * dim)).T
+    blocks, colour = {}, {}
+    for a in np.ndindex(*(n + 1,) * dim):
+        v = np.array(a) * order
+        key = int(np.ravel_multi_index(tuple(v), (M,) * dim))
+        on_plane = pts % order == 0
+        inside = np.where(on_plane, pts == v, np.abs(pts - v) < order).all(axis=1)
+        ks = set(np.flatnonzero(inside & ~bd).tolist()) | {key}
+        blocks[key] = ks
+        colour[key] = int(sum((a[k] & 1) << (dim - 1 - k) for k in range(dim)))
+    return blocks, colour
+
+
+@pytest.mark.parametrize("dim,order,n", [(2, 1, 4), (2, 2, 3), (3, 1, 3), (3, 2, 2)])
+def test_vertex_blocks_match_the_geometric_definition(dim, order, n):
+    """flo_vertex_blocks against the blocks built from cell closures, the colour order, and
+    the independence that makes any order within a colour give the same iterate."""
+    keys = po.vertex_blocks(dim, order, n)
+    geo, colour = _geometric_blocks(dim, order, n)
+    assert keys.shape == ((n + 1) ** dim, (2 * order - 1) ** dim)
+    W = keys.shape[1]
+    centres = keys[:, W // 2]
+    assert sorted(centres.tolist()) == sorted(geo)
+    for row, c in zip(keys, centres):
+        assert set(row[row >= 0].tolist()) == geo[int(c)]
+    cols = [colour[int(c)] for c in centres]
+    assert cols == sorted(cols)
+    rp, ci, _, _ = po.assemble(dim, order, n)
+    A = csr(rp, ci, np.ones(len(ci))).toarray() != 0
+    for col in set(cols):
+        mine = [keys[i][keys[i] >= 0] for i in range(len(cols)) if cols[i] == col]
+        for i in range(len(mine)):
+            for j in range(i + 1, len(mine)):
+                assert not A[np.ix_(mine[i], mine[j])].any()
+
+
+def _brute_ssc(A, blocks, colour, x, b, omega, rng):
+    """<ssc> iterate from its definition, blocks colour by colour in random order within a
+    colour, with numpy's dense solve; with omega = 1 the block's residual must vanish."""
+    x = x.copy()
+    for col in sorted(set(colour.values())):
+        vs = [v for v in blocks if colour[v] == col]
+        for v in rng.permutation(vs):
+            ks = np.array(sorted(blocks[v]))
+            r = b[ks] - A[ks] @ x
+            x[ks] += omega * np.linalg.solve(A[np.ix_(ks, ks)], r)
+            if omega == 1.0:
+                assert np.abs(b[ks] - A[ks] @ x).max() < 1e-12 * np.abs(b).max()
+    return x
+
+
+@pytest.mark.parametrize("dim,n", [(2, 4), (3, 2), (3, 3)])
+@pytest.mark.parametrize("omega", [1.0, 0.8])
+def test_ssc_matches_the_definition(dim, n, omega):
+    """One Q2 <geometric-ssc> iterate against the brute-force definition.  x is random at
+    the Dirichlet nodes too, so a block that wrongly holds a Dirichlet key other than its
+    vertex changes the result."""
+    rng = np.random.default_rng(dim * 10 + n)
+    rp, ci, v, _ = po.assemble(dim, 2, n, kappa=np.exp(rng.normal(size=n ** dim)))
+    A = csr(rp, ci, v).toarray()
+    N = A.shape[0]
+    x0, b = rng.normal(size=N), rng.normal(size=N)
+    x, _ = po.block_iterate(rp, ci, v, dim, 2, n, True, omega, x0, b, np.zeros(N))
+    geo, colour = _geometric_blocks(dim, 2, n)
+    ref = _brute_ssc(A, geo, colour, x0, b, omega, rng)
+    assert np.abs(x - ref).max() < 1e-12 * np.abs(ref).max()
+    # backwards: the colours last to first
+    xb, _ = po.block_iterate(rp, ci, v, dim, 2, n, True, omega, x0, b, np.zeros(N),
+                             backwards=True)
+    rev = {k: (1 << dim) - 1 - c for k, c in colour.items()}
+    refb = _brute_ssc(A, geo, rev, x0, b, omega, rng)
+    assert np.abs(xb - refb).max() < 1e-12 * np.abs(refb).max()
+
+
+@pytest.mark.parametrize("dim,n", [(2, 4), (3, 2)])
+def test_psc_matches_the_definition(dim, n):
+    """<geometric-psc>: x += damp sum_B R_B^T A_BB^-1 r_B with the residual as given."""
+    rng = np.random.default_rng(5)
+    rp, ci, v, _ = po.assemble(dim, 2, n, kappa=np.exp(rng.normal(size=n ** dim)))
+    A = csr(rp, ci, v).toarray()
+    N = A.shape[0]
+    x0, b = rng.normal(size=N), rng.normal(size=N)
+    r0 = b - A @ x0
+    x, r = po.block_iterate(rp, ci, v, dim, 2, n, False, 0.7, x0, b, r0)
+    assert np.array_equal(r, r0)
+    ref = x0.copy()
+    for ks in _geometric_blocks(dim, 2, n)[0].values():
+        ks = np.array(sorted(ks))
+        ref[ks] += 0.7 * np.linalg.solve(A[np.ix_(ks, ks)], r0[ks])
+    assert np.abs(x - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_q1_block_smoothers_are_point_smoothers(dim):
+    """For Q1 every vertex block is one DOF, so <geometric-ssc> is <sor> in colour order and
+    <geometric-psc> is <jacobi> with the same damping: whole V-cycles agree exactly."""
+    rng = np.random.default_rng(dim)
+    n0, lv = (2, 4) if dim == 2 else (1, 3)
+    nt = n0 << (lv - 1)
+    h = po.Hierarchy(dim, 1, n0, lv, kappa=np.exp(rng.normal(size=nt ** dim)))
+    b = rng.normal(size=h.ndofs())
+    for blk, point in ((po.SMOOTH_SSC, po.SMOOTH_SOR), (po.SMOOTH_PSC, po.SMOOTH_JACOBI)):
+        for steps in (1, 2):
+            kw = dict(omega=0.9, ordering=po.ORDER_COLOUR, pre_steps=steps, post_steps=steps,
+                      symmetric=1)
+            x1 = h.apply(po.params(smoother=blk, **kw), b)
+            x2 = h.apply(po.params(smoother=point, **kw), b)
+            assert np.array_equal(x1, x2)
+
+
+@pytest.mark.parametrize("dim,n0,levels", [(2, 1, 5), (3, 1, 3)])
+def test_ssc_vcycle_contracts_for_q2(dim, n0, levels):
+    """femlisp picks <geometric-ssc> for order > 1 (geomg.lisp:58-71); its own test asserts a
+    V-cycle rate below 0.06 for 2D order 4 (mg-cdr.lisp:102-103), which this Q1/Q2 oracle
+    cannot run.  Q2 with V(1,1) must contract at least that fast here; the measured rates
+    are 0.007 (2D, 16 cells per axis) and 0.010 (3D, 4 cells per axis)."""
+    h = po.Hierarchy(dim, 2, n0, levels)
+    p = po.params(outer=po.OUTER_MG, smoother=po.SMOOTH_SSC, omega=1.0, pre_steps=1,
+                  post_steps=1, stop=po.STOP_REDUCTION, tol=1e-10, maxsteps=30)
+    st, x, steps, dn = h.solve(p)
+    assert st == 0
+    rate = (dn[-1] / dn[0]) ** (1.0 / steps)
+    assert rate < 0.06, rate
