@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libflgpu.so")
 SOURCES = ["flgpu_core.hip", "flgpu_sparse.hip", "flgpu_lattice.hip", "flgpu_mg.hip",
-           "flgpu_unet.hip", "flgpu_io.hip"]
+           "flgpu_unet.hip", "flgpu_io.hip", "flgpu_block.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          # packed FP32 instructions are kept out of all device code

@@ -175,6 +175,10 @@ struct flgpu_mat {
   // interpolation (tp_kind 1) the entries of the fine nodes of c's block by parity class
   // (k_tp8_fill); 0 where there is no such entry
   void *tp27 = nullptr;
+  // block smoothers on a lattice matrix: inverses of the vertex-centred blocks, per vertex in
+  // lattice vertex order, blk_inv[v][j][i] = (A_BB^-1)_ij (prec); nullptr until needed and for
+  // order 1, whose blocks are single DOFs
+  void *blk_inv = nullptr;
 };
 
 namespace flgpu {
@@ -219,6 +223,13 @@ bool restrict_first_step(const flgpu_mat *PT, const void *r, void *rhs, const fl
 // zero, in one pass
 void stage_first_step(const flgpu_mat *A, int rprec, const void *r, void *b, double damp,
                       void *xo);
+// vertex-centred block smoothers (flgpu_block.hip): (re)compute the block inverses of a
+// lattice matrix, release them, and one <geometric-ssc> (ssc) / <geometric-psc> iterate
+// (backwards: colours last to first; b may be r)
+void blocks_prepare(flgpu_mat *A);
+void blocks_drop(flgpu_mat *A);
+void block_sweep(const flgpu_mat *A, bool ssc, double omega, int backwards, void *x,
+                 const void *b, void *r);
 // `steps` sweeps
 void sor_sweeps(const flgpu_mat *A, double omega, int steps, int backwards, void *x,
                 const void *b, void *r);

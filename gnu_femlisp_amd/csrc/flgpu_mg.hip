@@ -206,6 +206,18 @@ static void smooth(flgpu_mg *mg, int l, bool post, bool last = false) {
       jacobi_iterate(L.A, p.omega, x, r);
       if (steps > 1) spmv(L.A, -1.0, x, 1.0, b, r);  // residual-before and not residual-after
     }
+  } else if (p.smoother == FLGPU_SMOOTH_SSC || p.smoother == FLGPU_SMOOTH_PSC) {
+    // <block-iteration> has residual-before t and the product-iterator recomputes the residual
+    // after each of several steps (linit.lisp:116-119).  <ssc> overwrites the residual of every
+    // block DOF from b, so those computations are seen only when b is r (femlisp's aliased
+    // call); <psc> reads them.
+    bool ssc = p.smoother == FLGPU_SMOOTH_SSC;
+    bool seen = !ssc || b == r;
+    if (seen) ensure_residual(mg, l);
+    for (int k = 0; k < steps; k++) {
+      block_sweep(L.A, ssc, p.omega, post && p.symmetric, x, b, r);
+      if (steps > 1 && seen) spmv(L.A, -1.0, x, 1.0, b, r);
+    }
   } else {
     sor_sweeps(L.A, p.omega, steps, post && p.symmetric, x, b, r);
   }
@@ -368,7 +380,8 @@ static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work,
   bool jac = p.smoother == FLGPU_SMOOTH_JACOBI && T.tmp && p.pre_steps > 0 && !base_top;
   // :additive restricts before smoothing: the top residual is read and x must be stored
   bool add = p.additive != 0;
-  bool reads_r = base_top || add || !(jac || (p.smoother == FLGPU_SMOOTH_SOR && p.pre_steps > 0));
+  bool sweeps = p.smoother == FLGPU_SMOOTH_SOR || p.smoother == FLGPU_SMOOTH_SSC;
+  bool reads_r = base_top || add || !(jac || (sweeps && p.pre_steps > 0));
   bool zero = (jac && !add) || base_top;  // the first step writes x
   FLGPU_REQUIRE(!staged || mg_stages_first(mg, vprec), "this cycle does not take a staged rhs");
   if (vprec == p.prec) {
@@ -442,9 +455,14 @@ void mg_apply_cached(flgpu_mg *mg, int vprec, void *x, void *b, void *r) {
   mg_run_cached(mg, 0, vprec, x, b, r);
 }
 
+static bool block_smoother(const flgpu_mg_params &p) {
+  return p.smoother == FLGPU_SMOOTH_SSC || p.smoother == FLGPU_SMOOTH_PSC;
+}
+
 static void check_params(const flgpu_mg_params *p, int nlevels) {
   FLGPU_REQUIRE(p != nullptr, "params must not be NULL");
-  FLGPU_REQUIRE(p->smoother == FLGPU_SMOOTH_JACOBI || p->smoother == FLGPU_SMOOTH_SOR,
+  FLGPU_REQUIRE(p->smoother == FLGPU_SMOOTH_JACOBI || p->smoother == FLGPU_SMOOTH_SOR ||
+                    p->smoother == FLGPU_SMOOTH_SSC || p->smoother == FLGPU_SMOOTH_PSC,
                 "unknown smoother");
   FLGPU_REQUIRE(p->pre_steps >= 0 && p->post_steps >= 0, "negative smoothing steps");
   FLGPU_REQUIRE(p->base_level >= 0 && p->base_level < nlevels, "base_level out of range");
@@ -592,6 +610,8 @@ extern "C" int flgpu_mg_create(flgpu_ctx *ctx, int nlevels, flgpu_mat *const *A,
           int nc = 0;
           FLGPU_REQUIRE(flgpu_mat_colour_greedy(mg->lv[l].A, &nc) == 0, "greedy colouring failed");
         }
+        // block smoothers find their blocks from the lattice (blocks_prepare checks it)
+        if (block_smoother(*params) && l > params->base_level) blocks_prepare(mg->lv[l].A);
         if (l + 1 < nlevels) {
           mg->lv[l].I = mat_convert(I[l], params->prec);
           mat_build_transpose(mg->lv[l].I);
@@ -652,6 +672,7 @@ extern "C" int flgpu_mg_create_lattice(flgpu_ctx *ctx, int dim, int order, int n
         if (params->smoother == FLGPU_SMOOTH_SOR && l > params->base_level &&
             !mg->lv[l].A->alpha && !mg->lv[l].A->val27)
           lattice_colouring(mg->lv[l].A, dim, order, n);
+        if (block_smoother(*params) && l > params->base_level) blocks_prepare(mg->lv[l].A);
       }
       mg_alloc_work(mg);
     } catch (...) {

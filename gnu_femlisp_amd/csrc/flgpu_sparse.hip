@@ -217,6 +217,7 @@ void mat_free(flgpu_mat *A) {
   dev_free(A->alpha);
   dev_free(A->val27);
   dev_free(A->tp27);
+  dev_free(A->blk_inv);
   delete A;
 }
 
@@ -230,6 +231,7 @@ void mat_refresh_values(flgpu_mat *A) {
   diag_refresh(A);
   stencil_refresh(A);
   if (A->tp_kind) tp_prepare(A);
+  if (A->blk_inv) blocks_prepare(A);
   if (A->prec == FLGPU_F64) {
     sell_fill<double>(A->ctx, A, A->sell, 0);
     for (auto &s : A->colour_sell) sell_fill<double>(A->ctx, A, s, 0);

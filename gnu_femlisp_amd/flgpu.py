@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("FLGPU_LIB") or os.path.join(HERE, "libflgpu.so")
 
 F64, F32 = 0, 1
 ASM_F64, ASM_MFMA = 0, 1
-SMOOTH_JACOBI, SMOOTH_SOR = 0, 1
+SMOOTH_JACOBI, SMOOTH_SOR, SMOOTH_SSC, SMOOTH_PSC = 0, 1, 2, 3
 OUTER_MG, OUTER_CG, OUTER_CG_PLAIN, OUTER_FCG = 0, 1, 2, 3
 STOP_REDUCTION, STOP_DEFNORM = 0, 1
 STORAGE_SELL, STORAGE_STENCIL, STORAGE_CELL = 0, 1, 2
@@ -93,6 +93,7 @@ SIGNATURES = {
     "flgpu_jacobi_iterate": (_ci, [_vp, _cd, _vp, _vp]),
     "flgpu_jacobi_steps": (_ci, [_vp, _cd, _ci, _vp, _vp, _vp]),
     "flgpu_sor_iterate": (_ci, [_vp, _cd, _ci, _vp, _vp, _vp]),
+    "flgpu_block_iterate": (_ci, [_vp, _ci, _cd, _ci, _vp, _vp, _vp]),
     "flgpu_lattice_assemble": (_ci, [_vp, _ci, _ci, _ci, _vp, _ci, _cd, _ci, _ci, _P(_vp), _P(_vp)]),
     "flgpu_lattice_reassemble": (_ci, [_vp, _vp, _ci, _ci]),
     "flgpu_lattice_interpolation": (_ci, [_vp, _ci, _ci, _ci, _ci, _P(_vp)]),

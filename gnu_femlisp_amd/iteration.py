@@ -5,6 +5,11 @@ Classes and their femlisp counterparts:
   Jacobi(damp)                <jacobi>          src/iteration/linit.lisp:200-252
   SOR(omega), GaussSeidel()   <sor>/<gauss-seidel> with a colour :compare
                                                 src/iteration/linit.lisp:258-385
+  GeometricSSC(omega)         <geometric-ssc>   src/iteration/geoblock.lisp:47-49,57-59,
+                                                src/iteration/blockit.lisp:80-82,113-203
+  GeometricPSC(damp)          <geometric-psc>   src/iteration/geoblock.lisp:43-45,53-55,
+                                                src/iteration/blockit.lisp:76-78,113-203
+  select_smoother(order)      select-smoother   src/iteration/geomg.lisp:58-71
   GeometricCS(...)            <geometric-cs>    src/iteration/geomg.lisp:109-155
   GeometricFAS(...)           <geometric-fas>   src/iteration/geomg.lisp:174-194
   CG(preconditioner)          <cg>              src/iteration/krylow.lisp:67-132
@@ -49,6 +54,40 @@ class GaussSeidel(SOR):
         super().__init__(1.0)
 
 
+class GeometricSSC(LinearIteration):
+    """<geometric-ssc>: successive subspace correction over femlisp's vertex-centred blocks
+    (a mesh vertex and the non-Dirichlet DOFs of the cells around it), swept by colour; per
+    block the residual is updated and x_B += omega A_BB^-1 r_B.  Lattice matrices only."""
+
+    def __init__(self, omega=1.0):
+        self.omega = omega
+
+
+class GeometricPSC(LinearIteration):
+    """<geometric-psc>: the same blocks corrected additively from one residual,
+    x += damp sum_B A_BB^-1 r_B.  The blocks overlap, so damp must be small (about
+    2^-dim)."""
+
+    def __init__(self, damp=1.0):
+        self.damp = damp
+
+
+def geometric_ssc(**kw):
+    """femlisp's (geometric-ssc ...) constructor (geoblock.lisp:57-59)."""
+    return GeometricSSC(**kw)
+
+
+def geometric_psc(**kw):
+    """femlisp's (geometric-psc ...) constructor (geoblock.lisp:53-55)."""
+    return GeometricPSC(**kw)
+
+
+def select_smoother(order):
+    """select-smoother (geomg.lisp:58-71): <gauss-seidel> for order 1, <geometric-ssc>
+    for higher order."""
+    return GaussSeidel() if order == 1 else GeometricSSC()
+
+
 class LU(LinearIteration):
     """<lu>: only as the coarse-grid iteration (dense FP64 inverse on the GPU)."""
 
@@ -61,6 +100,9 @@ class GeometricCS(LinearIteration):
     levels: either a lattice description dict(dim=, order=, n0=, nlevels=,
     kappa=, mode=) assembled on the GPU, or explicit (A_list, I_list) of
     GpuSparseMatrix coarsest first (femlisp's :a-vec/:i-vec).
+
+    Without a smoother, lattice hierarchies take select_smoother(order) and explicit
+    matrices <gauss-seidel>.
     """
 
     def __init__(self, smoother=None, pre_steps=1, post_steps=1, base_level=0, gamma=1,
@@ -72,7 +114,7 @@ class GeometricCS(LinearIteration):
         if gamma < 0:
             raise ValueError("gamma must be >= 0")
         self.gamma = gamma
-        self.smoother = smoother or GaussSeidel()
+        self.smoother = smoother
         self.pre_steps, self.post_steps = pre_steps, post_steps
         self.base_level = base_level
         self.symmetric = symmetric
@@ -82,10 +124,18 @@ class GeometricCS(LinearIteration):
             raise ValueError("transfer must be 'geometric' or 'blackbox'")
         self.transfer = transfer
 
-    def params(self):
-        sm = self.smoother
-        kind = _f.SMOOTH_JACOBI if isinstance(sm, Jacobi) else _f.SMOOTH_SOR
-        omega = sm.damp if isinstance(sm, Jacobi) else sm.omega
+    def params(self, order=1):
+        sm = self.smoother or select_smoother(order)
+        if isinstance(sm, Jacobi):
+            kind, omega = _f.SMOOTH_JACOBI, sm.damp
+        elif isinstance(sm, SOR):
+            kind, omega = _f.SMOOTH_SOR, sm.omega
+        elif isinstance(sm, GeometricSSC):
+            kind, omega = _f.SMOOTH_SSC, sm.omega
+        elif isinstance(sm, GeometricPSC):
+            kind, omega = _f.SMOOTH_PSC, sm.damp
+        else:
+            raise TypeError(f"no GPU smoother for {type(sm).__name__}")
         return MgParams(kind, omega, self.pre_steps, self.post_steps, self.base_level,
                         int(self.symmetric), self.prec, int(self.galerkin),
                         _f.TRANSFER_BLACKBOX if self.transfer == "blackbox"
@@ -152,7 +202,7 @@ class Hierarchy:
         ctx = ctx or default_context()
         kv, nk = _kappa_vec(kappa, ctx)
         h, hb = ctypes.c_void_p(), ctypes.c_void_p()
-        prm = mgit.params()
+        prm = mgit.params(order)
         check(lib().flgpu_mg_create_lattice(ctx.h, dim, order, n0, nlevels, kv.h if kv else None,
                                             nk, f, mode, ctypes.byref(prm), ctypes.byref(h),
                                             ctypes.byref(hb)))
@@ -234,6 +284,15 @@ def make_iterator(linit, mat, steps=1, hierarchy=None):
         def step(x, b, r):
             check(lib().flgpu_sor_iterate(mat.h, linit.omega, 0, x.h, b.h, r.h))
         return Iterator(mat, _product(mat, step, steps, False, False), False, False)
+    if isinstance(linit, (GeometricSSC, GeometricPSC)):
+        ssc = isinstance(linit, GeometricSSC)
+        kind = _f.SMOOTH_SSC if ssc else _f.SMOOTH_PSC
+        w = linit.omega if ssc else linit.damp
+
+        def step(x, b, r):
+            check(lib().flgpu_block_iterate(mat.h, kind, w, 0, x.h, b.h, r.h))
+        # <block-iteration> iterators: residual-before t, residual-after nil (blockit.lisp:125,203)
+        return Iterator(mat, _product(mat, step, steps, True, False), True, False)
     if isinstance(linit, GeometricCS):
         if hierarchy is None:
             raise ValueError("a <geometric-cs> iterator needs its hierarchy")

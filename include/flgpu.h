@@ -164,6 +164,27 @@ int flgpu_jacobi_steps(const flgpu_mat *A, double damp, int steps, flgpu_vec *x,
 int flgpu_sor_iterate(const flgpu_mat *A, double omega, int backwards,
                       flgpu_vec *x, const flgpu_vec *b, flgpu_vec *r);
 
+/* ---- block smoothers (src/iteration/blockit.lisp:41-203,
+ *      src/iteration/geoblock.lisp:37-157) --------------------------------- */
+
+#define FLGPU_SMOOTH_SSC 2 /* <geometric-ssc>: successive subspace correction */
+#define FLGPU_SMOOTH_PSC 3 /* <geometric-psc>: parallel subspace correction */
+
+/* One iterate of femlisp's vertex-centred block smoother on a lattice matrix
+ * (flgpu_lattice_assemble, or a Galerkin level of a lattice hierarchy): the block of a
+ * mesh vertex v holds v and every DOF of a cell containing v that is not Dirichlet
+ * (find-vertex-centered-block, geoblock.lisp:65-91) -- one DOF for Q1, 3^dim for Q2.
+ * Blocks are swept by colour, the parity of the vertex coordinates (axis 0 most
+ * significant; backwards = 1: last colour first), which is <ssc> with an :ordering slot that
+ * sorts by colour; blocks of one colour do not couple, so the order inside a colour does
+ * not matter.  Per block B (blockit.lisp:113-203, store-p t: inverses formed once):
+ *   FLGPU_SMOOTH_SSC: r_B <- b_B - (A x)_B, x_B <- x_B + omega A_BB^-1 r_B;
+ *   FLGPU_SMOOTH_PSC: x_B <- x_B + omega A_BB^-1 r_B (omega = <psc>'s damp) with the
+ *   residual r = b - A x the caller provides (residual-before t); r is not written.
+ * b may be r (femlisp's (iterate q w w)); x must differ from both. */
+int flgpu_block_iterate(const flgpu_mat *A, int kind, double omega, int backwards,
+                        flgpu_vec *x, const flgpu_vec *b, flgpu_vec *r);
+
 /* ---- structured lattices: GPU assembly (src/discretization/fedisc.lisp:
  *      73-268, ellsys-fe.lisp:68-218, fe.lisp:235-244,502-535,
  *      constraints.lisp:323-411, sparseas.lisp:232-272) ------------------ */
@@ -246,8 +267,10 @@ int flgpu_system_write(const char *path, const flgpu_mat *A, const flgpu_vec *rh
 #define FLGPU_SMOOTH_SOR 1
 
 typedef struct flgpu_mg_params {
-  int smoother;    /* FLGPU_SMOOTH_JACOBI (damped) or FLGPU_SMOOTH_SOR (colour order) */
-  double omega;    /* <jacobi> damp / <sor> omega */
+  int smoother;    /* FLGPU_SMOOTH_JACOBI (damped), FLGPU_SMOOTH_SOR (colour order), or the
+                      block smoothers FLGPU_SMOOTH_SSC / FLGPU_SMOOTH_PSC (lattice matrices;
+                      femlisp's default for order > 1, geomg.lisp:58-71) */
+  double omega;    /* <jacobi> damp / <sor> omega / <ssc> omega / <psc> damp */
   int pre_steps;   /* product-iterator steps, linit.lisp:94-120 */
   int post_steps;
   int base_level;  /* coarse-grid solve level: dense FP64 LU/inverse */

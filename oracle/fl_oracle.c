@@ -885,7 +885,7 @@ static long *make_ordering(int dim, int order, const level_t *L, int mode,
 }
 
 /* setup-blocks of <geometric-blocking-mixin> with :vertex-centered blocks (geoblock.lisp:
- * 101-157) and find-vertex-centered-block (:57-86).  One block per mesh vertex v (a lattice
+ * 113-157) and find-vertex-centered-block (:65-91).  One block per mesh vertex v (a lattice
  * point whose coordinates are multiples of order): v itself and every DOF of a cell containing
  * v that is not Dirichlet -- the lattice points v + d with |d_k| <= order - 1 (a point with
  * |d_k| = order lies on a cell without v).  Uniform meshes have no hanging nodes, so the

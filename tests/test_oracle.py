@@ -511,7 +511,7 @@ def test_solution_matches_direct_solve():
     assert np.linalg.norm(x - u) / np.linalg.norm(u) < 1e-10
 
 
-# ---- vertex-centred block smoothers (blockit.lisp:113-203, geoblock.lisp:57-157) ----------
+# ---- vertex-centred block smoothers (blockit.lisp:113-203, geoblock.lisp:65-157) ----------
 
 def _geometric_blocks(dim, order, n):
     """Blocks from the definition: vertex v and every non-Dirichlet DOF whose carrying cell
