@@ -21,6 +21,8 @@
 // lattice vertex order as inv[v][j][i] = (A_BB^-1)_ij, identity rows and columns at absent
 // slots.  One iterate then costs per block the block's matrix rows (<ssc> only) and the W x W
 // inverse; order-1 blocks are single DOFs and take one lane each.
+#include <algorithm>
+
 #include "flgpu_internal.h"
 
 namespace flgpu {
@@ -77,22 +79,30 @@ __device__ __forceinline__ long blk_vertex_id(const BlkGeom &g, const int *a) {
   return v;
 }
 
+// the DOF of the slot with digits dg (axis k's offset dg[k] - h) of vertex a, or -1
+__device__ __forceinline__ int blk_key_d(const BlkGeom &g, const int *a, const int *dg) {
+  int key = 0;
+  bool centre = true, bdry = false;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (k >= g.dim) break;
+    int c = g.order * a[k] + dg[k] - g.h;
+    if (c < 0 || c >= g.M) return -1;
+    if (dg[k] != g.h) centre = false;
+    if (c == 0 || c == g.M - 1) bdry = true;
+    key = key * g.M + c;
+  }
+  return (bdry && !centre) ? -1 : key;
+}
+
 // the DOF of slot s of vertex a, or -1
 __device__ __forceinline__ int blk_key(const BlkGeom &g, const int *a, int s) {
-  int c[3];
-  bool centre = true, bdry = false;
+  int dg[3] = {0, 0, 0};
   for (int k = g.dim - 1; k >= 0; k--) {
-    int d = s % g.w1;
+    dg[k] = s % g.w1;
     s /= g.w1;
-    c[k] = g.order * a[k] + d - g.h;
-    if (c[k] < 0 || c[k] >= g.M) return -1;
-    if (d != g.h) centre = false;
-    if (c[k] == 0 || c[k] == g.M - 1) bdry = true;
   }
-  if (bdry && !centre) return -1;
-  long key = 0;
-  for (int k = 0; k < g.dim; k++) key = key * g.M + c[k];
-  return (int)key;
+  return blk_key_d(g, a, dg);
 }
 
 // Block inverses, one wave per vertex (grid: all (n+1)^dim vertices, colour 0's geometry with
@@ -162,52 +172,133 @@ __global__ __launch_bounds__(64) void k_blk_invert(BlkGeom g, const int *__restr
   }
 }
 
-__device__ __forceinline__ double lane_read(double v, int src) { return __shfl(v, src, 64); }
-__device__ __forceinline__ float lane_read(float v, int src) { return __shfl(v, src, 64); }
-
-template <class T>
-__device__ __forceinline__ T wave_allsum(T v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// v broadcast from lane j (j uniform over the wave)
+__device__ __forceinline__ float lane_bcast(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double lane_bcast(double v, int j) {
+  long long u = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), j);
+  int hi = __builtin_amdgcn_readlane((int)(u >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// One colour of an iterate with blocks of W > 1 slots, one wave per block.
+constexpr int kBlkWaves = 2;  // waves per workgroup of k_blk_sweep
+
+// Order-2 blocks in DIM dimensions: W = 3^DIM slots, slot digits base 3, last axis fastest.
+// Slots 3q, 3q + 1, 3q + 2 are three lattice neighbours along the last axis: their CSR rows
+// are contiguous, a run of at most 11 * 3^(DIM-1-z) * 5^z entries, z = the number of the first
+// DIM - 1 digits of q that sit on the vertex's planes.
+constexpr int pow_i(int b, int e) { return e == 0 ? 1 : b * pow_i(b, e - 1); }
+template <int DIM>
+constexpr int run_chunks(int q) {  // 64-entry chunks of run q
+  int z = 0;
+  for (int k = 0; k < DIM - 1; k++, q /= 3) z += (q % 3) == 1;
+  return (11 * pow_i(3, DIM - 1 - z) * pow_i(5, z) + 63) / 64;
+}
+template <int DIM>
+constexpr int all_chunks() {
+  int c = 0;
+  for (int q = 0; q < pow_i(3, DIM - 1); q++) c += run_chunks<DIM>(q);
+  return c;
+}
+
+// One colour of an iterate with order-2 blocks, one wave per block.
 // <ssc>: r_k = b_k - (A x)_k for the block's DOFs k (stored in r, as femlisp stores the block
 // residual), then x_B += omega A_BB^-1 r_B.  <psc>: x_B += omega A_BB^-1 r_B with r as given.
-// b may be r.
-template <class T, bool SSC>
-__global__ __launch_bounds__(256) void k_blk_sweep(BlkGeom g, long cnt,
-                                                   const int *__restrict__ rowptr,
-                                                   const int *__restrict__ colidx,
-                                                   const T *__restrict__ vals,
-                                                   const T *__restrict__ inv, T omega, T *x,
-                                                   const T *b, T *r) {
+// b may be r.  All memory reads of a block are issued in three waves of independent loads:
+// the row bounds and the block inverse; the CSR entries of every run, 64 at a time and every
+// lane its own entry; the x values they name.  Every lane adds its products into per-slot
+// partial sums, and the 64 partials of each slot are summed in lane order through LDS by the
+// slot's own lane.
+template <class T, bool SSC, int DIM>
+__global__ __launch_bounds__(64 * kBlkWaves) void k_blk_sweep(BlkGeom g, long cnt,
+                                                              const int *__restrict__ rowptr,
+                                                              const int *__restrict__ colidx,
+                                                              const T *__restrict__ vals,
+                                                              const T *__restrict__ inv, T omega,
+                                                              T *x, const T *b, T *r) {
+  constexpr int W = pow_i(3, DIM), NRUN = W / 3, NCH = all_chunks<DIM>();
+  __shared__ T part[kBlkWaves][64][W + 1];
   unsigned lb = xcd_block(blockIdx.x, gridDim.x);
-  long w = (long)lb * 4 + (threadIdx.x >> 6);
+  const int wv = threadIdx.x >> 6;
+  long w = (long)lb * kBlkWaves + wv;
   if (w >= cnt) return;  // uniform over the wave
-  const int lane = threadIdx.x & 63, W = g.W;
+  const int lane = threadIdx.x & 63;
   int av[3];
   blk_vertex(g, w, av);
   int key = lane < W ? blk_key(g, av, lane) : -1;
+  const T *iv = inv + blk_vertex_id(g, av) * W * W;
+  T ivr[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) ivr[j] = lane < W ? iv[j * W + lane] : (T)0;
   T myr = 0;
   if (SSC) {
-    for (int s = 0; s < W; s++) {
-      int ks = __builtin_amdgcn_readlane(key, s);
-      if (ks < 0) continue;
-      T acc = 0;
-      for (int e = rowptr[ks] + lane; e < rowptr[ks + 1]; e += 64) acc += vals[e] * x[colidx[e]];
-      acc = wave_allsum(acc);
-      if (lane == s) myr = b[ks] - acc;
+    int rs = 0, re = 0;
+    if (key >= 0) {
+      rs = rowptr[key];
+      re = rowptr[key + 1];
+    }
+    // per run: first present slot, and the entry bounds [e0, e3) with the ends e1, e2 of its
+    // first two rows (present rows are consecutive lattice points; none present: empty run)
+    int ta[NRUN], e0[NRUN], e1[NRUN], e2[NRUN], e3[NRUN];
+#pragma unroll
+    for (int q = 0; q < NRUN; q++) {
+      int k0 = __builtin_amdgcn_readlane(key, 3 * q);
+      int k1 = __builtin_amdgcn_readlane(key, 3 * q + 1);
+      int k2 = __builtin_amdgcn_readlane(key, 3 * q + 2);
+      int a = k0 >= 0 ? 0 : (k1 >= 0 ? 1 : (k2 >= 0 ? 2 : -1));
+      int z = k2 >= 0 ? 2 : (k1 >= 0 ? 1 : 0);
+      ta[q] = a < 0 ? 0 : a;
+      e0[q] = a < 0 ? 0 : __builtin_amdgcn_readlane(rs, 3 * q + a);
+      e3[q] = a < 0 ? 0 : __builtin_amdgcn_readlane(re, 3 * q + z);
+      e1[q] = (a >= 0 && a < z) ? __builtin_amdgcn_readlane(re, 3 * q + a) : e3[q];
+      e2[q] = (a >= 0 && a + 1 < z) ? __builtin_amdgcn_readlane(re, 3 * q + a + 1) : e3[q];
+    }
+    int cc[NCH];
+    T vv[NCH];
+#pragma unroll
+    for (int q = 0, c = 0; q < NRUN; q++)
+#pragma unroll
+      for (int i = 0; i < run_chunks<DIM>(q); i++, c++) {
+        int e = e0[q] + 64 * i + lane;
+        int ec = max(min(e, e3[q] - 1), 0);
+        cc[c] = colidx[ec];
+        vv[c] = vals[ec];
+      }
+    T xv[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; c++) xv[c] = x[cc[c]];
+    T acc[W];
+#pragma unroll
+    for (int s = 0; s < W; s++) acc[s] = 0;
+#pragma unroll
+    for (int q = 0, c = 0; q < NRUN; q++)
+#pragma unroll
+      for (int i = 0; i < run_chunks<DIM>(q); i++, c++) {
+        int e = e0[q] + 64 * i + lane;
+        T p = e < e3[q] ? vv[c] * xv[c] : (T)0;
+        int t = ta[q] + (e >= e1[q]) + (e >= e2[q]);
+        acc[3 * q] += t == 0 ? p : (T)0;
+        acc[3 * q + 1] += t == 1 ? p : (T)0;
+        acc[3 * q + 2] += t == 2 ? p : (T)0;
+      }
+#pragma unroll
+    for (int s = 0; s < W; s++) part[wv][lane][s] = acc[s];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (key >= 0) {
+      T sum = 0;
+      for (int l = 0; l < 64; l++) sum += part[wv][l][lane];
+      myr = b[key] - sum;
     }
   } else if (key >= 0) {
     myr = r[key];
   }
-  const T *iv = inv + blk_vertex_id(g, av) * W * W;
   T y = 0;
-  for (int j = 0; j < W; j++) {
-    T rj = lane_read(myr, j);
-    if (lane < W) y += iv[j * W + lane] * rj;
-  }
+#pragma unroll
+  for (int j = 0; j < W; j++) y += ivr[j] * lane_bcast(myr, j);
   if (key >= 0) {
     if (SSC) r[key] = myr;
     x[key] += omega * y;
@@ -288,15 +379,17 @@ static void block_sweep_t(const flgpu_mat *A, bool ssc, T omega, int backwards, 
                                                     (const T *)A->vals, (const T *)A->diag,
                                                     omega, x, b, r);
     } else {
-      unsigned grid = xcd_grid(grid_for(cnt, 4));
-      if (ssc)
-        k_blk_sweep<T, true><<<grid, 256, 0, st>>>(g, cnt, A->rowptr, A->colidx,
-                                                   (const T *)A->vals, (const T *)A->blk_inv,
-                                                   omega, x, b, r);
+      unsigned grid = xcd_grid(grid_for(cnt, kBlkWaves));
+      auto launch = [&](auto kern) {
+        kern<<<grid, 64 * kBlkWaves, 0, st>>>(g, cnt, A->rowptr, A->colidx, (const T *)A->vals,
+                                              (const T *)A->blk_inv, omega, x, b, r);
+      };
+      if (g.dim == 3)
+        launch(ssc ? k_blk_sweep<T, true, 3> : k_blk_sweep<T, false, 3>);
+      else if (g.dim == 2)
+        launch(ssc ? k_blk_sweep<T, true, 2> : k_blk_sweep<T, false, 2>);
       else
-        k_blk_sweep<T, false><<<grid, 256, 0, st>>>(g, cnt, A->rowptr, A->colidx,
-                                                    (const T *)A->vals, (const T *)A->blk_inv,
-                                                    omega, x, b, r);
+        launch(ssc ? k_blk_sweep<T, true, 1> : k_blk_sweep<T, false, 1>);
     }
     FLGPU_HIP(hipGetLastError());
   }
