@@ -6,7 +6,9 @@ coefficient diffusion at 128^3 cells (two-phase random inclusions, contrast
   mixed    FP32 V-cycle (BF16-MFMA coarse assembly) in FP64 flexible CG
   learned  mixed + the learned coarse correction (plain GMG-FCG fallback)
 
-plus V-cycle GDOF/s and the HBM roofline of the fine-level kernels.
+plus V-cycle GDOF/s and the HBM roofline of the fine-level kernels, and the
+3D Q2 48^3 configuration (kappa = exp(N(0, 1.5^2)) per cell, f = 1) solved with
+femlisp's default smoother for order 2, the vertex-centred block SSC.
 
 One step = one solve from x = 0 to ||b - A x|| / ||b|| < 1e-10 (FP64
 verified) with the hierarchy resident in HBM; setup (GPU assembly of all
@@ -96,6 +98,12 @@ def main():
                          "no JSON line), e.g. under a profiler")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the fine-level kernels and a copy of known size (counter runs)")
+    ap.add_argument("--no-q2", action="store_true", help="skip the Q2 48^3 configuration")
+    ap.add_argument("--q2-only", action="store_true",
+                    help="only the Q2 48^3 configuration (printed to stderr)")
+    ap.add_argument("--q2-cycles-only", action="store_true",
+                    help="only the Q2 48^3 hierarchies, V-cycles and block iterates, no solves "
+                         "(for kernel tracing)")
     args = ap.parse_args()
     if not args.solve_only and "plain" not in args.paths.split(","):
         ap.error("the JSON line compares with the plain path: include it or use --solve-only")
@@ -114,6 +122,96 @@ def main():
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
+
+    def q2_config(reps=3, solves=True):
+        """3D Q2, 48^3 cells (5 levels from 3^3), kappa = exp(g), g ~ N(0, 1.5^2) iid per cell
+        (seed 48), f = 1: FP64 <geometric-ssc> V(1,1) GMG-CG (femlisp's smoother for order 2,
+        geomg.lisp:58-71), the same cycle in FP32 inside FP64 FCG, and the FP32 multicolour GS
+        V(2,2) cycle for comparison; then one SSC iterate on the fine level alone."""
+        n0, lv, nq = 3, 5, 48
+        qctx = default_context()
+        kap = np.exp(np.random.default_rng(48).normal(scale=1.5, size=(nq,) * 3))
+        Aq, bq = lat.assemble(3, 2, nq, kap, prec=F.F64)
+        bn = norm(bq)
+        out = {"workload": f"3D Q2 {nq}^3 cells, N={Aq.nrows}, nnz={Aq.nnz}, "
+                           "kappa = exp(N(0, 1.5^2)) per cell (seed 48), f = 1, base 3^3"}
+        hq = {}
+        for name, sm, st, prec, itc in (
+                ("plain-ssc", it.GeometricSSC(1.0), 1, F.F64, it.CG),
+                ("mixed-ssc", it.GeometricSSC(1.0), 1, F.F32, it.FCG),
+                ("mixed-sor", it.SOR(1.0), 2, F.F32, it.FCG)):
+            cs = it.GeometricCS(sm, st, st, 0, symmetric=True, prec=prec)
+            t0 = time.perf_counter()
+            H = it.Hierarchy.lattice(cs, 3, 2, n0, lv, kappa=kap,
+                                     mode=F.ASM_MFMA if prec == F.F32 else F.ASM_F64)
+            qctx.synchronize()
+            su = time.perf_counter() - t0
+            if not args.no_graphs:
+                H.use_graphs(True)
+            hq[name] = H
+            # V-cycle on vectors of the hierarchy's precision (a rhs from b, zero start)
+            Nt = H.top_matrix().nrows
+            qv, w1 = GpuVector(Nt, prec), GpuVector.from_array(bq.numpy(), prec)
+            w2 = GpuVector.from_array(bq.numpy(), prec)
+            H.apply(qv, w1, w2)
+            qctx.timer_start()
+            for _ in range(10):
+                H.apply(qv, w1, w2)
+            vms = qctx.timer_stop() / 10
+            out[name] = {"vcycle_ms": vms, "vcycle_gdofs": Nt / (vms * 1e-3) / 1e9, "setup_s": su}
+            log(f"[bench] q2 {name}: V-cycle {vms:.3f} ms")
+            del qv, w1, w2
+            if not solves:
+                continue
+            ts = []
+            for _ in range(reps):
+                r = it.solve(it.LinearSolver(itc(cs), success_if=("<", "reduction", 1e-10),
+                                             failure_if=(">", "step", 300), true_residual=True),
+                             {"matrix": Aq, "rhs": bq, "hierarchy": H})
+                ts.append(r["seconds"])
+            rel = r["true_residual"] / bn
+            assert r["status"] == "success" and rel < 1e-10, (name, r["status"], rel)
+            out[name].update({"s": float(np.median(ts)), "steps": r["steps"], "true_rel": rel})
+            log(f"[bench] q2 {name}: {out[name]['s']:.4f}s, {r['steps']} iterations, "
+                f"true rel residual {rel:.2e}")
+        # the fine-level <geometric-ssc> iterate: 8 colour launches of k_blk_sweep.  Algorithmic
+        # bytes: the CSR rows of every block's DOFs (each block recomputes its residual),
+        # every block's inverse, and per DOF of a block b read, r and x written, x read
+        kern = {}
+        rp = Aq.csr()[0]
+        keys = lat.vertex_blocks(3, 2, nq)
+        present = keys >= 0
+        entries = int(np.sum((rp[1:] - rp[:-1])[keys[present]]))
+        for prec, vb, tag in ((F.F64, 8, "f64"), (F.F32, 4, "f32")):
+            H = hq["plain-ssc" if prec == F.F64 else "mixed-ssc"]
+            At = H.top_matrix()
+            Nq = At.nrows
+            xq = GpuVector.from_array(np.zeros(Nq), prec)
+            bb = GpuVector.from_array(bq.numpy(), prec)
+            rq = GpuVector(Nq, prec)
+            it_ = lambda: check(lib().flgpu_block_iterate(At.h, F.SMOOTH_SSC, 1.0, 0, xq.h,
+                                                          bb.h, rq.h))
+            it_()
+            reps_k = 10
+            qctx.timer_start()
+            for _ in range(reps_k):
+                it_()
+            us = qctx.timer_stop() * 1e3 / reps_k
+            byt = entries * (4 + vb) + keys.shape[0] * 27 * 27 * vb + int(present.sum()) * 4 * vb
+            kern[f"ssc_iterate_{tag}"] = {"us": us, "GBps": byt / (us * 1e-6) / 1e9,
+                                          "frac": byt / (us * 1e-6) / 8e12, "bytes": byt}
+            log(f"[bench] q2 ssc iterate {tag}: {us:.1f} us (8 colour launches), "
+                f"{byt / (us * 1e-6) / 1e9:.0f} GB/s")
+            del At
+        out["kernels"] = kern
+        if solves:
+            out["speedup_ssc_vs_sor"] = out["mixed-sor"]["s"] / out["mixed-ssc"]["s"]
+        del hq, Aq, bq
+        return out
+
+    if args.q2_only or args.q2_cycles_only:
+        q2_config(solves=not args.q2_cycles_only)
+        return
 
     n = args.n
     levels = int(round(np.log2(n))) + 1
@@ -357,6 +455,8 @@ def main():
         log(f"[bench] {k}: {v['ms'] * 1e3:.1f} us, {v['gbs']:.0f} GB/s "
             f"({v['gbs'] / 8000 * 100:.1f}% of 8 TB/s)")
 
+    q2 = None if args.no_q2 else q2_config()
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle, 1 thread) ...")
@@ -395,6 +495,7 @@ def main():
         "vcycle_ms": {k: v["ms"] for k, v in vcyc.items()},
         "kernels": {k: {"us": v["ms"] * 1e3, "GBps": v["gbs"], "frac": v["gbs"] / 8000.0}
                     for k, v in kern.items()},
+        "q2_48": q2,
         "roofline": roof if roof is not None else {
             "bound": "hbm", "kernel": "k_q1_sor<float> (fine-level <sor> sweep, per-cell form)",
             "achieved": dom["gbs"], "peak": 8000.0, "unit": "GB/s",

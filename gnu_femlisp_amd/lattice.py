@@ -92,3 +92,25 @@ def lattice_colours(dim, order, n):
     for k in range(dim):
         c = c * m + (idx[k] % m)
     return c
+
+
+def vertex_blocks(dim, order, n):
+    """Keys of the vertex-centred blocks that flgpu_block_iterate sweeps (femlisp's
+    find-vertex-centered-block, src/iteration/geoblock.lisp:65-91, on the lattice): one row
+    per mesh vertex, in colour order (the parity of the vertex coordinates, axis 0 most
+    significant) then index order; one column per slot, offsets -(order-1)..order-1 per axis
+    with the last axis fastest; -1 where the point lies outside the lattice or is a Dirichlet
+    DOF other than the vertex itself."""
+    M, h, w1 = order * n + 1, order - 1, 2 * order - 1
+    a = np.indices((n + 1,) * dim).reshape(dim, -1).T
+    colour = sum((a[:, k] & 1) << (dim - 1 - k) for k in range(dim))
+    a = a[np.argsort(colour, kind="stable")]
+    d = np.indices((w1,) * dim).reshape(dim, -1).T - h
+    c = a[:, None, :] * order + d[None, :, :]
+    inside = ((c >= 0) & (c < M)).all(-1)
+    bdry = ((c == 0) | (c == M - 1)).any(-1)
+    centre = (d == 0).all(-1)[None, :]
+    key = np.zeros(c.shape[:2], np.int64)
+    for k in range(dim):
+        key = key * M + c[..., k]
+    return np.where(inside & (centre | ~bdry), key, -1).astype(np.int32)
