@@ -1071,8 +1071,15 @@ extern "C" int flgpu_lattice_colouring(flgpu_mat *A, int dim, int order, int n) 
     check_lattice_args(dim, order, n);
     FLGPU_REQUIRE(A->nrows == ipow((long)order * n + 1, dim) && A->ncols == A->nrows,
                   "matrix size does not match the lattice");
+    FLGPU_REQUIRE(A->lat_dim == 0 ||
+                      (A->lat_dim == dim && A->lat_order == order && A->lat_n == n),
+                  "the matrix belongs to another lattice");
     CTX_LOCK(A->ctx);
     lattice_colouring(A, dim, order, n);
+    // a caller's matrix in lattice numbering: the block smoothers find their blocks from this
+    A->lat_dim = dim;
+    A->lat_order = order;
+    A->lat_n = n;
     return 0;
   });
 }

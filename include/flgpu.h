@@ -171,7 +171,8 @@ int flgpu_sor_iterate(const flgpu_mat *A, double omega, int backwards,
 #define FLGPU_SMOOTH_PSC 3 /* <geometric-psc>: parallel subspace correction */
 
 /* One iterate of femlisp's vertex-centred block smoother on a lattice matrix
- * (flgpu_lattice_assemble, or a Galerkin level of a lattice hierarchy): the block of a
+ * (flgpu_lattice_assemble, a level of a lattice hierarchy, or a caller's matrix in lattice
+ * numbering after flgpu_lattice_colouring): the block of a
  * mesh vertex v holds v and every DOF of a cell containing v that is not Dirichlet
  * (find-vertex-centered-block, geoblock.lisp:65-91) -- one DOF for Q1, 3^dim for Q2.
  * Blocks are swept by colour, the parity of the vertex coordinates (axis 0 most
@@ -232,7 +233,10 @@ int flgpu_lattice_projection(flgpu_ctx *ctx, int dim, int order, int nc, int pre
  * columns are zero.  For a constant coefficient it equals femlisp's trilinear
  * interpolation. */
 int flgpu_lattice_blackbox_interpolation(const flgpu_mat *A, flgpu_mat *P);
-/* lattice colouring for <sor>: coordinate parity (Q1) / coordinate mod 3 (Q2) */
+/* lattice colouring for <sor>: coordinate parity (Q1) / coordinate mod 3 (Q2).  A caller's
+ * matrix (flgpu_csr_upload, flgpu_system_read) in canonical lattice numbering is recorded as
+ * that lattice's matrix, which the block smoothers (flgpu_block_iterate) need; a matrix of
+ * another lattice is rejected. */
 int flgpu_lattice_colouring(flgpu_mat *A, int dim, int order, int n);
 /* The element kernel alone (discretize-locally for a batch of cells,
  * ellsys-fe.lisp:68-127): element matrices of cells first_cell ..

@@ -183,3 +183,48 @@ def test_ssc_graph_replays_and_reassembly():
     H.apply(q, GpuVector.from_array(r), GpuVector.from_array(r))
     H2.apply(q2, GpuVector.from_array(r), GpuVector.from_array(r))
     assert np.array_equal(q.numpy(), q2.numpy())
+
+
+def test_block_iterate_on_uploaded_lattice_matrix():
+    """A caller's CSR in lattice numbering (as femlisp's sparse-matrix->ccs in
+    sort-lexicographically order hands it over) takes block smoothing once
+    flgpu_lattice_colouring has recorded its lattice; the iterate equals the one on the
+    matrix assembled on the GPU."""
+    say("block iterate on an uploaded matrix")
+    dim, order, n = 3, 2, 3
+    rng = np.random.default_rng(41)
+    A, _ = lat.assemble(dim, order, n, kappa=np.exp(rng.normal(size=(n,) * dim)))
+    rp, ci, v = A.csr()
+    U = GpuSparseMatrix.from_csr(rp, ci, v)
+    N = A.nrows
+    x0, b = rng.normal(size=N), rng.normal(size=N)
+    x, bb, r = GpuVector.from_array(x0), GpuVector.from_array(b), GpuVector(N)
+    assert lib().flgpu_block_iterate(U.h, F.SMOOTH_SSC, 1.0, 0, x.h, bb.h, r.h) < 0
+    assert lib().flgpu_lattice_colouring(U.h, dim, order, n + 1) < 0  # wrong size
+    check(lib().flgpu_lattice_colouring(U.h, dim, order, n))
+    check(lib().flgpu_block_iterate(U.h, F.SMOOTH_SSC, 1.0, 0, x.h, bb.h, r.h))
+    x2, r2 = GpuVector.from_array(x0), GpuVector(N)
+    check(lib().flgpu_block_iterate(A.h, F.SMOOTH_SSC, 1.0, 0, x2.h, bb.h, r2.h))
+    assert np.array_equal(x.numpy(), x2.numpy())
+
+
+def test_ssc_mixed_fcg_eager_matches_graphs():
+    """FP32 <geometric-ssc> cycles inside FP64 FCG, launched eagerly and replayed from HIP
+    graphs: the same steps and the same solution, to 1e-10 in the FP64 true residual."""
+    say("ssc mixed fcg eager vs graphs")
+    dim, order, n0, levels = 3, 2, 1, 4
+    rng = np.random.default_rng(43)
+    kap = np.exp(rng.normal(scale=1.5, size=(8, 8, 8)))
+    A, b = lat.assemble(dim, order, 8, kappa=kap)
+    out = []
+    for graphs in (False, True):
+        cs, H = _hier(dim, order, n0, levels, it.GeometricSSC(), 1, 1, 0, symmetric=True,
+                      kappa=kap, prec=F.F32, mode=F.ASM_MFMA)
+        H.use_graphs(graphs)
+        bb = it.solve(it.LinearSolver(it.FCG(cs), success_if=("<", "reduction", 1e-10),
+                                      failure_if=(">", "step", 100), true_residual=True),
+                      {"matrix": A, "rhs": b, "hierarchy": H})
+        assert bb["status"] == "success"
+        assert bb["true_residual"] / np.linalg.norm(b.numpy()) < 1e-10
+        out.append((bb["steps"], bb["solution"].numpy()))
+    assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])

@@ -99,3 +99,30 @@ def test_config_q2_48_mixed_fcg():
     assert mixed["status"] == "success"
     assert mixed["true_residual"] / np.linalg.norm(b.numpy()) < 1e-10
     assert relerr(mixed["solution"].numpy(), plain["solution"].numpy()) < 1e-9
+
+
+def test_config_q2_48_block_ssc():
+    """The same Q2 48^3 system with femlisp's default smoother for order 2, the vertex-centred
+    block SSC (geomg.lisp:58-71), V(1,1): FP64 GMG-CG and the FP32 cycle inside FP64 FCG reach
+    1e-10 and agree."""
+    say("config: 3D Q2 48^3, block SSC")
+    n0, levels = 3, 5
+    n = n0 << (levels - 1)
+    kap = np.exp(np.random.default_rng(48).normal(scale=1.5, size=(n, n, n)))
+    A, b = lat.assemble(3, 2, n, kap, prec=F.F64)
+    sols, steps = [], []
+    for prec, itc in ((F.F64, it.CG), (F.F32, it.FCG)):
+        cs = it.GeometricCS(it.GeometricSSC(1.0), 1, 1, 0, symmetric=True, prec=prec)
+        H = it.Hierarchy.lattice(cs, 3, 2, n0, levels, kappa=kap,
+                                 mode=F.ASM_MFMA if prec == F.F32 else F.ASM_F64)
+        r = it.solve(it.LinearSolver(itc(cs), success_if=("<", "reduction", 1e-10),
+                                     failure_if=(">", "step", 100), true_residual=True),
+                     {"matrix": A, "rhs": b, "hierarchy": H})
+        assert r["status"] == "success"
+        assert r["true_residual"] / np.linalg.norm(b.numpy()) < 1e-10
+        sols.append(r["solution"].numpy())
+        steps.append(r["steps"])
+        del H
+    say(f"  ssc: FP64 CG {steps[0]} steps, FP32-in-FCG {steps[1]} steps")
+    assert steps[0] <= 40 and steps[1] <= 40
+    assert relerr(sols[1], sols[0]) < 1e-9
