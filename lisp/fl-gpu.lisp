@@ -70,6 +70,11 @@
   (fl.port:def-function ("flgpu_sor_iterate" flgpu-sor-iterate)
       ((a (* t)) (omega :double) (backwards :int) (x (* t)) (b (* t)) (r (* t)))
     :returning :int)
+  (fl.port:def-function ("flgpu_lattice_colouring" flgpu-lattice-colouring)
+      ((a (* t)) (dim :int) (order :int) (n :int)) :returning :int)
+  (fl.port:def-function ("flgpu_block_iterate" flgpu-block-iterate)
+      ((a (* t)) (kind :int) (omega :double) (backwards :int) (x (* t)) (b (* t)) (r (* t)))
+    :returning :int)
   (fl.port:def-function ("flgpu_mg_create" flgpu-mg-create)
       ((ctx (* t)) (nlevels :int) (a-vec (* (* t))) (i-vec (* (* t)))
        (params (* t)) (out (* (* t))))
@@ -136,7 +141,9 @@
   ((handle :initarg :handle :reader gpu-handle)
    (nrows :initarg :nrows :reader nrows)
    (ncols :initarg :ncols :reader ncols)
-   (keys :initarg :keys :reader gpu-keys :documentation "row keys in DOF-number order")))
+   (keys :initarg :keys :reader gpu-keys :documentation "row keys in DOF-number order")
+   (lattice :initarg :lattice :initform nil :reader gpu-lattice
+            :documentation "(dim order n) when the keys number that unit-cube lattice")))
 
 (defun canonical-keys (A)
   "The row keys of A in sort-lexicographically order of their DOF midpoints
@@ -144,14 +151,14 @@
 library's canonical numbering."
   (coerce (fl.mesh::sort-lexicographically (copy-list (row-keys A))) 'vector))
 
-(defun sparse-matrix->gpu (A &key (keys (canonical-keys A)) (prec 0))
+(defun sparse-matrix->gpu (A &key (keys (canonical-keys A)) (prec 0) lattice)
   "Uploads A: the CCS of A^T (sparse-matrix->ccs, src/matlisp/sparselu.lisp:235)
 is the CSR of A."
   (let ((ccs (sparse-matrix->ccs (transpose A) :keys keys)))
     (with-slots (sizes starts indices) (pattern ccs)
       (let ((n (aref sizes 0)))
         (make-instance '<gpu-sparse-matrix>
-                       :nrows n :ncols (aref sizes 1) :keys keys
+                       :nrows n :ncols (aref sizes 1) :keys keys :lattice lattice
                        :handle (flgpu-out-handle #'flgpu-csr-upload (flgpu-ctx)
                                                  n (aref sizes 1) (aref starts n)
                                                  starts indices (store ccs) prec))))))
@@ -192,6 +199,26 @@ is the CSR of A."
                                                0 (gpu-handle x) (gpu-handle b)
                                                (gpu-handle r))))
     :residual-after nil))
+
+(defun block-iterator (iter mat kind weight)
+  "<block-iteration> iterator (blockit.lisp:113-203) on the vertex-centred blocks of the
+lattice numbering MAT's keys: residual-before t, residual-after nil."
+  (destructuring-bind (dim order n)
+      (or (gpu-lattice mat) (error "block smoothing needs the matrix's lattice"))
+    (flgpu-check (flgpu-lattice-colouring (gpu-handle mat) dim order n)))
+  (make-instance '<iterator> :matrix mat :residual-before t :initialize nil
+    :iterate (lambda (x b r)
+               (flgpu-check (flgpu-block-iterate (gpu-handle mat) kind
+                                                 (float (slot-value iter weight) 1d0)
+                                                 0 (gpu-handle x) (gpu-handle b)
+                                                 (gpu-handle r))))
+    :residual-after nil))
+
+(defmethod make-iterator ((ssc fl.geomg::<geometric-ssc>) (mat <gpu-sparse-matrix>))
+  (block-iterator ssc mat 2 'fl.iteration::omega))     ; FLGPU_SMOOTH_SSC
+
+(defmethod make-iterator ((psc fl.geomg::<geometric-psc>) (mat <gpu-sparse-matrix>))
+  (block-iterator psc mat 3 'fl.iteration::damp))      ; FLGPU_SMOOTH_PSC
 
 ;;; ---- export for offline hand-over ----------------------------------------
 
