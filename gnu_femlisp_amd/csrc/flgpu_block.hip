@@ -197,18 +197,53 @@ constexpr int run_chunks(int q) {  // 64-entry chunks of run q
   return (11 * pow_i(3, DIM - 1 - z) * pow_i(5, z) + 63) / 64;
 }
 template <int DIM>
-constexpr int all_chunks() {
+constexpr int range_chunks(int q0, int q1) {
   int c = 0;
-  for (int q = 0; q < pow_i(3, DIM - 1); q++) c += run_chunks<DIM>(q);
+  for (int q = q0; q < q1; q++) c += run_chunks<DIM>(q);
   return c;
+}
+
+// runs [Q0, Q1) of a block: every chunk's entries loaded, then the x values they name, then
+// the products added into the slots' partial sums
+template <class T, int DIM, int Q0, int Q1>
+__device__ __forceinline__ void blk_runs(const int *__restrict__ colidx,
+                                         const T *__restrict__ vals, const T *x, int lane,
+                                         const int *ta, const int *e0, const int *e1,
+                                         const int *e2, const int *e3, T *acc) {
+  constexpr int NCH = range_chunks<DIM>(Q0, Q1);
+  int cc[NCH];
+  T vv[NCH];
+#pragma unroll
+  for (int q = Q0, c = 0; q < Q1; q++)
+#pragma unroll
+    for (int i = 0; i < run_chunks<DIM>(q); i++, c++) {
+      int e = e0[q] + 64 * i + lane;
+      int ec = max(min(e, e3[q] - 1), 0);
+      cc[c] = colidx[ec];
+      vv[c] = vals[ec];
+    }
+  T xv[NCH];
+#pragma unroll
+  for (int c = 0; c < NCH; c++) xv[c] = x[cc[c]];
+#pragma unroll
+  for (int q = Q0, c = 0; q < Q1; q++)
+#pragma unroll
+    for (int i = 0; i < run_chunks<DIM>(q); i++, c++) {
+      int e = e0[q] + 64 * i + lane;
+      T p = e < e3[q] ? vv[c] * xv[c] : (T)0;
+      int t = ta[q] + (e >= e1[q]) + (e >= e2[q]);
+      acc[3 * q] += t == 0 ? p : (T)0;
+      acc[3 * q + 1] += t == 1 ? p : (T)0;
+      acc[3 * q + 2] += t == 2 ? p : (T)0;
+    }
 }
 
 // One colour of an iterate with order-2 blocks, one wave per block.
 // <ssc>: r_k = b_k - (A x)_k for the block's DOFs k (stored in r, as femlisp stores the block
 // residual), then x_B += omega A_BB^-1 r_B.  <psc>: x_B += omega A_BB^-1 r_B with r as given.
-// b may be r.  All memory reads of a block are issued in three waves of independent loads:
-// the row bounds and the block inverse; the CSR entries of every run, 64 at a time and every
-// lane its own entry; the x values they name.  Every lane adds its products into per-slot
+// b may be r.  The memory reads of a block are issued in batches of independent loads: the
+// row bounds; the CSR entries of the runs, 64 at a time and every lane its own entry; the x
+// values they name; then the block inverse, while the partial sums are reduced.  Every lane adds its products into per-slot
 // partial sums, and the 64 partials of each slot are summed in lane order through LDS by the
 // slot's own lane.
 template <class T, bool SSC, int DIM>
@@ -218,7 +253,7 @@ __global__ __launch_bounds__(64 * kBlkWaves) void k_blk_sweep(BlkGeom g, long cn
                                                               const T *__restrict__ vals,
                                                               const T *__restrict__ inv, T omega,
                                                               T *x, const T *b, T *r) {
-  constexpr int W = pow_i(3, DIM), NRUN = W / 3, NCH = all_chunks<DIM>();
+  constexpr int W = pow_i(3, DIM), NRUN = W / 3;
   __shared__ T part[kBlkWaves][64][W + 1];
   unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const int wv = threadIdx.x >> 6;
@@ -230,8 +265,10 @@ __global__ __launch_bounds__(64 * kBlkWaves) void k_blk_sweep(BlkGeom g, long cn
   int key = lane < W ? blk_key(g, av, lane) : -1;
   const T *iv = inv + blk_vertex_id(g, av) * W * W;
   T ivr[W];
+  auto load_inverse = [&]() {
 #pragma unroll
-  for (int j = 0; j < W; j++) ivr[j] = lane < W ? iv[j * W + lane] : (T)0;
+    for (int j = 0; j < W; j++) ivr[j] = lane < W ? iv[j * W + lane] : (T)0;
+  };
   T myr = 0;
   if (SSC) {
     int rs = 0, re = 0;
@@ -255,36 +292,13 @@ __global__ __launch_bounds__(64 * kBlkWaves) void k_blk_sweep(BlkGeom g, long cn
       e1[q] = (a >= 0 && a < z) ? __builtin_amdgcn_readlane(re, 3 * q + a) : e3[q];
       e2[q] = (a >= 0 && a + 1 < z) ? __builtin_amdgcn_readlane(re, 3 * q + a + 1) : e3[q];
     }
-    int cc[NCH];
-    T vv[NCH];
-#pragma unroll
-    for (int q = 0, c = 0; q < NRUN; q++)
-#pragma unroll
-      for (int i = 0; i < run_chunks<DIM>(q); i++, c++) {
-        int e = e0[q] + 64 * i + lane;
-        int ec = max(min(e, e3[q] - 1), 0);
-        cc[c] = colidx[ec];
-        vv[c] = vals[ec];
-      }
-    T xv[NCH];
-#pragma unroll
-    for (int c = 0; c < NCH; c++) xv[c] = x[cc[c]];
     T acc[W];
 #pragma unroll
     for (int s = 0; s < W; s++) acc[s] = 0;
-#pragma unroll
-    for (int q = 0, c = 0; q < NRUN; q++)
-#pragma unroll
-      for (int i = 0; i < run_chunks<DIM>(q); i++, c++) {
-        int e = e0[q] + 64 * i + lane;
-        T p = e < e3[q] ? vv[c] * xv[c] : (T)0;
-        int t = ta[q] + (e >= e1[q]) + (e >= e2[q]);
-        acc[3 * q] += t == 0 ? p : (T)0;
-        acc[3 * q + 1] += t == 1 ? p : (T)0;
-        acc[3 * q + 2] += t == 2 ? p : (T)0;
-      }
+    blk_runs<T, DIM, 0, NRUN>(colidx, vals, x, lane, ta, e0, e1, e2, e3, acc);
 #pragma unroll
     for (int s = 0; s < W; s++) part[wv][lane][s] = acc[s];
+    load_inverse();  // in flight during the reduction, in the registers acc held
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -293,8 +307,9 @@ __global__ __launch_bounds__(64 * kBlkWaves) void k_blk_sweep(BlkGeom g, long cn
       for (int l = 0; l < 64; l++) sum += part[wv][l][lane];
       myr = b[key] - sum;
     }
-  } else if (key >= 0) {
-    myr = r[key];
+  } else {
+    load_inverse();
+    if (key >= 0) myr = r[key];
   }
   T y = 0;
 #pragma unroll
