@@ -21,8 +21,6 @@
 // lattice vertex order as inv[v][j][i] = (A_BB^-1)_ij, identity rows and columns at absent
 // slots.  One iterate then costs per block the block's matrix rows (<ssc> only) and the W x W
 // inverse; order-1 blocks are single DOFs and take one lane each.
-#include <algorithm>
-
 #include "flgpu_internal.h"
 
 namespace flgpu {
