@@ -217,6 +217,13 @@ void vdot_dev(flgpu_ctx *ctx, long n, int prec, const void *x, const void *y, in
   FLGPU_HIP(hipGetLastError());
 }
 
+// the sum of np partials (in k_dot_final's order) into reduction slot `slot`
+void dot_final(flgpu_ctx *ctx, const double *partial, int np, int slot) {
+  FLGPU_REQUIRE(slot >= 0 && slot < kRedSlots, "reduction slot out of range");
+  k_dot_final<<<1, kRedBlocks, 0, ctx->stream>>>(partial, np, ctx->red_out + slot);
+  FLGPU_HIP(hipGetLastError());
+}
+
 // dots (x[k], y[k]) into slots[k], k < count <= 3, in one pass; each equals vdot_dev's result
 void vdot_multi_dev(flgpu_ctx *ctx, long n, int prec, int count, const void *const *x,
                     const void *const *y, const int *slots) {

@@ -179,6 +179,9 @@ struct flgpu_mat {
   // lattice vertex order, blk_inv[v][j][i] = (A_BB^-1)_ij (prec); nullptr until needed and for
   // order 1, whose blocks are single DOFs
   void *blk_inv = nullptr;
+  // per-block partial sums of the per-cell product's fused dot (spmv_dot), dot_np entries
+  double *dot_part = nullptr;
+  long dot_np = 0;
 };
 
 namespace flgpu {
@@ -206,6 +209,9 @@ void spmv(const flgpu_mat *A, double alpha, const void *x, double beta,
           const void *z, void *out);
 void spmv_t(const flgpu_mat *A, double alpha, const void *x, double beta,
             void *out);
+// out = A x and the dot (out, x) into reduction slot `slot` in one pass over A, when A is
+// applied in the per-cell form; false (nothing launched) otherwise
+bool spmv_dot(const flgpu_mat *A, const void *x, void *out, int slot);
 void jacobi_iterate(const flgpu_mat *A, double damp, void *x, void *r);
 void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x,
                const void *b, void *r);
@@ -260,6 +266,8 @@ void vaxpy(flgpu_ctx *ctx, long n, int prec, double a, const void *x, void *y);
 void vdot_dev(flgpu_ctx *ctx, long n, int prec, const void *x, const void *y,
               int slot);
 // 1 to 3 dots in one pass, each identical to vdot_dev's
+// the sum of np per-block partials into reduction slot `slot`
+void dot_final(flgpu_ctx *ctx, const double *partial, int np, int slot);
 void vdot_multi_dev(flgpu_ctx *ctx, long n, int prec, int count, const void *const *x,
                     const void *const *y, const int *slots);
 double vdot_host(flgpu_ctx *ctx, long n, int prec, const void *x,

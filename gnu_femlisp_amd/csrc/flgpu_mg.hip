@@ -938,7 +938,7 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             initialized = true;
           }
           if (alpha != 0.0) {
-            if (!ap_ready) {
+            if (!ap_ready && !spmv_dot(A, W.p, W.a, 1)) {
               spmv(A, 1.0, W.p, 0.0, W.a, W.a);         // gemm! 1 A p 0 a
               vdot_dev(ctx, N, prec, W.a, W.p, 1);       // beta = a.p
             }
@@ -989,8 +989,10 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             FLGPU_HIP(hipGetLastError());
             ca = na;
             fetch_slots_async(ctx, 4);
-            spmv(A, 1.0, W.p, 0.0, W.a, W.a);  // the next step's product, ahead of the wait
-            vdot_dev(ctx, N, prec, W.a, W.p, 1);
+            if (!spmv_dot(A, W.p, W.a, 1)) {  // the next step's product, ahead of the wait
+              spmv(A, 1.0, W.p, 0.0, W.a, W.a);
+              vdot_dev(ctx, N, prec, W.a, W.p, 1);
+            }
             ap_ready = true;
             fetch_wait(ctx);
             alpha = ctx->host_out[ca];

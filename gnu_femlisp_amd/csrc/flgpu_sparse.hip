@@ -218,6 +218,7 @@ void mat_free(flgpu_mat *A) {
   dev_free(A->val27);
   dev_free(A->tp27);
   dev_free(A->blk_inv);
+  dev_free(A->dot_part);
   delete A;
 }
 
@@ -567,13 +568,16 @@ __device__ __forceinline__ T ld_at(const T *__restrict__ p, I byte_offset) {
 // position count 31 bits.  QZB rows per thread.
 // W2: every result is also stored to out2 in FP64 (the last step of a cycle run in FP32 for
 // an FP64 caller).
-template <class T, int OP, class I, int QZB, bool W2 = false>
+// DOT (Q1_SPMV): the block's share of sum_i out_i x_i goes to dpart[blockIdx.x] (each lane its
+// rows in order, then the wave, then the block's waves in order).
+template <class T, int OP, class I, int QZB, bool W2 = false, bool DOT = false>
 __global__ __launch_bounds__(256) void k_q1_zblock(int M, int n, const T *__restrict__ alpha,
                                                    Q1KT<T> kt, T coef, T beta,
                                                    const T *__restrict__ v,
                                                    const T *__restrict__ x,
                                                    T *__restrict__ out,
-                                                   double *__restrict__ out2 = nullptr) {
+                                                   double *__restrict__ out2 = nullptr,
+                                                   double *__restrict__ dpart = nullptr) {
   unsigned lb = xcd_block(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x & 63;
   const I Mi = (I)M, M2 = Mi * Mi;
@@ -672,6 +676,7 @@ __global__ __launch_bounds__(256) void k_q1_zblock(int M, int n, const T *__rest
   };
   const bool side = xbd || gy == 0 || gy == M - 1;
   const T cf = (OP == Q1_SPMV) ? coef : (T)-1;
+  double dsum = 0.0;
   prep_x(0);
   prep_x(1);
   prep_a(0);
@@ -711,7 +716,15 @@ __global__ __launch_bounds__(256) void k_q1_zblock(int M, int n, const T *__rest
     if (compute && zz < M) {
       *(T *)((char *)out + i) = res;
       if (W2) *(double *)((char *)out2 + i * (I)(sizeof(double) / sizeof(T))) = (double)res;
+      if (DOT) dsum += (double)res * (double)xi[r];
     }
+  }
+  if (DOT) {  // every lane gets here: no lane leaves the kernel early
+    __shared__ double ws[4];
+    for (int off = 32; off > 0; off >>= 1) dsum += __shfl_down(dsum, off, 64);
+    if (lane == 0) ws[threadIdx.x >> 6] = dsum;
+    __syncthreads();
+    if (threadIdx.x == 0) dpart[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
   }
 }
 
@@ -729,7 +742,7 @@ __global__ __launch_bounds__(256) void k_q1_zblock(int M, int n, const T *__rest
 
 template <class T, int OP>
 static void launch_q1_pass(const flgpu_mat *A, T coef, T beta, const void *v, const void *x,
-                           void *out, double *out2 = nullptr) {
+                           void *out, double *out2 = nullptr, double *dpart = nullptr) {
   constexpr int QZB = sizeof(T) == 4 ? FLGPU_QZB_F32
                       : (OP == Q1_SPMV ? FLGPU_QZB_F64_SPMV : FLGPU_QZB_F64_JACOBI);
   int M = A->lat_n + 1;
@@ -754,6 +767,21 @@ static void launch_q1_pass(const flgpu_mat *A, T coef, T beta, const void *v, co
     }
   }
   FLGPU_REQUIRE(out2 == nullptr, "an FP64 copy is written only by the FP32 <jacobi> pass");
+  if constexpr (OP == Q1_SPMV) {
+    if (dpart) {  // the product and its dot with x; 4 waves per block, as launched below
+      if (narrow)
+        k_q1_zblock<T, OP, unsigned, QZB, false, true><<<g, 256, 0, A->ctx->stream>>>(
+            M, A->lat_n, (const T *)A->alpha, kt, coef, beta, (const T *)v, (const T *)x,
+            (T *)out, nullptr, dpart);
+      else
+        k_q1_zblock<T, OP, unsigned long, QZB, false, true><<<g, 256, 0, A->ctx->stream>>>(
+            M, A->lat_n, (const T *)A->alpha, kt, coef, beta, (const T *)v, (const T *)x,
+            (T *)out, nullptr, dpart);
+      FLGPU_HIP(hipGetLastError());
+      return;
+    }
+  }
+  FLGPU_REQUIRE(dpart == nullptr, "the dot is formed only by the per-cell product");
   if (narrow)
     k_q1_zblock<T, OP, unsigned, QZB><<<g, 256, 0, A->ctx->stream>>>(
         M, A->lat_n, (const T *)A->alpha, kt, coef, beta, (const T *)v, (const T *)x, (T *)out);
@@ -1256,6 +1284,30 @@ void spmv(const flgpu_mat *A, double alpha, const void *x, double beta, const vo
   }
   if (A->prec == FLGPU_F64) launch_spmv<double>(A->ctx, A->sell, alpha, x, beta, z, out);
   else launch_spmv<float>(A->ctx, A->sell, alpha, x, beta, z, out);
+}
+
+// blocks of a per-cell pass over A (launch_q1_pass's grid)
+static long q1_blocks(const flgpu_mat *A, int qzb) {
+  long M = A->lat_n + 1;
+  long total = M * M * ((M + qzb - 1) / qzb);
+  return (long)xcd_grid(grid_for((total + 61) / 62 * 64, 256));
+}
+
+bool spmv_dot(const flgpu_mat *A, const void *x, void *out, int slot) {
+  if (!A->alpha || A->tp_kind) return false;
+  long nb = q1_blocks(A, A->prec == FLGPU_F64 ? FLGPU_QZB_F64_SPMV : FLGPU_QZB_F32);
+  flgpu_mat *M = const_cast<flgpu_mat *>(A);
+  if (M->dot_np < nb) {
+    dev_free(M->dot_part);
+    M->dot_part = (double *)dev_alloc(sizeof(double) * nb);
+    M->dot_np = nb;
+  }
+  if (A->prec == FLGPU_F64)
+    launch_q1_pass<double, Q1_SPMV>(A, 1.0, 0.0, out, x, out, nullptr, M->dot_part);
+  else
+    launch_q1_pass<float, Q1_SPMV>(A, 1.0f, 0.0f, out, x, out, nullptr, M->dot_part);
+  dot_final(A->ctx, M->dot_part, (int)nb, slot);
+  return true;
 }
 
 void spmv_t(const flgpu_mat *A, double alpha, const void *x, double beta, void *out) {
