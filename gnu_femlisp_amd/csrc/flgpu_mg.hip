@@ -530,25 +530,26 @@ __global__ void k_cg_p(long n, const double *a, const double *b, T *p, const T *
   p[i] = sc * p[i] + q[i];
 }
 
-// p <- q + ((zr - zrold)/alpha) p  (flexible CG, Polak-Ribiere)
+// p <- q + beta p, beta = -(q.a)/(a.p) (flexible CG): Polak-Ribiere's q.(r - r_old)/(q_old.r_old)
+// with r - r_old = -lam a and lam = (q_old.r_old)/(a.p), so that r_old need not be kept
 template <class T>
-__global__ void k_fcg_p(long n, const double *zr, const double *zro, const double *alpha, T *p,
+__global__ void k_fcg_p(long n, const double *qa, const double *ap, T *p,
                         const T *__restrict__ q) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  T beta = (T)((*zr - *zro) / *alpha);
+  T beta = (T)(-*qa / *ap);
   p[i] = q[i] + beta * p[i];
 }
 
-// <cg>'s update in one pass: rold <- r (FCG, when given), x <- x + lam p, r <- r - lam a with
-// lam = num / den rounded to T once per sign (femlisp's axpy! with (/ alpha beta))
+// <cg>'s update in one pass: x <- x + lam p, r <- r - lam a with lam = num / den rounded to T
+// once per sign (femlisp's axpy! with (/ alpha beta))
 // STAGE: also the next preconditioner call's staging pass on the new r (stage_first_step, for
 // an FP32 hierarchy under FP64 vectors): bs = (float)r, xo = damp (bs / diag), the values that
 // pass would write
 template <class T, bool STAGE = false>
 __global__ void k_cg_update(long n, const double *num, const double *den, const T *__restrict__ p,
                             const T *__restrict__ a, T *__restrict__ x, T *__restrict__ r,
-                            T *__restrict__ rold, float *__restrict__ bs = nullptr,
+                            float *__restrict__ bs = nullptr,
                             const float *__restrict__ diag = nullptr, float damp = 0.0f,
                             float *__restrict__ xo = nullptr) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -556,7 +557,6 @@ __global__ void k_cg_update(long n, const double *num, const double *den, const 
   double q = *num / *den;
   T lp = (T)(1.0 * q), lm = (T)(-1.0 * q);
   T ri = r[i];
-  if (rold) rold[i] = ri;
   x[i] = lp * p[i] + x[i];
   T rn = lm * a[i] + ri;
   r[i] = rn;
@@ -791,17 +791,18 @@ struct SolveWork {
   flgpu_ctx *ctx;
   long N;
   int prec;
+  // t: the true residual b - A x when it is checked
   void *r = nullptr, *p = nullptr, *a = nullptr, *w = nullptr, *w2 = nullptr, *q = nullptr,
-       *rold = nullptr;
+       *t = nullptr;
   SolveWork(flgpu_ctx *c, long n, int pr) : ctx(c), N(n), prec(pr) {
     size_t b = prec_bytes(pr) * n;
-    for (void **v : {&r, &p, &a, &w, &w2, &q, &rold}) {
+    for (void **v : {&r, &p, &a, &w, &w2, &q, &t}) {
       *v = dev_alloc(b);
       vfill(c, n, pr, *v, 0.0);
     }
   }
   ~SolveWork() {
-    for (void *v : {r, p, a, w, w2, q, rold}) dev_free(v);
+    for (void *v : {r, p, a, w, w2, q, t}) dev_free(v);
   }
 };
 
@@ -882,14 +883,14 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
       bool ok = (p->stop == FLGPU_STOP_REDUCTION) ? (reduction < p->tol) : (defnorm < p->tol);
       if (ok && p->true_residual && p->outer != FLGPU_OUTER_MG) {
         // confirm with the FP64 true residual; replace the recurrence if it lags
-        spmv(A, -1.0, x->d, 1.0, b->d, W.rold);
-        vdot_dev(ctx, N, prec, W.rold, W.rold, 5);
+        spmv(A, -1.0, x->d, 1.0, b->d, W.t);
+        vdot_dev(ctx, N, prec, W.t, W.t, 5);
         fetch_slots(ctx, 6);
         double tn = std::sqrt(ctx->host_out[5]);
         double tr = (tn == 0.0) ? 0.0 : tn / initial;
         ok = (p->stop == FLGPU_STOP_REDUCTION) ? (tr < p->tol) : (tn < p->tol);
         if (!ok) {
-          vcopy(ctx, N, prec, W.rold, prec, W.r);
+          vcopy(ctx, N, prec, W.t, prec, W.r);
           initialized = false;  // restart the recurrence from the true residual
           ap_ready = false;
         }
@@ -951,32 +952,31 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
               MGLevel &T = mg->lv[mg->top];
               k_cg_update<double, true><<<g, 256, 0, s>>>(
                   N, slot + ca, slot + 1, (const double *)W.p, (const double *)W.a,
-                  (double *)x->d, (double *)W.r, fcg ? (double *)W.rold : nullptr,
-                  (float *)mg->bs, (const float *)T.A->diag, (float)mg->p.omega, (float *)T.tmp);
+                  (double *)x->d, (double *)W.r, (float *)mg->bs, (const float *)T.A->diag,
+                  (float)mg->p.omega, (float *)T.tmp);
             } else if (prec == FLGPU_F64)
               k_cg_update<double><<<g, 256, 0, s>>>(N, slot + ca, slot + 1, (const double *)W.p,
                                                     (const double *)W.a, (double *)x->d,
-                                                    (double *)W.r,
-                                                    fcg ? (double *)W.rold : nullptr);
+                                                    (double *)W.r);
             else
               k_cg_update<float><<<g, 256, 0, s>>>(N, slot + ca, slot + 1, (const float *)W.p,
                                                    (const float *)W.a, (float *)x->d,
-                                                   (float *)W.r, fcg ? (float *)W.rold : nullptr);
+                                                   (float *)W.r);
             FLGPU_HIP(hipGetLastError());
             if (pc) precondition(mg, p, W, W.r, W.q, learned, staged);
             else vcopy(ctx, N, prec, W.r, prec, W.q);
             const int na = 2 - ca;  // the new alpha's slot
-            {  // new alpha = q.r, defect norm^2 = r.r, FCG's q.rold: one pass
-              const void *dx[3] = {W.q, W.r, W.q}, *dy[3] = {W.r, W.r, W.rold};
+            {  // new alpha = q.r, defect norm^2 = r.r, FCG's q.a: one pass
+              const void *dx[3] = {W.q, W.r, W.q}, *dy[3] = {W.r, W.r, W.a};
               const int slots[3] = {na, 3, 4};
               vdot_multi_dev(ctx, N, prec, fcg ? 3 : 2, dx, dy, slots);
             }
             if (fcg) {
               if (prec == FLGPU_F64)
-                k_fcg_p<double><<<g, 256, 0, s>>>(N, slot + na, slot + 4, slot + ca,
-                                                  (double *)W.p, (const double *)W.q);
+                k_fcg_p<double><<<g, 256, 0, s>>>(N, slot + 4, slot + 1, (double *)W.p,
+                                                  (const double *)W.q);
               else
-                k_fcg_p<float><<<g, 256, 0, s>>>(N, slot + na, slot + 4, slot + ca, (float *)W.p,
+                k_fcg_p<float><<<g, 256, 0, s>>>(N, slot + 4, slot + 1, (float *)W.p,
                                                  (const float *)W.q);
             } else {
               if (prec == FLGPU_F64)
@@ -1008,8 +1008,8 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
     }
     auto t1 = std::chrono::steady_clock::now();
     // true residual in the precision of A
-    spmv(A, -1.0, x->d, 1.0, b->d, W.rold);
-    double tn = std::sqrt(vdot_host(ctx, N, prec, W.rold, W.rold));
+    spmv(A, -1.0, x->d, 1.0, b->d, W.t);
+    double tn = std::sqrt(vdot_host(ctx, N, prec, W.t, W.t));
     if (report) {
       report->status = status;
       report->steps = step;
