@@ -89,15 +89,16 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int n, const double *__res
   double acc = 0.0;
   for (int j0 = 0; j0 < n; j0 += 64 * U) {
     double a[U], v[U];
+    // every load unconditional (the index clamped into the row) so that all are in flight
+    // together; the terms beyond the row are dropped from the sum, which keeps its order
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      int j = j0 + lane + 64 * u;
-      a[u] = j < n ? Ainv[(long)row * n + j] : 0.0;
-      v[u] = j < n ? (double)r[j] : 0.0;
+      int j = min(j0 + lane + 64 * u, n - 1);
+      a[u] = Ainv[(long)row * n + j];
+      v[u] = (double)r[j];
     }
 #pragma unroll
-    for (int u = 0; u < U; u++)
-      if (j0 + lane + 64 * u < n) acc += a[u] * v[u];
+    for (int u = 0; u < U; u++) acc = (j0 + lane + 64 * u < n) ? acc + a[u] * v[u] : acc;
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if (lane == 0) {
