@@ -75,6 +75,17 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partial(long n, const T *x,
   double acc = 0.0;
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long stride = (long)gridDim.x * blockDim.x;
+  // four strides' loads in flight together, summed in stride order
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    T xv[4], yv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      xv[u] = x[i + u * stride];
+      yv[u] = y[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc += (double)xv[u] * (double)yv[u];
+  }
   for (; i < n; i += stride) acc += (double)x[i] * (double)y[i];
   acc = wave_sum(acc);
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -101,6 +112,21 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partial_multi(long n, const
   for (int k = 0; k < K; k++) acc[k] = 0.0;
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long stride = (long)gridDim.x * blockDim.x;
+  // four strides' loads in flight together; each dot still sums in stride order
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    T xv[4][K], yv[4][K];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        xv[u][k] = xs[k][i + u * stride];
+        yv[u][k] = ys[k][i + u * stride];
+      }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int k = 0; k < K; k++) acc[k] += (double)xv[u][k] * (double)yv[u][k];
+  }
   for (; i < n; i += stride)
 #pragma unroll
     for (int k = 0; k < K; k++) acc[k] += (double)xs[k][i] * (double)ys[k][i];
