@@ -386,7 +386,8 @@ int flgpu_mg_attach_unet(flgpu_mg *mg, flgpu_unet *u, const flgpu_vec *kappa,
 #define FLGPU_OUTER_MG 0       /* the V-cycle is the iteration */
 #define FLGPU_OUTER_CG 1       /* <cg> with the V-cycle as preconditioner */
 #define FLGPU_OUTER_CG_PLAIN 2 /* <cg> without preconditioner */
-#define FLGPU_OUTER_FCG 3      /* flexible CG: beta = -(z.Ap)/(p.Ap) (Polak-Ribiere's form) */
+#define FLGPU_OUTER_FCG 3      /* flexible CG: beta = -(z.Ap)/(p.Ap), which equals Polak-Ribiere's
+                                  z.(r - r_old)/(z_old.r_old) */
 #define FLGPU_STOP_REDUCTION 0 /* success-if (< :reduction tol) */
 #define FLGPU_STOP_DEFNORM 1   /* success-if (< :defnorm tol) */
 
