@@ -144,6 +144,52 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partial_multi(long n, const
   }
 }
 
+// The CG/FCG step's dots q.r, r.r (and q.a for K = 3) with each vector loaded once: the same
+// products summed in the same order as k_dot_partial_multi over (q, r, q) x (r, r, a)
+template <typename T, int K>
+__global__ __launch_bounds__(kRedThreads) void k_dot_partial_qr(long n, const T *q, const T *r,
+                                                                const T *a, double *partial) {
+  __shared__ double s[K][kRedThreads / 64];
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) acc[k] = 0.0;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    T qv[4], rv[4], av[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      qv[u] = q[i + u * stride];
+      rv[u] = r[i + u * stride];
+      if (K == 3) av[u] = a[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      acc[0] += (double)qv[u] * (double)rv[u];
+      acc[1] += (double)rv[u] * (double)rv[u];
+      if (K == 3) acc[K - 1] += (double)qv[u] * (double)av[u];
+    }
+  }
+  for (; i < n; i += stride) {
+    double qd = (double)q[i], rd = (double)r[i];
+    acc[0] += qd * rd;
+    acc[1] += rd * rd;
+    if (K == 3) acc[K - 1] += qd * (double)a[i];
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    double v = wave_sum(acc[k]);
+    if (lane == 0) s[k][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double t = 0.0;
+    for (int j = 0; j < kRedThreads / 64; j++) t += s[threadIdx.x][j];
+    partial[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
 // block k reduces dot k's partials exactly as k_dot_final does, into out[slot[k]]
 __global__ __launch_bounds__(kRedBlocks) void k_dot_final_multi(const double *partial, int np,
                                                                  double *out, int s0, int s1,
@@ -259,11 +305,24 @@ void vdot_multi_dev(flgpu_ctx *ctx, long n, int prec, int count, const void *con
   const void *xx[3] = {x[0], count > 1 ? x[1] : x[0], count > 2 ? x[2] : x[0]};
   const void *yy[3] = {y[0], count > 1 ? y[1] : y[0], count > 2 ? y[2] : y[0]};
   hipStream_t st = ctx->stream;
+  // (q, r, q) x (r, r, a): the CG/FCG step's dots, each vector loaded once
+  const bool qr = count >= 2 && xx[1] == yy[0] && yy[1] == yy[0] && (count < 3 || xx[2] == xx[0]);
 #define FLGPU_DOTS(T, K)                                                                      \
   k_dot_partial_multi<T, K><<<kRedBlocks, kRedThreads, 0, st>>>(                              \
       n, (const T *)xx[0], (const T *)yy[0], (const T *)xx[1], (const T *)yy[1],              \
       (const T *)xx[2], (const T *)yy[2], ctx->red_partial)
-  if (prec == FLGPU_F64) {
+#define FLGPU_QR(T, K)                                                                        \
+  k_dot_partial_qr<T, K><<<kRedBlocks, kRedThreads, 0, st>>>(                                 \
+      n, (const T *)xx[0], (const T *)yy[0], (const T *)yy[2], ctx->red_partial)
+  if (qr) {
+    if (prec == FLGPU_F64) {
+      if (count == 2) FLGPU_QR(double, 2);
+      else FLGPU_QR(double, 3);
+    } else {
+      if (count == 2) FLGPU_QR(float, 2);
+      else FLGPU_QR(float, 3);
+    }
+  } else if (prec == FLGPU_F64) {
     if (count == 1) FLGPU_DOTS(double, 1);
     else if (count == 2) FLGPU_DOTS(double, 2);
     else FLGPU_DOTS(double, 3);
@@ -273,6 +332,7 @@ void vdot_multi_dev(flgpu_ctx *ctx, long n, int prec, int count, const void *con
     else FLGPU_DOTS(float, 3);
   }
 #undef FLGPU_DOTS
+#undef FLGPU_QR
   k_dot_final_multi<<<count, kRedBlocks, 0, st>>>(ctx->red_partial, kRedBlocks, ctx->red_out,
                                                   slots[0], count > 1 ? slots[1] : slots[0],
                                                   count > 2 ? slots[2] : slots[0]);
