@@ -265,9 +265,9 @@ void vaxpy(flgpu_ctx *ctx, long n, int prec, double a, const void *x, void *y);
 // result stays on device in ctx->red_out[slot]; dot_host syncs and returns it
 void vdot_dev(flgpu_ctx *ctx, long n, int prec, const void *x, const void *y,
               int slot);
-// 1 to 3 dots in one pass, each identical to vdot_dev's
 // the sum of np per-block partials into reduction slot `slot`
 void dot_final(flgpu_ctx *ctx, const double *partial, int np, int slot);
+// 1 to 3 dots in one pass, each identical to vdot_dev's
 void vdot_multi_dev(flgpu_ctx *ctx, long n, int prec, int count, const void *const *x,
                     const void *const *y, const int *slots);
 double vdot_host(flgpu_ctx *ctx, long n, int prec, const void *x,
