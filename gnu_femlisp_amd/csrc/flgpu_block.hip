@@ -358,7 +358,10 @@ void blocks_prepare(flgpu_mat *A) {
   FLGPU_REQUIRE(A->nrows == N && A->ncols == N, "matrix size differs from its lattice");
   if (g.W == 1) return;  // one-DOF blocks use the diagonal
   size_t bytes = (size_t)nv * g.W * g.W * prec_bytes(A->prec);
-  if (!A->blk_inv) A->blk_inv = dev_alloc(bytes);
+  if (!A->blk_inv) {
+    A->blk_inv = dev_alloc(bytes);
+    layout_changed(A);
+  }
   if (A->prec == FLGPU_F64)
     k_blk_invert<double><<<(unsigned)nv, 64, 0, A->ctx->stream>>>(
         g, A->rowptr, A->colidx, (const double *)A->vals, (double *)A->blk_inv);
@@ -369,6 +372,7 @@ void blocks_prepare(flgpu_mat *A) {
 }
 
 void blocks_drop(flgpu_mat *A) {
+  if (A->blk_inv) layout_changed(A);
   dev_free(A->blk_inv);
   A->blk_inv = nullptr;
 }

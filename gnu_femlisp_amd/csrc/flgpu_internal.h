@@ -182,9 +182,15 @@ struct flgpu_mat {
   // per-block partial sums of the per-cell product's fused dot (spmv_dot), dot_np entries
   double *dot_part = nullptr;
   long dot_np = 0;
+  // counts the changes of the buffers and forms the kernels read (a form dropped or built, a
+  // buffer freed or allocated, the colour subsets rebuilt, the stencil's symmetry changed);
+  // captured V-cycles address those buffers and are dropped when any count has moved
+  unsigned long layout = 0;
 };
 
 namespace flgpu {
+
+inline void layout_changed(flgpu_mat *A) { A->layout++; }
 
 // matrices (flgpu_sparse.hip)
 flgpu_mat *mat_new(flgpu_ctx *ctx, int nrows, int ncols, long nnz, int prec);

@@ -659,7 +659,10 @@ static void lattice_set_alpha(flgpu_mat *A, const flgpu_vec *kappa, int nk) {
     A->kt[b] = s;
   }
   long ncells = (long)n * n * n;
-  if (!A->alpha) A->alpha = dev_alloc(prec_bytes(A->prec) * ncells);
+  if (!A->alpha) {
+    A->alpha = dev_alloc(prec_bytes(A->prec) * ncells);
+    layout_changed(A);
+  }
   const double *kd = kappa ? (const double *)kappa->d : nullptr;
   if (A->prec == FLGPU_F64)
     k_alpha<double><<<grid_for(ncells, 256), 256, 0, A->ctx->stream>>>(ncells, 1.0 / n, kd,
@@ -742,10 +745,13 @@ void lattice_dirichlet_identity(flgpu_mat *A, int dim, int order, int n) {
   FLGPU_HIP(hipGetLastError());
 }
 
+// New values for a new kappa in the same pattern.  A matrix applied in the per-cell form gets
+// the new cell factors with its values (or drops that form when kappa is no longer constant
+// per cell: nk != n); a matrix whose form was capped below it stays capped.
 void lattice_reassemble(flgpu_mat *A, const flgpu_vec *kappa, int nk, int mode) {
   check_kappa(kappa, A->lat_dim, A->lat_n, nk);
   assemble_values(A, kappa, nk, mode, nullptr);
-  lattice_set_alpha(A, kappa, nk);
+  if (A->alpha) lattice_set_alpha(A, kappa, nk);
   mat_refresh_values(A);
 }
 
@@ -1026,10 +1032,8 @@ extern "C" int flgpu_lattice_reassemble(flgpu_mat *A, const flgpu_vec *kappa, in
     FLGPU_REQUIRE(A, "matrix must not be NULL");
     FLGPU_REQUIRE(A->lat_dim > 0, "matrix was not assembled on a lattice");
     FLGPU_REQUIRE(!kappa || kappa->ctx == A->ctx, "kappa belongs to another context");
-    check_kappa(kappa, A->lat_dim, A->lat_n, nk);
     CTX_LOCK(A->ctx);
-    assemble_values(A, kappa, nk, mode, nullptr);
-    mat_refresh_values(A);
+    lattice_reassemble(A, kappa, nk, mode);
     FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
     return 0;
   });

@@ -17,6 +17,8 @@ struct LearnedHook {
 // The hook of the network attached to mg (error if none is attached).
 std::unique_ptr<LearnedHook> learned_hook(flgpu_mg *mg, const flgpu_mat *A);
 void mg_apply_ptr(flgpu_mg *mg, int vprec, void *x, void *b, void *r);
+// the context a network was created on
+flgpu_ctx *unet_ctx(const flgpu_unet *u);
 // network attached to a hierarchy (flgpu_mg_attach_unet)
 void mg_unet(flgpu_mg *mg, flgpu_unet **u, const flgpu_vec **kappa, int *n);
 

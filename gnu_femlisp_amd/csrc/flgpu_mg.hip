@@ -69,6 +69,8 @@ struct flgpu_mg {
   };
   std::vector<Graph> graphs;
   int use_graphs = 1;
+  // the level matrices' layout counts when the cached graphs were captured
+  unsigned long graph_layout = 0;
 };
 
 namespace flgpu {
@@ -107,56 +109,87 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int n, const double *__res
   }
 }
 
-// Dense LU without pivoting (the coarse matrices are SPD after Dirichlet
-// elimination) and the inverse by forward/backward substitution.
-static std::vector<double> dense_inverse(int n, const std::vector<int> &rp,
-                                         const std::vector<int> &ci,
-                                         const std::vector<double> &v) {
-  std::vector<double> a((size_t)n * n, 0.0), inv((size_t)n * n, 0.0);
-  for (int r = 0; r < n; r++)
-    for (int e = rp[r]; e < rp[r + 1]; e++) a[(size_t)r * n + ci[e]] = v[e];
-  for (int k = 0; k < n; k++) {
-    double piv = a[(size_t)k * n + k];
-    FLGPU_REQUIRE(piv != 0.0, "coarse matrix is singular (zero pivot)");
-    for (int i = k + 1; i < n; i++) {
-      double l = a[(size_t)i * n + k] / piv;
-      a[(size_t)i * n + k] = l;
-      if (l != 0.0) {
-        double *ai = &a[(size_t)i * n], *ak = &a[(size_t)k * n];
-        for (int j = k + 1; j < n; j++) ai[j] -= l * ak[j];
-      }
+// ---- dense FP64 inverse on the GPU -------------------------------------------------------
+// Gauss-Jordan elimination without pivoting (the matrices inverted here -- coarse levels and
+// the <lu> fallback's system -- are SPD after the Dirichlet elimination), in place in the
+// textbook form: for each pivot k, row k is divided by a_kk (a_kk becomes 1 / a_kk) and
+// a_ik-multiples of it are subtracted from the other rows (a_ik becoming -a_ik / a_kk).
+// After the last pivot the array holds A^-1.  One launch per pivot reads one buffer and
+// writes the other, so that no element is read after it was overwritten.
+
+__global__ void k_dense_zero(long nn, double *M) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nn) M[i] = 0.0;
+}
+
+template <class T>
+__global__ void k_dense_scatter(int n, const int *__restrict__ rowptr,
+                                const int *__restrict__ colidx, const T *__restrict__ vals,
+                                double *M) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int e = rowptr[r]; e < rowptr[r + 1]; e++) M[(long)r * n + colidx[e]] = (double)vals[e];
+}
+
+__global__ __launch_bounds__(256) void k_gauss_jordan(int n, int k, const double *__restrict__ S,
+                                                      double *__restrict__ D, int *bad) {
+  // n <= 8192: every index fits 32 bits
+  unsigned idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (unsigned)n * (unsigned)n) return;
+  unsigned i = idx / (unsigned)n, j = idx - i * (unsigned)n;
+  double piv = S[(long)k * n + k];
+  if (idx == 0 && piv == 0.0) *bad = 1;
+  double rk = (j == (unsigned)k ? 1.0 : S[(long)k * n + j]) / piv;  // the new row k
+  double v = (i == (unsigned)k) ? rk : (j == (unsigned)k ? 0.0 : S[idx]) - S[(long)i * n + k] * rk;
+  D[idx] = v;
+}
+
+// A^-1 of a square matrix of at most 8192 rows as a new n x n FP64 device array (row-major)
+static double *dense_inverse_dev(const flgpu_mat *A) {
+  FLGPU_REQUIRE(A->nrows == A->ncols, "dense inverse of a non-square matrix");
+  FLGPU_REQUIRE(A->nrows <= 8192, "matrix too large for the dense inverse (> 8192 rows)");
+  int n = A->nrows;
+  long nn = (long)n * n;
+  hipStream_t s = A->ctx->stream;
+  double *M0 = (double *)dev_alloc(sizeof(double) * std::max<long>(nn, 1));
+  double *M1 = nullptr;
+  int *bad = nullptr;
+  try {
+    M1 = (double *)dev_alloc(sizeof(double) * std::max<long>(nn, 1));
+    bad = (int *)dev_alloc(sizeof(int));
+    FLGPU_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+    k_dense_zero<<<grid_for(nn, 256), 256, 0, s>>>(nn, M0);
+    if (A->prec == FLGPU_F64)
+      k_dense_scatter<double><<<grid_for(n, 256), 256, 0, s>>>(n, A->rowptr, A->colidx,
+                                                               (const double *)A->vals, M0);
+    else
+      k_dense_scatter<float><<<grid_for(n, 256), 256, 0, s>>>(n, A->rowptr, A->colidx,
+                                                              (const float *)A->vals, M0);
+    FLGPU_HIP(hipGetLastError());
+    for (int k = 0; k < n; k++) {
+      k_gauss_jordan<<<grid_for(nn, 256), 256, 0, s>>>(n, k, M0, M1, bad);
+      std::swap(M0, M1);
     }
+    FLGPU_HIP(hipGetLastError());
+    int hbad = 0;
+    FLGPU_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    FLGPU_HIP(hipStreamSynchronize(s));
+    FLGPU_REQUIRE(hbad == 0, "matrix is singular (zero pivot in the dense inverse)");
+  } catch (...) {
+    dev_free(M0);
+    dev_free(M1);
+    dev_free(bad);
+    throw;
   }
-  std::vector<double> col(n);
-  for (int c = 0; c < n; c++) {
-    for (int i = 0; i < n; i++) {
-      double s = (i == c) ? 1.0 : 0.0;
-      for (int j = 0; j < i; j++) s -= a[(size_t)i * n + j] * col[j];
-      col[i] = s;
-    }
-    for (int i = n - 1; i >= 0; i--) {
-      double s = col[i];
-      for (int j = i + 1; j < n; j++) s -= a[(size_t)i * n + j] * col[j];
-      col[i] = s / a[(size_t)i * n + i];
-    }
-    for (int i = 0; i < n; i++) inv[(size_t)i * n + c] = col[i];
-  }
-  return inv;
+  dev_free(M1);
+  dev_free(bad);
+  return M0;
 }
 
 static void coarse_setup(flgpu_mg *mg, MGLevel &L) {
-  const flgpu_mat *A = L.A;
-  FLGPU_REQUIRE(A->nrows <= 8192, "base level too large for the dense coarse solve (> 8192 DOFs)");
-  int n = A->nrows;
-  std::vector<int> rp(n + 1), ci(A->nnz);
-  std::vector<double> v(A->nnz);
-  FLGPU_REQUIRE(flgpu_csr_download(A, rp.data(), ci.data(), v.data()) == 0,
-                "coarse matrix download failed");
-  std::vector<double> inv = dense_inverse(n, rp, ci, v);
-  L.cinv = (double *)dev_alloc(sizeof(double) * (size_t)n * n);
-  FLGPU_HIP(hipMemcpyAsync(L.cinv, inv.data(), sizeof(double) * (size_t)n * n,
-                           hipMemcpyHostToDevice, mg->ctx->stream));
-  FLGPU_HIP(hipStreamSynchronize(mg->ctx->stream));
+  FLGPU_REQUIRE(L.A->nrows <= 8192,
+                "base level too large for the dense coarse solve (> 8192 DOFs)");
+  L.cinv = dense_inverse_dev(L.A);
 }
 
 static void coarse_apply(flgpu_mg *mg, MGLevel &L, void *x, void *r) {
@@ -418,12 +451,36 @@ static void mg_run(flgpu_mg *mg, int mode, int vprec, void *x, void *b, void *r)
 // mode 0: x <- x + V(r) on (x, b, r); mode 1: x <- V(b) from zero, b read only, r work; mode
 // 2: mode 1 with b already staged (mg_apply_zero's `staged`).
 // With graphs on, the first call on a (mode, x, b, r, precision) is captured and replayed.
+// Sum of the layout counts of every matrix a cycle reads.  The counts only grow, so the sum
+// moves whenever one of them does: a level matrix handed out by flgpu_mg_level_matrix whose
+// form, colouring or buffers the caller changed.
+static unsigned long mg_layout(const flgpu_mg *mg) {
+  unsigned long sum = 0;
+  auto add = [&](const flgpu_mat *M) {
+    if (!M) return;
+    sum += M->layout;
+    if (M->transpose) sum += M->transpose->layout;
+  };
+  for (const MGLevel &L : mg->lv) {
+    add(L.A);
+    add(L.I);
+    add(L.R);
+  }
+  return sum;
+}
+
 static void mg_run_cached(flgpu_mg *mg, int mode, int vprec, void *x, void *b, void *r) {
   if (!mg->use_graphs) {
     mg_run(mg, mode, vprec, x, b, r);
     return;
   }
   hipStream_t s = mg->ctx->stream;
+  unsigned long lay = mg_layout(mg);
+  if (lay != mg->graph_layout) {  // captured cycles may address freed or replaced buffers
+    FLGPU_HIP(hipStreamSynchronize(s));
+    graphs_clear(mg);
+    mg->graph_layout = lay;
+  }
   for (auto &g : mg->graphs)
     if (g.x == x && g.b == b && g.r == r && g.vprec == vprec && g.mode == mode) {
       FLGPU_HIP(hipGraphLaunch(g.exec, s));
@@ -723,6 +780,8 @@ extern "C" int flgpu_mg_attach_unet(flgpu_mg *mg, flgpu_unet *u, const flgpu_vec
     FLGPU_REQUIRE((mg->lat_n0 << mg->top) == n, "n does not match the top level");
     FLGPU_REQUIRE(!kappa || (kappa->prec == FLGPU_F64 && kappa->n == (long)n * n * n),
                   "kappa must be an FP64 vector of n^3 cell values");
+    FLGPU_REQUIRE(!u || unet_ctx(u) == mg->ctx, "the network belongs to another context");
+    FLGPU_REQUIRE(!kappa || kappa->ctx == mg->ctx, "kappa belongs to another context");
     mg->unet = u;
     mg->unet_kappa = kappa;
     mg->unet_n = n;
@@ -767,6 +826,8 @@ extern "C" int flgpu_mg_apply(flgpu_mg *mg, flgpu_vec *x, flgpu_vec *b, flgpu_ve
     FLGPU_REQUIRE(x->n == N && b->n == N && r->n == N, "vector length does not match the top level");
     FLGPU_REQUIRE(x->prec == b->prec && b->prec == r->prec, "vectors must share one precision");
     FLGPU_REQUIRE(x->d != r->d && x->d != b->d, "x must differ from b and r");
+    FLGPU_REQUIRE(x->ctx == mg->ctx && b->ctx == mg->ctx && r->ctx == mg->ctx,
+                  "x, b and r must belong to the hierarchy's context");
     CTX_LOCK(mg->ctx);
     mg_apply_cached(mg, x->prec, x->d, b->d, r->d);
     return 0;
@@ -843,6 +904,13 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
     FLGPU_REQUIRE(!(p->alias && mg && mg->p.prec != A->prec),
                   "alias mode needs the hierarchy in the precision of A");
     FLGPU_REQUIRE(!p->learned || p->outer == FLGPU_OUTER_FCG, "the learned correction needs FCG");
+    FLGPU_REQUIRE(p->restart_cycle >= 0, "restart_cycle must not be negative");
+    FLGPU_REQUIRE(p->fallback == FLGPU_FALLBACK_NONE || p->fallback == FLGPU_FALLBACK_LU,
+                  "unknown fallback iteration");
+    FLGPU_REQUIRE(p->fallback == FLGPU_FALLBACK_NONE || A->nrows <= 8192,
+                  "the <lu> fallback needs a system of at most 8192 unknowns");
+    FLGPU_REQUIRE(b->ctx == A->ctx && x->ctx == A->ctx, "b and x must belong to A's context");
+    FLGPU_REQUIRE(!mg || mg->ctx == A->ctx, "the hierarchy must belong to A's context");
     flgpu_ctx *ctx = A->ctx;
     CTX_LOCK(ctx);
     long N = A->nrows;
@@ -869,6 +937,10 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
     int fell_back = 0;
     double best_true = -1.0;
     int best_step = 0;
+    int count = 0;  // <cg>'s steps for restart-cycle (krylow.lisp:101-105,128-131)
+    // <safe-linear-solver> (linsolve.lisp:45-56,75-89): the <lu> fallback's inverse of A
+    bool fallback_p = false;
+    std::unique_ptr<double, void (*)(double *)> lu_inv(nullptr, [](double *q) { dev_free(q); });
     for (;;) {
       // intermediate :before: defect and its norm (solve.lisp:139-155)
       if (!residual_p) {
@@ -901,8 +973,37 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
         break;
       }
       if (step > p->maxsteps) {
-        status = 1;
-        break;
+        if (p->fallback == FLGPU_FALLBACK_NONE) {
+          status = 1;
+          break;
+        }
+        if (fallback_p) {  // "The fallback iteration has failed." (linsolve.lisp:84-85)
+          status = 2;
+          break;
+        }
+        // the iterator becomes <lu>'s; the step count and the residual at hand stay
+        fallback_p = true;
+        lu_inv.reset(dense_inverse_dev(A));
+      }
+      if (fallback_p) {  // <lu> iterate (linit.lisp:150-160): x += A^-1 r, residual-after nil
+        if (prec == FLGPU_F64)
+          k_coarse_solve<double><<<grid_for(N, 4), 256, 0, s>>>((int)N, lu_inv.get(),
+                                                                (const double *)W.r,
+                                                                (double *)x->d, 0);
+        else
+          k_coarse_solve<float><<<grid_for(N, 4), 256, 0, s>>>((int)N, lu_inv.get(),
+                                                               (const float *)W.r,
+                                                               (float *)x->d, 0);
+        FLGPU_HIP(hipGetLastError());
+        residual_p = false;
+        ap_ready = false;
+        step++;
+        continue;
+      }
+      // <cg> restarts at the start of the step after a completed restart cycle
+      if (p->restart_cycle > 0 && count > 0 && count % p->restart_cycle == 0) {
+        initialized = false;
+        ap_ready = false;
       }
       // learned stall fallback: plain GMG-FCG takes over when the defect has
       // not dropped 10x within stall_window steps
@@ -1001,7 +1102,9 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             vdot_dev(ctx, N, prec, W.r, W.r, 3);
             fetch_slots(ctx, 4);
           }
-          residual_p = true;
+          // a completed restart cycle leaves no :residual-after: the loop recomputes r
+          residual_p = !(p->restart_cycle > 0 && ++count % p->restart_cycle == 0);
+          if (!residual_p) ap_ready = false;
           break;
         }
       }
@@ -1019,6 +1122,7 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
       report->true_residual_norm = tn;
       report->fell_back = fell_back;
       report->seconds = std::chrono::duration<double>(t1 - t0).count();
+      report->fallback_used = fallback_p ? 1 : 0;
     }
     return status;
   });

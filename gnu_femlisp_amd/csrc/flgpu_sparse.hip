@@ -224,6 +224,7 @@ void mat_free(flgpu_mat *A) {
 
 // values no longer follow the per-cell form (new values or another ordering)
 void drop_alpha(flgpu_mat *A) {
+  if (A->alpha) layout_changed(A);
   dev_free(A->alpha);
   A->alpha = nullptr;
 }
@@ -325,6 +326,7 @@ void mat_build_transpose(const flgpu_mat *A) {
   upload_vals(T, tv.data());
   mat_finish(T);
   const_cast<flgpu_mat *>(A)->transpose = T;
+  layout_changed(const_cast<flgpu_mat *>(A));
 }
 
 flgpu_mat *mat_convert(const flgpu_mat *A, int prec) {
@@ -385,6 +387,7 @@ void set_colouring(flgpu_mat *A, int ncolours, const int *colour) {
                     "colouring couples two rows of one colour");
   for (auto &s : A->colour_sell) sell_free(s);
   A->colour_sell.clear();
+  layout_changed(A);
   for (int c = 0; c < ncolours; c++) A->colour_sell.push_back(sell_build(A, &rows[c]));
   A->ncolours = ncolours;
 }
@@ -963,10 +966,12 @@ void stencil_refresh(flgpu_mat *A) {
   FLGPU_HIP(hipMemcpyAsync(&h, asym, sizeof(int), hipMemcpyDeviceToHost, A->ctx->stream));
   FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
   dev_free(asym);
+  if (A->st_sym != (h == 0)) layout_changed(A);
   A->st_sym = (h == 0);
 }
 
 void drop_stencil(flgpu_mat *A) {
+  if (A->val27) layout_changed(A);
   dev_free(A->val27);
   A->val27 = nullptr;
   A->st_sym = false;
@@ -1157,7 +1162,10 @@ void tp_prepare(flgpu_mat *T) {
   if (T->tp_kind == 0) return;
   const int nc = T->tp_nc;
   const long Nc = (long)(nc + 1) * (nc + 1) * (nc + 1);
-  if (!T->tp27) T->tp27 = dev_alloc(27 * Nc * prec_bytes(T->prec));
+  if (!T->tp27) {
+    T->tp27 = dev_alloc(27 * Nc * prec_bytes(T->prec));
+    layout_changed(T);
+  }
   const unsigned g = grid_for(Nc, 256);
   if (T->tp_kind == 1) {
     if (T->prec == FLGPU_F64)
@@ -1179,6 +1187,7 @@ void tp_prepare(flgpu_mat *T) {
 
 void tp_drop(flgpu_mat *A) {
   if (A->tp_kind == 1 && A->transpose) tp_drop(A->transpose);
+  if (A->tp_kind || A->tp27) layout_changed(A);
   dev_free(A->tp27);
   A->tp27 = nullptr;
   A->tp_kind = 0;
@@ -1968,6 +1977,14 @@ extern "C" int flgpu_mat_storage(flgpu_mat *A, int max_form, int *form) {
       *form = A->alpha                   ? FLGPU_STORAGE_CELL
               : (A->val27 || A->tp_kind) ? FLGPU_STORAGE_STENCIL
                                          : FLGPU_STORAGE_SELL;
+    return 0;
+  });
+}
+
+extern "C" int flgpu_mat_stencil_planes(const flgpu_mat *A, int *planes) {
+  return guard([&] {
+    FLGPU_REQUIRE(A && planes, "arguments must not be NULL");
+    *planes = (A->alpha || !A->val27) ? 0 : (A->st_sym ? 14 : 27);
     return 0;
   });
 }

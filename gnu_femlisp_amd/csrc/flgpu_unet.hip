@@ -348,6 +348,8 @@ struct flgpu_unet {
 
 namespace flgpu {
 
+flgpu_ctx *unet_ctx(const flgpu_unet *u) { return u->ctx; }
+
 static long vox(int batch, int n) { return (long)batch * n * n * n; }
 
 static void unet_free_acts(flgpu_unet *u) {

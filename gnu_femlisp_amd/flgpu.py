@@ -21,6 +21,7 @@ OUTER_MG, OUTER_CG, OUTER_CG_PLAIN, OUTER_FCG = 0, 1, 2, 3
 STOP_REDUCTION, STOP_DEFNORM = 0, 1
 STORAGE_SELL, STORAGE_STENCIL, STORAGE_CELL = 0, 1, 2
 TRANSFER_GEOMETRIC, TRANSFER_BLACKBOX = 0, 1
+FALLBACK_NONE, FALLBACK_LU = 0, 1
 
 
 class FlgpuError(RuntimeError):
@@ -45,14 +46,15 @@ class SolveParams(ctypes.Structure):
     _fields_ = [("outer", ctypes.c_int), ("alias", ctypes.c_int), ("stop", ctypes.c_int),
                 ("tol", ctypes.c_double), ("maxsteps", ctypes.c_int),
                 ("true_residual", ctypes.c_int), ("learned", ctypes.c_int),
-                ("stall_window", ctypes.c_int)]
+                ("stall_window", ctypes.c_int), ("restart_cycle", ctypes.c_int),
+                ("fallback", ctypes.c_int)]
 
 
 class SolveReport(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("steps", ctypes.c_int),
                 ("initial_defnorm", ctypes.c_double), ("final_defnorm", ctypes.c_double),
                 ("true_residual_norm", ctypes.c_double), ("fell_back", ctypes.c_int),
-                ("seconds", ctypes.c_double)]
+                ("seconds", ctypes.c_double), ("fallback_used", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); every symbol of include/flgpu.h
@@ -84,6 +86,7 @@ SIGNATURES = {
     "flgpu_mat_convert": (_ci, [_vp, _ci, _P(_vp)]),
     "flgpu_mat_destroy": (_ci, [_vp]),
     "flgpu_mat_storage": (_ci, [_vp, _ci, _P(_ci)]),
+    "flgpu_mat_stencil_planes": (_ci, [_vp, _P(_ci)]),
     "flgpu_gemm_nn": (_ci, [_cd, _vp, _vp, _cd, _vp]),
     "flgpu_gemm_tn": (_ci, [_cd, _vp, _vp, _cd, _vp]),
     "flgpu_compute_residual": (_ci, [_vp, _vp, _vp, _vp]),

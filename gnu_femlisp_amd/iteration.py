@@ -89,7 +89,8 @@ def select_smoother(order):
 
 
 class LU(LinearIteration):
-    """<lu>: only as the coarse-grid iteration (dense FP64 inverse on the GPU)."""
+    """<lu>: the coarse-grid iteration and the fallback of a SafeLinearSolver
+    (dense FP64 inverse formed on the GPU)."""
 
 
 class GeometricCS(LinearIteration):
@@ -165,9 +166,13 @@ def fas(**kw):
 
 
 class CG(LinearIteration):
-    def __init__(self, preconditioner=None, alias=False):
+    """<cg> (krylow.lisp:67-132); restart_cycle as femlisp's slot: every
+    restart_cycle steps the residual is recomputed and the recurrence restarts."""
+
+    def __init__(self, preconditioner=None, alias=False, restart_cycle=None):
         self.preconditioner = preconditioner
         self.alias = alias
+        self.restart_cycle = restart_cycle
 
 
 class FCG(CG):
@@ -325,8 +330,28 @@ class LinearSolver:
             outer, alias = _f.OUTER_MG, 0
         else:
             raise TypeError(f"unsupported iteration {type(it).__name__}")
+        restart = int(getattr(it, "restart_cycle", None) or 0)
+        fallback = self.fallback_kind()
         return SolveParams(outer, alias, stop, float(tol), maxsteps, int(self.true_residual),
-                           int(self.learned), int(self.stall_window))
+                           int(self.learned), int(self.stall_window), restart, fallback)
+
+    def fallback_kind(self):
+        return _f.FALLBACK_NONE
+
+
+class SafeLinearSolver(LinearSolver):
+    """<safe-linear-solver> (linsolve.lisp:45-56,75-89): on failure the iteration
+    becomes the fallback, <lu> by default, and the loop goes on; a second failure
+    is an error (status "fallback-failed")."""
+
+    def __init__(self, iteration, fallback=None, **kw):
+        super().__init__(iteration, **kw)
+        self.fallback = LU() if fallback is None else fallback
+
+    def fallback_kind(self):
+        if isinstance(self.fallback, LU):
+            return _f.FALLBACK_LU
+        raise TypeError(f"no GPU fallback for {type(self.fallback).__name__}")
 
 
 def solve(solver, blackboard):
@@ -352,7 +377,8 @@ def solve(solver, blackboard):
     red = rep.final_defnorm / rep.initial_defnorm if rep.initial_defnorm else 0.0
     blackboard.update({
         "solution": x,
-        "status": "success" if st == 0 else "failure",
+        "status": {0: "success", 1: "failure"}.get(st, "fallback-failed"),
+        "fallback_used": bool(rep.fallback_used),
         "steps": steps,
         "defnorms": dn[: min(steps + 1, cap)].copy(),
         "defnorm": rep.final_defnorm,

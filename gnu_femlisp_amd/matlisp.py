@@ -171,6 +171,13 @@ class GpuSparseMatrix:
         check(lib().flgpu_mat_storage(self.h, int(max_form), ctypes.byref(form)))
         return form.value
 
+    def stencil_planes(self):
+        """value planes the stencil-form kernels stream per row (27, 14 when symmetric, 0 when
+        the matrix is not in the stencil form)"""
+        p = ctypes.c_int()
+        check(lib().flgpu_mat_stencil_planes(self.h, ctypes.byref(p)))
+        return p.value
+
     def colour_greedy(self):
         n = ctypes.c_int()
         check(lib().flgpu_mat_colour_greedy(self.h, ctypes.byref(n)))

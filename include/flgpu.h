@@ -116,6 +116,11 @@ int flgpu_mat_convert(const flgpu_mat *A, int prec, flgpu_mat **out);
 #define FLGPU_STORAGE_STENCIL 1
 #define FLGPU_STORAGE_CELL 2
 int flgpu_mat_storage(flgpu_mat *A, int max_form, int *form);
+/* Value planes the stencil-form kernels stream per row (no femlisp counterpart: byte
+ * accounting for measurements): 27, or 14 when the stencil is exactly symmetric
+ * (A(i, i + d) == A(i + d, i)) and the 13 lower coefficients are read from the mirrored upper
+ * planes of the neighbour rows; 0 when A is not in the stencil form. */
+int flgpu_mat_stencil_planes(const flgpu_mat *A, int *planes);
 int flgpu_mat_destroy(flgpu_mat *A);
 
 /* gemm-nn! alpha A y beta x : x <- beta x + alpha A y
@@ -402,7 +407,21 @@ typedef struct flgpu_solve_params {
   int learned;         /* 1: add the learned coarse correction (FCG only) */
   int stall_window;    /* learned: fall back to plain GMG-FCG when the true
                           residual has not dropped by 10x in this many steps */
+  int restart_cycle;   /* <cg>'s restart-cycle (krylow.lisp:68-70,101-105,128-131): 0 never;
+                          else every restart_cycle CG/FCG steps the step leaves no
+                          :residual-after, the loop recomputes r = b - A x and the next step
+                          restarts the recurrence from it */
+  int fallback;        /* FLGPU_FALLBACK_NONE: <linear-solver>; FLGPU_FALLBACK_LU:
+                          <safe-linear-solver> (linsolve.lisp:45-56,75-89): on the first
+                          failure the iteration becomes <lu> on A (linit.lisp:132-162:
+                          x <- x + A^-1 r with the residual at hand, residual-after nil;
+                          dense FP64 inverse formed on the GPU, at most 8192 unknowns) and the
+                          loop goes on with the same step count; a second failure returns
+                          status 2 (femlisp: "The fallback iteration has failed.") */
 } flgpu_solve_params;
+
+#define FLGPU_FALLBACK_NONE 0
+#define FLGPU_FALLBACK_LU 1
 
 typedef struct flgpu_solve_report {
   int status;                 /* 0 success, 1 failure (step limit) */
@@ -412,11 +431,14 @@ typedef struct flgpu_solve_report {
   double true_residual_norm;  /* ||b - A x||_2 in FP64 at exit */
   int fell_back;              /* learned: 1 if the plain fallback took over */
   double seconds;             /* wall time of the iteration loop */
+  int fallback_used;          /* <safe-linear-solver>: 1 if the fallback iteration ran */
 } flgpu_solve_report;
 
 /* Solve A x = b from x = 0 with femlisp's <linear-solver> loop.  mg may be
  * NULL for FLGPU_OUTER_CG_PLAIN.  defnorms (may be NULL) receives
- * ||r_k||_2 for k = 0..steps, at most max_defnorms values. */
+ * ||r_k||_2 for k = 0..steps, at most max_defnorms values.  Returns 0 on
+ * success, 1 on failure, 2 when a <safe-linear-solver>'s fallback failed too.
+ * mg, b and x must belong to A's context. */
 int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_params *p,
                 const flgpu_vec *b, flgpu_vec *x, flgpu_solve_report *report,
                 double *defnorms, int max_defnorms);

@@ -5,6 +5,7 @@
  * product library; see fl_oracle.h.  Each routine names the femlisp source it
  * restates (paths relative to the femlisp tree).
  */
+#include <omp.h>
 #include "fl_oracle.h"
 
 #include <math.h>
@@ -535,6 +536,9 @@ int flo_blackbox_interpolation(int nf, const int *rowptr, const int *colidx,
 void flo_spmv(int nrows, const int *rowptr, const int *colidx,
               const double *vals, double alpha, const double *y, double beta,
               double *x) {
+  /* rows in parallel, as femlisp's :parallel t row loop (sparse-matrix.lisp:397-405); every
+   * row's sum keeps its order */
+#pragma omp parallel for schedule(static) if (nrows > 4096)
   for (int i = 0; i < nrows; i++) {
     double acc = (beta == 0.0) ? 0.0 : beta * x[i];
     for (int e = rowptr[i]; e < rowptr[i + 1]; e++)
@@ -570,9 +574,26 @@ void flo_csr_transpose(int nrows, int ncols, const int *rowptr,
   free(cnt);
 }
 
+/* Host threads for the parallel loops (the CPU baseline times 1 and all cores). */
+void flo_set_threads(int n) { omp_set_num_threads(n > 0 ? n : omp_get_num_procs()); }
+int flo_get_threads(void) { return omp_get_max_threads(); }
+
+/* dot (sparse-vector.lisp:260-264 with with-accumulators, parallel-adaptions.lisp:30-42):
+ * partial sums over DOT_CHUNKS fixed index ranges, each in index order, added in range order.
+ * The ranges do not depend on the number of threads, so neither does the result. */
+#define DOT_CHUNKS 64
 static double dot(long n, const double *x, const double *y) {
+  double part[DOT_CHUNKS];
+  long len = (n + DOT_CHUNKS - 1) / DOT_CHUNKS;
+#pragma omp parallel for schedule(static) if (n > 65536)
+  for (int c = 0; c < DOT_CHUNKS; c++) {
+    double s = 0.0;
+    long lo = c * len, hi = lo + len < n ? lo + len : n;
+    for (long i = lo; i < hi; i++) s += x[i] * y[i];
+    part[c] = s;
+  }
   double s = 0.0;
-  for (long i = 0; i < n; i++) s += x[i] * y[i];
+  for (int c = 0; c < DOT_CHUNKS; c++) s += part[c];
   return s;
 }
 
@@ -830,6 +851,7 @@ static void compute_residual(const csr_t *A, const double *x, const double *b,
 /* <jacobi> iterate on <sparse-matrix> (linit.lisp:220-250): per row,
  * gesv! overwrites r_i with A_ii^-1 r_i, then x_i += damp * r_i. */
 static void jacobi_iterate(const level_t *L, double omega, double *x, double *r) {
+#pragma omp parallel for schedule(static) if (L->N > 4096)
   for (long i = 0; i < L->N; i++) {
     r[i] = r[i] / L->diag[i];
     x[i] += omega * r[i];
@@ -839,17 +861,30 @@ static void jacobi_iterate(const level_t *L, double omega, double *x, double *r)
 /* <sor> iterate on <sparse-matrix> (linit.lisp:288-323), store-p nil: for
  * each row key in the ordering, corr := r_i := b_i, corr += (-x_j) A_ij over
  * the row (diagonal included), gesv! corr /= A_ii, x_i += omega corr. */
-static void sor_iterate(const level_t *L, double omega, const long *ordering,
-                        double *x, const double *b, double *r) {
+static void sor_row(const level_t *L, double omega, long i, double *x, const double *b,
+                    double *r) {
   const csr_t *A = &L->A;
-  for (long k = 0; k < L->N; k++) {
-    long i = ordering[k];
-    double corr = b[i];
-    for (int e = A->rowptr[i]; e < A->rowptr[i + 1]; e++)
-      corr += (-x[A->colidx[e]]) * A->vals[e];
-    corr = corr / L->diag[i];
-    r[i] = corr;
-    x[i] += omega * corr;
+  double corr = b[i];
+  for (int e = A->rowptr[i]; e < A->rowptr[i + 1]; e++)
+    corr += (-x[A->colidx[e]]) * A->vals[e];
+  corr = corr / L->diag[i];
+  r[i] = corr;
+  x[i] += omega * corr;
+}
+
+/* seg (NULL for the index order): nseg runs of the ordering, one per colour; rows of one
+ * colour do not couple, so each run's rows are updated in parallel with the result of the
+ * sequential sweep (femlisp's <parallel-sor> idea, linit.lisp:325-376, on a colouring) */
+static void sor_iterate(const level_t *L, double omega, const long *ordering,
+                        const long *seg, int nseg, double *x, const double *b, double *r) {
+  if (!seg) {
+    for (long k = 0; k < L->N; k++) sor_row(L, omega, ordering[k], x, b, r);
+    return;
+  }
+  for (int c = 0; c < nseg; c++) {
+    long lo = seg[c], hi = seg[c + 1];
+#pragma omp parallel for schedule(static) if (hi - lo > 4096)
+    for (long k = lo; k < hi; k++) sor_row(L, omega, ordering[k], x, b, r);
   }
 }
 
@@ -857,16 +892,20 @@ static void sor_iterate(const level_t *L, double omega, const long *ordering,
  * <sor> :compare by lattice colour (coordinate parity for Q1, coordinate
  * mod 3 for Q2), then index (linit.lisp:258-263,293-295). */
 static long *make_ordering(int dim, int order, const level_t *L, int mode,
-                           int backwards) {
+                           int backwards, long **seg_out, int *nseg_out) {
   long *ord = (long *)malloc(sizeof(long) * L->N);
   int M = order * L->n + 1;
   int m = order + 1; /* colours per axis */
+  long *seg = NULL;
+  int ncol = 0;
   if (mode == FLO_ORDER_INDEX) {
     for (long i = 0; i < L->N; i++) ord[i] = i;
   } else {
-    int ncol = (int)ipow(m, dim);
+    ncol = (int)ipow(m, dim);
+    seg = (long *)malloc(sizeof(long) * (ncol + 1));
     long pos = 0;
-    for (int col = 0; col < ncol; col++)
+    for (int col = 0; col < ncol; col++) {
+      seg[col] = pos;
       for (long i = 0; i < L->N; i++) {
         int c[FLO_MAXDIM];
         unrank(i, dim, M, c);
@@ -874,13 +913,25 @@ static long *make_ordering(int dim, int order, const level_t *L, int mode,
         for (int k = 0; k < dim; k++) cc = cc * m + c[k] % m;
         if (cc == col) ord[pos++] = i;
       }
+    }
+    seg[ncol] = pos;
   }
-  if (backwards)
+  if (backwards) {
     for (long i = 0; i < L->N / 2; i++) {
       long t = ord[i];
       ord[i] = ord[L->N - 1 - i];
       ord[L->N - 1 - i] = t;
     }
+    if (seg) { /* the runs in reverse order: run c starts where run ncol-1-c ended */
+      long *rs = (long *)malloc(sizeof(long) * (ncol + 1));
+      for (int c = 0; c <= ncol; c++) rs[c] = L->N - seg[ncol - c];
+      free(seg);
+      seg = rs;
+    }
+  }
+  if (seg_out) *seg_out = seg;
+  else free(seg);
+  if (nseg_out) *nseg_out = ncol;
   return ord;
 }
 
@@ -980,13 +1031,51 @@ static void dense_gesv(int m, double *a, double *b) {
  *  both then solve A_BB y = r_B by gesv! on a copy (:179-185) and set x_B += omega y, where
  *  omega is <ssc>'s omega and <psc>'s damp (:139-142).
  * <psc> reads r as the caller left it (residual-before t) and never writes it. */
+static void one_block(const csr_t *A, const int *blk, int W, int ssc, double omega,
+                      double *x, const double *b, double *r, double *loc, double *y, int *ks);
+
+/* the colour of block q: the parity of its vertex coordinates divided by the order (the vertex
+ * is the centre slot of the key list) */
+static int block_colour(const int *keys, long q, int W, int dim, int order, int n) {
+  int M = order * n + 1, c[FLO_MAXDIM], cc = 0;
+  unrank(keys[q * W + (W - 1) / 2], dim, M, c);
+  for (int k = 0; k < dim; k++) cc = cc * 2 + (c[k] / order) % 2;
+  return cc;
+}
+
 static void block_iterate(const csr_t *A, const int *keys, long nb, int W, int ssc,
-                          double omega, int backwards, double *x, const double *b, double *r) {
-  double *loc = (double *)malloc(sizeof(double) * W * W);
-  double *y = (double *)malloc(sizeof(double) * W);
-  int *ks = (int *)malloc(sizeof(int) * W);
-  for (long q = 0; q < nb; q++) {
-    const int *blk = keys + (backwards ? nb - 1 - q : q) * W;
+                          double omega, int backwards, double *x, const double *b, double *r,
+                          int dim, int order, int n) {
+  /* runs of blocks of one colour (flo_vertex_blocks sorts by colour): their blocks neither
+   * share a key nor read one another's keys, so each run is swept in parallel with the
+   * result of the sequential sweep */
+  long lo = 0;
+  while (lo < nb) {
+    long qlo = backwards ? nb - 1 - lo : lo;
+    int col = block_colour(keys, qlo, W, dim, order, n);
+    long hi = lo + 1;
+    while (hi < nb && block_colour(keys, backwards ? nb - 1 - hi : hi, W, dim, order, n) == col)
+      hi++;
+#pragma omp parallel if (hi - lo > 256)
+    {
+      double *loc = (double *)malloc(sizeof(double) * W * W);
+      double *y = (double *)malloc(sizeof(double) * W);
+      int *ks = (int *)malloc(sizeof(int) * W);
+#pragma omp for schedule(static)
+      for (long q = lo; q < hi; q++)
+        one_block(A, keys + (backwards ? nb - 1 - q : q) * W, W, ssc, omega, x, b, r, loc, y,
+                  ks);
+      free(loc);
+      free(y);
+      free(ks);
+    }
+    lo = hi;
+  }
+}
+
+static void one_block(const csr_t *A, const int *blk, int W, int ssc, double omega,
+                      double *x, const double *b, double *r, double *loc, double *y, int *ks) {
+  {
     int m = 0;
     for (int s = 0; s < W; s++)
       if (blk[s] >= 0) ks[m++] = blk[s];
@@ -1005,9 +1094,6 @@ static void block_iterate(const csr_t *A, const int *keys, long nb, int W, int s
     dense_gesv(m, loc, y);
     for (int i = 0; i < m; i++) x[ks[i]] += omega * y[i];
   }
-  free(loc);
-  free(y);
-  free(ks);
 }
 
 int flo_block_iterate(int nrows, const int *rowptr, const int *colidx, const double *vals,
@@ -1019,7 +1105,7 @@ int flo_block_iterate(int nrows, const int *rowptr, const int *colidx, const dou
   int *keys = (int *)malloc(sizeof(int) * nb * W);
   flo_vertex_blocks(dim, order, n, keys);
   csr_t A = {nrows, nrows, (int *)rowptr, (int *)colidx, (double *)vals};
-  block_iterate(&A, keys, nb, W, ssc, omega, backwards, x, b, r);
+  block_iterate(&A, keys, nb, W, ssc, omega, backwards, x, b, r, dim, order, n);
   free(keys);
   return 0;
 }
@@ -1063,6 +1149,8 @@ typedef struct {
   flo_mg *mg;
   const flo_params *p;
   long **pre_ord, **post_ord;
+  long **pre_seg, **post_seg; /* colour runs of the orderings (NULL: index order) */
+  int nseg;
   int **blk;   /* vertex-centred blocks per level (block smoothers) */
   long *nblk;
   int blk_w;
@@ -1094,12 +1182,15 @@ static void smoother_run(mgrun_t *R, int l, int post) {
     }
     for (int k = 0; k < steps; k++) {
       block_iterate(&L->A, R->blk[l], R->nblk[l], R->blk_w, p->smoother == FLO_SMOOTH_SSC,
-                    p->omega, post && p->symmetric, L->sol, L->rhs, L->res);
+                    p->omega, post && p->symmetric, L->sol, L->rhs, L->res, R->mg->dim,
+                    R->mg->order, L->n);
       if (steps > 1) compute_residual(&L->A, L->sol, L->rhs, L->res);
     }
   } else {
     long *ord = post ? R->post_ord[l] : R->pre_ord[l];
-    for (int k = 0; k < steps; k++) sor_iterate(L, p->omega, ord, L->sol, L->rhs, L->res);
+    long *seg = post ? R->post_seg[l] : R->pre_seg[l];
+    for (int k = 0; k < steps; k++)
+      sor_iterate(L, p->omega, ord, seg, R->nseg, L->sol, L->rhs, L->res);
   }
   /* residual-after nil for every smoother */
   L->residual_p = 0;
@@ -1178,12 +1269,20 @@ static void mgrun_init(mgrun_t *R, flo_mg *mg, const flo_params *p) {
   R->top = mg->levels - 1;
   R->pre_ord = (long **)calloc(mg->levels, sizeof(long *));
   R->post_ord = (long **)calloc(mg->levels, sizeof(long *));
+  R->pre_seg = (long **)calloc(mg->levels, sizeof(long *));
+  R->post_seg = (long **)calloc(mg->levels, sizeof(long *));
+  R->nseg = 0;
   if (p->smoother == FLO_SMOOTH_SOR)
     for (int l = p->base_level + 1; l < mg->levels; l++) {
-      R->pre_ord[l] = make_ordering(mg->dim, mg->order, &mg->lv[l], p->ordering, 0);
-      R->post_ord[l] = p->symmetric
-                           ? make_ordering(mg->dim, mg->order, &mg->lv[l], p->ordering, 1)
-                           : R->pre_ord[l];
+      R->pre_ord[l] = make_ordering(mg->dim, mg->order, &mg->lv[l], p->ordering, 0,
+                                    &R->pre_seg[l], &R->nseg);
+      if (p->symmetric) {
+        R->post_ord[l] = make_ordering(mg->dim, mg->order, &mg->lv[l], p->ordering, 1,
+                                       &R->post_seg[l], &R->nseg);
+      } else {
+        R->post_ord[l] = R->pre_ord[l];
+        R->post_seg[l] = R->pre_seg[l];
+      }
     }
   R->blk = (int **)calloc(mg->levels, sizeof(int *));
   R->nblk = (long *)calloc(mg->levels, sizeof(long));
@@ -1203,10 +1302,14 @@ static void mgrun_free(mgrun_t *R) {
   for (int l = 0; l < R->mg->levels; l++) {
     if (R->post_ord[l] != R->pre_ord[l]) free(R->post_ord[l]);
     free(R->pre_ord[l]);
+    if (R->post_seg[l] != R->pre_seg[l]) free(R->post_seg[l]);
+    free(R->pre_seg[l]);
     free(R->blk[l]);
   }
   free(R->pre_ord);
   free(R->post_ord);
+  free(R->pre_seg);
+  free(R->post_seg);
   free(R->blk);
   free(R->nblk);
 }
@@ -1264,7 +1367,19 @@ int flo_mg_apply(flo_mg *mg, const flo_params *p, const double *b, double *x) {
 /* <linear-solver> loop (iterate.lisp:239-250; solve.lisp:126-155;
  * linsolve.lisp:58-73; problem.lisp:109-117) around the iteration chosen by
  * p->outer: the V-cycle itself, <cg> with the V-cycle as preconditioner
- * (krylow.lisp:72-132), plain <cg>, or flexible CG (Polak-Ribiere beta). */
+ * (krylow.lisp:72-132), plain <cg>, or flexible CG (Polak-Ribiere beta).
+ *
+ * restart_cycle restates <cg>'s restart-cycle (krylow.lisp:101-105,128-131): the
+ * iterator counts its steps, and when the count reaches a multiple of the cycle the
+ * step returns no :residual-after flag, so the loop recomputes r = b - A x
+ * (solve.lisp:139-145), and the next step restarts the recurrence from it.
+ *
+ * fallback restates <safe-linear-solver> (linsolve.lisp:45-56,75-89): when the
+ * failure test holds the first time, the iterator is replaced by the fallback's,
+ * <lu> (linit.lisp:132-162: r <- A^-1 r, x <- x + r, residual-after nil), and the
+ * loop goes on with the same blackboard (step count, defnorms, the residual the
+ * failed iteration left); a second failure is femlisp's "The fallback iteration has
+ * failed." error, status 2 here. */
 int flo_mg_solve(flo_mg *mg, const flo_params *p, const double *b, double *x,
                  int *steps_out, double *defnorms) {
   if (p->base_level < 0 || p->base_level >= mg->levels) return -1;
@@ -1283,6 +1398,10 @@ int flo_mg_solve(flo_mg *mg, const flo_params *p, const double *b, double *x,
   double *rold = (double *)calloc(N, sizeof(double));
   memset(x, 0, sizeof(double) * N);
   int residual_p = 0, step = 0, initialized = 0, status = 1;
+  int count = 0;      /* <cg>'s step count for the restart cycle */
+  int fallback_p = 0; /* the <lu> fallback iterator is in use */
+  level_t F;          /* the top-level matrix's LU for the fallback */
+  memset(&F, 0, sizeof F);
   double initial = -1.0, alpha = 0.0;
   for (;;) {
     /* intermediate :before */
@@ -1300,9 +1419,30 @@ int flo_mg_solve(flo_mg *mg, const flo_params *p, const double *b, double *x,
       break;
     }
     if (step > p->maxsteps) {
-      status = 1;
-      break;
+      if (p->fallback == FLO_FALLBACK_NONE) {
+        status = 1;
+        break;
+      }
+      if (fallback_p) { /* "The fallback iteration has failed." */
+        status = 2;
+        break;
+      }
+      /* change to the fallback iteration; nothing else on the blackboard changes */
+      fallback_p = 1;
+      F = *T;
+      F.lu = NULL;
+      coarse_factor(&F);
     }
+    if (fallback_p) { /* the <lu> iterate on the residual at hand */
+      coarse_solve(&F, r);
+      #pragma omp parallel for simd if (N > 65536)
+      for (long i = 0; i < N; i++) x[i] += r[i];
+      residual_p = 0;
+      step++;
+      continue;
+    }
+    /* <cg>'s restart at the start of a step after a completed cycle */
+    if (p->restart_cycle > 0 && count > 0 && count % p->restart_cycle == 0) initialized = 0;
     /* next-step */
     switch (p->outer) {
       case FLO_OUTER_MG:
@@ -1322,16 +1462,19 @@ int flo_mg_solve(flo_mg *mg, const flo_params *p, const double *b, double *x,
           flo_spmv((int)N, T->A.rowptr, T->A.colidx, T->A.vals, 1.0, pv, 0.0, a);
           double beta = dot(N, a, pv);
           double lam = alpha / beta;
+          #pragma omp parallel for simd if (N > 65536)
           for (long i = 0; i < N; i++) x[i] += lam * pv[i];
+          #pragma omp parallel for simd if (N > 65536)
           for (long i = 0; i < N; i++) r[i] += (-lam) * a[i];
           if (pc) precond(&R, N, r, q, w, w2);
           else memcpy(q, r, sizeof(double) * N);
           double new_alpha = dot(N, q, r);
           double sc = new_alpha / alpha;
+          #pragma omp parallel for simd if (N > 65536)
           for (long i = 0; i < N; i++) pv[i] = sc * pv[i] + q[i];
           alpha = new_alpha;
         }
-        residual_p = 1;
+        residual_p = !(p->restart_cycle > 0 && ++count % p->restart_cycle == 0);
         break;
       }
       case FLO_OUTER_FCG: {
@@ -1344,16 +1487,19 @@ int flo_mg_solve(flo_mg *mg, const flo_params *p, const double *b, double *x,
         if (alpha != 0.0) {
           flo_spmv((int)N, T->A.rowptr, T->A.colidx, T->A.vals, 1.0, pv, 0.0, a);
           double lam = alpha / dot(N, pv, a);
+          #pragma omp parallel for simd if (N > 65536)
           for (long i = 0; i < N; i++) x[i] += lam * pv[i];
           memcpy(rold, r, sizeof(double) * N);
+          #pragma omp parallel for simd if (N > 65536)
           for (long i = 0; i < N; i++) r[i] += (-lam) * a[i];
           precond(&R, N, r, q, w, w2);
           double zr = dot(N, q, r);
           double beta = (zr - dot(N, q, rold)) / alpha;
+          #pragma omp parallel for simd if (N > 65536)
           for (long i = 0; i < N; i++) pv[i] = q[i] + beta * pv[i];
           alpha = zr;
         }
-        residual_p = 1;
+        residual_p = !(p->restart_cycle > 0 && ++count % p->restart_cycle == 0);
         break;
       }
       default:
@@ -1364,6 +1510,7 @@ int flo_mg_solve(flo_mg *mg, const flo_params *p, const double *b, double *x,
   }
 done:
   if (steps_out) *steps_out = step;
+  free(F.lu);
   free(bb);
   free(r);
   free(pv);

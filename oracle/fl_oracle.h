@@ -117,6 +117,12 @@ int flo_block_iterate(int nrows, const int *rowptr, const int *colidx, const dou
                       int dim, int order, int n, int ssc, double omega, int backwards,
                       double *x, const double *b, double *r);
 
+/* Host threads of the parallel loops (n <= 0: all cores); the results do not depend on it:
+ * rows, one colour's <sor> rows and one colour's blocks run in parallel, each with its own
+ * sequential arithmetic, and dots add fixed index ranges in a fixed order. */
+void flo_set_threads(int n);
+int flo_get_threads(void);
+
 /* ---- multigrid hierarchy + solvers (src/iteration/ multigrid, linit, krylow) */
 typedef struct flo_mg flo_mg;
 
@@ -171,13 +177,23 @@ typedef struct flo_params {
                        262-287): restrict first, then smooth; post-smooth, then prolongate */
   int fas;          /* 0 correction scheme, 1 Brandt's FAS (<geometric-fas>, geomg.lisp:174-194;
                        restrict/prolongate methods multigrid.lisp:226-247) */
+  int restart_cycle; /* <cg> restart-cycle (krylow.lisp:68-70,101-105,128-131): 0 never, else
+                       every restart_cycle steps the residual is recomputed from x and the
+                       recurrence restarts from it */
+  int fallback;     /* 0 <linear-solver>; FLO_FALLBACK_LU: <safe-linear-solver>
+                       (linsolve.lisp:45-56,75-89) whose fallback iteration, taken on failure,
+                       is <lu> (linit.lisp:132-162) on the top-level matrix */
 } flo_params;
+
+enum { FLO_FALLBACK_NONE = 0, FLO_FALLBACK_LU = 1 };
 
 /* Solve A x = b from x = 0 with femlisp's <linear-solver> loop
  * (src/iteration/iterate.lisp:239-250, solve.lisp:126-165,
  * linsolve.lisp:58-73).  defnorms[0..steps] receives ||r_k||_2 (the norm
  * femlisp observes: recurrence residual for CG, true residual otherwise).
- * Returns 0 on success, 1 on failure (step limit), <0 on argument errors. */
+ * Returns 0 on success, 1 on failure (step limit), 2 when the fallback
+ * iteration of a <safe-linear-solver> has failed too (femlisp signals an
+ * error, linsolve.lisp:84-85), <0 on argument errors. */
 int flo_mg_solve(flo_mg *mg, const flo_params *p, const double *b, double *x,
                  int *steps, double *defnorms);
 

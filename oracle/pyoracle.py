@@ -16,6 +16,7 @@ OUTER_MG, OUTER_CG, OUTER_CG_PLAIN, OUTER_FCG = 0, 1, 2, 3
 SMOOTH_JACOBI, SMOOTH_SOR, SMOOTH_SSC, SMOOTH_PSC = 0, 1, 2, 3
 ORDER_INDEX, ORDER_COLOUR = 0, 1
 STOP_REDUCTION, STOP_DEFNORM = 0, 1
+FALLBACK_NONE, FALLBACK_LU = 0, 1
 
 
 class Params(ctypes.Structure):
@@ -35,14 +36,27 @@ class Params(ctypes.Structure):
         ("gamma", ctypes.c_int),
         ("additive", ctypes.c_int),
         ("fas", ctypes.c_int),
+        ("restart_cycle", ctypes.c_int),
+        ("fallback", ctypes.c_int),
     ]
 
 
 def params(outer=OUTER_MG, smoother=SMOOTH_JACOBI, omega=1.0, ordering=ORDER_INDEX,
            pre_steps=1, post_steps=1, base_level=0, symmetric=0, alias=0,
-           stop=STOP_REDUCTION, tol=1e-10, maxsteps=100, gamma=1, additive=0, fas=0):
+           stop=STOP_REDUCTION, tol=1e-10, maxsteps=100, gamma=1, additive=0, fas=0,
+           restart_cycle=0, fallback=FALLBACK_NONE):
     return Params(outer, smoother, omega, ordering, pre_steps, post_steps,
-                  base_level, symmetric, alias, stop, tol, maxsteps, gamma, additive, fas)
+                  base_level, symmetric, alias, stop, tol, maxsteps, gamma, additive, fas,
+                  restart_cycle, fallback)
+
+
+def set_threads(n):
+    """host threads of the oracle's parallel loops (n <= 0: all cores)"""
+    lib().flo_set_threads(int(n))
+
+
+def get_threads():
+    return lib().flo_get_threads()
 
 
 def lib():
@@ -54,6 +68,8 @@ def lib():
         L = ctypes.CDLL(path)
         P = ctypes.POINTER
         ci, cl, cd, vp = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_void_p
+        L.flo_set_threads.argtypes = [ci]
+        L.flo_get_threads.restype = ci
         L.flo_lattice_ndofs.restype = cl
         L.flo_lattice_ndofs.argtypes = [ci, ci, ci]
         L.flo_lattice_nnz.restype = cl

@@ -11,6 +11,11 @@
   this size, so the result is checked through size-independent properties: the
   FP64 true residual, and agreement with the GPU's plain FP64 GMG-CG (whose
   parity with the oracle the smaller cases pin) to 1e-9.
+- Deploy: 3D Q1, 128^3 cells, two-phase inclusions (contrast 1e4, 30 % volume fraction,
+  radii 3-8 cells, SplitMix64 seed 2024): femlisp's plain FP64 GMG-CG follows the oracle's
+  defect history over its first steps (the oracle needs about 0.5 s per step here), and the
+  plain and the mixed black-box Jacobi paths both reach an FP64 true residual below 1e-10 and
+  agree to 1e-9.
 """
 import sys
 
@@ -20,6 +25,7 @@ import pytest
 from gnu_femlisp_amd import flgpu as F
 from gnu_femlisp_amd import iteration as it
 from gnu_femlisp_amd import lattice as lat
+from gnu_femlisp_amd.fields import inclusions
 from gnu_femlisp_amd.matlisp import GpuVector
 from oracle import pyoracle as po
 
@@ -126,3 +132,47 @@ def test_config_q2_48_block_ssc():
     say(f"  ssc: FP64 CG {steps[0]} steps, FP32-in-FCG {steps[1]} steps")
     assert steps[0] <= 40 and steps[1] <= 40
     assert relerr(sols[1], sols[0]) < 1e-9
+
+
+def test_config_deploy_128_inclusions():
+    say("config: 3D Q1 128^3 two-phase inclusions, contrast 1e4")
+    n, levels, base = 128, 8, 3
+    kap = inclusions(n)
+    assert abs(np.mean(kap > 1) - 0.30) < 0.01
+    # (a) femlisp's plain FP64 GMG-CG, V(2,2) multicolour GS, over its first 5 steps
+    oh = po.Hierarchy(3, 1, 1, levels, kappa=kap)
+    assert oh.ndofs() == 2146689
+    b = oh.rhs()
+    prm = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_SOR, omega=1.0,
+                    ordering=po.ORDER_COLOUR, pre_steps=2, post_steps=2, base_level=base,
+                    symmetric=1, alias=0, tol=1e-10, maxsteps=4)
+    st, xo, so, dno = oh.solve(prm, b)
+    del oh
+    assert st == 1 and so == 5
+    cs = it.GeometricCS(it.SOR(1.0), 2, 2, base, symmetric=True)
+    H = it.Hierarchy.lattice(cs, 3, 1, 1, levels, kappa=kap)
+    A, bg = H.top_matrix(), H.rhs
+    assert np.allclose(bg.numpy(), b, rtol=1e-13, atol=0)
+    first = it.solve(it.LinearSolver(it.CG(cs), success_if=("<", "reduction", 1e-10),
+                                     failure_if=(">", "step", 4)),
+                     {"matrix": A, "rhs": bg, "hierarchy": H})
+    say(f"  first steps: gpu {first['defnorms']}, oracle {dno}")
+    assert first["steps"] == so
+    assert np.allclose(first["defnorms"], dno, rtol=1e-8)
+    assert relerr(first["solution"].numpy(), xo) < 1e-8
+    # (b) plain FP64 GMG-CG and the value path (FP32 black-box Jacobi V(2,2) cycle in FP64
+    # FCG) to an FP64 true residual below 1e-10
+    plain = it.solve(it.LinearSolver(it.CG(cs), success_if=("<", "reduction", 1e-10),
+                                     failure_if=(">", "step", 600), true_residual=True),
+                     {"matrix": A, "rhs": bg, "hierarchy": H})
+    bn = np.linalg.norm(b)
+    assert plain["status"] == "success" and plain["true_residual"] / bn < 1e-10
+    cs32 = it.GeometricCS(it.Jacobi(0.9), 2, 2, base, prec=F.F32, transfer="blackbox")
+    H32 = it.Hierarchy.lattice(cs32, 3, 1, 1, levels, kappa=kap, mode=F.ASM_MFMA)
+    mixed = it.solve(it.LinearSolver(it.FCG(cs32), success_if=("<", "reduction", 1e-10),
+                                     failure_if=(">", "step", 600), true_residual=True),
+                     {"matrix": A, "rhs": bg, "hierarchy": H32})
+    say(f"  plain {plain['steps']} steps, mixed-bb-jac {mixed['steps']} steps")
+    assert mixed["status"] == "success" and mixed["true_residual"] / bn < 1e-10
+    assert mixed["steps"] < plain["steps"]
+    assert relerr(mixed["solution"].numpy(), plain["solution"].numpy()) < 1e-9

@@ -511,6 +511,106 @@ def test_solution_matches_direct_solve():
     assert np.linalg.norm(x - u) / np.linalg.norm(u) < 1e-10
 
 
+# ---- <cg> restart-cycle (krylow.lisp:68-70,101-105,128-131) -------------------------------
+
+def _steepest_descent(A, b, steps):
+    """textbook steepest descent from zero with the true residual each step:
+    x += (r.r / r.Ar) r, r = b - A x; returns the residual norms ||r_0..r_steps||"""
+    x = np.zeros_like(b)
+    out = []
+    for _ in range(steps + 1):
+        r = b - A @ x
+        out.append(np.linalg.norm(r))
+        x = x + (r @ r) / (r @ (A @ r)) * r
+    return np.array(out)
+
+
+def test_cg_restart_cycle_one_is_steepest_descent():
+    """With restart-cycle 1 every step restarts from the recomputed residual: plain <cg>
+    becomes steepest descent (a dropped restart or a missing residual refresh differs)."""
+    h = po.Hierarchy(2, 1, 1, 4)
+    A = csr(*h.matrix())
+    b = h.rhs()
+    p = po.params(outer=po.OUTER_CG_PLAIN, tol=1e-30, maxsteps=11, restart_cycle=1)
+    st, x, steps, dn = h.solve(p, b)
+    assert st == 1 and steps == 12
+    ref = _steepest_descent(A, b, 12)
+    assert np.allclose(dn, ref[: len(dn)], rtol=1e-10, atol=0)
+    # without the restart the same solver is CG, which is much faster here
+    _, _, _, dcg = h.solve(po.params(outer=po.OUTER_CG_PLAIN, tol=1e-30, maxsteps=11), b)
+    assert dcg[-1] < 1e-3 * dn[-1]
+
+
+@pytest.mark.parametrize("outer", [po.OUTER_CG, po.OUTER_CG_PLAIN])
+def test_cg_restart_is_cg_on_the_residual_equation(outer):
+    """After a restart cycle of k steps, CG starts afresh on A d = b - A x_k: two cycles
+    give x_k + (k CG steps on the residual equation from zero)."""
+    k = 3
+    h = po.Hierarchy(3, 1, 1, 4)
+    A = csr(*h.matrix())
+    b = h.rhs()
+    kw = dict(outer=outer, smoother=po.SMOOTH_JACOBI, omega=0.6, pre_steps=2, post_steps=2,
+              base_level=1, tol=1e-30)
+    # maxsteps m stops after m + 1 steps
+    _, xk, sk, _ = h.solve(po.params(maxsteps=k - 1, **kw), b)
+    _, d, _, _ = h.solve(po.params(maxsteps=k - 1, **kw), b - A @ xk)
+    _, x2k, s2k, dn = h.solve(po.params(maxsteps=2 * k - 1, restart_cycle=k, **kw), b)
+    assert sk == k and s2k == 2 * k
+    assert np.linalg.norm(x2k - (xk + d)) <= 1e-12 * np.linalg.norm(x2k)
+    # the defect observed after the cycle is the recomputed true residual
+    assert np.isclose(dn[k], np.linalg.norm(b - A @ xk), rtol=1e-12)
+    # a cycle longer than the run changes nothing
+    _, x1, _, d1 = h.solve(po.params(maxsteps=2 * k - 1, **kw), b)
+    _, x2, _, d2 = h.solve(po.params(maxsteps=2 * k - 1, restart_cycle=50, **kw), b)
+    assert np.array_equal(x1, x2) and np.array_equal(d1, d2)
+
+
+def test_cg_restart_recomputes_the_residual():
+    """Past convergence the recurrence residual falls far below rounding while the true one
+    stalls near 1e-16: the defect after a completed cycle is the recomputed b - A x."""
+    h = po.Hierarchy(2, 1, 1, 4)
+    A = csr(*h.matrix())
+    b = h.rhs()
+    k = 40
+    _, _, _, d0 = h.solve(po.params(outer=po.OUTER_CG_PLAIN, tol=1e-300, maxsteps=k - 1), b)
+    _, xk, _, _ = h.solve(po.params(outer=po.OUTER_CG_PLAIN, tol=1e-300, maxsteps=k - 1), b)
+    _, _, _, d1 = h.solve(po.params(outer=po.OUTER_CG_PLAIN, tol=1e-300, maxsteps=2 * k,
+                                    restart_cycle=k), b)
+    true_k = np.linalg.norm(b - A @ xk)
+    assert d0[k] < 1e-20 * true_k  # the recurrence has drifted off
+    assert np.isclose(d1[k], true_k, rtol=1e-2)  # rounding-level value, summation order
+    assert np.array_equal(d1[:k], d0[:k])
+
+
+# ---- <safe-linear-solver> (linsolve.lisp:45-56,75-89) --------------------------------------
+
+def test_safe_linear_solver_falls_back_to_lu():
+    """On failure the iteration becomes <lu> and the loop goes on with the same step count:
+    one <lu> step on the residual at hand solves the system."""
+    h = po.Hierarchy(2, 1, 1, 6)  # 2D Q1 32^2, 1089 DOFs
+    A = csr(*h.matrix())
+    b = h.rhs()
+    kw = dict(outer=po.OUTER_CG, smoother=po.SMOOTH_JACOBI, omega=0.6, pre_steps=2,
+              post_steps=2, base_level=0, tol=1e-10, maxsteps=2)
+    st0, x0, s0, d0 = h.solve(po.params(**kw), b)
+    st, x, steps, dn = h.solve(po.params(fallback=po.FALLBACK_LU, **kw), b)
+    assert st0 == 1 and s0 == 3
+    assert st == 0 and steps == s0 + 1
+    assert np.array_equal(dn[: s0 + 1], d0)  # the failed iteration's history is kept
+    u = sla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - u) <= 1e-12 * np.linalg.norm(u)
+    # the <lu> step used the recurrence residual: x = x_3 + A^-1 r_3
+    assert dn[-1] < 1e-10 * dn[0]
+
+
+def test_safe_linear_solver_fallback_failure():
+    """A second failure is femlisp's 'The fallback iteration has failed.' error."""
+    h = po.Hierarchy(2, 1, 1, 3)
+    p = po.params(outer=po.OUTER_CG_PLAIN, tol=1e-300, maxsteps=1, fallback=po.FALLBACK_LU)
+    st, x, steps, dn = h.solve(p)
+    assert st == 2 and steps == 3
+
+
 # ---- vertex-centred block smoothers (blockit.lisp:113-203, geoblock.lisp:65-157) ----------
 
 def _geometric_blocks(dim, order, n):
@@ -643,3 +743,28 @@ def test_ssc_vcycle_contracts_for_q2(dim, n0, levels):
     assert st == 0
     rate = (dn[-1] / dn[0]) ** (1.0 / steps)
     assert rate < 0.06, rate
+
+
+# ---- host threads ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("smoother,order", [(po.SMOOTH_SOR, 1), (po.SMOOTH_JACOBI, 1),
+                                            (po.SMOOTH_SSC, 2)])
+def test_oracle_results_do_not_depend_on_threads(smoother, order):
+    """the CPU baseline's parallel loops (rows, one colour's rows or blocks, fixed dot ranges)
+    give the 1-thread results bit for bit"""
+    n0, levels = (1, 5) if order == 1 else (1, 4)
+    rng = np.random.default_rng(9)
+    ntop = n0 << (levels - 1)
+    kap = np.exp(rng.normal(size=(ntop,) * 3))
+    h = po.Hierarchy(3, order, n0, levels, kappa=kap)
+    p = po.params(outer=po.OUTER_CG, smoother=smoother, omega=1.0 if smoother else 0.6,
+                  ordering=po.ORDER_COLOUR, pre_steps=2, post_steps=2, base_level=1,
+                  symmetric=1, tol=1e-10, maxsteps=100)
+    out = []
+    for t in (1, 0):
+        po.set_threads(t)
+        out.append(h.solve(p))
+    po.set_threads(0)
+    (s1, x1, n1, d1), (s2, x2, n2, d2) = out
+    assert s1 == s2 == 0 and n1 == n2
+    assert np.array_equal(x1, x2) and np.array_equal(d1, d2)
