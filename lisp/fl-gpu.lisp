@@ -3,10 +3,11 @@
 ;;; Load after femlisp (SBCL).  It binds the C-ABI with fl.port:def-function
 ;;; (src/basic/port.lisp:442-455) in the manner of femlisp's SuperLU binding
 ;;; (src/alien/superlu.lisp:57-75), defines <gpu-sparse-matrix> and <gpu-vector>
-;;; with the matlisp generics the solver path calls, make-iterator methods for
-;;; <jacobi> and <sor> on GPU matrices, the flgpu_mg_create binding (the hierarchy
-;;; and its parameters are built by the caller), and EXPORT-SYSTEM, which writes a
-;;; femlisp system in the layout flgpu_system_read reads.
+;;; with the matlisp generics the solver path calls (vector.lisp:91-152,224-326),
+;;; make-iterator methods for <jacobi>, <sor>, <geometric-ssc>/<geometric-psc> and
+;;; <geometric-cs> on GPU matrices, a SOLVE method that hands a <linear-solver> (or
+;;; <safe-linear-solver>) whose matrix lives on the GPU to flgpu_solve, and
+;;; EXPORT-SYSTEM, which writes a femlisp system in the layout flgpu_system_read reads.
 ;;;
 ;;; No Lisp implementation is installed where this library is built, so this file
 ;;; has not been executed; the same C-ABI is exercised by the Python host side and
@@ -51,6 +52,8 @@
   (fl.port:def-function ("flgpu_vec_fill" flgpu-vec-fill) ((x (* t)) (a :double)) :returning :int)
   (fl.port:def-function ("flgpu_vec_dot" flgpu-vec-dot)
       ((x (* t)) (y (* t)) (out (* :double))) :returning :int)
+  (fl.port:def-function ("flgpu_vec_norm2" flgpu-vec-norm2)
+      ((x (* t)) (out (* :double))) :returning :int)
   (fl.port:def-function ("flgpu_csr_upload" flgpu-csr-upload)
       ((ctx (* t)) (nrows :int) (ncols :int) (nnz #.+long+)
        (rowptr (* :int)) (colidx (* :int)) (vals (* :double))
@@ -81,6 +84,15 @@
     :returning :int)
   (fl.port:def-function ("flgpu_mg_apply" flgpu-mg-apply)
       ((mg (* t)) (x (* t)) (b (* t)) (r (* t))) :returning :int)
+  (fl.port:def-function ("flgpu_mg_create_lattice" flgpu-mg-create-lattice)
+      ((ctx (* t)) (dim :int) (order :int) (n0 :int) (nlevels :int) (kappa (* t)) (nk :int)
+       (f :double) (asm-mode :int) (params (* t)) (out (* (* t))) (rhs (* (* t))))
+    :returning :int)
+  (fl.port:def-function ("flgpu_mg_destroy" flgpu-mg-destroy) ((mg (* t))) :returning :int)
+  (fl.port:def-function ("flgpu_solve" flgpu-solve)
+      ((mg (* t)) (a (* t)) (params (* t)) (b (* t)) (x (* t)) (report (* t))
+       (defnorms (* :double)) (max-defnorms :int))
+    :returning :int)
   (fl.port:def-function ("flgpu_system_read" flgpu-system-read)
       ((ctx (* t)) (path sb-alien:c-string) (prec :int) (a (* (* t))) (rhs (* (* t))))
     :returning :int))
@@ -137,6 +149,24 @@
     (flgpu-check (fl.port:foreign-call #'flgpu-vec-dot (gpu-handle x) (gpu-handle y) out))
     (aref out 0)))
 
+;;; The remaining generics the solver loop and <cg> call on vectors (vector.lisp:91-152,
+;;; 224-326): their default methods walk blocks with vref, which a device vector has not.
+(defmethod scalar-type ((x <gpu-vector>)) 'double-float)
+(defmethod make-analog ((x <gpu-vector>)) (gpu-vector (gpu-size x)))
+(defmethod copy ((x <gpu-vector>)) (copy! x (make-analog x)))
+(defmethod fill! ((x <gpu-vector>) s)
+  (flgpu-check (flgpu-vec-fill (gpu-handle x) (float s 1d0))) x)
+(defmethod m+! ((x <gpu-vector>) (y <gpu-vector>))   ; y <- x + y
+  (axpy! 1d0 x y))
+(defmethod l2-norm ((x <gpu-vector>))
+  (let ((out (make-array 1 :element-type 'double-float)))
+    (flgpu-check (fl.port:foreign-call #'flgpu-vec-norm2 (gpu-handle x) out))
+    (aref out 0)))
+(defmethod norm ((x <gpu-vector>) &optional (p 2))
+  (if (eql p 2)
+      (l2-norm x)
+      (norm (gpu-vector-to-host x) p)))
+
 (defclass <gpu-sparse-matrix> (<matrix>)
   ((handle :initarg :handle :reader gpu-handle)
    (nrows :initarg :nrows :reader nrows)
@@ -190,8 +220,13 @@ is the CSR of A."
     :residual-after nil))
 
 (defmethod make-iterator ((sor <sor>) (mat <gpu-sparse-matrix>))
-  (let ((n (make-array 1 :element-type '(signed-byte 32))))
-    (flgpu-check (fl.port:foreign-call #'flgpu-mat-colour-greedy (gpu-handle mat) n)))
+  ;; <sor> with a :compare by colour (linit.lisp:258-263): the lattice colouring when the
+  ;; keys number a lattice (it keeps the per-cell and stencil forms), else a greedy one
+  (if (gpu-lattice mat)
+      (destructuring-bind (dim order n) (gpu-lattice mat)
+        (flgpu-check (flgpu-lattice-colouring (gpu-handle mat) dim order n)))
+      (let ((n (make-array 1 :element-type '(signed-byte 32))))
+        (flgpu-check (fl.port:foreign-call #'flgpu-mat-colour-greedy (gpu-handle mat) n))))
   (make-instance '<iterator> :matrix mat :residual-before nil :initialize nil
     :iterate (lambda (x b r)
                (flgpu-check (flgpu-sor-iterate (gpu-handle mat)
@@ -219,6 +254,219 @@ lattice numbering MAT's keys: residual-before t, residual-after nil."
 
 (defmethod make-iterator ((psc fl.geomg::<geometric-psc>) (mat <gpu-sparse-matrix>))
   (block-iterator psc mat 3 'fl.iteration::damp))      ; FLGPU_SMOOTH_PSC
+
+;;; ---- C structs as byte vectors ---------------------------------------------
+;;; The parameter and report structs of include/flgpu.h, laid out by byte offset (int 4,
+;;; double 8 aligned to 8, little endian) and passed pinned like any other Lisp array.
+
+(defparameter +mg-params-layout+
+  '((smoother 0 :int) (omega 8 :double) (pre-steps 16 :int) (post-steps 20 :int)
+    (base-level 24 :int) (symmetric 28 :int) (prec 32 :int) (galerkin 36 :int)
+    (transfer 40 :int) (gamma 44 :int) (additive 48 :int) (fas 52 :int))
+  "flgpu_mg_params: field, byte offset, type; 56 bytes")
+
+(defparameter +solve-params-layout+
+  '((outer 0 :int) (alias 4 :int) (stop 8 :int) (tol 16 :double) (maxsteps 24 :int)
+    (true-residual 28 :int) (learned 32 :int) (stall-window 36 :int)
+    (restart-cycle 40 :int) (fallback 44 :int))
+  "flgpu_solve_params: field, byte offset, type; 48 bytes")
+
+(defparameter +solve-report-layout+
+  '((status 0 :int) (steps 4 :int) (initial-defnorm 8 :double) (final-defnorm 16 :double)
+    (true-residual-norm 24 :double) (fell-back 32 :int) (seconds 40 :double)
+    (fallback-used 48 :int))
+  "flgpu_solve_report: field, byte offset, type; 56 bytes")
+
+(defun pack-struct (layout size plist)
+  (let ((buf (make-array size :element-type '(unsigned-byte 8) :initial-element 0)))
+    (loop for (field offset type) in layout
+          for value = (getf plist (intern (symbol-name field) :keyword) 0)
+          do (ecase type
+               (:int
+                (loop for i below 4
+                      do (setf (aref buf (+ offset i)) (ldb (byte 8 (* 8 i)) value))))
+               (:double
+                (let ((bits (logior (ash (ldb (byte 32 0)
+                                              (sb-kernel:double-float-high-bits
+                                               (float value 1d0)))
+                                         32)
+                                    (sb-kernel:double-float-low-bits (float value 1d0)))))
+                  (loop for i below 8
+                        do (setf (aref buf (+ offset i)) (ldb (byte 8 (* 8 i)) bits)))))))
+    buf))
+
+(defun unpack-struct (layout buf)
+  (loop for (field offset type) in layout
+        collect (intern (symbol-name field) :keyword)
+        collect (ecase type
+                  (:int (let ((u (loop for i below 4
+                                       sum (ash (aref buf (+ offset i)) (* 8 i)))))
+                          (if (logbitp 31 u) (- u (ash 1 32)) u)))
+                  (:double (let ((u (loop for i below 8
+                                          sum (ash (aref buf (+ offset i)) (* 8 i)))))
+                             (sb-kernel:make-double-float
+                              (let ((hi (ldb (byte 32 32) u)))
+                                (if (logbitp 31 hi) (- hi (ash 1 32)) hi))
+                              (ldb (byte 32 0) u)))))))
+
+;;; ---- <geometric-cs> on the GPU (multigrid.lisp:303-393, geomg.lisp:109-142) ----------
+
+(defun smoother-kind (smoother)
+  "FLGPU_SMOOTH_* and the damping of femlisp's smoother object"
+  (typecase smoother
+    (fl.geomg::<geometric-ssc> (values 2 (slot-value smoother 'fl.iteration::omega)))
+    (fl.geomg::<geometric-psc> (values 3 (slot-value smoother 'fl.iteration::damp)))
+    (<jacobi> (values 0 (slot-value smoother 'fl.iteration::damp)))
+    (<sor> (values 1 (slot-value smoother 'fl.iteration::omega)))
+    (t (error "no GPU smoother for ~A" smoother))))
+
+(defun mg-params (mgit &key (prec 0) (transfer 0) (symmetric 1))
+  "flgpu_mg_params of a femlisp <geometric-cs> / <geometric-fas> object"
+  (multiple-value-bind (kind omega) (smoother-kind (fl.iteration::pre-smoother mgit))
+    (pack-struct +mg-params-layout+ 56
+                 (list :smoother kind :omega omega
+                       :pre-steps (fl.iteration::pre-steps mgit)
+                       :post-steps (fl.iteration::post-steps mgit)
+                       :base-level (fl.iteration::base-level mgit)
+                       :symmetric symmetric :prec prec
+                       :galerkin (if (fl.geomg::galerkin-p mgit) 1 0)
+                       :transfer transfer :gamma (fl.iteration::gamma mgit)
+                       :additive (if (eq (fl.iteration::combination-type mgit) :additive) 1 0)
+                       :fas (if (typep mgit 'fl.iteration::<fas>) 1 0)))))
+
+(defclass <gpu-hierarchy> ()
+  ((handle :initarg :handle :reader gpu-handle)
+   (rhs :initarg :rhs :initform nil :reader gpu-hierarchy-rhs
+        :documentation "the rhs assembled with a lattice hierarchy")))
+
+(defun gpu-hierarchy (mgit mat &key kappa-cells (n0 1) (f 1d0) (asm-mode 0) (prec 0)
+                                    a-vec i-vec)
+  "flgpu_mg of the <geometric-cs> MGIT for MAT.  When MAT's keys number a lattice,
+(dim order n), the levels are assembled on the GPU from n0 cells per axis up to n,
+with the coefficient KAPPA-CELLS (a <gpu-vector> of n^dim cell values, or NIL for
+kappa = 1), as femlisp's re-discretisation does (geomg.lisp:122-126).  Otherwise
+femlisp's own multilevel decomposition is uploaded: A-VEC and I-VEC are the
+blackboard's :a-vec and :i-vec (multigrid.lisp:139-141), coarsest level first."
+  (let ((params (mg-params mgit :prec prec)))
+    (if (gpu-lattice mat)
+        (destructuring-bind (dim order n) (gpu-lattice mat)
+          (let ((mg-box (make-array 1 :element-type '(unsigned-byte 64)))
+                (rhs-box (make-array 1 :element-type '(unsigned-byte 64)))
+                (nlevels (1+ (round (log (/ n n0) 2)))))
+            (flgpu-check (fl.port:foreign-call
+                          #'flgpu-mg-create-lattice (flgpu-ctx) dim order n0 nlevels
+                          (if kappa-cells (gpu-handle kappa-cells) (sb-sys:int-sap 0))
+                          (if kappa-cells n 0) (float f 1d0) asm-mode params mg-box rhs-box))
+            (make-instance '<gpu-hierarchy>
+                           :handle (sb-sys:int-sap (aref mg-box 0))
+                           :rhs (make-instance '<gpu-vector> :n (nrows mat)
+                                               :handle (sb-sys:int-sap (aref rhs-box 0))))))
+        (let* ((levels (length a-vec))
+               (as (map 'vector (lambda (A) (sap-int (gpu-handle (sparse-matrix->gpu A))))
+                        a-vec))
+               (is (map 'vector (lambda (I)
+                                  (sap-int (gpu-handle
+                                            (sparse-matrix->gpu I :keys (canonical-keys I)))))
+                        i-vec))
+               (a-arr (make-array levels :element-type '(unsigned-byte 64)
+                                         :initial-contents as))
+               (i-arr (make-array (max 1 (1- levels)) :element-type '(unsigned-byte 64)
+                                                       :initial-contents
+                                                       (if (plusp (length is)) is '(0)))))
+          (make-instance '<gpu-hierarchy>
+                         :handle (flgpu-out-handle #'flgpu-mg-create (flgpu-ctx) levels
+                                                   a-arr i-arr params))))))
+
+(defvar *gpu-hierarchies* (make-hash-table :test 'eq :weakness :key)
+  "the flgpu_mg built for a (mg-iteration . matrix) pair, so that <cg> and SOLVE share it")
+
+(defun ensure-gpu-hierarchy (mgit mat)
+  (let ((per-mat (or (gethash mgit *gpu-hierarchies*)
+                     (setf (gethash mgit *gpu-hierarchies*)
+                           (make-hash-table :test 'eq :weakness :key)))))
+    (or (gethash mat per-mat)
+        (setf (gethash mat per-mat) (gpu-hierarchy mgit mat)))))
+
+(defmethod make-iterator ((mgit fl.geomg::<geometric-cs>) (mat <gpu-sparse-matrix>))
+  "The <mg-iteration> iterator (multigrid.lisp:368-391) as one flgpu_mg_apply per step."
+  (let ((mg (ensure-gpu-hierarchy mgit mat)))
+    (make-instance '<iterator> :matrix mat :residual-before t :initialize nil
+      :iterate (lambda (x b r)
+                 (flgpu-check (flgpu-mg-apply (gpu-handle mg) (gpu-handle x) (gpu-handle b)
+                                              (gpu-handle r))))
+      :residual-after nil)))
+
+;;; ---- solve (solve.lisp:46-58, linsolve.lisp:58-89) --------------------------------------
+
+(defun stop-form (success-if)
+  "FLGPU_STOP_* and tolerance of a success-if form (< :reduction tol) / (< :defnorm tol)"
+  (when (and (consp success-if) (eq (first success-if) '<) (= (length success-if) 3))
+    (case (second success-if)
+      (:reduction (values 0 (third success-if)))
+      (:defnorm (values 1 (third success-if))))))
+
+(defun step-limit (failure-if)
+  "maxsteps of a failure-if form (> :step n)"
+  (when (and (consp failure-if) (eq (first failure-if) '>) (eq (second failure-if) :step))
+    (third failure-if)))
+
+(defun gpu-solvable-p (solver A b)
+  (and (typep A '<gpu-sparse-matrix>) (typep b '<gpu-vector>)
+       (stop-form (slot-value solver 'fl.iteration::success-if))
+       (step-limit (slot-value solver 'fl.iteration::failure-if))
+       (let ((it (slot-value solver 'iteration)))
+         (or (typep it 'fl.geomg::<geometric-cs>)
+             (and (typep it '<cg>)
+                  (or (null (slot-value it 'fl.iteration::preconditioner))
+                      (typep (slot-value it 'fl.iteration::preconditioner)
+                             'fl.geomg::<geometric-cs>)))))
+       (or (not (typep solver '<safe-linear-solver>))
+           (typep (fallback solver) '<lu>))))
+
+(defmethod solve :around ((solver <linear-solver>) &optional blackboard)
+  "A <linear-solver> on a system whose matrix and rhs live on the GPU runs as one
+flgpu_solve call (the step loop, stopping tests and, for <safe-linear-solver>, the
+<lu> fallback inside the library); anything else takes femlisp's own loop, which
+also works on GPU matrices and vectors through the methods above."
+  (let* ((problem (getbb blackboard :problem))
+         (A (if problem (matrix problem) (getbb blackboard :matrix)))
+         (b (if problem (rhs problem) (getbb blackboard :rhs))))
+    (if (not (gpu-solvable-p solver A b))
+        (call-next-method)
+        (multiple-value-bind (stop tol) (stop-form (slot-value solver 'fl.iteration::success-if))
+          (let* ((it (slot-value solver 'iteration))
+                 (pc (if (typep it '<cg>) (slot-value it 'fl.iteration::preconditioner) it))
+                 (mg (and pc (ensure-gpu-hierarchy pc A)))
+                 (maxsteps (step-limit (slot-value solver 'fl.iteration::failure-if)))
+                 (params (pack-struct
+                          +solve-params-layout+ 48
+                          (list :outer (cond ((not (typep it '<cg>)) 0) (pc 1) (t 2))
+                                :alias 1 :stop stop :tol tol :maxsteps maxsteps
+                                :restart-cycle (if (typep it '<cg>)
+                                                   (or (restart-cycle it) 0)
+                                                   0)
+                                :fallback (if (typep solver '<safe-linear-solver>) 1 0))))
+                 (report (make-array 56 :element-type '(unsigned-byte 8)))
+                 (defnorms (make-array (+ maxsteps 3) :element-type 'double-float))
+                 (x (or (getbb blackboard :solution) (make-domain-vector-for A)))
+                 (status (fl.port:foreign-call #'flgpu-solve
+                                               (if mg (gpu-handle mg) (sb-sys:int-sap 0))
+                                               (gpu-handle A) params (gpu-handle b)
+                                               (gpu-handle x) report defnorms
+                                               (length defnorms)))
+                 (rep (unpack-struct +solve-report-layout+ report)))
+            (when (minusp status) (flgpu-check status))
+            (when (= status 2) (error "The fallback iteration has failed."))
+            (setf (getbb blackboard :solution) x
+                  (getbb blackboard :status) (if (zerop status) :success :failure)
+                  (getbb blackboard :step) (getf rep :steps)
+                  (getbb blackboard :defnorm) (getf rep :final-defnorm)
+                  (getbb blackboard :initial-defnorm) (getf rep :initial-defnorm)
+                  (getbb blackboard :reduction) (if (zerop (getf rep :initial-defnorm))
+                                                    0d0
+                                                    (/ (getf rep :final-defnorm)
+                                                       (getf rep :initial-defnorm))))
+            blackboard)))))
 
 ;;; ---- export for offline hand-over ----------------------------------------
 
