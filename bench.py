@@ -27,6 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 WEIGHTS = os.path.join(ROOT, "gnu_femlisp_amd", "unet_weights.npz")
 PATH_DESC = {
+    "learned": "mixed-bb-jac with the learned correction: the U-Net, trained on 32^3 random-"
+               "coefficient problems, maps the residual left by the V-cycle, restricted to the "
+               "32^3 level, to a correction there, prolongated and added along the energy-optimal "
+               "step inside FP64 flexible CG; plain GMG-FCG takes over on stall",
     "mixed-bb-jac": "FP32 damped-Jacobi V(2,2) V-cycle, omega 0.9, black-box transfers, Galerkin levels, "
                     "in FP64 flexible CG",
     "plain-bb-jac": "FP64 damped-Jacobi V(2,2) V-cycle, omega 0.9, black-box transfers, Galerkin levels, "
@@ -59,20 +63,120 @@ def cell_sor_bytes(ncells, N, vb):
     return ncells * vb + 4 * N * vb
 
 
-def cpu_baseline(kappa, n0, levels, base, b, iters=24):
-    """The oracle (C restatement of femlisp's algorithm, one thread) running the
-    same plain FP64 GMG-CG on the same system for a bounded sample of `iters`
-    iterations (about 13 s at 128^3); the caller scales the time per iteration."""
+def host_info():
+    """host CPU of this box: the threads the oracle may use (the box's share, OMP_NUM_THREADS),
+    nproc and the processor model"""
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS") or nproc)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": threads, "nproc": nproc, "cpu": model}
+
+
+def cpu_time(h, prm, b, threads, runs):
+    """median wall time of `runs` oracle solves on `threads` host threads, after one warm-up"""
     from oracle import pyoracle as po
+    po.set_threads(threads)
+    h.solve(prm, b)
+    ts, steps = [], 0
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        st, x, steps, dn = h.solve(prm, b)
+        ts.append(time.perf_counter() - t0)
+    po.set_threads(0)
+    return float(np.median(ts)), steps, st
+
+
+def cpu_baselines(threads, deploy, gpu_steps, log=lambda *a: None):
+    """The oracle (the C restatement of femlisp's algorithm: FP64, femlisp's pattern and
+    iteration semantics, OpenMP over rows and over one colour's rows or blocks) timed on
+    `threads` host threads and on one, median of 5 after a warm-up, setup excluded:
+
+      2D Q1 32^2 Poisson, V(2,2) <jacobi> 0.6 PCG to 1e-10: the full solve;
+      3D Q1 64^3 Poisson, random rhs, V(2,2) multicolour GS PCG to 1e-10: the full solve;
+      3D Q2 48^3, kappa = exp(N(0, 1.5^2)), <geometric-ssc> V(1,1) PCG: runs of 1 and 3
+        iterations (1 and 2 on one thread), extrapolated to the GPU's iteration count;
+      3D Q1 128^3 inclusions (the deploy workload), femlisp's plain GMG-CG: runs of 2 and 8
+        iterations (1 and 3 on one thread), extrapolated to the plain path's iteration count.
+    """
+    from oracle import pyoracle as po
+    out = {}
+
+    def sample(name, h, prm, b, ks_all, ks_one, scale_to, runs_one=3):
+        """t(k) = t0 + k c for k iterations (t0: the oracle's per-solve set-up, orderings
+        and the coarse factorisation); two sample lengths give c and t0, and the estimate
+        of the whole solve is t0 + c x the GPU's iteration count"""
+        res = {}
+        for tag, th, ks, runs in (("all", threads, ks_all, 5), ("one", 1, ks_one, runs_one)):
+            t = []
+            for k in ks:
+                prm.maxsteps = k - 1
+                tk, steps, st = cpu_time(h, prm, b, th, runs)
+                assert steps == k
+                t.append(tk)
+            c = (t[1] - t[0]) / (ks[1] - ks[0])
+            t0 = max(t[0] - ks[0] * c, 0.0)
+            res[tag] = {"s": t0 + c * scale_to, "per_iteration_s": c, "setup_s": t0}
+        log(f"[bench] cpu {name}: {res['all']['s']:.2f}s on {threads} threads, "
+            f"{res['one']['s']:.2f}s on one")
+        out[name] = {"s": res["all"]["s"], "s_1core": res["one"]["s"], "threads": threads,
+                     "iterations": scale_to,
+                     "per_iteration_s": res["all"]["per_iteration_s"],
+                     "per_iteration_s_1core": res["one"]["per_iteration_s"],
+                     "sample": f"runs of {ks_all[0]} and {ks_all[1]} iterations on {threads} "
+                               f"threads (median of 5), {ks_one[0]} and {ks_one[1]} on one "
+                               f"(median of {runs_one}): per-solve set-up + per-iteration "
+                               f"time x {scale_to} iterations"}
+
+    # 2D Q1 32^2 and 3D Q1 64^3: whole solves
+    for name, dim, lv, sm, om, sym, rhs in (("2d_q1_32", 2, 6, po.SMOOTH_JACOBI, 0.6, 0, None),
+                                           ("3d_q1_64", 3, 7, po.SMOOTH_SOR, 1.0, 1, "random")):
+        h = po.Hierarchy(dim, 1, 1, lv)
+        b = h.rhs()
+        if rhs == "random":
+            b = np.random.default_rng(2024).uniform(-1, 1, h.ndofs())
+            b[po.boundary(dim, 1, 1 << (lv - 1))] = 0
+        prm = po.params(outer=po.OUTER_CG, smoother=sm, omega=om, ordering=po.ORDER_COLOUR,
+                        pre_steps=2, post_steps=2, base_level=0, symmetric=sym, tol=1e-10,
+                        maxsteps=200)
+        ta, steps, st = cpu_time(h, prm, b, threads, 5)
+        t1, _, _ = cpu_time(h, prm, b, 1, 5)
+        assert st == 0
+        out[name] = {"s": ta, "s_1core": t1, "threads": threads, "iterations": steps,
+                     "sample": "the whole solve, median of 5"}
+        log(f"[bench] cpu {name}: {ta:.3f}s on {threads} threads, {t1:.3f}s on one")
+        del h
+    # 3D Q2 48^3 with the block SSC femlisp selects for order 2
+    if gpu_steps.get("q2") is not None:
+        kap = np.exp(np.random.default_rng(48).normal(scale=1.5, size=(48,) * 3))
+        t0 = time.perf_counter()
+        h = po.Hierarchy(3, 2, 3, 5, kappa=kap)
+        su = time.perf_counter() - t0
+        log(f"[bench] cpu 3d_q2_48_ssc: oracle setup {su:.1f}s")
+        prm = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_SSC, omega=1.0, pre_steps=1,
+                        post_steps=1, base_level=0, symmetric=1, tol=1e-10)
+        sample("3d_q2_48_ssc", h, prm, h.rhs(), (1, 3), (1, 2), gpu_steps["q2"], runs_one=1)
+        out["3d_q2_48_ssc"]["setup_s"] = su
+        del h
+    # the deploy workload
+    kappa, n0, levels, base, b = deploy
     t0 = time.perf_counter()
     h = po.Hierarchy(3, 1, n0, levels, kappa=kappa)
-    t1 = time.perf_counter()
-    p = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_SOR, omega=1.0, ordering=po.ORDER_COLOUR,
-                  pre_steps=2, post_steps=2, base_level=base, symmetric=1, tol=1e-10,
-                  maxsteps=iters - 1)
-    st, x, steps, dn = h.solve(p, b)
-    t2 = time.perf_counter()
-    return {"setup_s": t1 - t0, "sample_s": t2 - t1, "steps": steps, "status": st}
+    su = time.perf_counter() - t0
+    log(f"[bench] cpu 3d_q1_128_inclusions: oracle setup {su:.1f}s")
+    prm = po.params(outer=po.OUTER_CG, smoother=po.SMOOTH_SOR, omega=1.0,
+                    ordering=po.ORDER_COLOUR, pre_steps=2, post_steps=2, base_level=base,
+                    symmetric=1, tol=1e-10)
+    sample("3d_q1_128_inclusions", h, prm, b, (2, 8), (1, 3), gpu_steps["plain"], runs_one=3)
+    out["3d_q1_128_inclusions"]["setup_s"] = su
+    return out
 
 
 def main():
@@ -84,8 +188,12 @@ def main():
     ap.add_argument("--base", type=int, default=3, help="coarse-solve level (2^base cells/axis)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--train-problems", type=int, default=256)
-    ap.add_argument("--train-epochs", type=int, default=4)
+    ap.add_argument("--train-problems", type=int, default=2048,
+                    help="random-coefficient 32^3 problems of the training set (one residual/"
+                         "error pair each)")
+    ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--learned-level", type=int, default=32,
+                    help="cells per axis of the level the network runs on")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--paths", default="plain,mixed,plain-bb,mixed-bb,plain-bb-jac,mixed-bb-jac,"
                                        "learned",
@@ -98,6 +206,10 @@ def main():
                          "no JSON line), e.g. under a profiler")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the fine-level kernels and a copy of known size (counter runs)")
+    ap.add_argument("--assembly-only", action="store_true",
+                    help="only the MFMA work (counter runs): 3D Q1 assembly at n^3 with FP64 and "
+                         "BF16 MFMA contractions, 3D Q2 48^3 assembly, 8 U-Net training steps at "
+                         "32^3 with batch 32")
     ap.add_argument("--no-q2", action="store_true", help="skip the Q2 48^3 configuration")
     ap.add_argument("--q2-only", action="store_true",
                     help="only the Q2 48^3 configuration (printed to stderr)")
@@ -245,8 +357,12 @@ def main():
                     sb, wb = cell_spmv_bytes(n ** 3, N, vb), cell_sor_bytes(n ** 3, N, vb)
                     jb = n ** 3 * vb + 3 * N * vb
                 elif form == F.STORAGE_STENCIL:
-                    sb, wb = 27 * N * vb + 2 * N * vb, 27 * N * vb + 5 * N * vb
-                    jb = 27 * N * vb + 4 * N * vb
+                    # the value planes the kernels stream: 14 of 27 when the stencil is
+                    # exactly symmetric (the lower ones come from the neighbours' upper planes)
+                    pl = M.stencil_planes()
+                    assert pl in (14, 27)
+                    sb, wb = pl * N * vb + 2 * N * vb, pl * N * vb + 5 * N * vb
+                    jb = pl * N * vb + 4 * N * vb
                 else:
                     sb, wb = spmv_bytes(nnz, N, vb), sor_sweep_bytes(nnz, N, vb)
                     jb = nnz * (vb + 4) + 4 * (N + 1) + 4 * N * vb
@@ -285,6 +401,47 @@ def main():
             del P
         return out
 
+    def assembly_entry(reps=5):
+        """Assembly of the value path's fine matrix: 3D Q1 n^3, FP32 values, BF16-MFMA element
+        contractions (k_asm_q1: element matrices formed in LDS, one pass).  Timed: whole
+        re-assemblies for the coefficient (values, load of the per-cell factors, refresh of the
+        stored forms), HIP events over `reps` calls.  Algorithmic flops (the reference's per-qp
+        chain, ellsys-fe.lisp:120-127): cells x nq x 2 (nloc d^2 + nloc^2 d); bytes: the
+        coefficient read and the CSR values written once."""
+        A, _ = lat.assemble(3, 1, n, (kv, n), prec=F.F32, mode=F.ASM_MFMA, rhs=False)
+        lat.reassemble(A, (kv, n), mode=F.ASM_MFMA)
+        ctx.timer_start()
+        for _ in range(reps):
+            lat.reassemble(A, (kv, n), mode=F.ASM_MFMA)
+        ms = ctx.timer_stop() / reps
+        cells, nq, nloc, d = n ** 3, 8, 8, 3
+        flops = cells * nq * 2 * (nloc * d * d + nloc * nloc * d)
+        byt = cells * 8 + A.nnz * 4
+        out = {"kernel": "k_asm_q1<float, 2> (BF16-MFMA element contractions in LDS) + refresh",
+               "ms": ms, "flops": flops, "bytes": byt,
+               "tflops": flops / (ms * 1e-3) / 1e12, "gbs": byt / (ms * 1e-3) / 1e9}
+        out["frac_hbm"] = out["gbs"] / 8000.0
+        del A
+        return out
+
+    if args.assembly_only:
+        for prec in (F.F64, F.F32):
+            A, _ = lat.assemble(3, 1, n, (kv, n), prec=prec, mode=F.ASM_MFMA, rhs=False)
+            lat.reassemble(A, (kv, n), mode=F.ASM_MFMA)
+            del A
+        kq = np.exp(np.random.default_rng(48).normal(scale=1.5, size=(48,) * 3))
+        for prec in (F.F64, F.F32):
+            A, _ = lat.assemble(3, 2, 48, kq, prec=prec, mode=F.ASM_MFMA, rhs=False)
+            del A
+        net = learned.UNet(2, 3)
+        xb = GpuVector.from_array(np.random.default_rng(1).normal(size=32 * 32 ** 3 * 2), F.F32)
+        tb = GpuVector.from_array(np.random.default_rng(2).normal(size=32 * 32 ** 3), F.F32)
+        for _ in range(8):
+            net.train_step(xb, tb, 32, 32, 1e-3)
+        ctx.synchronize()
+        log("[bench] assembly-only done")
+        return
+
     if args.kernels_only:
         for k, v in fine_kernels(args.kernel_reps).items():
             log(f"[bench] {k}: {v['ms'] * 1e3:.1f} us, {v['gbs']:.0f} GB/s")
@@ -310,7 +467,7 @@ def main():
     hier, setup = {}, {}
     paths = args.paths.split(",")
     for name, (prec, mode, tr, sm) in spec.items():
-        if name not in paths and not (name == "mixed" and "learned" in paths):
+        if name not in paths and not (name == "mixed-bb-jac" and "learned" in paths):
             continue
         t = time.perf_counter()
         if sm == "jac":
@@ -342,7 +499,11 @@ def main():
             train_info = {"source": "gnu_femlisp_amd/unet_weights.npz"}
         else:
             t = time.perf_counter()
-            X, T, cnt = learned.generate_pairs(n=32, problems=args.train_problems, seed=11)
+            # the training configuration: 32^3 cells, kappa = exp(g), g a smoothed Gaussian
+            # field (correlation length 4 cells, sigma 2), pairs from the FP64 multigrid,
+            # Adam lr 1e-3, batch 32
+            X, T, cnt = learned.generate_pairs(n=32, problems=args.train_problems,
+                                               snapshots=(1,), kinds=("gaussian",), seed=11)
             t_data = time.perf_counter() - t
             hist, vloss = learned.train(net, X, T, cnt, 32, epochs=args.train_epochs, batch=32,
                                         lr=1e-3, seed=2)
@@ -350,11 +511,13 @@ def main():
                           "train_s": time.perf_counter() - t - t_data,
                           "loss": hist, "holdout": vloss}
             del X, T
-        learned.attach(hier["mixed"][1], net, kv, n)
+        # on the value path's hierarchy, at the level with learned_level cells per axis (the
+        # coefficient there: the geometric mean of the fine cells')
+        net_kappa = learned.attach_coarse(hier["mixed-bb-jac"][1], net, kappa, args.learned_level)
 
     def run(path):
         learned_p = path == "learned"
-        cs, H = hier["mixed" if learned_p else path]
+        cs, H = hier["mixed-bb-jac" if learned_p else path]
         if path.startswith("plain"):
             solver = it.LinearSolver(it.CG(cs), failure_if=(">", "step", 600), true_residual=True)
         else:
@@ -450,6 +613,9 @@ def main():
 
     b_host = b.numpy()
     del hier, A, b, H
+    asm = assembly_entry()
+    log(f"[bench] assembly 3D Q1 {n}^3 FP32 (BF16 MFMA), whole reassembly: {asm['ms']:.2f} ms, "
+        f"{asm['tflops']:.2f} TFLOP/s, {asm['gbs']:.0f} GB/s of the value write")
     kern = fine_kernels(args.kernel_reps)
     for k, v in kern.items():
         log(f"[bench] {k}: {v['ms'] * 1e3:.1f} us, {v['gbs']:.0f} GB/s "
@@ -457,17 +623,52 @@ def main():
 
     q2 = None if args.no_q2 else q2_config()
 
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] cpu baseline (oracle, 1 thread) ...")
-        cpu = cpu_baseline(kappa, n0, levels, args.base, b_host)
-        # time to 1e-10: the time per iteration of the sample times the iteration count the
-        # same plain GMG-CG needs on the GPU to the same (true) residual
-        cpu["solve_s"] = cpu["sample_s"] / max(cpu["steps"], 1) * results["plain"]["steps"]
-        log(f"[bench] cpu: setup {cpu['setup_s']:.1f}s, {cpu['steps']} iterations in "
-            f"{cpu['sample_s']:.1f}s -> {cpu['solve_s']:.1f}s for {results['plain']['steps']}")
+    # the first two reference configurations on the GPU, with the oracle's algorithm: 2D Q1
+    # 32^2 Poisson, V(2,2) <jacobi> 0.6 PCG, and 3D Q1 64^3 Poisson with a random rhs, V(2,2)
+    # multicolour GS PCG, both FP64 to 1e-10 (median of 5 solves after a warm-up)
+    small = {}
+    for name, dim, lv, sm, sym, rhs in (("2d_q1_32", 2, 6, it.Jacobi(0.6), False, None),
+                                       ("3d_q1_64", 3, 7, it.SOR(1.0), True, "random")):
+        cs = it.GeometricCS(sm, 2, 2, 0, symmetric=sym)
+        Hs = it.Hierarchy.lattice(cs, dim, 1, 1, lv)
+        bs = Hs.rhs
+        if rhs == "random":
+            from oracle import pyoracle as po
+            bh = np.random.default_rng(2024).uniform(-1, 1, Hs.top_matrix().nrows)
+            bh[po.boundary(dim, 1, 1 << (lv - 1))] = 0
+            bs = GpuVector.from_array(bh)
+        ts = []
+        for k in range(6):
+            r = it.solve(it.LinearSolver(it.CG(cs), failure_if=(">", "step", 200)),
+                         {"matrix": Hs.top_matrix(), "rhs": bs, "hierarchy": Hs})
+            assert r["status"] == "success"
+            if k:
+                ts.append(r["seconds"])
+        small[name] = {"s": float(np.median(ts)), "steps": r["steps"]}
+        log(f"[bench] {name}: GPU {small[name]['s'] * 1e3:.2f} ms, {r['steps']} iterations")
+        del Hs, bs
 
-    fast = min((p for p in results if p != "plain"), key=lambda p: results[p]["s"])
+    cpu, host = None, host_info()
+    if not args.no_cpu:
+        log(f"[bench] cpu baselines (oracle on {host['threads']} threads and on one, "
+            f"{host['cpu']}, nproc {host['nproc']}) ...")
+        cpu = cpu_baselines(host["threads"], (kappa, n0, levels, args.base, b_host),
+                            {"plain": results["plain"]["steps"],
+                             "q2": q2["plain-ssc"]["steps"] if q2 else None}, log)
+        for k, v in cpu.items():
+            log(f"[bench] cpu {k}: {v['s']:.3f}s on {v['threads']} threads, "
+                f"{v['s_1core']:.3f}s on one ({v['sample']})")
+        gpu_s = {"2d_q1_32": small["2d_q1_32"]["s"], "3d_q1_64": small["3d_q1_64"]["s"],
+                 "3d_q1_128_inclusions": results["plain"]["s"]}
+        if q2:
+            gpu_s["3d_q2_48_ssc"] = q2["plain-ssc"]["s"]
+        for k in cpu:
+            cpu[k]["gpu_plain_s"] = gpu_s.get(k)
+
+    # the metric's path: the learned correction with the mixed-precision cycle; without it (a
+    # run without the learned path), the fastest path, named in config
+    fast = "learned" if "learned" in results else min(
+        (p for p in results if p != "plain"), key=lambda p: results[p]["s"])
     dom = kern["cell_sor_f32"]
     line = {
         "metric": "s to 1e-10 rel. residual, 3D Q1 128^3; V-cycle GDOF/s; learned+mixed vs plain",
@@ -490,6 +691,12 @@ def main():
         "paths": {p: {k: v for k, v in r.items()} for p, r in results.items()},
         "speedup_vs_plain": {p: results["plain"]["s"] / results[p]["s"] for p in results},
         "learned_training": train_info,
+        "learned_gain": None if not ("learned" in results and "mixed-bb-jac" in results) else {
+            "steps_without": results["mixed-bb-jac"]["steps"],
+            "steps_with": results["learned"]["steps"],
+            "time_ratio_without_over_with": results["mixed-bb-jac"]["s"] / results["learned"]["s"],
+            "fell_back": results["learned"]["fell_back"]},
+        "assembly": asm,
         "setup_s": setup,
         "vcycle_gdofs": {k: v["gdofs"] for k, v in vcyc.items()},
         "vcycle_ms": {k: v["ms"] for k, v in vcyc.items()},
@@ -501,10 +708,14 @@ def main():
             "achieved": dom["gbs"], "peak": 8000.0, "unit": "GB/s",
             "frac": dom["gbs"] / 8000.0, "traffic": None},
         "cpu_baseline": None if cpu is None else {
-            "value": cpu["solve_s"], "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"plain FP64 GMG-CG on the same {n}^3 system, {cpu['steps']} iterations "
-                      f"in {cpu['sample_s']:.1f}s, scaled to the {results['plain']['steps']} "
-                      f"iterations of the plain path (setup {cpu['setup_s']:.1f}s excluded)"},
+            "value": cpu["3d_q1_128_inclusions"]["s"], "unit": "s", "cores": host["threads"],
+            "kind": "port",
+            "sample": f"the oracle's plain FP64 GMG-CG on the same {n}^3 system (OpenMP over "
+                      f"rows and colours): {cpu['3d_q1_128_inclusions']['sample']} of the plain "
+                      f"path; setup excluded",
+            "cores_1": cpu["3d_q1_128_inclusions"]["s_1core"],
+            "nproc": host["nproc"], "cpu": host["cpu"],
+            "configs": cpu},
     }
     print(json.dumps(line), flush=True)
 

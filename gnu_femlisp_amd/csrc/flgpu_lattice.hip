@@ -543,6 +543,156 @@ __global__ void k_gather(long N, int dim, int order, int n, DevTable pat, const 
   }
 }
 
+// ---- 3D Q1 assembly in one pass, element matrices in LDS only ---------------------------
+// A workgroup owns a 4 x 4 x 4 block of lattice nodes.  It samples the coefficient at the
+// quadrature points of the 5 x 5 x 5 cells around the block (S, as k_coef), forms their element
+// matrices E = S T in LDS with the contraction of the chosen mode (k_elem_fma's loop, or the
+// 16 x 16 MFMA tiles of k_elem_mfma_f64 / k_elem_mfma_bf16 with the same operands in the same
+// lanes), and then forms its nodes' CSR rows from LDS exactly as k_gather does: each entry is
+// the sum, in ascending cell order and in FP64, of E over the cells holding both DOFs, with
+// femlisp's Dirichlet elimination, and the load vector likewise.  So the values equal the
+// two-pass assembly's bit for bit, while the element matrices never reach device memory.
+constexpr int kAsmB = 4;                         // nodes per axis of a block
+constexpr int kAsmC = kAsmB + 1;                 // cells per axis around it
+constexpr int kAsmNC = kAsmC * kAsmC * kAsmC;    // 125 cells
+constexpr int kAsmNCP = (kAsmNC + 15) / 16 * 16; // padded to MFMA tiles of 16 cells
+
+enum { ASM_FMA = 0, ASM_MFMA64 = 1, ASM_MFMA_BF16 = 2 };
+
+template <class T, int MODE>
+__global__ __launch_bounds__(256) void k_asm_q1(int n, const double *__restrict__ g,
+                                                const double *__restrict__ kappa, int nk,
+                                                double scale, const double *__restrict__ Tt,
+                                                const int *__restrict__ rowptr, T *vals,
+                                                double rhs_scale, const double *__restrict__ mload,
+                                                T *rhs) {
+  // E in the precision the two-pass path stores it in: FP64 for FP64 matrices, FP32 otherwise
+  __shared__ double Ss[kAsmNCP][8];
+  __shared__ T Es[kAsmNCP][64];
+  const int M = n + 1;
+  const int nbpa = (M + kAsmB - 1) / kAsmB;
+  int blk = blockIdx.x;
+  const int bx = blk % nbpa;
+  blk /= nbpa;
+  const int by = blk % nbpa, bz = blk / nbpa;
+  const int cz0 = bz * kAsmB - 1, cy0 = by * kAsmB - 1, cx0 = bx * kAsmB - 1;  // first cell
+  const int tid = threadIdx.x;
+  // S[c][q] = scale kappa(x_q) (k_coef); zero for cells outside the lattice (never read)
+  for (int e = tid; e < kAsmNCP * 8; e += blockDim.x) {
+    const int lc = e >> 3, q = e & 7;
+    double v = 0.0;
+    if (lc < kAsmNC) {
+      const int cz = cz0 + lc / (kAsmC * kAsmC), cy = cy0 + (lc / kAsmC) % kAsmC,
+                cx = cx0 + lc % kAsmC;
+      if (cz >= 0 && cy >= 0 && cx >= 0 && cz < n && cy < n && cx < n) {
+        double kap = 1.0;
+        if (kappa) {
+          const int r = nk / n;
+          const int qz = q >> 2, qy = (q >> 1) & 1, qx = q & 1;
+          long idx = (long)cz * r + (int)floor(g[qz] * r);
+          idx = idx * nk + (long)cy * r + (int)floor(g[qy] * r);
+          idx = idx * nk + (long)cx * r + (int)floor(g[qx] * r);
+          kap = kappa[idx];
+        }
+        v = scale * kap;
+      }
+    }
+    Ss[lc][q] = v;
+  }
+  __syncthreads();
+  if (MODE == ASM_FMA) {
+    for (int e = tid; e < kAsmNCP * 64; e += blockDim.x) {
+      const int lc = e >> 6, ab = e & 63;
+      double acc = 0.0;
+      for (int q = 0; q < 8; q++) acc += Ss[lc][q] * Tt[q * 64 + ab];
+      Es[lc][ab] = (T)acc;
+    }
+  } else {
+    // tiles of 16 cells x 16 entries, one per wave at a time
+    const int lane = tid & 63, w = tid >> 6;
+    const int i = lane & 15, kg = lane >> 4;
+    for (int job = w; job < (kAsmNCP / 16) * 4; job += 4) {
+      const int c0 = (job >> 2) * 16, n0 = (job & 3) * 16;
+      const int bcol = n0 + i;
+      if (MODE == ASM_MFMA64) {
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < 8; k0 += 4) {
+          const int k = k0 + kg;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ss[c0 + i][k], Tt[k * 64 + bcol], acc, 0,
+                                                     0, 0);
+        }
+        for (int v = 0; v < 4; v++) Es[c0 + kg + 4 * v][bcol] = (T)acc[v];
+      } else {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+        v4s ah, al, bh, bl;
+        for (int v = 0; v < 4; v++) {
+          const int k = 4 * kg + v;
+          const float av = (float)(k < 8 ? Ss[c0 + i][k] : 0.0);
+          const float bv = (float)(k < 8 ? Tt[k * 64 + bcol] : 0.0);
+          ah[v] = f2bf(av);
+          al[v] = f2bf(av - bf2f(ah[v]));
+          bh[v] = f2bf(bv);
+          bl[v] = f2bf(bv - bf2f(bh[v]));
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+        for (int v = 0; v < 4; v++) Es[c0 + 4 * kg + v][bcol] = (T)acc[v];
+      }
+    }
+  }
+  __syncthreads();
+  // rows: four threads per node, each taking every fourth of its (up to) 27 entries
+  const int node = tid >> 2, part = tid & 3;
+  const int Pz = bz * kAsmB + node / (kAsmB * kAsmB), Py = by * kAsmB + (node / kAsmB) % kAsmB,
+            Px = bx * kAsmB + node % kAsmB;
+  if (Pz >= M || Py >= M || Px >= M) return;
+  const long r = ((long)Pz * M + Py) * M + Px;
+  const bool rb = Pz == 0 || Py == 0 || Px == 0 || Pz == M - 1 || Py == M - 1 || Px == M - 1;
+  // the row's neighbours inside the lattice, in ascending column order: entry position = the
+  // number of such neighbours before it
+  unsigned live = 0;
+  for (int k = 0; k < 27; k++) {
+    const int Qz = Pz + k / 9 - 1, Qy = Py + (k / 3) % 3 - 1, Qx = Px + k % 3 - 1;
+    if (Qz >= 0 && Qy >= 0 && Qx >= 0 && Qz < M && Qy < M && Qx < M) live |= 1u << k;
+  }
+  const int pos0 = rowptr[r];
+  for (int k = part; k < 27; k += 4) {
+    if (!(live >> k & 1u)) continue;
+    const int Qz = Pz + k / 9 - 1, Qy = Py + (k / 3) % 3 - 1, Qx = Px + k % 3 - 1;
+    const bool cb = Qz == 0 || Qy == 0 || Qx == 0 || Qz == M - 1 || Qy == M - 1 || Qx == M - 1;
+    double acc = 0.0;
+    if (!(rb || cb)) {
+      // cells c with max(P, Q) - 1 <= c <= min(P, Q) per axis, ascending (z slowest)
+      const int zlo = max(max(Pz, Qz) - 1, 0), zhi = min(min(Pz, Qz), n - 1);
+      const int ylo = max(max(Py, Qy) - 1, 0), yhi = min(min(Py, Qy), n - 1);
+      const int xlo = max(max(Px, Qx) - 1, 0), xhi = min(min(Px, Qx), n - 1);
+      for (int cz = zlo; cz <= zhi; cz++)
+        for (int cy = ylo; cy <= yhi; cy++)
+          for (int cx = xlo; cx <= xhi; cx++) {
+            const int lc = ((cz - cz0) * kAsmC + (cy - cy0)) * kAsmC + (cx - cx0);
+            const int la = ((Pz - cz) * 2 + (Py - cy)) * 2 + (Px - cx);
+            const int lb = ((Qz - cz) * 2 + (Qy - cy)) * 2 + (Qx - cx);
+            acc += (double)Es[lc][la * 8 + lb];
+          }
+    } else {
+      acc = (rb && k == 13) ? 1.0 : 0.0;
+    }
+    vals[pos0 + __popc(live & ((1u << k) - 1u))] = (T)acc;
+  }
+  if (rhs && part == 0) {
+    double bsum = 0.0;
+    if (!rb) {
+      for (int cz = max(Pz - 1, 0); cz <= min(Pz, n - 1); cz++)
+        for (int cy = max(Py - 1, 0); cy <= min(Py, n - 1); cy++)
+          for (int cx = max(Px - 1, 0); cx <= min(Px, n - 1); cx++)
+            bsum += mload[((Pz - cz) * 2 + (Py - cy)) * 2 + (Px - cx)];
+      bsum *= rhs_scale;
+    }
+    rhs[r] = (T)bsum;
+  }
+}
+
 // ---- drivers ---------------------------------------------------------------------------
 
 struct LatticeBuild {
@@ -603,11 +753,62 @@ static void element_matrices(flgpu_ctx *ctx, const RefElement &R, int n, const f
   dev_free(dg);
 }
 
+// 3D Q1: the one-pass assembly (k_asm_q1)
+static void assemble_values_q1(flgpu_mat *A, const flgpu_vec *kappa, int nk, int mode,
+                               void *rhs_dev) {
+  flgpu_ctx *ctx = A->ctx;
+  const int n = A->lat_n;
+  RefElement R = ref_element(3, 1);
+  double *dT = (double *)dev_alloc(sizeof(double) * R.T.size());
+  double *dg = nullptr, *dm = nullptr;
+  try {
+    dg = (double *)dev_alloc(sizeof(double) * R.s);
+    dm = (double *)dev_alloc(sizeof(double) * R.nloc);
+    FLGPU_HIP(hipMemcpyAsync(dT, R.T.data(), sizeof(double) * R.T.size(), hipMemcpyHostToDevice,
+                             ctx->stream));
+    FLGPU_HIP(hipMemcpyAsync(dg, R.pts, sizeof(double) * R.s, hipMemcpyHostToDevice,
+                             ctx->stream));
+    FLGPU_HIP(hipMemcpyAsync(dm, R.m.data(), sizeof(double) * R.nloc, hipMemcpyHostToDevice,
+                             ctx->stream));
+    const int nbpa = (n + 1 + kAsmB - 1) / kAsmB;
+    const unsigned g = (unsigned)nbpa * nbpa * nbpa;
+    const double scale = 1.0 / n;  // H^(d-2)
+    const double rscale = A->lat_f * std::pow(1.0 / n, 3);
+    const double *kd = kappa ? (const double *)kappa->d : nullptr;
+    hipStream_t st = ctx->stream;
+#define FLGPU_ASM(T, MODE)                                                                   \
+  k_asm_q1<T, MODE><<<g, 256, 0, st>>>(n, dg, kd, nk, scale, dT, A->rowptr, (T *)A->vals, \
+                                       rscale, dm, (T *)rhs_dev)
+    if (A->prec == FLGPU_F64) {
+      if (mode == FLGPU_ASM_MFMA) FLGPU_ASM(double, ASM_MFMA64);
+      else FLGPU_ASM(double, ASM_FMA);
+    } else {
+      if (mode == FLGPU_ASM_MFMA) FLGPU_ASM(float, ASM_MFMA_BF16);
+      else FLGPU_ASM(float, ASM_FMA);
+    }
+#undef FLGPU_ASM
+    FLGPU_HIP(hipGetLastError());
+    FLGPU_HIP(hipStreamSynchronize(st));
+  } catch (...) {
+    dev_free(dT);
+    dev_free(dg);
+    dev_free(dm);
+    throw;
+  }
+  dev_free(dT);
+  dev_free(dg);
+  dev_free(dm);
+}
+
 // values (and rhs) of a lattice matrix whose pattern is already built
 static void assemble_values(flgpu_mat *A, const flgpu_vec *kappa, int nk, int mode,
                             void *rhs_dev) {
   flgpu_ctx *ctx = A->ctx;
   int dim = A->lat_dim, order = A->lat_order, n = A->lat_n;
+  if (dim == 3 && order == 1) {
+    assemble_values_q1(A, kappa, nk, mode, rhs_dev);
+    return;
+  }
   RefElement R = ref_element(dim, order);
   Table1D pat = table_pattern(order, n);
   long ncells = ipow(n, dim);
@@ -661,6 +862,7 @@ static void lattice_set_alpha(flgpu_mat *A, const flgpu_vec *kappa, int nk) {
   long ncells = (long)n * n * n;
   if (!A->alpha) {
     A->alpha = dev_alloc(prec_bytes(A->prec) * ncells);
+    A->sor_tmp = dev_alloc(prec_bytes(A->prec) * A->nrows);
     layout_changed(A);
   }
   const double *kd = kappa ? (const double *)kappa->d : nullptr;

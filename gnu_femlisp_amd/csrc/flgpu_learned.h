@@ -21,5 +21,13 @@ void mg_apply_ptr(flgpu_mg *mg, int vprec, void *x, void *b, void *r);
 flgpu_ctx *unet_ctx(const flgpu_unet *u);
 // network attached to a hierarchy (flgpu_mg_attach_unet)
 void mg_unet(flgpu_mg *mg, flgpu_unet **u, const flgpu_vec **kappa, int *n);
+// lattice hierarchies: the level with n cells per axis (-1 if none), the top level, the
+// cycle's precision, a level's rows, and the cycle's transfers between levels l and l - 1
+int mg_level_cells(const flgpu_mg *mg, int n);
+int mg_top_level(const flgpu_mg *mg);
+int mg_precision(const flgpu_mg *mg);
+long mg_level_rows(const flgpu_mg *mg, int l);
+void mg_restrict(flgpu_mg *mg, int l, const void *fine, void *coarse);
+void mg_prolong(flgpu_mg *mg, int l, const void *coarse, void *fine);
 
 }  // namespace flgpu

@@ -558,6 +558,28 @@ void mg_unet(flgpu_mg *mg, flgpu_unet **u, const flgpu_vec **kappa, int *n) {
   *n = mg->unet_n;
 }
 
+int mg_level_cells(const flgpu_mg *mg, int n) {
+  if (mg->lat_dim == 0) return -1;
+  for (int l = 0; l <= mg->top; l++)
+    if ((mg->lat_n0 << l) == n) return l;
+  return -1;
+}
+int mg_top_level(const flgpu_mg *mg) { return mg->top; }
+int mg_precision(const flgpu_mg *mg) { return mg->p.prec; }
+long mg_level_rows(const flgpu_mg *mg, int l) { return mg->lv[l].N; }
+
+// coarse = I^T fine from level l to l - 1, as the cycle restricts (beta 0)
+void mg_restrict(flgpu_mg *mg, int l, const void *fine, void *coarse) {
+  const MGLevel &C = mg->lv[l - 1];
+  spmv(C.I->transpose, 1.0, fine, 0.0, coarse, coarse);
+}
+
+// fine = I coarse from level l - 1 to l (beta 0)
+void mg_prolong(flgpu_mg *mg, int l, const void *coarse, void *fine) {
+  const MGLevel &C = mg->lv[l - 1];
+  spmv(C.I, 1.0, coarse, 0.0, fine, fine);
+}
+
 void mg_free(flgpu_mg *mg) {
   if (!mg) return;
   graphs_clear(mg);
@@ -777,7 +799,8 @@ extern "C" int flgpu_mg_attach_unet(flgpu_mg *mg, flgpu_unet *u, const flgpu_vec
     FLGPU_REQUIRE(mg, "hierarchy must not be NULL");
     FLGPU_REQUIRE(mg->lat_dim == 3 && mg->lat_order == 1,
                   "the learned correction is defined for 3D Q1 lattices");
-    FLGPU_REQUIRE((mg->lat_n0 << mg->top) == n, "n does not match the top level");
+    FLGPU_REQUIRE(!u || mg_level_cells(mg, n) >= 0, "n does not match a level of the hierarchy");
+    FLGPU_REQUIRE(!u || n % 4 == 0, "the network grid needs n divisible by 4");
     FLGPU_REQUIRE(!kappa || (kappa->prec == FLGPU_F64 && kappa->n == (long)n * n * n),
                   "kappa must be an FP64 vector of n^3 cell values");
     FLGPU_REQUIRE(!u || unet_ctx(u) == mg->ctx, "the network belongs to another context");

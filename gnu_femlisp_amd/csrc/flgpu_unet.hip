@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <vector>
 
 #include "flgpu_learned.h"
 
@@ -712,7 +713,8 @@ namespace flgpu {
 // network input of sample s: channel 0 = r / rms(r) on nodes (z,y,x) < n of
 // the (n+1)^3 lattice, channel 1 = log(kappa) / 4 of the cell with that lower
 // corner (0 without kappa).  rms from the dot in *ss: sqrt(ss / n^3).
-__global__ void k_pack(int n, const double *__restrict__ r, const double *__restrict__ kappa,
+template <class T>
+__global__ void k_pack(int n, const T *__restrict__ r, const double *__restrict__ kappa,
                        const double *ss, float *x, long sample) {
   long nv = (long)n * n * n;
   long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -722,7 +724,7 @@ __global__ void k_pack(int n, const double *__restrict__ r, const double *__rest
   double rms = sqrt(*ss / (double)nv);
   double inv = rms > 0.0 ? 1.0 / rms : 0.0;
   float *o = x + (sample * nv + v) * 2;
-  o[0] = (float)(r[idx] * inv);
+  o[0] = (float)((double)r[idx] * inv);
   o[1] = kappa ? (float)(log(kappa[v]) * 0.25) : 0.f;
 }
 
@@ -761,35 +763,62 @@ __global__ void k_add_opt(long N, const double *num, const double *den, const T 
   q[i] += beta * c[i];
 }
 
+// The network's correction inside flexible CG.  After the V-cycle z1 = q, the residual
+// r1 = r - A z1 is taken to the level the network runs on (the cycle's own restrictions, in
+// the cycle's precision), the network maps it to an error estimate there, the cycle's
+// prolongations bring that back (c), and q += beta c with the energy-optimal beta.  On the
+// top level this is z = z1 + beta N(r1).
 struct UnetHook : LearnedHook {
   flgpu_ctx *ctx;
+  flgpu_mg *mg;
   const flgpu_mat *A;
   flgpu_unet *u;
   const flgpu_vec *kappa;
-  int n;
+  int n, lc, top, mprec;
   long N;
   int prec;
   void *r1 = nullptr, *c = nullptr, *Ac = nullptr;
+  std::vector<void *> lv;  // levels lc..top in the cycle's precision (below the top)
   float *xin = nullptr;
   ~UnetHook() override {
     dev_free(r1);
     dev_free(c);
     dev_free(Ac);
+    for (void *p : lv) dev_free(p);
     dev_free(xin);
   }
-  // z1 = q (the V-cycle result), r1 = r - A z1, c = N(r1), q += beta c
   void correct(void *r, void *q) override {
     hipStream_t s = ctx->stream;
-    spmv(A, -1.0, q, 1.0, r, r1);
     FLGPU_REQUIRE(prec == FLGPU_F64, "the learned correction runs inside an FP64 outer iteration");
-    vdot_dev(ctx, N, prec, r1, r1, 10);
+    spmv(A, -1.0, q, 1.0, r, r1);
     long nv = (long)n * n * n;
-    k_pack<<<grid_for(nv, 256), 256, 0, s>>>(n, (const double *)r1,
-                                             kappa ? (const double *)kappa->d : nullptr,
-                                             ctx->red_out + 10, xin, 0);
+    const double *kd = kappa ? (const double *)kappa->d : nullptr;
     unet_alloc_acts(u, 1, n, false);
-    unet_forward(u, 1, n, xin);
-    k_unpack<double><<<grid_for(N, 256), 256, 0, s>>>(n, u->a[8], (double *)c);
+    if (lc == top) {
+      vdot_dev(ctx, N, prec, r1, r1, 10);
+      k_pack<double><<<grid_for(nv, 256), 256, 0, s>>>(n, (const double *)r1, kd,
+                                                       ctx->red_out + 10, xin, 0);
+      unet_forward(u, 1, n, xin);
+      k_unpack<double><<<grid_for(N, 256), 256, 0, s>>>(n, u->a[8], (double *)c);
+    } else {
+      vcopy(ctx, N, FLGPU_F64, r1, mprec, lv[top - lc]);
+      for (int l = top; l > lc; l--) mg_restrict(mg, l, lv[l - lc], lv[l - 1 - lc]);
+      long Nc = mg_level_rows(mg, lc);
+      vdot_dev(ctx, Nc, mprec, lv[0], lv[0], 10);
+      if (mprec == FLGPU_F64)
+        k_pack<double><<<grid_for(nv, 256), 256, 0, s>>>(n, (const double *)lv[0], kd,
+                                                         ctx->red_out + 10, xin, 0);
+      else
+        k_pack<float><<<grid_for(nv, 256), 256, 0, s>>>(n, (const float *)lv[0], kd,
+                                                        ctx->red_out + 10, xin, 0);
+      unet_forward(u, 1, n, xin);
+      if (mprec == FLGPU_F64)
+        k_unpack<double><<<grid_for(Nc, 256), 256, 0, s>>>(n, u->a[8], (double *)lv[0]);
+      else
+        k_unpack<float><<<grid_for(Nc, 256), 256, 0, s>>>(n, u->a[8], (float *)lv[0]);
+      for (int l = lc + 1; l <= top; l++) mg_prolong(mg, l, lv[l - 1 - lc], lv[l - lc]);
+      vcopy(ctx, N, mprec, lv[top - lc], FLGPU_F64, c);
+    }
     spmv(A, 1.0, c, 0.0, Ac, Ac);
     vdot_dev(ctx, N, prec, c, r1, 11);
     vdot_dev(ctx, N, prec, c, Ac, 12);
@@ -809,17 +838,27 @@ std::unique_ptr<LearnedHook> learned_hook(flgpu_mg *mg, const flgpu_mat *A) {
   FLGPU_REQUIRE(n % 4 == 0, "the network grid needs n divisible by 4");
   std::unique_ptr<UnetHook> h(new UnetHook);
   h->ctx = A->ctx;
+  h->mg = mg;
   h->A = A;
   h->u = u;
   h->kappa = kappa;
   h->n = n;
   h->N = A->nrows;
   h->prec = A->prec;
-  FLGPU_REQUIRE(h->N == (long)(n + 1) * (n + 1) * (n + 1), "A does not match the lattice");
+  h->lc = mg_level_cells(mg, n);
+  h->top = mg_top_level(mg);
+  h->mprec = mg_precision(mg);
+  FLGPU_REQUIRE(h->lc >= 0, "the network's grid is no level of the hierarchy");
+  FLGPU_REQUIRE(mg_level_rows(mg, h->lc) == (long)(n + 1) * (n + 1) * (n + 1),
+                "the network's level is not a 3D Q1 lattice of n cells per axis");
+  FLGPU_REQUIRE(mg_level_rows(mg, h->top) == h->N, "A does not match the hierarchy");
   size_t b = prec_bytes(A->prec) * h->N;
   h->r1 = dev_alloc(b);
   h->c = dev_alloc(b);
   h->Ac = dev_alloc(b);
+  if (h->lc < h->top)
+    for (int l = h->lc; l <= h->top; l++)
+      h->lv.push_back(dev_alloc(prec_bytes(h->mprec) * mg_level_rows(mg, l)));
   h->xin = (float *)dev_alloc(sizeof(float) * 2 * (long)n * n * n);
   return std::unique_ptr<LearnedHook>(h.release());
 }
@@ -846,7 +885,7 @@ extern "C" int flgpu_unet_pack(const flgpu_vec *r, const flgpu_vec *kappa, int n
                                                                 ctx->red_out + 14, (float *)x->d,
                                                                 sample);
     else
-      k_pack<<<grid_for(nv, 256), 256, 0, ctx->stream>>>(
+      k_pack<double><<<grid_for(nv, 256), 256, 0, ctx->stream>>>(
           n, (const double *)r->d, kappa ? (const double *)kappa->d : nullptr, ctx->red_out + 14,
           (float *)x->d, sample);
     FLGPU_HIP(hipGetLastError());

@@ -170,6 +170,28 @@ def train(net, X, T, count, n, epochs=10, batch=32, lr=1e-3, seed=0, holdout=0.0
     return hist, vloss
 
 
+def coarse_kappa(kappa, n):
+    """the coefficient on n cells per axis from a finer per-cell field: the geometric mean
+    over each coarse cell's fine cells (the network reads log(kappa))"""
+    nf = kappa.shape[0]
+    f = nf // n
+    assert f * n == nf and kappa.shape == (nf, nf, nf)
+    lk = np.log(kappa).reshape(n, f, n, f, n, f).mean(axis=(1, 3, 5))
+    return np.exp(lk)
+
+
 def attach(hierarchy, net, kappa_vec, n):
+    """attach the network to a lattice hierarchy: it runs on the level with n cells per axis
+    (the top level or a coarser one) and reads kappa_vec, n^3 cell values on that level"""
     check(lib().flgpu_mg_attach_unet(hierarchy.h, net.h, kappa_vec.h if kappa_vec else None, n))
     hierarchy._unet = (net, kappa_vec)
+
+
+def attach_coarse(hierarchy, net, kappa, n, ctx=None):
+    """attach the network on the level with n cells per axis of a hierarchy whose top-level
+    coefficient is the per-cell field kappa (finer): the level's coefficient is its
+    geometric mean"""
+    kc = coarse_kappa(np.asarray(kappa), n)
+    kv = GpuVector.from_array(kc.ravel(), F.F64, ctx or hierarchy.ctx)
+    attach(hierarchy, net, kv, n)
+    return kv

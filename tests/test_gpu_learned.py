@@ -84,6 +84,93 @@ def test_unet_training_reduces_loss():
     assert hist[-1] < 0.8 * hist[0], hist
 
 
+def bf16(a):
+    """round to BF16 (nearest, ties to even) as the kernels do when operands enter MFMA"""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def test_unet_forward_closed_forms():
+    """the parameter sets that reduce the checker to closed forms (tests/test_oracle_unet.py)
+    on the GPU network: a single tap is the shift with zero padding, bit for bit on BF16
+    inputs; the pools, upsamplings and [up, skip] concatenations give the block means"""
+    say("unet closed forms")
+    from tests.test_oracle_unet import CENTER, block_mean, positive_input, tap, theta_from
+    n, batch = 8, 2
+    net = learned.UNet(2, 0)
+    x = positive_input(n, batch, 5)
+    xv = GpuVector.from_array(x.ravel(), F.F32)
+    for d in ((1, 0, -1), (-1, -1, -1)):
+        net.set_params(theta_from([(0, tap(*d), 0, 0, 1.0), (1, CENTER, 0, 0, 1.0),
+                                   (7, CENTER, 32, 0, 1.0), (8, CENTER, 0, 0, 1.0)]))
+        y = net.forward(xv, batch, n).numpy().reshape(batch, n, n, n)
+        ref = np.zeros((batch, n, n, n))
+        zs, ys, xs = (slice(max(0, -k), n - max(0, k)) for k in d)
+        zd, yd, xd = (slice(max(0, k), n - max(0, -k)) for k in d)
+        ref[:, zs, ys, xs] = bf16(x[..., 0])[:, zd, yd, xd]
+        assert np.array_equal(y, ref), d
+    ent = [(l, CENTER, 0, 0, 1.0) for l in (0, 1, 2, 3, 4, 5)]
+    ent += [(6, CENTER, 0, 0, 1.0), (6, CENTER, 64, 0, 2.0), (7, CENTER, 0, 0, 1.0),
+            (7, CENTER, 32, 0, 3.0), (8, CENTER, 0, 0, 1.0)]
+    net.set_params(theta_from(ent, biases=[(8, 0, 0.25)]))
+    y = net.forward(xv, batch, n).numpy().reshape(batch, n, n, n)
+    x0 = x[..., 0]
+    ref = block_mean(x0, 4) + 2 * block_mean(x0, 2) + 3 * x0 + 0.25
+    # every layer rounds its input to BF16 (relative 2^-9 each, a few on every path)
+    assert np.abs(y - ref).max() <= 2e-2 * np.abs(ref).max()
+
+
+def test_unet_adam_step_matches_checker():
+    """one training step on the GPU: the parameters and moments Adam leaves equal the
+    checker's Adam (po.adam, FP64) applied to the GPU's own gradient of the same batch"""
+    say("unet adam step")
+    n, batch, lr, step = 8, 2, 1e-3, 5
+    rng = np.random.default_rng(8)
+    theta = he_theta(4).astype(np.float32).astype(np.float64)
+    m = (rng.normal(size=theta.size) * 1e-3).astype(np.float32).astype(np.float64)
+    v = (rng.uniform(size=theta.size) * 1e-6).astype(np.float32).astype(np.float64)
+    net = learned.UNet(2, 0)
+    net.set_params(theta, m, v, step)
+    x = GpuVector.from_array(rng.normal(size=batch * n ** 3 * 2), F.F32)
+    t = GpuVector.from_array(rng.normal(size=batch * n ** 3), F.F32)
+    loss0, g = net.loss_grad(x, t, batch, n)
+    loss1 = net.train_step(x, t, batch, n, lr)
+    assert abs(loss1 - loss0) <= 1e-6 * abs(loss0)  # the loss before the update
+    th_gpu, m_gpu, v_gpu, st = net.params()
+    assert st == step + 1
+    th_ref, m_ref, v_ref = po.adam(theta, g, m, v, step + 1, lr=lr)
+    # FP32 moments: m = b1 m + (1 - b1) g may cancel, so its error is bounded by the rounding
+    # of its terms, not of the result; v adds positive terms
+    u = 2.0 ** -23
+    assert np.all(np.abs(m_gpu - m_ref) <= 3 * u * (0.9 * np.abs(m) + 0.1 * np.abs(g)) + 1e-30)
+    assert np.allclose(v_gpu, v_ref, rtol=4 * u, atol=1e-30)
+    # the update from the GPU's own moments, to FP32 rounding of theta and of the update
+    b1, b2, eps, t = 0.9, 0.999, 1e-8, step + 1
+    d_exp = -lr * (m_gpu / (1 - b1 ** t)) / (np.sqrt(v_gpu / (1 - b2 ** t)) + eps)
+    d_gpu = th_gpu - theta
+    assert np.all(np.abs(d_gpu - d_exp) <= 1e-5 * np.abs(d_exp) + 2 * u * np.abs(theta) + 1e-12)
+    # and the checker's update agrees wherever the moments do not cancel
+    d_ref = th_ref - theta
+    sel = np.abs(m_ref) > 1e-3 * np.abs(m_ref).max()
+    assert np.allclose(d_gpu[sel], d_ref[sel], rtol=1e-4, atol=2 * u * np.abs(theta).max())
+
+
+def test_unet_training_batch_32_on_32_cubed():
+    """the training configuration's shapes: random-coefficient Q1 problems on 32^3 cells
+    (kappa = exp(g), g a smoothed Gaussian field, correlation length 4 cells, sigma 2),
+    pairs from the FP64 multigrid, Adam with lr 1e-3 and batch 32: the loss falls"""
+    say("unet training, batch 32, 32^3")
+    n = 32
+    X, T, cnt = learned.generate_pairs(n=n, problems=256, snapshots=(1,), kinds=("gaussian",),
+                                       seed=21)
+    assert cnt == 256
+    net = learned.UNet(2, 5)
+    hist, vloss = learned.train(net, X, T, cnt, n, epochs=8, batch=32, lr=1e-3, seed=3)
+    say(f"losses {hist} holdout {vloss}")
+    assert hist[-1] < 0.9 * hist[0], hist
+
+
 def _problem(n, seed, kind):
     kap = learned.coefficient(kind, n, seed)
     return kap

@@ -275,3 +275,78 @@ def test_concurrent_contexts_and_shared_context():
     assert not errs, errs
     for z, s in zip(zs, seq):
         assert np.array_equal(z.numpy(), s)
+
+
+@pytest.mark.parametrize("n", [40, 70])
+@pytest.mark.parametrize("prec", [F.F64, F.F32])
+def test_cell_sor_sweeps_over_many_tiles(n, prec):
+    """the per-cell <sor> sweep (two plane-parity launches, 32 x 32 tiles with recomputed
+    halos) on lattices of several and ragged tiles equals the SELL colour sweep, forwards and
+    backwards, and with rhs and residual one vector (femlisp's (iterate q w w))"""
+    say(f"cell sor n={n} prec={prec}")
+    rng = np.random.default_rng(n)
+    kap = np.exp(rng.normal(scale=1.5, size=(n, n, n)))
+    A, b = lat.assemble(3, 1, n, kap, prec=prec)
+    B, _ = lat.assemble(3, 1, n, kap, prec=prec)
+    assert A.storage() == F.STORAGE_CELL and B.storage(F.STORAGE_SELL) == F.STORAGE_SELL
+    tol = 1e-13 if prec == F.F64 else 2e-6
+    y = rng.uniform(-1, 1, A.nrows)
+    bh = b.numpy() + rng.uniform(-1, 1, A.nrows)
+    for bw in (0, 1):
+        for alias in (False, True):
+            xs = [GpuVector.from_array(y, prec) for _ in range(2)]
+            rs = [GpuVector.from_array(bh, prec) for _ in range(2)]
+            bs = [GpuVector.from_array(bh, prec) for _ in range(2)]
+            for M, xv, rv, bv in ((A, xs[0], rs[0], bs[0]), (B, xs[1], rs[1], bs[1])):
+                for _ in range(2):
+                    check(lib().flgpu_sor_iterate(M.h, 1.1, bw, xv.h, (rv if alias else bv).h,
+                                                  rv.h))
+            assert relerr(xs[0].numpy(), xs[1].numpy()) < tol, (bw, alias)
+            assert relerr(rs[0].numpy(), rs[1].numpy()) < 10 * tol, (bw, alias)
+
+
+def _gather_host(E, n, M):
+    """femlisp's scatter of element matrices (global-local-matrix-operation) restated: for every
+    row P and lattice neighbour Q, the sum in ascending cell order, in FP64, of E over the cells
+    holding both; Dirichlet rows and columns eliminated (identity diagonal)"""
+    vals = []
+    for P in np.ndindex(M, M, M):
+        rb = any(p in (0, M - 1) for p in P)
+        for d in np.ndindex(3, 3, 3):
+            Q = tuple(p + dd - 1 for p, dd in zip(P, d))
+            if any(q < 0 or q >= M for q in Q):
+                continue
+            cb = any(q in (0, M - 1) for q in Q)
+            if rb or cb:
+                vals.append(1.0 if (rb and Q == P) else 0.0)
+                continue
+            acc = 0.0
+            rng = [range(max(max(p, q) - 1, 0), min(min(p, q), n - 1) + 1) for p, q in zip(P, Q)]
+            for c in np.ndindex(*[len(r) for r in rng]):
+                cc = tuple(r[i] for r, i in zip(rng, c))
+                cell = (cc[0] * n + cc[1]) * n + cc[2]
+                la = ((P[0] - cc[0]) * 2 + (P[1] - cc[1])) * 2 + (P[2] - cc[2])
+                lb = ((Q[0] - cc[0]) * 2 + (Q[1] - cc[1])) * 2 + (Q[2] - cc[2])
+                acc += E[cell, la, lb]
+            vals.append(acc)
+    return np.array(vals)
+
+
+@pytest.mark.parametrize("prec,mode", [(F.F64, F.ASM_F64), (F.F64, F.ASM_MFMA),
+                                       (F.F32, F.ASM_F64), (F.F32, F.ASM_MFMA)])
+@pytest.mark.parametrize("nk_factor", [1, 2])
+def test_one_pass_q1_assembly_equals_element_gather(prec, mode, nk_factor):
+    """3D Q1 values from the one-pass kernel (element matrices in LDS only) equal the element
+    kernel's matrices gathered on the host in ascending cell order, bit for bit"""
+    n = 9
+    say(f"one-pass assembly prec {prec} mode {mode} nk {nk_factor * n}")
+    kap = _kap(11, n, nk_factor * n)
+    A, b = lat.assemble(3, 1, n, kap, prec=prec, mode=mode)
+    E = lat.element_matrices(3, 1, n, kap, prec=prec, mode=mode).reshape(n ** 3, 8, 8)
+    ref = _gather_host(E, n, n + 1)
+    if prec == F.F32:
+        ref = ref.astype(np.float32).astype(np.float64)
+    assert np.array_equal(A.csr()[2], ref)
+    # the load vector of f = 1 against the oracle
+    orp, oci, ov, ob = po.assemble(3, 1, n, kap, 1.0)
+    assert relerr(b.numpy(), ob) < (1e-14 if prec == F.F64 else 1e-7)
