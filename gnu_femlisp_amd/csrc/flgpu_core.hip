@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "flgpu_internal.h"
 
@@ -31,6 +33,12 @@ void dev_free(void *p) {
 constexpr int kRedBlocks = 1024;
 constexpr int kRedThreads = 256;
 constexpr int kRedSlots = 64;
+
+// Scratch of flgpu_vec_dot_noalloc: the per-block partials of one dot (kRedBlocks values)
+// and its result, 8.2 KB on every device the library runs on, held until the process ends.
+// Host code never passes this name to a kernel (on the host it is a stand-in, not a device
+// address): it asks each device for the array's address once (scratch_on) and uses that.
+static __device__ __align__(256) double scratch[kRedBlocks + 32];
 
 // ---- kernels ------------------------------------------------------------------
 
@@ -353,6 +361,23 @@ void fetch_slots_async(flgpu_ctx *ctx, int nslots) {
 
 void fetch_wait(flgpu_ctx *ctx) { FLGPU_HIP(hipEventSynchronize(ctx->ev_fetch)); }
 
+// The address of `scratch` on `device`, which must be current: hipGetSymbolAddress on the
+// first call for that device, then the kept value (each device holds its own copy).
+static double *scratch_on(int device) {
+  static std::mutex mu;
+  static std::vector<double *> ptr;
+  std::lock_guard<std::mutex> g(mu);
+  FLGPU_REQUIRE(device >= 0, "bad device index");
+  if ((int)ptr.size() <= device) ptr.resize(device + 1, nullptr);
+  if (!ptr[device]) {
+    void *p = nullptr;
+    FLGPU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FLGPU_REQUIRE(p != nullptr, "no device address for the scratch array");
+    ptr[device] = (double *)p;
+  }
+  return ptr[device];
+}
+
 double vdot_host(flgpu_ctx *ctx, long n, int prec, const void *x, const void *y) {
   vdot_dev(ctx, n, prec, x, y, 0);
   fetch_slots(ctx, 1);
@@ -597,6 +622,31 @@ extern "C" int flgpu_vec_dot(const flgpu_vec *x, const flgpu_vec *y, double *out
     FLGPU_REQUIRE(x->ctx == y->ctx, "vectors belong to different contexts");
     CTX_LOCK(x->ctx);
     *out = vdot_host(x->ctx, x->n, x->prec, x->d, y->d);
+    return 0;
+  });
+}
+
+extern "C" int flgpu_vec_dot_noalloc(const flgpu_vec *x, const flgpu_vec *y, double *out) {
+  return guard([&] {
+    FLGPU_REQUIRE(x && y && out, "arguments must not be NULL");
+    FLGPU_REQUIRE(x->n == y->n, "dot: length mismatch");
+    FLGPU_REQUIRE(x->prec == y->prec, "dot: precision mismatch");
+    FLGPU_REQUIRE(x->ctx == y->ctx, "vectors belong to different contexts");
+    CTX_LOCK(x->ctx);
+    double *sp = scratch_on(x->ctx->device);
+    hipStream_t s = x->ctx->stream;
+    // the kernels and grid of flgpu_vec_dot: every block writes its partial and the final
+    // pass writes the result, so nothing in the scratch needs clearing between calls
+    if (x->prec == FLGPU_F64)
+      k_dot_partial<double><<<kRedBlocks, kRedThreads, 0, s>>>(
+          x->n, (const double *)x->d, (const double *)y->d, sp);
+    else
+      k_dot_partial<float><<<kRedBlocks, kRedThreads, 0, s>>>(
+          x->n, (const float *)x->d, (const float *)y->d, sp);
+    k_dot_final<<<1, kRedBlocks, 0, s>>>(sp, kRedBlocks, sp + kRedBlocks);
+    FLGPU_HIP(hipGetLastError());
+    FLGPU_HIP(hipMemcpyAsync(out, sp + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+    FLGPU_HIP(hipStreamSynchronize(s));
     return 0;
   });
 }

@@ -78,6 +78,7 @@ SIGNATURES = {
     "flgpu_vec_scal": (_ci, [_cd, _vp]),
     "flgpu_vec_axpy": (_ci, [_cd, _vp, _vp]),
     "flgpu_vec_dot": (_ci, [_vp, _vp, _P(_cd)]),
+    "flgpu_vec_dot_noalloc": (_ci, [_vp, _vp, _P(_cd)]),
     "flgpu_vec_norm2": (_ci, [_vp, _P(_cd)]),
     "flgpu_csr_upload": (_ci, [_vp, _ci, _ci, _cl, _vp, _vp, _vp, _ci, _P(_vp)]),
     "flgpu_csr_update_values": (_ci, [_vp, _vp]),

@@ -80,6 +80,13 @@ int flgpu_vec_axpy(double a, const flgpu_vec *x, flgpu_vec *y);
 int flgpu_vec_dot(const flgpu_vec *x, const flgpu_vec *y, double *out);
 /* norm x 2 / l2-norm */
 int flgpu_vec_norm2(const flgpu_vec *x, double *out);
+/* flgpu_vec_dot without any allocation: no femlisp counterpart; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).  The same
+ * kernels, grid and result as flgpu_vec_dot, for any length; the 1024 per-block partials and
+ * the result go to a fixed 8.2 KB array each device holds for the life of the process, not
+ * to the context's buffers.  Calls that overlap on one device (from two contexts) race on
+ * that array. */
+int flgpu_vec_dot_noalloc(const flgpu_vec *x, const flgpu_vec *y, double *out);
 
 /* ---- sparse matrices (src/matlisp/sparse-matrix.lisp:378-442) ------------ */
 
