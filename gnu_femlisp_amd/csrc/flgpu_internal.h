@@ -162,7 +162,6 @@ struct flgpu_mat {
   // by DOFs i and j), d = lattice offset; kernels apply A from alpha
   void *alpha = nullptr;  // n^3 cell factors (prec), nullptr if not available
   double kt[8] = {};      // Kref[0][b], b = bz*4 + by*2 + bx
-  void *sor_tmp = nullptr;  // with alpha: nrows values, the <sor> sweep's second buffer
   // 3D Q1 lattice matrices otherwise: the 27 values of every row, SoA
   // val27[k][row], k = (dz+1)*9 + (dy+1)*3 + (dx+1); columns are implied
   void *val27 = nullptr;

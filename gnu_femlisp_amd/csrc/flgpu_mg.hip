@@ -870,6 +870,11 @@ extern "C" int flgpu_mg_use_graphs(flgpu_mg *mg, int enable) {
 
 // ---- C-ABI: solver --------------------------------------------------------------------------
 
+// The learned correction is dropped when its step has added less than this fraction of the
+// preconditioned step's energy in this many FCG steps in a row.
+constexpr double kLearnedMinGain = 1e-2;
+constexpr int kLearnedWeakSteps = 5;
+
 namespace {
 
 struct SolveWork {
@@ -961,6 +966,7 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
     double best_true = -1.0;
     int best_step = 0;
     int count = 0;  // <cg>'s steps for restart-cycle (krylow.lisp:101-105,128-131)
+    int weak_steps = 0;  // consecutive steps whose learned correction added little
     // <safe-linear-solver> (linsolve.lisp:45-56,75-89): the <lu> fallback's inverse of A
     bool fallback_p = false;
     std::unique_ptr<double, void (*)(double *)> lu_inv(nullptr, [](double *q) { dev_free(q); });
@@ -1113,7 +1119,8 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             }
             FLGPU_HIP(hipGetLastError());
             ca = na;
-            fetch_slots_async(ctx, 4);
+            // with the network on, its step's dots (slots 11, 12) come along
+            fetch_slots_async(ctx, learned ? 13 : 4);
             if (!spmv_dot(A, W.p, W.a, 1)) {  // the next step's product, ahead of the wait
               spmv(A, 1.0, W.p, 0.0, W.a, W.a);
               vdot_dev(ctx, N, prec, W.a, W.p, 1);
@@ -1121,6 +1128,21 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             ap_ready = true;
             fetch_wait(ctx);
             alpha = ctx->host_out[ca];
+            if (learned) {
+              // the energy the network's step adds, (c.r1)^2 / (c.Ac), against that of the
+              // whole preconditioned step, z.r: after kLearnedWeakSteps steps in a row below
+              // kLearnedMinGain the network is dropped and plain GMG-FCG goes on
+              const double num = ctx->host_out[11], den = ctx->host_out[12];
+              const double gain = (den > 0.0 && alpha > 0.0) ? num * num / den / alpha : 0.0;
+              weak_steps = gain < kLearnedMinGain ? weak_steps + 1 : 0;
+              if (weak_steps >= kLearnedWeakSteps) {
+                // as the stall fallback: plain GMG-FCG from the residual at hand
+                learned = nullptr;
+                fell_back = 1;
+                initialized = false;
+                ap_ready = false;
+              }
+            }
           } else {
             vdot_dev(ctx, N, prec, W.r, W.r, 3);
             fetch_slots(ctx, 4);

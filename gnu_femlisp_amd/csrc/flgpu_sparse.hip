@@ -215,7 +215,6 @@ void mat_free(flgpu_mat *A) {
   for (auto &s : A->colour_sell) sell_free(s);
   if (A->transpose) mat_free(A->transpose);
   dev_free(A->alpha);
-  dev_free(A->sor_tmp);
   dev_free(A->val27);
   dev_free(A->tp27);
   dev_free(A->blk_inv);
@@ -228,8 +227,6 @@ void drop_alpha(flgpu_mat *A) {
   if (A->alpha) layout_changed(A);
   dev_free(A->alpha);
   A->alpha = nullptr;
-  dev_free(A->sor_tmp);
-  A->sor_tmp = nullptr;
 }
 
 void mat_refresh_values(flgpu_mat *A) {
@@ -351,7 +348,6 @@ flgpu_mat *mat_convert(const flgpu_mat *A, int prec) {
   if (A->alpha) {
     long nc = (long)A->lat_n * A->lat_n * A->lat_n;
     B->alpha = dev_alloc(prec_bytes(prec) * nc);
-    B->sor_tmp = dev_alloc(prec_bytes(prec) * B->nrows);
     vcopy(ctx, nc, A->prec, A->alpha, prec, B->alpha);
     for (int j = 0; j < 8; j++) B->kt[j] = A->kt[j];
   }
@@ -818,149 +814,6 @@ __global__ __launch_bounds__(256) void k_q1_sor(int M, int n, int cz, int cy, in
   corr = corr / d;
   r[i] = corr;
   x[i] += omega * corr;
-}
-
-// One <sor> sweep in the per-cell form in two launches instead of eight.  The colours are
-// c = cz*4 + cy*2 + cx; those with the same z parity cz update rows of planes that never touch
-// one another (planes z and z + 2 share no cell), and the rows of one plane read the planes
-// z - 1, z + 1 of the other parity, whose values the sweep has not changed yet (the first
-// parity) or has finished (the second).  So the four colours of a parity are a sequence of
-// in-plane steps: a block takes a TY x TX tile of one plane, stages x of the planes
-// z - 1..z + 1, b and the cell factors of the tile plus a halo in LDS, and runs the four
-// colours in order with a barrier between them.  Colour j's rows need the in-plane values of
-// colours < j one node away, so every colour is also computed on a halo of 3 nodes: the values
-// there match those of the block owning them as far as the tile needs them (colour j is exact
-// up to 3 - j nodes out).  Every row is formed with q1_row's arithmetic on the same values, so
-// the sweep equals the eight colour launches bit for bit.
-//
-// Blocks stage values other blocks of the same launch update, so no launch writes what it
-// reads: the first parity reads src = x and writes its new planes to dst = tmp, the second
-// reads src = tmp and writes to dst = x.  Each launch also copies the other parity's planes
-// from src to dst (a block copies plane z - 1, and plane z + 1 when that is the last plane), so
-// that tmp holds the whole vector for the second launch and x the result after it.  b is staged
-// as well, so it must not be r.
-constexpr int kSorTY = 32, kSorTX = 32, kSorH = 3;
-constexpr int kSorRY = kSorTY + 2 * kSorH, kSorRX = kSorTX + 2 * kSorH;  // rows updated
-constexpr int kSorLY = kSorRY + 2, kSorLX = kSorRX + 2;                  // x staged (+1 ring)
-constexpr int kSorCY = kSorRY + 1, kSorCX = kSorRX + 1;                  // cells staged
-
-template <class T>
-__global__ __launch_bounds__(256) void k_q1_sor_planes(int M, int n, int pz, int backwards,
-                                                       const T *__restrict__ alpha, Q1K kt,
-                                                       T omega, const T *__restrict__ src,
-                                                       T *__restrict__ dst,
-                                                       const T *__restrict__ b,
-                                                       T *__restrict__ r) {
-  __shared__ T xs[3][kSorLY][kSorLX];  // planes z-1, z, z+1 over the region and one more node
-  __shared__ T al[2][kSorCY][kSorCX];  // cells of the cell planes z-1 and z
-  __shared__ T bs[kSorRY][kSorRX];
-  const int nty = (M + kSorTY - 1) / kSorTY, ntx = (M + kSorTX - 1) / kSorTX;
-  int blk = blockIdx.x;
-  const int tx = blk % ntx;
-  blk /= ntx;
-  const int ty = blk % nty;
-  const int z = 2 * (blk / nty) + pz;
-  const int y0 = ty * kSorTY, x0 = tx * kSorTX;
-  const int ry0 = y0 - kSorH, rx0 = x0 - kSorH;  // region origin (may lie outside)
-  // the other parity's planes this block hands on from src to dst: its tile of plane z - 1,
-  // and of plane z + 1 when that is the lattice's last plane
-  for (int pl = -1; pl <= 1; pl += 2) {
-    const int zz = z + pl;
-    if (zz < 0 || zz >= M || (pl == 1 && zz != M - 1)) continue;
-    for (int e = threadIdx.x; e < kSorTY * kSorTX; e += blockDim.x) {
-      const int yy = y0 + e / kSorTX, xx = x0 + e % kSorTX;
-      if (yy >= M || xx >= M) continue;
-      const long i = ((long)zz * M + yy) * M + xx;
-      dst[i] = src[i];
-    }
-  }
-  // stage: x over [ry0 - 1, ry0 + RY] x [rx0 - 1, rx0 + RX] for the three planes (entries
-  // outside the lattice are never read), cells over [ry0 - 1, ry0 + RY) and b over the region
-  for (int e = threadIdx.x; e < 3 * kSorLY * kSorLX; e += blockDim.x) {
-    const int pl = e / (kSorLY * kSorLX), q = e % (kSorLY * kSorLX);
-    const int yy = ry0 - 1 + q / kSorLX, xx = rx0 - 1 + q % kSorLX, zz = z - 1 + pl;
-    T v = (T)0;
-    if (zz >= 0 && zz < M && yy >= 0 && yy < M && xx >= 0 && xx < M)
-      v = src[((long)zz * M + yy) * M + xx];
-    xs[pl][q / kSorLX][q % kSorLX] = v;
-  }
-  for (int e = threadIdx.x; e < 2 * kSorCY * kSorCX; e += blockDim.x) {
-    const int pl = e / (kSorCY * kSorCX), q = e % (kSorCY * kSorCX);
-    const int yc = ry0 - 1 + q / kSorCX, xc = rx0 - 1 + q % kSorCX, zc = z - 1 + pl;
-    T v = (T)0;
-    if (zc >= 0 && zc < n && yc >= 0 && yc < n && xc >= 0 && xc < n)
-      v = alpha[((long)zc * n + yc) * n + xc];
-    al[pl][q / kSorCX][q % kSorCX] = v;
-  }
-  for (int e = threadIdx.x; e < kSorRY * kSorRX; e += blockDim.x) {
-    const int yy = ry0 + e / kSorRX, xx = rx0 + e % kSorRX;
-    T v = (T)0;
-    if (yy >= 0 && yy < M && xx >= 0 && xx < M) v = b[((long)z * M + yy) * M + xx];
-    bs[e / kSorRX][e % kSorRX] = v;
-  }
-  __syncthreads();
-  const bool zbd = z == 0 || z == M - 1;
-  const bool lz[3] = {z - 1 != 0, true, z + 1 != M - 1};
-  // each colour's nodes of the region: at most (RY/2 + 1) x (RX/2 + 1)
-  constexpr int CY = kSorRY / 2 + 1, CX = kSorRX / 2 + 1;
-  for (int j = 0; j < 4; j++) {
-    const int c = backwards ? 3 - j : j;
-    const int cy = c >> 1, cx = c & 1;
-    // the first region row / column of this colour (ry0 may be negative)
-    const int fy = (ry0 - cy) & 1, fx = (rx0 - cx) & 1;
-    for (int e = threadIdx.x; e < CY * CX; e += blockDim.x) {
-      const int ly = fy + 2 * (e / CX), lx = fx + 2 * (e % CX);  // region coordinates
-      const int yy = ry0 + ly, xx = rx0 + lx;
-      if (ly >= kSorRY || lx >= kSorRX || yy < 0 || xx < 0 || yy >= M || xx >= M) continue;
-      const bool bd = zbd || yy == 0 || xx == 0 || yy == M - 1 || xx == M - 1;
-      const T xi = xs[1][ly + 1][lx + 1];
-      T corr = bs[ly][lx];
-      T d = (T)1;
-      if (bd) {
-        corr += (-xi) * (T)1;
-      } else {
-        T a[2][2][2];
-#pragma unroll
-        for (int pp = 0; pp < 2; pp++)
-#pragma unroll
-          for (int qq = 0; qq < 2; qq++)
-#pragma unroll
-            for (int ss = 0; ss < 2; ss++) a[pp][qq][ss] = al[pp][ly + qq][lx + ss];
-        const bool ly3[3] = {yy - 1 != 0, true, yy + 1 != M - 1};
-        const bool lx3[3] = {xx - 1 != 0, true, xx + 1 != M - 1};
-        T acc = corr;
-#pragma unroll
-        for (int k = 0; k < 27; k++) {
-          const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
-          const bool live = lz[dz + 1] && ly3[dy + 1] && lx3[dx + 1];
-          const T xn = live ? xs[dz + 1][ly + 1 + dy][lx + 1 + dx] : xi;
-          const int pz0 = dz > 0 ? 1 : 0, pz1 = dz < 0 ? 0 : 1;
-          const int py0 = dy > 0 ? 1 : 0, py1 = dy < 0 ? 0 : 1;
-          const int px0 = dx > 0 ? 1 : 0, px1 = dx < 0 ? 0 : 1;
-          T sum = (T)0;
-#pragma unroll
-          for (int pp = pz0; pp <= pz1; pp++)
-#pragma unroll
-            for (int qq = py0; qq <= py1; qq++)
-#pragma unroll
-              for (int ss = px0; ss <= px1; ss++) sum += a[pp][qq][ss];
-          const T aij = (T)kt.k[(dz != 0) * 4 + (dy != 0) * 2 + (dx != 0)] * sum;
-          if (k == 13) d = aij;
-          acc += ((T)-1 * xn) * (live ? aij : (T)0);
-        }
-        corr = acc;
-      }
-      corr = corr / d;
-      const T xnew = xi + omega * corr;
-      xs[1][ly + 1][lx + 1] = xnew;
-      if (ly >= kSorH && ly < kSorH + kSorTY && lx >= kSorH && lx < kSorH + kSorTX) {
-        const long i = ((long)z * M + yy) * M + xx;
-        r[i] = corr;
-        dst[i] = xnew;
-      }
-    }
-    __syncthreads();
-  }
 }
 
 // ---- 3D Q1 lattice matrices in 27-point stencil storage ------------------------------
@@ -1777,31 +1630,7 @@ __global__ __launch_bounds__(256) void k_sor_colour(int nsub, int nslices, const
   x[row] += omega * corr;
 }
 
-template <class T>
-static void q1_sor_planes(const flgpu_mat *A, T omega, int backwards, T *x, const T *b, T *r) {
-  const int M = A->lat_n + 1;
-  const int nty = (M + kSorTY - 1) / kSorTY, ntx = (M + kSorTX - 1) / kSorTX;
-  T *tmp = (T *)A->sor_tmp;
-  for (int k = 0; k < 2; k++) {
-    const int pz = backwards ? 1 - k : k;
-    const unsigned g = (unsigned)((M - pz + 1) / 2) * nty * ntx;
-    const T *src = k == 0 ? x : tmp;
-    T *dst = k == 0 ? tmp : x;
-    k_q1_sor_planes<T><<<g, 256, 0, A->ctx->stream>>>(M, A->lat_n, pz, backwards,
-                                                      (const T *)A->alpha, q1k(A), omega, src,
-                                                      dst, b, r);
-    FLGPU_HIP(hipGetLastError());
-  }
-}
-
 void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const void *b, void *r) {
-  if (A->alpha && A->sor_tmp && b != r) {
-    if (A->prec == FLGPU_F64)
-      q1_sor_planes<double>(A, omega, backwards, (double *)x, (const double *)b, (double *)r);
-    else
-      q1_sor_planes<float>(A, (float)omega, backwards, (float *)x, (const float *)b, (float *)r);
-    return;
-  }
   if (A->alpha || A->val27) {
     // lattice colours in the order of the lattice colouring: cz*4 + cy*2 + cx
     int M = A->lat_n + 1;

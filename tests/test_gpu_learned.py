@@ -139,14 +139,17 @@ def test_unet_adam_step_matches_checker():
     assert abs(loss1 - loss0) <= 1e-6 * abs(loss0)  # the loss before the update
     th_gpu, m_gpu, v_gpu, st = net.params()
     assert st == step + 1
-    th_ref, m_ref, v_ref = po.adam(theta, g, m, v, step + 1, lr=lr)
+    # the GPU's Adam runs in FP32 with the FP32 roundings of b1, b2 and eps (0.999f differs
+    # from 0.999 by 1.3e-5 relative): the checker gets the same constants
+    b1, b2, eps = (float(np.float32(c)) for c in (0.9, 0.999, 1e-8))
+    th_ref, m_ref, v_ref = po.adam(theta, g, m, v, step + 1, lr=lr, b1=b1, b2=b2, eps=eps)
     # FP32 moments: m = b1 m + (1 - b1) g may cancel, so its error is bounded by the rounding
     # of its terms, not of the result; v adds positive terms
     u = 2.0 ** -23
     assert np.all(np.abs(m_gpu - m_ref) <= 3 * u * (0.9 * np.abs(m) + 0.1 * np.abs(g)) + 1e-30)
     assert np.allclose(v_gpu, v_ref, rtol=4 * u, atol=1e-30)
     # the update from the GPU's own moments, to FP32 rounding of theta and of the update
-    b1, b2, eps, t = 0.9, 0.999, 1e-8, step + 1
+    t = step + 1
     d_exp = -lr * (m_gpu / (1 - b1 ** t)) / (np.sqrt(v_gpu / (1 - b2 ** t)) + eps)
     d_gpu = th_gpu - theta
     assert np.all(np.abs(d_gpu - d_exp) <= 1e-5 * np.abs(d_exp) + 2 * u * np.abs(theta) + 1e-12)

@@ -280,9 +280,9 @@ def test_concurrent_contexts_and_shared_context():
 @pytest.mark.parametrize("n", [40, 70])
 @pytest.mark.parametrize("prec", [F.F64, F.F32])
 def test_cell_sor_sweeps_over_many_tiles(n, prec):
-    """the per-cell <sor> sweep (two plane-parity launches, 32 x 32 tiles with recomputed
-    halos) on lattices of several and ragged tiles equals the SELL colour sweep, forwards and
-    backwards, and with rhs and residual one vector (femlisp's (iterate q w w))"""
+    """the per-cell <sor> sweep on larger lattices (n = 40, 70: several thread blocks per
+    colour, ragged ends) equals the SELL colour sweep, forwards and backwards, and with rhs and
+    residual one vector (femlisp's (iterate q w w))"""
     say(f"cell sor n={n} prec={prec}")
     rng = np.random.default_rng(n)
     kap = np.exp(rng.normal(scale=1.5, size=(n, n, n)))
