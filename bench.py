@@ -620,6 +620,11 @@ def main():
     for k, v in kern.items():
         log(f"[bench] {k}: {v['ms'] * 1e3:.1f} us, {v['gbs']:.0f} GB/s "
             f"({v['gbs'] / 8000 * 100:.1f}% of 8 TB/s)")
+    # the byte models count each value once; a rate above the HBM peak means a model counts
+    # too few bytes (or the data came from the Infinity Cache), so it is flagged in the line
+    over_peak = sorted(k for k, v in kern.items() if v["gbs"] > 8000.0)
+    if over_peak:
+        log(f"[bench] WARNING: above the 8 TB/s HBM peak by the byte model: {over_peak}")
 
     q2 = None if args.no_q2 else q2_config()
 
@@ -702,6 +707,7 @@ def main():
         "vcycle_ms": {k: v["ms"] for k, v in vcyc.items()},
         "kernels": {k: {"us": v["ms"] * 1e3, "GBps": v["gbs"], "frac": v["gbs"] / 8000.0}
                     for k, v in kern.items()},
+        "kernels_above_hbm_peak": over_peak,
         "q2_48": q2,
         "roofline": roof if roof is not None else {
             "bound": "hbm", "kernel": "k_q1_sor<float> (fine-level <sor> sweep, per-cell form)",

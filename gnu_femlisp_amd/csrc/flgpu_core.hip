@@ -446,6 +446,7 @@ extern "C" int flgpu_ctx_destroy(flgpu_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     dev_free(ctx->red_partial);
     dev_free(ctx->red_out);
+    dev_free(ctx->solve_buf);
     if (ctx->host_out) (void)hipHostFree(ctx->host_out);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);

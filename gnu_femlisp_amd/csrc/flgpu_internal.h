@@ -115,6 +115,9 @@ struct flgpu_ctx {
   double *host_out = nullptr;     // pinned host copy of red_out
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   hipEvent_t ev_fetch = nullptr;  // marks the end of an asynchronous slot fetch
+  // flgpu_solve's work vectors, kept between solves (the context lock serialises solves)
+  void *solve_buf = nullptr;
+  size_t solve_bytes = 0;
   std::string name;
 };
 
@@ -162,6 +165,10 @@ struct flgpu_mat {
   // by DOFs i and j), d = lattice offset; kernels apply A from alpha
   void *alpha = nullptr;  // n^3 cell factors (prec), nullptr if not available
   double kt[8] = {};      // Kref[0][b], b = bz*4 + by*2 + bx
+  // with alpha: the <sor> sweep's strips per plane (0: one launch per colour) and its copy of
+  // x and b around the strip boundaries, 2 (M (S - 1) 8 M) values (prec), nullptr if S < 2
+  int sor_strips = 0, sor_h = 0;
+  void *sor_ghost = nullptr;
   // 3D Q1 lattice matrices otherwise: the 27 values of every row, SoA
   // val27[k][row], k = (dz+1)*9 + (dy+1)*3 + (dx+1); columns are implied
   void *val27 = nullptr;
@@ -199,6 +206,8 @@ void mat_finish(flgpu_mat *A);  // SELL + diagonal from the device CSR
 void mat_refresh_values(flgpu_mat *A);  // SELL/diag/colour values after a CSR value change
 void mat_build_transpose(const flgpu_mat *A);
 void drop_alpha(flgpu_mat *A);
+// the strips and boundary copy of the per-cell <sor> sweep, after alpha is allocated
+void sor_strips_alloc(flgpu_mat *A);
 void tp_prepare(flgpu_mat *T);  // (re)fill tp27 of an interpolation or its transpose
 void tp_drop(flgpu_mat *A);     // products of A (and its transpose) go back to SELL
 bool is_q1_lattice(const flgpu_mat *A);

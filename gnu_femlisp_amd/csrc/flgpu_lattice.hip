@@ -862,6 +862,7 @@ static void lattice_set_alpha(flgpu_mat *A, const flgpu_vec *kappa, int nk) {
   long ncells = (long)n * n * n;
   if (!A->alpha) {
     A->alpha = dev_alloc(prec_bytes(A->prec) * ncells);
+    sor_strips_alloc(A);
     layout_changed(A);
   }
   const double *kd = kappa ? (const double *)kappa->d : nullptr;

@@ -56,6 +56,11 @@ struct flgpu_mg {
   // attached learned correction
   flgpu_unet *unet = nullptr;
   const flgpu_vec *unet_kappa = nullptr;
+  // the learned correction's hook (its buffers) for the matrix hook_A of hook_N rows, kept
+  // between solves; dropped when another network is attached
+  std::unique_ptr<flgpu::LearnedHook> hook;
+  const flgpu_mat *hook_A = nullptr;
+  long hook_N = 0;
   int unet_n = 0;
   // an FP32 cycle for FP64 vectors: where the last top-level step may also store its result
   // in FP64 (set for one cycle; x64_done tells whether it did)
@@ -805,6 +810,10 @@ extern "C" int flgpu_mg_attach_unet(flgpu_mg *mg, flgpu_unet *u, const flgpu_vec
                   "kappa must be an FP64 vector of n^3 cell values");
     FLGPU_REQUIRE(!u || unet_ctx(u) == mg->ctx, "the network belongs to another context");
     FLGPU_REQUIRE(!kappa || kappa->ctx == mg->ctx, "kappa belongs to another context");
+    CTX_LOCK(mg->ctx);
+    mg->hook.reset();  // the next learned solve builds the hook for this network
+    mg->hook_A = nullptr;
+    mg->hook_N = 0;
     mg->unet = u;
     mg->unet_kappa = kappa;
     mg->unet_n = n;
@@ -873,7 +882,7 @@ extern "C" int flgpu_mg_use_graphs(flgpu_mg *mg, int enable) {
 // The learned correction is dropped when its step has added less than this fraction of the
 // preconditioned step's energy in this many FCG steps in a row.
 constexpr double kLearnedMinGain = 1e-2;
-constexpr int kLearnedWeakSteps = 5;
+constexpr int kLearnedWeakSteps = 3;
 
 namespace {
 
@@ -884,15 +893,24 @@ struct SolveWork {
   // t: the true residual b - A x when it is checked
   void *r = nullptr, *p = nullptr, *a = nullptr, *w = nullptr, *w2 = nullptr, *q = nullptr,
        *t = nullptr;
+  // the vectors are slices of the context's solve buffer, grown when too small and kept for
+  // the next solve: no allocation or release per solve once the size is reached
   SolveWork(flgpu_ctx *c, long n, int pr) : ctx(c), N(n), prec(pr) {
-    size_t b = prec_bytes(pr) * n;
+    const size_t b = (prec_bytes(pr) * (size_t)std::max(n, 1L) + 255) & ~(size_t)255;
+    if (c->solve_bytes < 7 * b) {
+      FLGPU_HIP(hipStreamSynchronize(c->stream));
+      dev_free(c->solve_buf);
+      c->solve_buf = nullptr;
+      c->solve_bytes = 0;
+      c->solve_buf = dev_alloc(7 * b);
+      c->solve_bytes = 7 * b;
+    }
+    char *base = (char *)c->solve_buf;
+    int k = 0;
     for (void **v : {&r, &p, &a, &w, &w2, &q, &t}) {
-      *v = dev_alloc(b);
+      *v = base + (k++) * b;
       vfill(c, n, pr, *v, 0.0);
     }
-  }
-  ~SolveWork() {
-    for (void *v : {r, p, a, w, w2, q, t}) dev_free(v);
   }
 };
 
@@ -945,9 +963,16 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
     int prec = A->prec;
     hipStream_t s = ctx->stream;
     SolveWork W(ctx, N, prec);
-    std::unique_ptr<LearnedHook> hook;
-    if (p->learned) hook = learned_hook(mg, A);
-    LearnedHook *learned = hook.get();
+    LearnedHook *learned = nullptr;
+    if (p->learned) {
+      if (!mg->hook || mg->hook_A != A || mg->hook_N != A->nrows) {
+        mg->hook.reset();
+        mg->hook = learned_hook(mg, A);
+        mg->hook_A = A;
+        mg->hook_N = A->nrows;
+      }
+      learned = mg->hook.get();
+    }
     double *slot = ctx->red_out;
     unsigned g = grid_for(N, 256);
     auto t0 = std::chrono::steady_clock::now();

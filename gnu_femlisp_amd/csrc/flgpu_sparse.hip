@@ -61,6 +61,47 @@ __global__ void k_sell_fill(int nsub, const int *rows, const int *rowptr, const 
   }
 }
 
+// The same for all rows (rows == nullptr) and rows of at most kSellLdsW entries: a block takes
+// one slice, reads the slice's CSR entries (contiguous) into LDS with coalesced loads, and
+// writes the slice column by column.
+constexpr int kSellLdsW = 32;
+
+template <class T>
+__global__ __launch_bounds__(64) void k_sell_fill_lds(int nrows, const int *__restrict__ rowptr,
+                                                      const int *__restrict__ colidx,
+                                                      const T *__restrict__ vals,
+                                                      const long *__restrict__ slice_ptr,
+                                                      int *scol, T *sval, int fill_cols) {
+  __shared__ T sv[64 * kSellLdsW];
+  __shared__ int sc[64 * kSellLdsW];
+  const int s = blockIdx.x, lane = threadIdx.x;
+  const int r0 = s * 64, r = r0 + lane;
+  const int e0 = rowptr[r0], e1 = rowptr[min(r0 + 64, nrows)];
+  for (int e = e0 + lane; e < e1; e += 64) {
+    sv[e - e0] = vals[e];
+    sc[e - e0] = colidx[e];
+  }
+  int b = 0, len = 0;
+  if (r < nrows) {
+    b = rowptr[r] - e0;
+    len = rowptr[r + 1] - rowptr[r];
+  }
+  __syncthreads();
+  const long base = slice_ptr[s];
+  const int w = (int)((slice_ptr[s + 1] - base) >> 6);
+  const int padc = len > 0 ? sc[b + len - 1] : 0;
+  for (int k = 0; k < w; k++) {
+    const long p = base + (long)k * 64 + lane;
+    if (k < len) {
+      if (fill_cols) scol[p] = sc[b + k];
+      sval[p] = sv[b + k];
+    } else {
+      if (fill_cols) scol[p] = padc;
+      sval[p] = (T)0;
+    }
+  }
+}
+
 void sell_free(Sell &s) {
   dev_free(s.rows);
   dev_free(s.slice_ptr);
@@ -72,6 +113,13 @@ void sell_free(Sell &s) {
 template <class T>
 static void sell_fill(flgpu_ctx *ctx, const flgpu_mat *A, Sell &s, int fill_cols) {
   if (s.nrows == 0) return;
+  if (!s.rows && s.maxw <= kSellLdsW) {
+    k_sell_fill_lds<T><<<s.nslices, 64, 0, ctx->stream>>>(s.nrows, A->rowptr, A->colidx,
+                                                          (const T *)A->vals, s.slice_ptr, s.col,
+                                                          (T *)s.val, fill_cols);
+    FLGPU_HIP(hipGetLastError());
+    return;
+  }
   int nthreads = s.nslices * 64;
   k_sell_fill<T><<<grid_for(nthreads, 256), 256, 0, ctx->stream>>>(
       s.nrows, s.rows, A->rowptr, A->colidx, (const T *)A->vals, s.slice_ptr, s.col, (T *)s.val,
@@ -215,6 +263,7 @@ void mat_free(flgpu_mat *A) {
   for (auto &s : A->colour_sell) sell_free(s);
   if (A->transpose) mat_free(A->transpose);
   dev_free(A->alpha);
+  dev_free(A->sor_ghost);
   dev_free(A->val27);
   dev_free(A->tp27);
   dev_free(A->blk_inv);
@@ -227,6 +276,9 @@ void drop_alpha(flgpu_mat *A) {
   if (A->alpha) layout_changed(A);
   dev_free(A->alpha);
   A->alpha = nullptr;
+  dev_free(A->sor_ghost);
+  A->sor_ghost = nullptr;
+  A->sor_strips = A->sor_h = 0;
 }
 
 void mat_refresh_values(flgpu_mat *A) {
@@ -350,6 +402,7 @@ flgpu_mat *mat_convert(const flgpu_mat *A, int prec) {
     B->alpha = dev_alloc(prec_bytes(prec) * nc);
     vcopy(ctx, nc, A->prec, A->alpha, prec, B->alpha);
     for (int j = 0; j < 8; j++) B->kt[j] = A->kt[j];
+    sor_strips_alloc(B);
   }
   mat_finish(B);
   if (A->transpose) B->transpose = mat_convert(A->transpose, prec);
@@ -816,6 +869,225 @@ __global__ __launch_bounds__(256) void k_q1_sor(int M, int n, int cz, int cy, in
   x[i] += omega * corr;
 }
 
+// ---- the per-cell <sor> sweep in plane strips ----------------------------------------------
+// Colours 0-3 lie on the even z planes, 4-7 on the odd ones, and a row couples to the other z
+// parity only through the planes z - 1 and z + 1, which no row of its own parity changes.  So
+// one launch sweeps the four colours of one parity.  A block owns the lines [y0, y1) (whole
+// lines in x) of one interior plane z.  It stages the plane's x over the lines
+// [y0 - 4, y1 + 4), b plus the terms of the planes z - 1 and z + 1 for every row of
+// [y0 - 3, y1 + 3), and the cell factors of the cell planes z - 1 and z summed per cell column.
+// Then it runs the four in-plane colours with a barrier between them.  Colour j of the four
+// needs the in-plane values of the colours before it one line away, so the block computes it
+// on 3 - j lines beyond its own on each side.  It does so from the same values with the same
+// arithmetic as the block owning those lines, so the values it uses equal the owner's bit for
+// bit.  The owner overwrites those lines in the same launch, so a block reads them from a copy
+// that k_q1_sor_prep takes before the sweep (gx, gb: for every plane z and strip boundary
+// sb = 1..S-1 the lines sb H - 4 .. sb H + 3).  k_q1_sor_prep also updates the Dirichlet
+// planes z = 0 and z = M - 1, whose rows couple to nothing.  The row sums b, then the
+// neighbours of planes z - 1 and z + 1, then the in-plane ones, and divide by A_ii.  That is
+// q1_row's arithmetic (linit.lisp:309-318) with the in-plane cell sums taken over cell columns.
+// A row updated here gets r_i = corr and x_i += omega corr, as in k_q1_sor.  b may be r: every
+// row's b is read before its r is written, by the owner or from the copy.
+template <class T>
+__global__ __launch_bounds__(256) void k_q1_sor_prep(int M, int H, int S, T omega, T *x,
+                                                     const T *b, T *r, T *gx, T *gb) {
+  const long M2 = (long)M * M;
+  const long nb = (long)(M - 2) * (S - 1) * 8 * M;  // boundary lines of the interior planes
+  const long nt = nb + 2 * M2;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nt;
+       t += (long)gridDim.x * blockDim.x) {
+    if (t < nb) {
+      const int xc = (int)(t % M);
+      long u = t / M;
+      const int k = (int)(u % 8);
+      u /= 8;
+      const int sb = (int)(u % (S - 1)) + 1, z = (int)(u / (S - 1)) + 1;
+      const int y = sb * H - 4 + k;
+      if (y < 0 || y >= M) continue;
+      const long g = ((long)(z * (S - 1) + sb - 1) * 8 + k) * M + xc;
+      const long i = ((long)z * M + y) * M + xc;
+      gx[g] = x[i];
+      gb[g] = b[i];
+    } else {
+      const long e = t - nb;
+      const long i = (e < M2 ? 0 : (long)(M - 1) * M2) + e % M2;
+      T corr = b[i];
+      corr += (-x[i]) * (T)1;
+      corr = corr / (T)1;
+      r[i] = corr;
+      x[i] += omega * corr;
+    }
+  }
+}
+
+constexpr int kSorBatch = 6;  // staged elements per thread and load batch
+
+// Index arithmetic is 32-bit (the lattice has fewer than 2^31 nodes) and divides by the
+// lattice widths through FastDiv (fM: M, fn: n, fx0 / fx1: the colour columns (M + 1) / 2 and
+// M / 2, 0 when that width is 1): a runtime integer division costs tens of VALU instructions,
+// and the kernel is bound by VALU issue.
+template <class T, int CAP>
+__global__ __launch_bounds__(1024) void k_q1_sor_strips(int M, int n, int pz, int H, int S,
+                                                        int backwards, FastDiv fM, FastDiv fn,
+                                                        FastDiv fx0, FastDiv fx1,
+                                                        const T *__restrict__ alpha, Q1K kt,
+                                                        T omega, T *__restrict__ x, const T *b,
+                                                        T *r, const T *__restrict__ gx,
+                                                        const T *__restrict__ gb) {
+  __shared__ T lds[CAP];
+  T *xs = lds;                   // x of plane z, line y0 - 4 + l
+  T *ps = xs + (H + 8) * M;      // b plus the off-plane terms, line y0 - 3 + l
+  T *bs = ps + (H + 6) * M;      // alpha(z - 1) + alpha(z) per cell column, cell line y0 - 4 + l
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = (int)(lb % S);
+  const int z = (pz == 0 ? 2 : 1) + 2 * (int)(lb / S);
+  if (z > M - 2) return;  // the grid's padding (whole blocks)
+  const int y0 = s * H, y1 = min(M, y0 + H);
+  const unsigned uM = (unsigned)M, un = (unsigned)n, M2 = uM * uM;
+  const unsigned zb = (unsigned)z * M2;                    // plane z
+  const unsigned cb0 = (unsigned)(z - 1) * un * un, cb1 = (unsigned)z * un * un;  // cell planes
+  // the copies of boundary s (lines y0 - 4 ..) and s + 1 (lines y1 - 4 ..); only read where
+  // that boundary exists
+  const unsigned ol = (unsigned)(z * (S - 1) + s - 1) * 8u * uM, ou = ol + 8u * uM;
+  // Staging is a chain of global loads; every thread issues its loads of x and of the cell
+  // columns for kSorBatch elements before it stores any, and the off-plane terms of two rows
+  // at a time, so that a block waits for a few memory latencies instead of one per element.
+  const int nxs = (H + 8) * M, nbs = (H + 7) * n;
+  for (int e0 = threadIdx.x; e0 < max(nxs, nbs); e0 += kSorBatch * blockDim.x) {
+    T xv[kSorBatch], bv[kSorBatch];
+#pragma unroll
+    for (int u = 0; u < kSorBatch; u++) {
+      const int e = e0 + u * blockDim.x;
+      xv[u] = (T)0;
+      bv[u] = (T)0;
+      if (e < nxs) {
+        const int l = (int)fdiv((unsigned)e, fM), xc = e - l * M, y = y0 - 4 + l;
+        if (y >= 0 && y < M) {
+          if (y < y0) xv[u] = gx[ol + (unsigned)((y - y0 + 4) * M + xc)];
+          else if (y >= y1) xv[u] = gx[ou + (unsigned)((y - y1 + 4) * M + xc)];
+          else xv[u] = x[zb + (unsigned)(y * M + xc)];
+        }
+      }
+      if (e < nbs) {
+        const int l = (int)fdiv((unsigned)e, fn), xc = e - l * n, yc = y0 - 4 + l;
+        if (yc >= 0 && yc < n) {
+          const unsigned o = (unsigned)(yc * n + xc);
+          bv[u] = alpha[cb0 + o] + alpha[cb1 + o];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSorBatch; u++) {
+      const int e = e0 + u * blockDim.x;
+      if (e < nxs) xs[e] = xv[u];
+      if (e < nbs) bs[e] = bv[u];
+    }
+  }
+  const bool lz[2] = {z - 1 != 0, z + 1 != M - 1};
+  const int nps = (H + 6) * M;
+#pragma unroll 2
+  for (int e = threadIdx.x; e < nps; e += blockDim.x) {
+    const int l = (int)fdiv((unsigned)e, fM), xc = e - l * M, y = y0 - 3 + l;
+    if (y < 0 || y >= M) continue;
+    const unsigned i = zb + (unsigned)(y * M + xc);
+    T acc;
+    if (y < y0) acc = gb[ol + (unsigned)((y - y0 + 4) * M + xc)];
+    else if (y >= y1) acc = gb[ou + (unsigned)((y - y1 + 4) * M + xc)];
+    else acc = b[i];
+    if (y != 0 && y != M - 1 && xc != 0 && xc != M - 1) {
+      const unsigned c0 = (unsigned)((y - 1) * n + (xc - 1));  // cell (y - 1, xc - 1) in-plane
+      T a[2][2][2];
+#pragma unroll
+      for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+          for (int c = 0; c < 2; c++)
+            a[p][q][c] = alpha[(p ? cb1 : cb0) + c0 + (unsigned)(q * n + c)];
+      const bool ly[3] = {y - 1 != 0, true, y + 1 != M - 1};
+      const bool lx[3] = {xc - 1 != 0, true, xc + 1 != M - 1};
+      T xn[2][9];
+#pragma unroll
+      for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          const int dy = k / 3 - 1, dx = k % 3 - 1;
+          const bool live = lz[p] && ly[dy + 1] && lx[dx + 1];
+          // read-only planes z - 1 and z + 1; a dead neighbour reads the row's own column
+          const unsigned j = (p ? i + M2 : i - M2) + (unsigned)(live ? dy * M + dx : 0);
+          xn[p][k] = x[j];
+        }
+#pragma unroll
+      for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          const int dy = k / 3 - 1, dx = k % 3 - 1;
+          const int q0 = dy > 0 ? 1 : 0, q1 = dy < 0 ? 0 : 1;
+          const int c0 = dx > 0 ? 1 : 0, c1 = dx < 0 ? 0 : 1;
+          T sum = (T)0;
+#pragma unroll
+          for (int q = q0; q <= q1; q++)
+#pragma unroll
+            for (int c = c0; c <= c1; c++) sum += a[p][q][c];
+          const T aij = (T)kt.k[4 + (dy != 0) * 2 + (dx != 0)] * sum;
+          const bool live = lz[p] && ly[dy + 1] && lx[dx + 1];
+          acc += ((T)-1 * xn[p][k]) * (live ? aij : (T)0);
+        }
+    }
+    ps[e] = acc;
+  }
+  __syncthreads();
+  for (int j = 0; j < 4; j++) {
+    const int c = backwards ? 3 - j : j;
+    const int cy = c >> 1, cx = c & 1;
+    int lo = max(0, y0 - (3 - j));
+    const int hi = min(M, y1 + (3 - j));
+    lo += (lo - cy) & 1;  // the first line of parity cy
+    const int nyl = hi > lo ? (hi - lo + 1) / 2 : 0, nxl = (M - cx + 1) / 2;
+    const FastDiv fx = cx ? fx1 : fx0;
+    for (int e = threadIdx.x; e < nyl * nxl; e += blockDim.x) {
+      const int ey = nxl > 1 ? (int)fdiv((unsigned)e, fx) : e;
+      const int y = lo + 2 * ey, xc = cx + 2 * (e - ey * nxl);
+      const int li = (y - y0 + 4) * M + xc;
+      const T xi = xs[li];
+      T acc = ps[(y - y0 + 3) * M + xc];
+      T d = (T)1;
+      if (y == 0 || y == M - 1 || xc == 0 || xc == M - 1) {
+        acc += (-xi) * (T)1;
+      } else {
+        const T *bc = bs + (y - y0 + 3) * n + (xc - 1);  // cell column (y - 1, xc - 1)
+        const T be[2][2] = {{bc[0], bc[1]}, {bc[n], bc[n + 1]}};
+        const bool ly[3] = {y - 1 != 0, true, y + 1 != M - 1};
+        const bool lx[3] = {xc - 1 != 0, true, xc + 1 != M - 1};
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          const int dy = k / 3 - 1, dx = k % 3 - 1;
+          const int q0 = dy > 0 ? 1 : 0, q1 = dy < 0 ? 0 : 1;
+          const int c0 = dx > 0 ? 1 : 0, c1 = dx < 0 ? 0 : 1;
+          T sum = (T)0;
+#pragma unroll
+          for (int q = q0; q <= q1; q++)
+#pragma unroll
+            for (int cc = c0; cc <= c1; cc++) sum += be[q][cc];
+          const T aij = (T)kt.k[(dy != 0) * 2 + (dx != 0)] * sum;
+          if (k == 4) d = aij;
+          const bool live = ly[dy + 1] && lx[dx + 1];
+          acc += ((T)-1 * xs[live ? li + dy * M + dx : li]) * (live ? aij : (T)0);
+        }
+      }
+      const T corr = acc / d;
+      const T xnew = xi + omega * corr;
+      xs[li] = xnew;
+      if (y >= y0 && y < y1) {
+        const unsigned i = zb + (unsigned)(y * M + xc);
+        r[i] = corr;
+        x[i] = xnew;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---- 3D Q1 lattice matrices in 27-point stencil storage ------------------------------
 
 // val27[k][i] = A(i, i + off_k) for the CSR entries of row i, 0 elsewhere
@@ -834,6 +1106,47 @@ __global__ void k_st27_fill(int M, const int *__restrict__ rowptr, const int *__
     // lattice patterns only hold neighbours; anything else is never stored
     if ((unsigned)ax > 2u || (unsigned)ay > 2u || (unsigned)az > 2u) continue;
     val27[(long)(az * 9 + ay * 3 + ax) * N + i] = vals[e];
+  }
+}
+
+// The same for rows of at most 27 entries on lattices of M >= 3: a block takes 128 rows, reads
+// their CSR entries (contiguous) into LDS with coalesced loads, and every thread walks its row
+// once along the 27 neighbour columns in ascending order (columns that are no lattice neighbour
+// are passed over), writing every plane with coalesced stores.
+constexpr int kSt27Rows = 128;
+
+template <class T>
+__global__ __launch_bounds__(kSt27Rows) void k_st27_fill_lds(int M,
+                                                             const int *__restrict__ rowptr,
+                                                             const int *__restrict__ colidx,
+                                                             const T *__restrict__ vals,
+                                                             T *val27) {
+  __shared__ T sv[kSt27Rows * 27];
+  __shared__ int sc[kSt27Rows * 27];
+  const long N = (long)M * M * M;
+  const long r0 = (long)blockIdx.x * kSt27Rows, i = r0 + threadIdx.x;
+  const int e0 = rowptr[r0], e1 = rowptr[min(r0 + kSt27Rows, N)];
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += kSt27Rows) {
+    sv[e - e0] = vals[e];
+    sc[e - e0] = colidx[e];
+  }
+  __syncthreads();
+  if (i >= N) return;
+  const long M2 = (long)M * M;
+  const int x = (int)(i % M), y = (int)((i / M) % M), z = (int)(i / M2);
+  int e = rowptr[i] - e0;
+  const int eend = rowptr[i + 1] - e0;
+#pragma unroll
+  for (int k = 0; k < 27; k++) {
+    const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+    T v = (T)0;
+    if ((unsigned)(x + dx) < (unsigned)M && (unsigned)(y + dy) < (unsigned)M &&
+        (unsigned)(z + dz) < (unsigned)M) {
+      const long j = i + dz * M2 + dy * M + dx;
+      while (e < eend && sc[e] < j) e++;
+      if (e < eend && sc[e] == j) v = sv[e++];
+    }
+    val27[k * N + i] = v;
   }
 }
 
@@ -950,14 +1263,24 @@ void stencil_refresh(flgpu_mat *A) {
   long N = A->nrows;
   int *asym = (int *)dev_alloc(sizeof(int));
   FLGPU_HIP(hipMemsetAsync(asym, 0, sizeof(int), A->ctx->stream));
+  const bool lds = M >= 3 && A->sell.maxw <= 27;
+  const unsigned gl = (unsigned)((N + kSt27Rows - 1) / kSt27Rows);
   if (A->prec == FLGPU_F64) {
-    k_st27_fill<double><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
-        M, A->rowptr, A->colidx, (const double *)A->vals, (double *)A->val27);
+    if (lds)
+      k_st27_fill_lds<double><<<gl, kSt27Rows, 0, A->ctx->stream>>>(
+          M, A->rowptr, A->colidx, (const double *)A->vals, (double *)A->val27);
+    else
+      k_st27_fill<double><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
+          M, A->rowptr, A->colidx, (const double *)A->vals, (double *)A->val27);
     k_st27_symcheck<double><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
         M, (const double *)A->val27, asym);
   } else {
-    k_st27_fill<float><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
-        M, A->rowptr, A->colidx, (const float *)A->vals, (float *)A->val27);
+    if (lds)
+      k_st27_fill_lds<float><<<gl, kSt27Rows, 0, A->ctx->stream>>>(
+          M, A->rowptr, A->colidx, (const float *)A->vals, (float *)A->val27);
+    else
+      k_st27_fill<float><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
+          M, A->rowptr, A->colidx, (const float *)A->vals, (float *)A->val27);
     k_st27_symcheck<float><<<grid_for(N, 256), 256, 0, A->ctx->stream>>>(
         M, (const float *)A->val27, asym);
   }
@@ -1630,7 +1953,69 @@ __global__ __launch_bounds__(256) void k_sor_colour(int nsub, int nslices, const
   x[row] += omega * corr;
 }
 
+// LDS of the strip sweep (values): 128 KB in FP64, 64 KB in FP32
+constexpr int kSorCapF64 = 16384, kSorCapF32 = 16384;
+
+// values the strip sweep stages for strips of H lines on an M^3 lattice
+static long sor_strip_lds(int M, int H) {
+  return (long)(H + 8) * M + (long)(H + 6) * M + (long)(H + 7) * (M - 1);
+}
+
+void sor_strips_alloc(flgpu_mat *A) {
+  dev_free(A->sor_ghost);
+  A->sor_ghost = nullptr;
+  A->sor_strips = A->sor_h = 0;
+  const int M = A->lat_n + 1;
+  if (A->lat_dim != 3 || A->lat_order != 1 || M < 3) return;
+  const long cap = A->prec == FLGPU_F64 ? kSorCapF64 : kSorCapF32;
+  // about one block per CU and launch, strips of at least 8 lines, within the LDS (two blocks
+  // per CU in FP32 with strips half as high measured slower: 63 against 48 us at 129^3)
+  const int planes = (M - 1) / 2;
+  int S = std::max(1, std::min((256 + planes - 1) / planes, (M + 7) / 8));
+  while (S <= M && sor_strip_lds(M, (M + S - 1) / S) > cap) S++;
+  if (S > M) return;  // lattices this wide keep the colour launches
+  const int H = (M + S - 1) / S;
+  S = (M + H - 1) / H;  // no empty strip
+  A->sor_strips = S;
+  A->sor_h = H;
+  if (S > 1)
+    A->sor_ghost = dev_alloc(prec_bytes(A->prec) * 2 * (size_t)M * (S - 1) * 8 * M);
+}
+
+template <class T, int CAP>
+static void q1_sor_strips(const flgpu_mat *A, T omega, int backwards, T *x, const T *b, T *r) {
+  const int M = A->lat_n + 1, S = A->sor_strips, H = A->sor_h;
+  FLGPU_REQUIRE(sor_strip_lds(M, H) <= CAP, "strip sweep: LDS size");
+  T *gx = (T *)A->sor_ghost, *gb = gx ? gx + (long)M * (S - 1) * 8 * M : nullptr;
+  const long prep = (long)(M - 2) * (S - 1) * 8 * M + 2L * M * M;
+  k_q1_sor_prep<T><<<grid_for(prep, 256), 256, 0, A->ctx->stream>>>(M, H, S, omega, x, b, r, gx,
+                                                                      gb);
+  FLGPU_HIP(hipGetLastError());
+  for (int k = 0; k < 2; k++) {
+    const int pz = backwards ? 1 - k : k;
+    const int z0 = pz == 0 ? 2 : 1;
+    if (z0 > M - 2) continue;
+    const int planes = (M - 2 - z0) / 2 + 1;
+    const unsigned g = xcd_grid((unsigned)(planes * S));
+    const FastDiv fx1 = M / 2 > 1 ? fast_div((unsigned)(M / 2)) : FastDiv{1, 0, 0};
+    k_q1_sor_strips<T, CAP><<<g, 1024, 0, A->ctx->stream>>>(
+        M, A->lat_n, pz, H, S, backwards, fast_div((unsigned)M), fast_div((unsigned)A->lat_n),
+        fast_div((unsigned)((M + 1) / 2)), fx1, (const T *)A->alpha, q1k(A), omega, x, b, r, gx,
+        gb);
+    FLGPU_HIP(hipGetLastError());
+  }
+}
+
 void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const void *b, void *r) {
+  if (A->alpha && A->sor_strips > 0) {
+    if (A->prec == FLGPU_F64)
+      q1_sor_strips<double, kSorCapF64>(A, omega, backwards, (double *)x, (const double *)b,
+                                        (double *)r);
+    else
+      q1_sor_strips<float, kSorCapF32>(A, (float)omega, backwards, (float *)x,
+                                       (const float *)b, (float *)r);
+    return;
+  }
   if (A->alpha || A->val27) {
     // lattice colours in the order of the lattice colouring: cz*4 + cy*2 + cx
     int M = A->lat_n + 1;
@@ -1736,6 +2121,72 @@ __global__ void k_rap(int nc, const int *__restrict__ ptrp, const int *__restric
   }
 }
 
+// The same with the coarse row's columns and accumulators in LDS (coarse rows of at most
+// kRapW entries): the column searches and the FP64 sums stay on the CU instead of reaching
+// device memory for every contribution.  Same loops, same order, same results.
+constexpr int kRapW = 27, kRapB = 128;
+
+template <class T>
+__global__ __launch_bounds__(kRapB) void k_rap_lds(
+    int nc, const int *__restrict__ ptrp, const int *__restrict__ ptci, const T *__restrict__ ptv,
+    const int *__restrict__ arp, const int *__restrict__ aci, const T *__restrict__ av,
+    const int *__restrict__ prp, const int *__restrict__ pci, const T *__restrict__ pv,
+    const int *__restrict__ crp, const int *__restrict__ cci, double *acc, int *bad, int upper) {
+  __shared__ double sacc[kRapW * kRapB];  // entry j of this thread's row at j * kRapB + tid
+  __shared__ int scol[kRapW * kRapB];
+  const int tid = threadIdx.x;
+  const int k = blockIdx.x * kRapB + tid;
+  if (k >= nc) return;
+  const int c0 = crp[k], w = crp[k + 1] - c0;
+  for (int j = 0; j < w; j++) {
+    scol[j * kRapB + tid] = cci[c0 + j];
+    sacc[j * kRapB + tid] = 0.0;
+  }
+  for (int a = ptrp[k]; a < ptrp[k + 1]; a++) {
+    const int l = ptci[a];
+    const double plk = (double)ptv[a];
+    for (int b = arp[l]; b < arp[l + 1]; b++) {
+      const int m = aci[b];
+      const double wv = plk * (double)av[b];
+      for (int c = prp[m]; c < prp[m + 1]; c++) {
+        const int n = pci[c];
+        if (upper && n < k) continue;  // the lower triangle is mirrored afterwards
+        int lo = 0, hi = w - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (scol[mid * kRapB + tid] < n) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo >= w || scol[lo * kRapB + tid] != n) {
+          *bad = 1;
+          continue;
+        }
+        sacc[lo * kRapB + tid] += wv * (double)pv[c];
+      }
+    }
+  }
+  for (int j = 0; j < w; j++) acc[c0 + j] = sacc[j * kRapB + tid];
+}
+
+__global__ void k_max_row(int n, const int *__restrict__ rp, int *mx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicMax(mx, rp[i + 1] - rp[i]);
+}
+
+// the longest row of A's CSR (A's SELL may not be built yet)
+static int max_row_length(const flgpu_mat *A) {
+  if (A->nrows == 0) return 0;
+  int *dm = (int *)dev_alloc(sizeof(int));
+  FLGPU_HIP(hipMemsetAsync(dm, 0, sizeof(int), A->ctx->stream));
+  k_max_row<<<grid_for(A->nrows, 256), 256, 0, A->ctx->stream>>>(A->nrows, A->rowptr, dm);
+  FLGPU_HIP(hipGetLastError());
+  int h = 0;
+  FLGPU_HIP(hipMemcpyAsync(&h, dm, sizeof(int), hipMemcpyDeviceToHost, A->ctx->stream));
+  FLGPU_HIP(hipStreamSynchronize(A->ctx->stream));
+  dev_free(dm);
+  return h;
+}
+
 template <class T>
 __global__ void k_store(long n, const double *acc, T *v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1818,8 +2269,20 @@ void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac) {
   FLGPU_HIP(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
   int nc = Ac->nrows;
   int *missing = (int *)dev_alloc(sizeof(int));
+  const bool lds = max_row_length(Ac) <= kRapW;
   auto rap = [&](int upper) {
-    if (A->prec == FLGPU_F64)
+    const unsigned gl = (unsigned)((nc + kRapB - 1) / kRapB);
+    if (lds && A->prec == FLGPU_F64)
+      k_rap_lds<double><<<gl, kRapB, 0, ctx->stream>>>(
+          nc, PT->rowptr, PT->colidx, (const double *)PT->vals, A->rowptr, A->colidx,
+          (const double *)A->vals, P->rowptr, P->colidx, (const double *)P->vals, Ac->rowptr,
+          Ac->colidx, acc, bad, upper);
+    else if (lds)
+      k_rap_lds<float><<<gl, kRapB, 0, ctx->stream>>>(
+          nc, PT->rowptr, PT->colidx, (const float *)PT->vals, A->rowptr, A->colidx,
+          (const float *)A->vals, P->rowptr, P->colidx, (const float *)P->vals, Ac->rowptr,
+          Ac->colidx, acc, bad, upper);
+    else if (A->prec == FLGPU_F64)
       k_rap<double><<<grid_for(nc, 128), 128, 0, ctx->stream>>>(
           nc, PT->rowptr, PT->colidx, (const double *)PT->vals, A->rowptr, A->colidx,
           (const double *)A->vals, P->rowptr, P->colidx, (const double *)P->vals, Ac->rowptr,

@@ -11,8 +11,9 @@
 // implicit GEMMs on v_mfma_f32_16x16x16bf16_1k: M = voxels (tiles of 16),
 // N = output channels (tiles of 16), K = 27 taps x input channels (chunks of
 // 16); activations and weights are rounded to BF16 when they enter the
-// matrix cores, products accumulate in FP32.  Weight gradients are FP32 sums
-// staged through LDS with per-chunk partials reduced in a fixed order.
+// matrix cores, products accumulate in FP32.  Weight gradients are implicit
+// GEMMs on the same instruction (voxels as K) with per-chunk partials reduced
+// in a fixed order.
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -110,84 +111,118 @@ __global__ __launch_bounds__(256) void k_conv_mfma(int batch, int n, int ci, int
 }
 
 // ---- weight gradients: gW[t][c][o] = sum_v in[v + off(t)][c] * gout[v][o] -------------
+// An implicit GEMM on v_mfma_f32_16x16x16bf16_1k per tap: M = input channels (tiles of 16),
+// N = output channels (tiles of 16), K = voxels (16 per product).  A block takes one tap and a
+// chunk of kWgVox voxels; it stages kWgChunk voxels at a time in LDS (the tap-shifted inputs
+// and the output gradients, rounded to BF16), and each wave accumulates up to kWgTiles
+// (c, o) tiles in FP32.  Blocks of the 27 taps of a chunk are adjacent in the grid, so the
+// chunk's values are read from HBM once and from L2 by the other taps.  The centre tap's
+// blocks also sum gout per channel in FP32 (the bias gradient).  Every block writes its sums
+// as a partial [chunk][27 ci co + co], reduced over the chunks in a fixed order.
 
 constexpr int kWgChunk = 64;  // voxels staged per LDS round
 constexpr int kWgVox = 4096;  // voxels per block (partial sum)
+constexpr int kWgTiles = 4;   // (c, o) tiles per wave: 16 tiles of 16 x 16 for 4 waves
 
 __global__ __launch_bounds__(256) void k_conv_wgrad(int batch, int n, int ci, int co,
                                                     const float *__restrict__ in,
                                                     const float *__restrict__ gout,
                                                     float *partial) {
-  extern __shared__ float sh[];
-  float *sin = sh;                    // [kWgChunk][ci]
-  float *sg = sh + kWgChunk * ci;     // [kWgChunk][co]
-  int tap = blockIdx.y;
-  long chunk = blockIdx.x;
-  long nv = (long)n * n * n, total = batch * nv;
-  int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-  int npair = ci * co;
-  float acc[16];
-  for (int j = 0; j < 16; j++) acc[j] = 0.f;
-  long vbeg = chunk * kWgVox, vend = min(vbeg + kWgVox, total);
+  __shared__ short sin[kWgChunk * 96];  // [voxel][c], c padded to a multiple of 16
+  __shared__ short sg[kWgChunk * 64];   // [voxel][o]
+  __shared__ long soff[kWgChunk];       // element offset of the shifted input voxel, -1: zero
+  __shared__ float sb[256];
+  const int cip = (ci + 15) & ~15, cop = (co + 15) & ~15;
+  const int tap = (int)(blockIdx.x % 27);
+  const long chunk = blockIdx.x / 27;
+  const long nv = (long)n * n * n, total = batch * nv;
+  const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+  const int nto = cop / 16, ntiles = (cip / 16) * nto;
+  v4f acc[kWgTiles];
+  for (int t = 0; t < kWgTiles; t++) acc[t] = (v4f){0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;  // centre tap: this thread's channel o = threadIdx.x % cop
+  const long vbeg = chunk * kWgVox, vend = min(vbeg + kWgVox, total);
   for (long vs = vbeg; vs < vend; vs += kWgChunk) {
     __syncthreads();
-    for (int e = threadIdx.x; e < kWgChunk * ci; e += blockDim.x) {
-      int k = e / ci, c = e % ci;
-      long v = vs + k;
-      float val = 0.f;
+    if (threadIdx.x < kWgChunk) {
+      const long v = vs + threadIdx.x;
+      long off = -1;
       if (v < vend) {
-        int b = (int)(v / nv);
-        long r = v % nv;
-        int z = (int)(r / ((long)n * n)) + dz, y = (int)((r / n) % n) + dy, x = (int)(r % n) + dx;
+        const int b = (int)(v / nv);
+        const long r = v % nv;
+        const int z = (int)(r / ((long)n * n)) + dz, y = (int)((r / n) % n) + dy,
+                  x = (int)(r % n) + dx;
         if (z >= 0 && z < n && y >= 0 && y < n && x >= 0 && x < n)
-          val = in[((long)b * nv + ((long)z * n + y) * n + x) * ci + c];
+          off = ((long)b * nv + ((long)z * n + y) * n + x) * ci;
       }
-      sin[e] = val;
-    }
-    for (int e = threadIdx.x; e < kWgChunk * co; e += blockDim.x) {
-      int k = e / co, o = e % co;
-      long v = vs + k;
-      sg[e] = (v < vend) ? gout[v * co + o] : 0.f;
+      soff[threadIdx.x] = off;
     }
     __syncthreads();
-    for (int j = 0; j < 16; j++) {
-      int p = threadIdx.x + j * 256;
-      if (p >= npair) break;
-      int c = p / co, o = p % co;
-      float s = acc[j];
-      for (int k = 0; k < kWgChunk; k++) s += sin[k * ci + c] * sg[k * co + o];
-      acc[j] = s;
+    for (int e = threadIdx.x; e < kWgChunk * cip; e += blockDim.x) {
+      const int k = e / cip, c = e - k * cip;
+      const long off = soff[k];
+      sin[e] = bf16_bits(off >= 0 && c < ci ? in[off + c] : 0.f);
+    }
+    for (int e = threadIdx.x; e < kWgChunk * cop; e += blockDim.x) {
+      const int k = e / cop, o = e - k * cop;
+      const long v = vs + k;
+      const float val = v < vend && o < co ? gout[v * co + o] : 0.f;
+      bsum += val;
+      sg[e] = bf16_bits(val);
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int k0 = 0; k0 < kWgChunk; k0 += 16) {
+#pragma unroll
+      for (int t = 0; t < kWgTiles; t++) {
+        const int tile = w + 4 * t;
+        if (tile >= ntiles) break;  // wave-uniform
+        const int c0 = (tile / nto) * 16, o0 = (tile % nto) * 16;
+        v4s a, bb;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          a[j] = sin[(k0 + 4 * g + j) * cip + c0 + i];   // A[c0 + i][voxel 4g + j]
+          bb[j] = sg[(k0 + 4 * g + j) * cop + o0 + i];   // B[voxel 4g + j][o0 + i]
+        }
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[t], 0, 0, 0);
+      }
     }
   }
-  for (int j = 0; j < 16; j++) {
-    int p = threadIdx.x + j * 256;
-    if (p >= npair) break;
-    partial[(chunk * 27 + tap) * (long)npair + p] = acc[j];
+  const long npair = (long)ci * co, stride = 27 * npair + co;
+  float *part = partial + chunk * stride;
+  // D[4g + q][i]: input channel c0 + 4g + q, output channel o0 + i
+#pragma unroll
+  for (int t = 0; t < kWgTiles; t++) {
+    const int tile = w + 4 * t;
+    if (tile >= ntiles) break;
+    const int c0 = (tile / nto) * 16, o = (tile % nto) * 16 + i;
+    for (int q = 0; q < 4; q++) {
+      const int c = c0 + 4 * g + q;
+      if (c < ci && o < co) part[tap * npair + (long)c * co + o] = acc[t][q];
+    }
+  }
+  if (tap == 13) {  // bias: threads with the same channel summed in thread order
+    sb[threadIdx.x] = bsum;
+    __syncthreads();
+    if (threadIdx.x < co) {
+      float s = 0.f;
+      for (int k = threadIdx.x; k < 256; k += cop) s += sb[k];
+      part[27 * npair + threadIdx.x] = s;
+    }
   }
 }
 
-// gW[t][c][o] = sum over chunks (fixed order) of the partials
-__global__ void k_wgrad_reduce(long nchunks, int npair27, const float *partial, float *gw) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npair27) return;
+// gW[t][c][o] and gb[o] = sums over the chunks (fixed order) of the partials
+__global__ void k_wgrad_reduce(long nchunks, int np27, int co, const float *partial, float *gw,
+                               float *gb) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np27 + co) return;
+  const long stride = (long)np27 + co;
   float s = 0.f;
-  for (long c = 0; c < nchunks; c++) s += partial[c * npair27 + p];
-  gw[p] = s;
-}
-
-// gb[o] = sum_v gout[v][o]  (one block per output channel, fixed tree)
-__global__ __launch_bounds__(256) void k_bgrad(long total, int co, const float *gout, float *gb) {
-  __shared__ float s[256];
-  int o = blockIdx.x;
-  float acc = 0.f;
-  for (long v = threadIdx.x; v < total; v += 256) acc += gout[v * co + o];
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) gb[o] = s[0];
+  for (long c = 0; c < nchunks; c++) s += partial[c * stride + p];
+  if (p < np27) gw[p] = s;
+  else gb[p - np27] = s;
 }
 
 // ---- elementwise / resampling ------------------------------------------------------------
@@ -432,20 +467,22 @@ static void conv_wgrad(flgpu_unet *u, int l, int batch, int n, int ci, const flo
   int co = L_COUT[l];
   long total = vox(batch, n);
   long nchunks = (total + kWgVox - 1) / kWgVox;
-  long need = nchunks * 27L * ci * co;
+  const int np27 = 27 * ci * co;
+  long need = nchunks * (np27 + (long)co);
   if (need > u->partial_cap) {
     dev_free(u->partial);
     u->partial = (float *)dev_alloc(sizeof(float) * need);
     u->partial_cap = need;
   }
-  FLGPU_REQUIRE(ci * co <= 16 * 256, "layer too wide for the weight-gradient kernel");
-  size_t shm = sizeof(float) * kWgChunk * (ci + co);
+  const int cip = (ci + 15) & ~15, cop = (co + 15) & ~15;
+  FLGPU_REQUIRE(cip <= 96 && cop <= 64 && (cip / 16) * (cop / 16) <= 4 * kWgTiles &&
+                    256 % cop == 0,
+                "layer too wide for the weight-gradient kernel");
   hipStream_t s = u->ctx->stream;
-  k_conv_wgrad<<<dim3((unsigned)nchunks, 27), 256, shm, s>>>(batch, n, ci, co, in, gout, u->partial);
-  int np27 = 27 * ci * co;
-  k_wgrad_reduce<<<grid_for(np27, 256), 256, 0, s>>>(nchunks, np27, u->partial,
-                                                     u->grad + u->woff[l]);
-  k_bgrad<<<co, 256, 0, s>>>(total, co, gout, u->grad + u->boff[l]);
+  k_conv_wgrad<<<(unsigned)(nchunks * 27), 256, 0, s>>>(batch, n, ci, co, in, gout, u->partial);
+  k_wgrad_reduce<<<grid_for(np27 + co, 256), 256, 0, s>>>(nchunks, np27, co, u->partial,
+                                                          u->grad + u->woff[l],
+                                                          u->grad + u->boff[l]);
   FLGPU_HIP(hipGetLastError());
 }
 

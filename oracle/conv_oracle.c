@@ -19,8 +19,9 @@
  *
  * Loss: mean over all output values of (y - t)^2.  Double precision
  * throughout; `bf16` = 1 rounds the operands that the GPU feeds to its BF16
- * matrix cores (activations entering a convolution, weights, and the
- * incoming gradient of the input-gradient convolution) to bfloat16 first.
+ * matrix cores (activations entering a convolution, weights, the incoming
+ * gradient of the input-gradient convolution, and both factors of the weight
+ * gradient's products) to bfloat16 first.
  */
 #include <math.h>
 #include <stdint.h>
@@ -106,7 +107,7 @@ static void conv_bwd(int batch, int n, int ci, int co, const double *in, const d
               double *gw = gW + ((long)t * ci + c) * co;
               double s = 0.0;
               for (int k = 0; k < co; k++) {
-                gw[k] += ip[c] * g[k];
+                gw[k] += bf16 ? bf16r(ip[c]) * bf16r(g[k]) : ip[c] * g[k];
                 if (gi) s += (bf16 ? bf16r(g[k]) : g[k]) * (bf16 ? bf16r(wr[k]) : wr[k]);
               }
               if (gi) gi[c] += s;

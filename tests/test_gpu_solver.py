@@ -277,12 +277,14 @@ def test_concurrent_contexts_and_shared_context():
         assert np.array_equal(z.numpy(), s)
 
 
-@pytest.mark.parametrize("n", [40, 70])
-@pytest.mark.parametrize("prec", [F.F64, F.F32])
+@pytest.mark.parametrize("n,prec", [(40, F.F64), (40, F.F32), (70, F.F64), (70, F.F32),
+                                    (128, F.F32)])
 def test_cell_sor_sweeps_over_many_tiles(n, prec):
-    """the per-cell <sor> sweep on larger lattices (n = 40, 70: several thread blocks per
-    colour, ragged ends) equals the SELL colour sweep, forwards and backwards, and with rhs and
-    residual one vector (femlisp's (iterate q w w))"""
+    """the per-cell <sor> sweep on larger lattices (strips per plane: n = 40, 6 of 7 lines;
+    n = 70, 8 of 9; n = 128, 4 of 33; the last strip one to three lines shorter) equals the
+    SELL colour
+    sweep, forwards and backwards, and with rhs and residual one vector (femlisp's
+    (iterate q w w))"""
     say(f"cell sor n={n} prec={prec}")
     rng = np.random.default_rng(n)
     kap = np.exp(rng.normal(scale=1.5, size=(n, n, n)))
