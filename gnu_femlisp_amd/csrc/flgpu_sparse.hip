@@ -2214,6 +2214,28 @@ __global__ void k_csr_symcheck(int n, const int *__restrict__ rp, const int *__r
   if (bad) asym[0] = 1;
 }
 
+// The same with one lane per entry (32 lanes per row, rows of at most 32 entries): the binary
+// searches of a row run side by side instead of one after another.
+template <class T>
+__global__ __launch_bounds__(256) void k_csr_symcheck32(int n, const int *__restrict__ rp,
+                                                        const int *__restrict__ ci,
+                                                        const T *__restrict__ v, int *asym) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(t >> 5), k = (int)(t & 31);
+  if (i >= n) return;
+  const int e = rp[i] + k;
+  if (e >= rp[i + 1]) return;
+  const int j = ci[e];
+  if (j == i) return;
+  int lo = rp[j], hi = rp[j + 1] - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (ci[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo > hi || ci[lo] != i || !(v[lo] == v[e])) asym[0] = 1;
+}
+
 // the lower triangle of the coarse accumulators from the upper one (missing[0] = 1 when the
 // pattern has no transposed position)
 __global__ void k_mirror(int nc, const int *__restrict__ crp, const int *__restrict__ cci,
@@ -2239,12 +2261,23 @@ static bool csr_symmetric(const flgpu_mat *A) {
   flgpu_ctx *ctx = A->ctx;
   int *asym = (int *)dev_alloc(sizeof(int));
   FLGPU_HIP(hipMemsetAsync(asym, 0, sizeof(int), ctx->stream));
-  if (A->prec == FLGPU_F64)
-    k_csr_symcheck<double><<<grid_for(A->nrows, 256), 256, 0, ctx->stream>>>(
-        A->nrows, A->rowptr, A->colidx, (const double *)A->vals, asym);
-  else
-    k_csr_symcheck<float><<<grid_for(A->nrows, 256), 256, 0, ctx->stream>>>(
-        A->nrows, A->rowptr, A->colidx, (const float *)A->vals, asym);
+  const bool lanes = max_row_length(A) <= 32;
+  const unsigned g32 = grid_for(32L * A->nrows, 256);
+  if (A->prec == FLGPU_F64) {
+    if (lanes)
+      k_csr_symcheck32<double><<<g32, 256, 0, ctx->stream>>>(A->nrows, A->rowptr, A->colidx,
+                                                              (const double *)A->vals, asym);
+    else
+      k_csr_symcheck<double><<<grid_for(A->nrows, 256), 256, 0, ctx->stream>>>(
+          A->nrows, A->rowptr, A->colidx, (const double *)A->vals, asym);
+  } else {
+    if (lanes)
+      k_csr_symcheck32<float><<<g32, 256, 0, ctx->stream>>>(A->nrows, A->rowptr, A->colidx,
+                                                             (const float *)A->vals, asym);
+    else
+      k_csr_symcheck<float><<<grid_for(A->nrows, 256), 256, 0, ctx->stream>>>(
+          A->nrows, A->rowptr, A->colidx, (const float *)A->vals, asym);
+  }
   FLGPU_HIP(hipGetLastError());
   int h = 1;
   FLGPU_HIP(hipMemcpyAsync(&h, asym, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
