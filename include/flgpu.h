@@ -414,8 +414,8 @@ typedef struct flgpu_solve_params {
   int learned;         /* 1: add the learned coarse correction (FCG only) */
   int stall_window;    /* learned: fall back to plain GMG-FCG when the true
                           residual has not dropped by 10x in this many steps, or
-                          when the network's step has added less than 1% of the
-                          preconditioned step's energy in 3 steps in a row */
+                          as soon as the network's step has added less than 1% of the
+                          preconditioned step's energy (a restart from the residual) */
   int restart_cycle;   /* <cg>'s restart-cycle (krylow.lisp:68-70,101-105,128-131): 0 never;
                           else every restart_cycle CG/FCG steps the step leaves no
                           :residual-after, the loop recomputes r = b - A x and the next step

@@ -215,3 +215,33 @@ def test_learned_fcg_and_fallback():
                   {"matrix": A, "rhs": b, "hierarchy": H})
     assert fb["status"] == "success" and fb["true_residual"] / bn < 1e-10
     assert fb["fell_back"]
+
+
+def test_learned_solves_reuse_the_hook():
+    """The hierarchy keeps the learned correction's buffers between solves: two learned solves
+    in a row give the same result bit for bit, a solve after another network is attached
+    equals one on a fresh hierarchy with that network, and the first network again gives the
+    first result."""
+    say("learned hook reuse")
+    n = 16
+    kap = _problem(n, 4242, "inclusions")
+    kv = GpuVector.from_array(kap.ravel(), F.F64)
+    nets = [learned.UNet(2, 5), learned.UNet(2, 6)]
+
+    def solve(H, net):
+        learned.attach(H, net, kv, n)
+        r = it.solve(it.LinearSolver(it.FCG(cs), true_residual=True, learned=True,
+                                     stall_window=1000, failure_if=(">", "step", 200)),
+                     {"matrix": H.top_matrix(), "rhs": H.rhs, "hierarchy": H})
+        assert r["status"] == "success"
+        return r["steps"], r["solution"].numpy()
+
+    cs = it.GeometricCS(it.SOR(1.0), 2, 2, 1, symmetric=True)
+    H = it.Hierarchy.lattice(cs, 3, 1, 1, 5, kappa=(kv, n))
+    a1, a2 = solve(H, nets[0]), solve(H, nets[0])
+    b = solve(H, nets[1])
+    a3 = solve(H, nets[0])
+    Hf = it.Hierarchy.lattice(cs, 3, 1, 1, 5, kappa=(kv, n))
+    bf = solve(Hf, nets[1])
+    for x, y in ((a1, a2), (a1, a3), (b, bf)):
+        assert x[0] == y[0] and np.array_equal(x[1], y[1])

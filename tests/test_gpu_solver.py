@@ -213,6 +213,33 @@ def test_graphs_follow_level_matrix_changes(smoother):
         assert np.array_equal(g["solution"].numpy(), e["solution"].numpy())
 
 
+def test_solve_work_vectors_reused_across_sizes():
+    """A context keeps flgpu_solve's work vectors between solves and grows them when a larger
+    system comes: solves of different sizes and precisions in turn on one context equal the
+    same solves on fresh contexts bit for bit."""
+    say("solve work reuse")
+    shared = Context(0)
+
+    def run(ctx, n, prec):
+        kap = _kap(n, n, n)
+        if prec == F.F64:
+            cs = it.GeometricCS(it.SOR(1.0), 2, 2, 1, symmetric=True)
+            H = it.Hierarchy.lattice(cs, 3, 1, 1, int(np.log2(n)) + 1, kappa=kap, ctx=ctx)
+            r = it.solve(it.LinearSolver(it.CG(cs), failure_if=(">", "step", 100)),
+                         {"matrix": H.top_matrix(), "rhs": H.rhs, "hierarchy": H})
+        else:
+            A, b = lat.assemble(3, 1, n, kap, prec=prec, ctx=ctx)
+            r = it.solve(it.LinearSolver(it.CG(), success_if=("<", "reduction", 1e-5),
+                                         failure_if=(">", "step", 400)),
+                         {"matrix": A, "rhs": b})
+        assert r["status"] == "success"
+        return r["steps"], r["solution"].numpy()
+
+    for n, prec in ((16, F.F64), (8, F.F64), (16, F.F32), (32, F.F64), (8, F.F32)):
+        a, b = run(shared, n, prec), run(Context(0), n, prec)
+        assert a[0] == b[0] and np.array_equal(a[1], b[1]), (n, prec)
+
+
 def test_concurrent_contexts_and_shared_context():
     """Calls from several threads (femlisp's lparallel workers): solves on two contexts run
     concurrently, and products on one shared context are serialised by its lock; both give
