@@ -889,28 +889,29 @@ __global__ __launch_bounds__(256) void k_q1_sor(int M, int n, int cz, int cy, in
 // A row updated here gets r_i = corr and x_i += omega corr, as in k_q1_sor.  b may be r: every
 // row's b is read before its r is written, by the owner or from the copy.
 template <class T>
-__global__ __launch_bounds__(256) void k_q1_sor_prep(int M, int H, int S, T omega, T *x,
-                                                     const T *b, T *r, T *gx, T *gb) {
-  const long M2 = (long)M * M;
-  const long nb = (long)(M - 2) * (S - 1) * 8 * M;  // boundary lines of the interior planes
-  const long nt = nb + 2 * M2;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nt;
-       t += (long)gridDim.x * blockDim.x) {
+__global__ __launch_bounds__(256) void k_q1_sor_prep(int M, int H, int S, FastDiv fM,
+                                                     FastDiv fS1, T omega, T *x, const T *b,
+                                                     T *r, T *gx, T *gb) {
+  const unsigned uM = (unsigned)M, M2 = uM * uM;
+  const unsigned nb = (unsigned)(M - 2) * (unsigned)(S - 1) * 8u * uM;  // boundary lines
+  const unsigned nt = nb + 2u * M2;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
     if (t < nb) {
-      const int xc = (int)(t % M);
-      long u = t / M;
-      const int k = (int)(u % 8);
-      u /= 8;
-      const int sb = (int)(u % (S - 1)) + 1, z = (int)(u / (S - 1)) + 1;
-      const int y = sb * H - 4 + k;
+      unsigned u = fdiv(t, fM);
+      const unsigned xc = t - u * uM, k = u & 7u;
+      u >>= 3;  // (z - 1) (S - 1) + sb - 1
+      const unsigned zq = S > 2 ? fdiv(u, fS1) : u;
+      const int sb = (int)(u - zq * (unsigned)(S - 1)) + 1, z = (int)zq + 1;
+      const int y = sb * H - 4 + (int)k;
       if (y < 0 || y >= M) continue;
-      const long g = ((long)(z * (S - 1) + sb - 1) * 8 + k) * M + xc;
-      const long i = ((long)z * M + y) * M + xc;
+      // copy index ((z (S - 1) + sb - 1) 8 + k) M + xc, u = (z - 1) (S - 1) + sb - 1
+      const unsigned g = ((u + (unsigned)(S - 1)) * 8u + k) * uM + xc;
+      const unsigned i = ((unsigned)z * uM + (unsigned)y) * uM + xc;
       gx[g] = x[i];
       gb[g] = b[i];
     } else {
-      const long e = t - nb;
-      const long i = (e < M2 ? 0 : (long)(M - 1) * M2) + e % M2;
+      const unsigned e = t - nb;
+      const unsigned i = (e < M2 ? 0u : (uM - 1u) * M2) + (e < M2 ? e : e - M2);
       T corr = b[i];
       corr += (-x[i]) * (T)1;
       corr = corr / (T)1;
@@ -983,58 +984,78 @@ __global__ __launch_bounds__(1024) void k_q1_sor_strips(int M, int n, int pz, in
       if (e < nbs) bs[e] = bv[u];
     }
   }
+  // The off-plane terms: a wave takes 62 consecutive positions of the lines [y0 - 3, y1 + 3)
+  // (lanes 1..62; lanes 0 and 63 only load), each lane loading the x column and the cell
+  // column of its own position and taking the x - 1 / x + 1 columns from its neighbouring
+  // lanes (DPP shifts): 11 loads per row instead of 27.  A row's x - 1 and x + 1 are on its
+  // own line (a line ends only at a Dirichlet row, which reads none).  Every lane of a wave
+  // runs every iteration (the shifts read all lanes); loads are clamped into the lattice.
   const bool lz[2] = {z - 1 != 0, z + 1 != M - 1};
   const int nps = (H + 6) * M;
-#pragma unroll 2
-  for (int e = threadIdx.x; e < nps; e += blockDim.x) {
-    const int l = (int)fdiv((unsigned)e, fM), xc = e - l * M, y = y0 - 3 + l;
-    if (y < 0 || y >= M) continue;
-    const unsigned i = zb + (unsigned)(y * M + xc);
-    T acc;
-    if (y < y0) acc = gb[ol + (unsigned)((y - y0 + 4) * M + xc)];
-    else if (y >= y1) acc = gb[ou + (unsigned)((y - y1 + 4) * M + xc)];
-    else acc = b[i];
-    if (y != 0 && y != M - 1 && xc != 0 && xc != M - 1) {
-      const unsigned c0 = (unsigned)((y - 1) * n + (xc - 1));  // cell (y - 1, xc - 1) in-plane
-      T a[2][2][2];
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int c0 = (threadIdx.x >> 6) * 62; c0 < nps; c0 += nw * 62) {
+    const int e = c0 + lane - 1;
+    const int ec = min(max(e, 0), nps - 1);
+    const int l = (int)fdiv((unsigned)ec, fM), xc = ec - l * M, y = y0 - 3 + l;
+    const int yl = min(max(y, 0), M - 1), xcc = min(xc, n - 1);
+    T xcol[2][3], acol[2][2];
 #pragma unroll
-      for (int p = 0; p < 2; p++)
+    for (int p = 0; p < 2; p++) {
 #pragma unroll
-        for (int q = 0; q < 2; q++)
+      for (int d = 0; d < 3; d++) {
+        const int yy = min(max(yl + d - 1, 0), M - 1);
+        xcol[p][d] = x[(p ? zb + M2 : zb - M2) + (unsigned)(yy * M + xc)];
+      }
 #pragma unroll
-          for (int c = 0; c < 2; c++)
-            a[p][q][c] = alpha[(p ? cb1 : cb0) + c0 + (unsigned)(q * n + c)];
+      for (int q = 0; q < 2; q++) {
+        const int yc = min(max(yl - 1 + q, 0), n - 1);
+        acol[p][q] = alpha[(p ? cb1 : cb0) + (unsigned)(yc * n + xcc)];
+      }
+    }
+    const bool row = lane >= 1 && lane <= 62 && e < nps && y >= 0 && y < M;
+    T acc = (T)0;
+    if (row) {
+      const unsigned i = zb + (unsigned)(y * M + xc);
+      if (y < y0) acc = gb[ol + (unsigned)((y - y0 + 4) * M + xc)];
+      else if (y >= y1) acc = gb[ou + (unsigned)((y - y1 + 4) * M + xc)];
+      else acc = b[i];
+    }
+    T xn[2][3][3], a[2][2][2];
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        xn[p][d][1] = xcol[p][d];
+        xn[p][d][0] = lane_up(xcol[p][d]);
+        xn[p][d][2] = lane_down(xcol[p][d]);
+      }
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        a[p][q][1] = acol[p][q];
+        a[p][q][0] = lane_up(acol[p][q]);
+      }
+    }
+    if (row && y != 0 && y != M - 1 && xc != 0 && xc != M - 1) {
       const bool ly[3] = {y - 1 != 0, true, y + 1 != M - 1};
       const bool lx[3] = {xc - 1 != 0, true, xc + 1 != M - 1};
-      T xn[2][9];
-#pragma unroll
-      for (int p = 0; p < 2; p++)
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-          const int dy = k / 3 - 1, dx = k % 3 - 1;
-          const bool live = lz[p] && ly[dy + 1] && lx[dx + 1];
-          // read-only planes z - 1 and z + 1; a dead neighbour reads the row's own column
-          const unsigned j = (p ? i + M2 : i - M2) + (unsigned)(live ? dy * M + dx : 0);
-          xn[p][k] = x[j];
-        }
 #pragma unroll
       for (int p = 0; p < 2; p++)
 #pragma unroll
         for (int k = 0; k < 9; k++) {
           const int dy = k / 3 - 1, dx = k % 3 - 1;
           const int q0 = dy > 0 ? 1 : 0, q1 = dy < 0 ? 0 : 1;
-          const int c0 = dx > 0 ? 1 : 0, c1 = dx < 0 ? 0 : 1;
+          const int c0c = dx > 0 ? 1 : 0, c1c = dx < 0 ? 0 : 1;
           T sum = (T)0;
 #pragma unroll
           for (int q = q0; q <= q1; q++)
 #pragma unroll
-            for (int c = c0; c <= c1; c++) sum += a[p][q][c];
+            for (int c = c0c; c <= c1c; c++) sum += a[p][q][c];
           const T aij = (T)kt.k[4 + (dy != 0) * 2 + (dx != 0)] * sum;
           const bool live = lz[p] && ly[dy + 1] && lx[dx + 1];
-          acc += ((T)-1 * xn[p][k]) * (live ? aij : (T)0);
+          acc += ((T)-1 * xn[p][dy + 1][dx + 1]) * (live ? aij : (T)0);
         }
     }
-    ps[e] = acc;
+    if (row) ps[e] = acc;
   }
   __syncthreads();
   for (int j = 0; j < 4; j++) {
@@ -1988,8 +2009,9 @@ static void q1_sor_strips(const flgpu_mat *A, T omega, int backwards, T *x, cons
   FLGPU_REQUIRE(sor_strip_lds(M, H) <= CAP, "strip sweep: LDS size");
   T *gx = (T *)A->sor_ghost, *gb = gx ? gx + (long)M * (S - 1) * 8 * M : nullptr;
   const long prep = (long)(M - 2) * (S - 1) * 8 * M + 2L * M * M;
-  k_q1_sor_prep<T><<<grid_for(prep, 256), 256, 0, A->ctx->stream>>>(M, H, S, omega, x, b, r, gx,
-                                                                      gb);
+  const FastDiv fS1 = S > 2 ? fast_div((unsigned)(S - 1)) : FastDiv{1, 0, 0};
+  k_q1_sor_prep<T><<<grid_for(prep, 256), 256, 0, A->ctx->stream>>>(
+      M, H, S, fast_div((unsigned)M), fS1, omega, x, b, r, gx, gb);
   FLGPU_HIP(hipGetLastError());
   for (int k = 0; k < 2; k++) {
     const int pz = backwards ? 1 - k : k;
