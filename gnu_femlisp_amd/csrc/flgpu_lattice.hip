@@ -556,19 +556,29 @@ constexpr int kAsmB = 4;                         // nodes per axis of a block
 constexpr int kAsmC = kAsmB + 1;                 // cells per axis around it
 constexpr int kAsmNC = kAsmC * kAsmC * kAsmC;    // 125 cells
 constexpr int kAsmNCP = (kAsmNC + 15) / 16 * 16; // padded to MFMA tiles of 16 cells
+constexpr int kAsmThreads = 256;                 // four threads per node of the block
+constexpr int kAsmEsW = 65;  // row stride of the element matrices in LDS: one bank per cell
 
 enum { ASM_FMA = 0, ASM_MFMA64 = 1, ASM_MFMA_BF16 = 2 };
 
 template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_asm_q1(int n, const double *__restrict__ g,
+__global__ __launch_bounds__(kAsmThreads) void k_asm_q1(int n, const double *__restrict__ g,
                                                 const double *__restrict__ kappa, int nk,
                                                 double scale, const double *__restrict__ Tt,
                                                 const int *__restrict__ rowptr, T *vals,
                                                 double rhs_scale, const double *__restrict__ mload,
                                                 T *rhs) {
-  // E in the precision the two-pass path stores it in: FP64 for FP64 matrices, FP32 otherwise
-  __shared__ double Ss[kAsmNCP][8];
-  __shared__ T Es[kAsmNCP][64];
+  // E in the precision the two-pass path stores it in: FP64 for FP64 matrices, FP32 otherwise;
+  // the rows of E are kAsmEsW apart, so that the nodes of a wave, whose cells differ, read
+  // different banks.  S is kept in FP64 (FMA and FP64 MFMA) or as its BF16 high and low parts
+  // (BF16 MFMA), formed once per value.
+  constexpr bool kBF = MODE == ASM_MFMA_BF16;
+  constexpr int kSBytes = kBF ? kAsmNCP * 8 * 2 * (int)sizeof(short) : kAsmNCP * 8 * (int)sizeof(double);
+  __shared__ __align__(16) unsigned char smem[kSBytes + kAsmNC * kAsmEsW * sizeof(T)];
+  double(*Ss)[8] = (double(*)[8])smem;
+  short(*Sh)[8] = (short(*)[8])smem;                   // BF16 high parts
+  short(*Sl)[8] = (short(*)[8])(smem + kSBytes / 2);   // BF16 low parts
+  T(*Es)[kAsmEsW] = (T(*)[kAsmEsW])(smem + kSBytes);
   const int M = n + 1;
   const int nbpa = (M + kAsmB - 1) / kAsmB;
   int blk = blockIdx.x;
@@ -577,119 +587,162 @@ __global__ __launch_bounds__(256) void k_asm_q1(int n, const double *__restrict_
   const int by = blk % nbpa, bz = blk / nbpa;
   const int cz0 = bz * kAsmB - 1, cy0 = by * kAsmB - 1, cx0 = bx * kAsmB - 1;  // first cell
   const int tid = threadIdx.x;
-  // S[c][q] = scale kappa(x_q) (k_coef); zero for cells outside the lattice (never read)
-  for (int e = tid; e < kAsmNCP * 8; e += blockDim.x) {
-    const int lc = e >> 3, q = e & 7;
-    double v = 0.0;
-    if (lc < kAsmNC) {
+  // S[c][q] = scale kappa(x_q) (k_coef); zero for cells outside the lattice (never read).
+  // A thread fills kAsmFill entries; their coefficient loads are all issued before any is used.
+  constexpr int kAsmFill = kAsmNCP * 8 / kAsmThreads;
+  {
+    const int r = nk / n;
+    int gq[2] = {0, 0};  // the quadrature points' sub-cell offsets floor(g r)
+    if (kappa) {
+      gq[0] = (int)floor(g[0] * r);
+      gq[1] = (int)floor(g[1] * r);
+    }
+    double kv[kAsmFill];
+    bool in[kAsmFill];
+#pragma unroll
+    for (int u = 0; u < kAsmFill; u++) {
+      const int e = tid + u * kAsmThreads, lc = e >> 3, q = e & 7;
       const int cz = cz0 + lc / (kAsmC * kAsmC), cy = cy0 + (lc / kAsmC) % kAsmC,
                 cx = cx0 + lc % kAsmC;
-      if (cz >= 0 && cy >= 0 && cx >= 0 && cz < n && cy < n && cx < n) {
-        double kap = 1.0;
-        if (kappa) {
-          const int r = nk / n;
-          const int qz = q >> 2, qy = (q >> 1) & 1, qx = q & 1;
-          long idx = (long)cz * r + (int)floor(g[qz] * r);
-          idx = idx * nk + (long)cy * r + (int)floor(g[qy] * r);
-          idx = idx * nk + (long)cx * r + (int)floor(g[qx] * r);
-          kap = kappa[idx];
-        }
-        v = scale * kap;
+      in[u] = lc < kAsmNC && cz >= 0 && cy >= 0 && cx >= 0 && cz < n && cy < n && cx < n;
+      long idx = 0;  // cells outside the lattice load the first value, which is not used
+      if (in[u]) {
+        idx = (long)cz * r + gq[q >> 2];
+        idx = idx * nk + (long)cy * r + gq[(q >> 1) & 1];
+        idx = idx * nk + (long)cx * r + gq[q & 1];
+      }
+      kv[u] = kappa ? kappa[idx] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kAsmFill; u++) {
+      const int e = tid + u * kAsmThreads;
+      const double sv = in[u] ? scale * kv[u] : 0.0;
+      if (kBF) {
+        const float av = (float)sv;
+        const short h = f2bf(av);
+        Sh[e >> 3][e & 7] = h;
+        Sl[e >> 3][e & 7] = f2bf(av - bf2f(h));
+      } else {
+        Ss[e >> 3][e & 7] = sv;
       }
     }
-    Ss[lc][q] = v;
   }
   __syncthreads();
   if (MODE == ASM_FMA) {
-    for (int e = tid; e < kAsmNCP * 64; e += blockDim.x) {
-      const int lc = e >> 6, ab = e & 63;
+    // entry ab = tid mod 64 of every cell this thread forms: its 8 values of T stay in registers
+    const int ab = tid & 63;
+    double tq[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) tq[q] = Tt[q * 64 + ab];
+    for (int lc = tid >> 6; lc < kAsmNC; lc += kAsmThreads / 64) {
       double acc = 0.0;
-      for (int q = 0; q < 8; q++) acc += Ss[lc][q] * Tt[q * 64 + ab];
+#pragma unroll
+      for (int q = 0; q < 8; q++) acc += Ss[lc][q] * tq[q];
       Es[lc][ab] = (T)acc;
     }
   } else {
-    // tiles of 16 cells x 16 entries, one per wave at a time
+    // tiles of 16 cells x 16 entries; wave w forms the entries 16 w .. 16 w + 15 of every cell
+    // (tile jobs w, w + 4, ...), so its operand from T is the same for all its tiles
+    static_assert(kAsmThreads == 256, "four waves, one per 16-entry column of T");
     const int lane = tid & 63, w = tid >> 6;
     const int i = lane & 15, kg = lane >> 4;
-    for (int job = w; job < (kAsmNCP / 16) * 4; job += 4) {
-      const int c0 = (job >> 2) * 16, n0 = (job & 3) * 16;
-      const int bcol = n0 + i;
-      if (MODE == ASM_MFMA64) {
+    const int bcol = w * 16 + i;
+    if (MODE == ASM_MFMA64) {
+      const double b0 = Tt[kg * 64 + bcol], b1 = Tt[(4 + kg) * 64 + bcol];
+      for (int c0 = 0; c0 < kAsmNCP; c0 += 16) {
         v4d acc = {0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < 8; k0 += 4) {
-          const int k = k0 + kg;
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ss[c0 + i][k], Tt[k * 64 + bcol], acc, 0,
-                                                     0, 0);
-        }
-        for (int v = 0; v < 4; v++) Es[c0 + kg + 4 * v][bcol] = (T)acc[v];
-      } else {
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ss[c0 + i][kg], b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ss[c0 + i][4 + kg], b1, acc, 0, 0, 0);
+        for (int v = 0; v < 4; v++)
+          if (c0 + kg + 4 * v < kAsmNC) Es[c0 + kg + 4 * v][bcol] = (T)acc[v];
+      }
+    } else {
+      v4s bh, bl;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int k = 4 * kg + v;
+        const float bv = (float)(k < 8 ? Tt[k * 64 + bcol] : 0.0);
+        bh[v] = f2bf(bv);
+        bl[v] = f2bf(bv - bf2f(bh[v]));
+      }
+      for (int c0 = 0; c0 < kAsmNCP; c0 += 16) {
         v4f acc = {0.f, 0.f, 0.f, 0.f};
-        v4s ah, al, bh, bl;
-        for (int v = 0; v < 4; v++) {
-          const int k = 4 * kg + v;
-          const float av = (float)(k < 8 ? Ss[c0 + i][k] : 0.0);
-          const float bv = (float)(k < 8 ? Tt[k * 64 + bcol] : 0.0);
-          ah[v] = f2bf(av);
-          al[v] = f2bf(av - bf2f(ah[v]));
-          bh[v] = f2bf(bv);
-          bl[v] = f2bf(bv - bf2f(bh[v]));
-        }
+        const v4s zero = {0, 0, 0, 0};  // k >= 8: the padding of K to 16
+        const v4s ah = kg < 2 ? *(const v4s *)&Sh[c0 + i][4 * kg] : zero;
+        const v4s al = kg < 2 ? *(const v4s *)&Sl[c0 + i][4 * kg] : zero;
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
-        for (int v = 0; v < 4; v++) Es[c0 + 4 * kg + v][bcol] = (T)acc[v];
+        for (int v = 0; v < 4; v++)
+          if (c0 + 4 * kg + v < kAsmNC) Es[c0 + 4 * kg + v][bcol] = (T)acc[v];
       }
     }
   }
   __syncthreads();
-  // rows: four threads per node, each taking every fourth of its (up to) 27 entries
+  // rows: four threads per node.  An interior row has all 27 neighbours (entry k at
+  // rowptr + k) and all 8 cells around it, so thread p < 3 forms the 9 entries of z-offset
+  // p - 1 with every cell and local index a constant offset from the node's (the loops over
+  // the offsets and the cells are unrolled), and thread 3 the load vector.  A Dirichlet row
+  // (on the lattice's boundary) is the identity in its own column and zero elsewhere, with
+  // fewer neighbours: its threads take every fourth entry at its position among the present
+  // ones.  Each entry is the sum, in ascending cell order and in FP64, of E over the cells
+  // holding both DOFs, as k_gather forms it.
   const int node = tid >> 2, part = tid & 3;
-  const int Pz = bz * kAsmB + node / (kAsmB * kAsmB), Py = by * kAsmB + (node / kAsmB) % kAsmB,
-            Px = bx * kAsmB + node % kAsmB;
+  const int lz = node / (kAsmB * kAsmB), ly = (node / kAsmB) % kAsmB, lx = node % kAsmB;
+  const int Pz = bz * kAsmB + lz, Py = by * kAsmB + ly, Px = bx * kAsmB + lx;
   if (Pz >= M || Py >= M || Px >= M) return;
   const long r = ((long)Pz * M + Py) * M + Px;
   const bool rb = Pz == 0 || Py == 0 || Px == 0 || Pz == M - 1 || Py == M - 1 || Px == M - 1;
-  // the row's neighbours inside the lattice, in ascending column order: entry position = the
-  // number of such neighbours before it
-  unsigned live = 0;
-  for (int k = 0; k < 27; k++) {
-    const int Qz = Pz + k / 9 - 1, Qy = Py + (k / 3) % 3 - 1, Qx = Px + k % 3 - 1;
-    if (Qz >= 0 && Qy >= 0 && Qx >= 0 && Qz < M && Qy < M && Qx < M) live |= 1u << k;
-  }
   const int pos0 = rowptr[r];
-  for (int k = part; k < 27; k += 4) {
-    if (!(live >> k & 1u)) continue;
-    const int Qz = Pz + k / 9 - 1, Qy = Py + (k / 3) % 3 - 1, Qx = Px + k % 3 - 1;
-    const bool cb = Qz == 0 || Qy == 0 || Qx == 0 || Qz == M - 1 || Qy == M - 1 || Qx == M - 1;
-    double acc = 0.0;
-    if (!(rb || cb)) {
-      // cells c with max(P, Q) - 1 <= c <= min(P, Q) per axis, ascending (z slowest)
-      const int zlo = max(max(Pz, Qz) - 1, 0), zhi = min(min(Pz, Qz), n - 1);
-      const int ylo = max(max(Py, Qy) - 1, 0), yhi = min(min(Py, Qy), n - 1);
-      const int xlo = max(max(Px, Qx) - 1, 0), xhi = min(min(Px, Qx), n - 1);
-      for (int cz = zlo; cz <= zhi; cz++)
-        for (int cy = ylo; cy <= yhi; cy++)
-          for (int cx = xlo; cx <= xhi; cx++) {
-            const int lc = ((cz - cz0) * kAsmC + (cy - cy0)) * kAsmC + (cx - cx0);
-            const int la = ((Pz - cz) * 2 + (Py - cy)) * 2 + (Px - cx);
-            const int lb = ((Qz - cz) * 2 + (Qy - cy)) * 2 + (Qx - cx);
-            acc += (double)Es[lc][la * 8 + lb];
-          }
-    } else {
-      acc = (rb && k == 13) ? 1.0 : 0.0;
+  if (rb) {
+    // the row's neighbours inside the lattice, in ascending column order: entry position =
+    // the number of such neighbours before it
+    unsigned live = 0;
+    for (int k = 0; k < 27; k++) {
+      const int Qz = Pz + k / 9 - 1, Qy = Py + (k / 3) % 3 - 1, Qx = Px + k % 3 - 1;
+      if (Qz >= 0 && Qy >= 0 && Qx >= 0 && Qz < M && Qy < M && Qx < M) live |= 1u << k;
     }
-    vals[pos0 + __popc(live & ((1u << k) - 1u))] = (T)acc;
+    for (int k = part; k < 27; k += 4)
+      if (live >> k & 1u) vals[pos0 + __popc(live & ((1u << k) - 1u))] = (T)(k == 13 ? 1.0 : 0.0);
+    if (rhs && part == 0) rhs[r] = (T)0.0;
+    return;
   }
-  if (rhs && part == 0) {
-    double bsum = 0.0;
-    if (!rb) {
-      for (int cz = max(Pz - 1, 0); cz <= min(Pz, n - 1); cz++)
-        for (int cy = max(Py - 1, 0); cy <= min(Py, n - 1); cy++)
-          for (int cx = max(Px - 1, 0); cx <= min(Px, n - 1); cx++)
-            bsum += mload[((Pz - cz) * 2 + (Py - cy)) * 2 + (Px - cx)];
-      bsum *= rhs_scale;
+  if (part < 3) {
+    const int dz = part - 1;
+    // the cell (P - 1) in the block's cell grid: cells start at (bz, by, bx) kAsmB - 1
+    const int lbase = (lz * kAsmC + ly) * kAsmC + lx;
+    const bool qz = (Pz + dz == 0) || (Pz + dz == M - 1);
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+      const int dy = e / 3 - 1, dx = e % 3 - 1;
+      const bool cb = qz || Py + dy == 0 || Py + dy == M - 1 || Px + dx == 0 || Px + dx == M - 1;
+      double acc = 0.0;
+      if (!cb) {
+        // shared cells per axis: offset -1 -> the lower cell (a = 0), +1 -> the upper (a = 1),
+        // 0 -> both; ascending (z slowest)
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          if ((dz < 0 && a == 1) || (dz > 0 && a == 0)) continue;
+#pragma unroll
+          for (int b = (dy > 0 ? 1 : 0); b <= (dy < 0 ? 0 : 1); b++)
+#pragma unroll
+            for (int c = (dx > 0 ? 1 : 0); c <= (dx < 0 ? 0 : 1); c++) {
+              const int lc = lbase + (a * kAsmC + b) * kAsmC + c;
+              const int la = ((1 - a) * 2 + (1 - b)) * 2 + (1 - c);
+              const int lb = ((dz + 1 - a) * 2 + (dy + 1 - b)) * 2 + (dx + 1 - c);
+              acc += (double)Es[lc][la * 8 + lb];
+            }
+        }
+      }
+      vals[pos0 + part * 9 + e] = (T)acc;
     }
-    rhs[r] = (T)bsum;
+  } else if (rhs) {
+    double bsum = 0.0;
+    for (int cz = Pz - 1; cz <= Pz; cz++)
+      for (int cy = Py - 1; cy <= Py; cy++)
+        for (int cx = Px - 1; cx <= Px; cx++)
+          bsum += mload[((Pz - cz) * 2 + (Py - cy)) * 2 + (Px - cx)];
+    rhs[r] = (T)(bsum * rhs_scale);
   }
 }
 
@@ -777,7 +830,7 @@ static void assemble_values_q1(flgpu_mat *A, const flgpu_vec *kappa, int nk, int
     const double *kd = kappa ? (const double *)kappa->d : nullptr;
     hipStream_t st = ctx->stream;
 #define FLGPU_ASM(T, MODE)                                                                   \
-  k_asm_q1<T, MODE><<<g, 256, 0, st>>>(n, dg, kd, nk, scale, dT, A->rowptr, (T *)A->vals, \
+  k_asm_q1<T, MODE><<<g, kAsmThreads, 0, st>>>(n, dg, kd, nk, scale, dT, A->rowptr, (T *)A->vals, \
                                        rscale, dm, (T *)rhs_dev)
     if (A->prec == FLGPU_F64) {
       if (mode == FLGPU_ASM_MFMA) FLGPU_ASM(double, ASM_MFMA64);
