@@ -304,20 +304,20 @@ def test_concurrent_contexts_and_shared_context():
         assert np.array_equal(z.numpy(), s)
 
 
+@pytest.mark.parametrize("form", [F.STORAGE_CELL, F.STORAGE_STENCIL])
 @pytest.mark.parametrize("n,prec", [(40, F.F64), (40, F.F32), (70, F.F64), (70, F.F32),
                                     (128, F.F32)])
-def test_cell_sor_sweeps_over_many_tiles(n, prec):
-    """the per-cell <sor> sweep on larger lattices (strips per plane: n = 40, 6 of 7 lines;
-    n = 70, 8 of 9; n = 128, 4 of 33; the last strip one to three lines shorter) equals the
-    SELL colour
-    sweep, forwards and backwards, and with rhs and residual one vector (femlisp's
-    (iterate q w w))"""
-    say(f"cell sor n={n} prec={prec}")
+def test_cell_sor_sweeps_over_many_tiles(n, prec, form):
+    """the per-cell <sor> sweep in plane strips on larger lattices (strips per plane: n = 40,
+    6 of 7 lines; n = 70, 8 of 9; n = 128, 4 of 33; the last strip one to three lines shorter)
+    and the stencil-form colour launches equal the SELL colour sweep, forwards and backwards,
+    and with rhs and residual one vector (femlisp's (iterate q w w))"""
+    say(f"sor strips n={n} prec={prec} form={form}")
     rng = np.random.default_rng(n)
     kap = np.exp(rng.normal(scale=1.5, size=(n, n, n)))
     A, b = lat.assemble(3, 1, n, kap, prec=prec)
     B, _ = lat.assemble(3, 1, n, kap, prec=prec)
-    assert A.storage() == F.STORAGE_CELL and B.storage(F.STORAGE_SELL) == F.STORAGE_SELL
+    assert A.storage(form) == form and B.storage(F.STORAGE_SELL) == F.STORAGE_SELL
     tol = 1e-13 if prec == F.F64 else 2e-6
     y = rng.uniform(-1, 1, A.nrows)
     bh = b.numpy() + rng.uniform(-1, 1, A.nrows)
