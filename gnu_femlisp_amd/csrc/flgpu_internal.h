@@ -217,7 +217,9 @@ flgpu_mat *mat_convert(const flgpu_mat *A, int prec);
 void sell_free(Sell &s);
 Sell sell_build_rows(flgpu_mat *A, const std::vector<int> &rows);
 void set_colouring(flgpu_mat *A, int ncolours, const int *colour_host);
-void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac);
+// lattice: P is a 3D Q1 lattice interpolation and A, Ac hold the lattice patterns of the fine
+// and the coarse lattice (the product then computes positions instead of searching them)
+void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac, bool lattice = false);
 
 // out[row] = beta*z[row] + alpha*(A x)[row]  (z may be out; beta == 0 ignores z)
 void spmv(const flgpu_mat *A, double alpha, const void *x, double beta,

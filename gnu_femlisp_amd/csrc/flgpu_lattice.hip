@@ -1016,7 +1016,7 @@ flgpu_mat *lattice_galerkin(const flgpu_mat *P, const flgpu_mat *Afine, int dim,
                             int nc) {
   flgpu_mat *Ac = lattice_pattern(Afine->ctx, dim, order, nc, Afine->prec);
   try {
-    galerkin_product(P, Afine, Ac);
+    galerkin_product(P, Afine, Ac, true);
     long N = Ac->nrows;
     int M = order * nc + 1;
     if (Ac->prec == FLGPU_F64)

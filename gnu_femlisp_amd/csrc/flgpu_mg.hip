@@ -784,7 +784,7 @@ extern "C" int flgpu_mg_reassemble(flgpu_mg *mg, const flgpu_vec *kappa, int nk)
         lattice_reassemble(L.A, kappa, nk, mg->asm_mode);
       } else {
         if (bbox) lattice_blackbox(mg->lv[l + 1].A, L.I);
-        galerkin_product(L.I, mg->lv[l + 1].A, L.A);
+        galerkin_product(L.I, mg->lv[l + 1].A, L.A, true);
         lattice_dirichlet_identity(L.A, mg->lat_dim, mg->lat_order, mg->lat_n0 << l);
         mat_refresh_values(L.A);
       }

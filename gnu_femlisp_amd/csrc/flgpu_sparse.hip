@@ -2190,9 +2190,80 @@ __global__ __launch_bounds__(kRapB) void k_rap_lds(
   for (int j = 0; j < w; j++) acc[c0 + j] = sacc[j * kRapB + tid];
 }
 
+// The same product on 3D Q1 lattices, one thread per coarse entry (k, n): the fine matrix holds
+// the lattice pattern (every lattice neighbour, ascending), P the pattern of the lattice
+// interpolation (fine node m takes the coarse nodes floor(m / 2) and ceil(m / 2) per axis,
+// ascending) and the coarse matrix the coarse lattice pattern, so every position is computed
+// instead of searched.  The entry sums the contributions (P_lk A_lm) P_mn of the fine l around
+// 2k (ascending) and the m next to l and to 2n (ascending): exactly the contributions, in the
+// order, that k_rap's loops bring to this entry, so the sums are equal bit for bit.
+__device__ __forceinline__ int lat_pos3(const int lo[3], const int cnt[3], int a, int b, int c) {
+  return ((a - lo[0]) * cnt[1] + (b - lo[1])) * cnt[2] + (c - lo[2]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_rap_q1(int Mc, int M, const int *__restrict__ arp,
+                                                const T *__restrict__ av,
+                                                const int *__restrict__ prp,
+                                                const T *__restrict__ pv,
+                                                const int *__restrict__ crp, double *acc,
+                                                int upper) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long Nc = (long)Mc * Mc * Mc;
+  if (t >= Nc * 27) return;
+  const int k = (int)(t / 27), d = (int)(t - (long)k * 27);
+  const int K[3] = {k / (Mc * Mc), (k / Mc) % Mc, k % Mc};
+  const int D[3] = {d / 9 - 1, (d / 3) % 3 - 1, d % 3 - 1};
+  int Nn[3], clo[3], ccnt[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    Nn[a] = K[a] + D[a];
+    if (Nn[a] < 0 || Nn[a] >= Mc) return;
+    clo[a] = K[a] > 0 ? -1 : 0;
+    ccnt[a] = (K[a] < Mc - 1 ? 1 : 0) - clo[a] + 1;
+  }
+  const int n = (Nn[0] * Mc + Nn[1]) * Mc + Nn[2];
+  if (upper && n < k) return;  // the lower triangle is mirrored afterwards
+  double s = 0.0;
+  const int l0[3] = {max(2 * K[0] - 1, 0), max(2 * K[1] - 1, 0), max(2 * K[2] - 1, 0)};
+  const int l1[3] = {min(2 * K[0] + 1, M - 1), min(2 * K[1] + 1, M - 1), min(2 * K[2] + 1, M - 1)};
+  for (int lz = l0[0]; lz <= l1[0]; lz++)
+    for (int ly = l0[1]; ly <= l1[1]; ly++)
+      for (int lx = l0[2]; lx <= l1[2]; lx++) {
+        const int L[3] = {lz, ly, lx};
+        const long l = ((long)lz * M + ly) * M + lx;
+        // P_lk: row l holds floor(l / 2) (and ceil(l / 2) on odd coordinates) per axis
+        int plo[3], pcnt[3], alo[3], acnt[3], m0[3], m1[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          plo[a] = L[a] >> 1;
+          pcnt[a] = (L[a] & 1) + 1;
+          alo[a] = L[a] > 0 ? L[a] - 1 : 0;
+          acnt[a] = min(L[a] + 1, M - 1) - alo[a] + 1;
+          m0[a] = max(max(L[a] - 1, 2 * Nn[a] - 1), 0);
+          m1[a] = min(min(L[a] + 1, 2 * Nn[a] + 1), M - 1);
+        }
+        const double plk = (double)pv[prp[l] + lat_pos3(plo, pcnt, K[0], K[1], K[2])];
+        const int ab = arp[l];
+        for (int mz = m0[0]; mz <= m1[0]; mz++)
+          for (int my = m0[1]; my <= m1[1]; my++)
+            for (int mx = m0[2]; mx <= m1[2]; mx++) {
+              const double w = plk * (double)av[ab + lat_pos3(alo, acnt, mz, my, mx)];
+              const long m = ((long)mz * M + my) * M + mx;
+              const int qlo[3] = {mz >> 1, my >> 1, mx >> 1};
+              const int qcnt[3] = {(mz & 1) + 1, (my & 1) + 1, (mx & 1) + 1};
+              s += w * (double)pv[prp[m] + lat_pos3(qlo, qcnt, Nn[0], Nn[1], Nn[2])];
+            }
+      }
+  acc[crp[k] + lat_pos3(clo, ccnt, D[0], D[1], D[2])] = s;
+}
+
+// the longest row: a wave's maximum first, then one atomic per wave
 __global__ void k_max_row(int n, const int *__restrict__ rp, int *mx) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicMax(mx, rp[i + 1] - rp[i]);
+  int len = i < n ? rp[i + 1] - rp[i] : 0;
+  for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, 64));
+  if ((threadIdx.x & 63) == 0 && len > 0) atomicMax(mx, len);
 }
 
 // the longest row of A's CSR (A's SELL may not be built yet)
@@ -2312,22 +2383,38 @@ static bool csr_symmetric(const flgpu_mat *A) {
 // exactly symmetric, only the upper triangle is accumulated and mirrored, so Ac is exactly
 // symmetric too (the two triangles of P^T A P otherwise round differently), which lets the
 // stencil kernels stream half the coefficients.
-void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac) {
+void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac, bool lattice) {
   FLGPU_REQUIRE(P->nrows == A->nrows && A->nrows == A->ncols, "galerkin: P and A do not match");
   FLGPU_REQUIRE(Ac->nrows == P->ncols && Ac->ncols == P->ncols, "galerkin: Ac does not match P");
   FLGPU_REQUIRE(P->prec == A->prec && A->prec == Ac->prec, "galerkin: precisions differ");
   flgpu_ctx *ctx = A->ctx;
-  mat_build_transpose(P);
-  const flgpu_mat *PT = P->transpose;
   double *acc = (double *)dev_alloc(sizeof(double) * std::max<long>(Ac->nnz, 1));
   int *bad = (int *)dev_alloc(sizeof(int));
   FLGPU_HIP(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
   int nc = Ac->nrows;
   int *missing = (int *)dev_alloc(sizeof(int));
-  const bool lds = max_row_length(Ac) <= kRapW;
+  // the structured product: 3D Q1, the fine lattice of an even n, the patterns' sizes as the
+  // lattices give them
+  const long nf = A->lat_n, ncl = nf / 2;
+  const bool q1 = lattice && is_q1_lattice(A) && nf % 2 == 0 &&
+                  A->nnz == (3 * nf + 1) * (3 * nf + 1) * (3 * nf + 1) &&
+                  Ac->nrows == (ncl + 1) * (ncl + 1) * (ncl + 1) &&
+                  Ac->nnz == (3 * ncl + 1) * (3 * ncl + 1) * (3 * ncl + 1) &&
+                  P->nnz == (3 * ncl + 1) * (3 * ncl + 1) * (3 * ncl + 1);
+  if (!q1) mat_build_transpose(P);  // the general product walks the rows of P^T
+  const flgpu_mat *PT = P->transpose;
+  const bool lds = !q1 && max_row_length(Ac) <= kRapW;
   auto rap = [&](int upper) {
     const unsigned gl = (unsigned)((nc + kRapB - 1) / kRapB);
-    if (lds && A->prec == FLGPU_F64)
+    if (q1 && A->prec == FLGPU_F64)
+      k_rap_q1<double><<<grid_for(27L * nc, 256), 256, 0, ctx->stream>>>(
+          (int)ncl + 1, (int)nf + 1, A->rowptr, (const double *)A->vals, P->rowptr,
+          (const double *)P->vals, Ac->rowptr, acc, upper);
+    else if (q1)
+      k_rap_q1<float><<<grid_for(27L * nc, 256), 256, 0, ctx->stream>>>(
+          (int)ncl + 1, (int)nf + 1, A->rowptr, (const float *)A->vals, P->rowptr,
+          (const float *)P->vals, Ac->rowptr, acc, upper);
+    else if (lds && A->prec == FLGPU_F64)
       k_rap_lds<double><<<gl, kRapB, 0, ctx->stream>>>(
           nc, PT->rowptr, PT->colidx, (const double *)PT->vals, A->rowptr, A->colidx,
           (const double *)A->vals, P->rowptr, P->colidx, (const double *)P->vals, Ac->rowptr,

@@ -654,6 +654,36 @@ def test_blackbox_hierarchy_solve_parity():
     assert np.allclose(bb["defnorms"], dno, rtol=1e-8)
 
 
+@pytest.mark.parametrize("prec", [F.F64, F.F32])
+def test_lattice_galerkin_equals_csr_product(prec):
+    """3D Q1: a black-box hierarchy's Galerkin levels (the product with every position taken
+    from the lattices, one coarse entry per thread) equal the CSR product of the same P and A
+    (flgpu_galerkin_product, which searches the rows) bit for bit, apart from the unit diagonal
+    the hierarchy puts on Dirichlet rows"""
+    say(f"lattice galerkin vs csr prec={prec}")
+    from gnu_femlisp_amd.fields import inclusions
+    from gnu_femlisp_amd.flgpu import check, lib
+    n0, levels = 4, 4
+    ntop = n0 << (levels - 1)
+    kap = inclusions(ntop, seed=9, rmin=1.5, rmax=4.0)
+    cs = it.GeometricCS(it.Jacobi(0.9), 2, 2, 0, prec=prec, transfer="blackbox")
+    H = it.Hierarchy.lattice(cs, 3, 1, n0, levels, kappa=kap)
+    for l in range(levels - 1):
+        n = n0 << l
+        Af, P = H.level_matrix(l + 1), H.level_matrix(l, interp=True)
+        Ac, _ = lat.assemble(3, 1, n, prec=prec)  # the coarse lattice pattern
+        check(lib().flgpu_galerkin_product(P.h, Af.h, Ac.h))
+        rp, ci, v = H.level_matrix(l).csr()
+        grp, gci, gv = Ac.csr()
+        assert np.array_equal(rp, grp) and np.array_equal(ci, gci)
+        rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+        M = n + 1
+        z, y, x = rows // (M * M), (rows // M) % M, rows % M
+        dirichlet = ((z == 0) | (z == n) | (y == 0) | (y == n) | (x == 0) | (x == n)) & (ci == rows)
+        assert np.array_equal(v[~dirichlet], gv[~dirichlet]), l
+        assert np.all(v[dirichlet] == 1.0) and np.all(gv[dirichlet] == 0.0), l
+
+
 def test_blackbox_mixed_fcg_reassemble():
     """FP32 black-box hierarchy inside FP64 FCG reaches a 1e-10 true residual
     with fewer steps than the geometric transfers on a 1e4-contrast problem;
