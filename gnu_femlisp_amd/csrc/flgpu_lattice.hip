@@ -928,6 +928,29 @@ static void lattice_set_alpha(flgpu_mat *A, const flgpu_vec *kappa, int nk) {
   FLGPU_HIP(hipGetLastError());
 }
 
+// The columns of the 3D Q1 lattice pattern (every lattice neighbour, ascending), one lane per
+// neighbour and 32 lanes per row: lane k writes neighbour k at its position among the present
+// ones, so a row's columns are written side by side.  The same pattern as k_tp_fill's tensor
+// product of the 1-D pattern {i - 1, i, i + 1}.
+__global__ __launch_bounds__(256) void k_q1_pattern(int M, const int *__restrict__ rowptr,
+                                                    int *colidx) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long N = (long)M * M * M;
+  const long r = t >> 5;
+  const int k = (int)(t & 31);
+  if (r >= N || k >= 27) return;
+  const int x = (int)(r % M), y = (int)((r / M) % M), z = (int)(r / ((long)M * M));
+  unsigned live = 0;
+#pragma unroll
+  for (int j = 0; j < 27; j++) {
+    const int qz = z + j / 9 - 1, qy = y + (j / 3) % 3 - 1, qx = x + j % 3 - 1;
+    if (qz >= 0 && qy >= 0 && qx >= 0 && qz < M && qy < M && qx < M) live |= 1u << j;
+  }
+  if (!(live >> k & 1u)) return;
+  const long c = r + (long)(k / 9 - 1) * M * M + (long)((k / 3) % 3 - 1) * M + (k % 3 - 1);
+  colidx[rowptr[r] + __popc(live & ((1u << k) - 1u))] = (int)c;
+}
+
 flgpu_mat *lattice_pattern(flgpu_ctx *ctx, int dim, int order, int n, int prec) {
   check_lattice_args(dim, order, n);
   Table1D pat = table_pattern(order, n);
@@ -939,8 +962,11 @@ flgpu_mat *lattice_pattern(flgpu_ctx *ctx, int dim, int order, int n, int prec) 
   A->lat_n = n;
   DevTable d = upload_table(ctx, pat);
   k_tp_rowptr<<<grid_for(N + 1, 256), 256, 0, ctx->stream>>>(N, dim, d, A->rowptr);
-  k_tp_fill<double><<<grid_for(N, 128), 128, 0, ctx->stream>>>(N, dim, d, A->rowptr, A->colidx,
-                                                               nullptr, 0);
+  if (dim == 3 && order == 1)
+    k_q1_pattern<<<grid_for(32 * N, 256), 256, 0, ctx->stream>>>(n + 1, A->rowptr, A->colidx);
+  else
+    k_tp_fill<double><<<grid_for(N, 128), 128, 0, ctx->stream>>>(N, dim, d, A->rowptr,
+                                                                 A->colidx, nullptr, 0);
   FLGPU_HIP(hipGetLastError());
   FLGPU_HIP(hipStreamSynchronize(ctx->stream));
   free_table(d);

@@ -149,6 +149,25 @@ __global__ __launch_bounds__(256) void k_gauss_jordan(int n, int k, const double
   D[idx] = v;
 }
 
+// trivial[k] = 1 when row k is the unit row e_k and column k is +0 apart from its diagonal
+// (a Dirichlet row and column of a lattice matrix).  Such a pivot is skipped: row k stays e_k
+// under every pivot, every other row subtracts zero multiples of it, and its column keeps
+// zeros, so the inverse is the same.  With positive pivots (SPD matrices) it is the same bit
+// for bit, since an entry that starts at +0 only ever has +0 or -0 subtracted from it; a
+// negative pivot can turn a zero of column k into -0, which the skipped pivot would have made
+// +0 again.
+__global__ __launch_bounds__(256) void k_dense_trivial(int n, const double *__restrict__ M,
+                                                       int *trivial) {
+  const int k = blockIdx.x;  // one block per pivot, its threads stride over row and column k
+  bool t = true;
+  for (int j = threadIdx.x; j < n; j += blockDim.x)
+    t = t && (j == k ? M[(long)k * n + k] == 1.0
+                     : __double_as_longlong(M[(long)k * n + j]) == 0 &&
+                           __double_as_longlong(M[(long)j * n + k]) == 0);
+  t = __syncthreads_and(t);
+  if (threadIdx.x == 0) trivial[k] = t ? 1 : 0;
+}
+
 // A^-1 of a square matrix of at most 8192 rows as a new n x n FP64 device array (row-major)
 static double *dense_inverse_dev(const flgpu_mat *A) {
   FLGPU_REQUIRE(A->nrows == A->ncols, "dense inverse of a non-square matrix");
@@ -171,7 +190,17 @@ static double *dense_inverse_dev(const flgpu_mat *A) {
       k_dense_scatter<float><<<grid_for(n, 256), 256, 0, s>>>(n, A->rowptr, A->colidx,
                                                               (const float *)A->vals, M0);
     FLGPU_HIP(hipGetLastError());
+    std::vector<int> trivial(std::max(n, 1), 0);
+    if (n > 0) {
+      int *dt = (int *)dev_alloc(sizeof(int) * n);
+      k_dense_trivial<<<n, 256, 0, s>>>(n, M0, dt);
+      FLGPU_HIP(hipGetLastError());
+      FLGPU_HIP(hipMemcpyAsync(trivial.data(), dt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+      FLGPU_HIP(hipStreamSynchronize(s));
+      dev_free(dt);
+    }
     for (int k = 0; k < n; k++) {
+      if (trivial[k]) continue;
       k_gauss_jordan<<<grid_for(nn, 256), 256, 0, s>>>(n, k, M0, M1, bad);
       std::swap(M0, M1);
     }

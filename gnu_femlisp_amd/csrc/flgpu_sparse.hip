@@ -170,13 +170,18 @@ Sell sell_build_rows(flgpu_mat *A, const std::vector<int> &rows) { return sell_b
 
 // ---- diagonal -----------------------------------------------------------------------
 
-__global__ void k_diagpos(int n, const int *rowptr, const int *colidx, int *pos) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
+// pos[r] = the position of A(r, r) in the CSR (the last one if repeated), -1 if absent: 32
+// lanes per row read the row's columns side by side (coalesced)
+__global__ __launch_bounds__(256) void k_diagpos32(int n, const int *__restrict__ rowptr,
+                                                   const int *__restrict__ colidx, int *pos) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(t >> 5), k = (int)(t & 31);
+  if (r >= n) return;  // the 32 lanes of a row leave together
   int p = -1;
-  for (int e = rowptr[r]; e < rowptr[r + 1]; e++)
+  for (int e = rowptr[r] + k; e < rowptr[r + 1]; e += 32)
     if (colidx[e] == r) p = e;
-  pos[r] = p;
+  for (int off = 16; off > 0; off >>= 1) p = max(p, __shfl_xor(p, off, 32));
+  if (k == 0) pos[r] = p;
 }
 
 template <class T>
@@ -241,7 +246,8 @@ void mat_finish(flgpu_mat *A) {
   A->diagpos = (int *)dev_alloc(sizeof(int) * std::max(n, 1));
   A->diag = dev_alloc(prec_bytes(A->prec) * std::max(n, 1));
   if (n > 0) {
-    k_diagpos<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, A->rowptr, A->colidx, A->diagpos);
+    k_diagpos32<<<grid_for(32L * n, 256), 256, 0, ctx->stream>>>(n, A->rowptr, A->colidx,
+                                                                 A->diagpos);
     FLGPU_HIP(hipGetLastError());
     diag_refresh(A);
   }
@@ -2194,20 +2200,29 @@ __global__ __launch_bounds__(kRapB) void k_rap_lds(
 // the lattice pattern (every lattice neighbour, ascending), P the pattern of the lattice
 // interpolation (fine node m takes the coarse nodes floor(m / 2) and ceil(m / 2) per axis,
 // ascending) and the coarse matrix the coarse lattice pattern, so every position is computed
-// instead of searched.  The entry sums the contributions (P_lk A_lm) P_mn of the fine l around
-// 2k (ascending) and the m next to l and to 2n (ascending): exactly the contributions, in the
-// order, that k_rap's loops bring to this entry, so the sums are equal bit for bit.
+// instead of searched, the row starts included.  The entry sums the contributions
+// (P_lk A_lm) P_mn of the fine l around 2k (ascending) and the m next to l and to 2n
+// (ascending): exactly the contributions, in the order, that k_rap's loops bring to this
+// entry, so the sums are equal bit for bit.
 __device__ __forceinline__ int lat_pos3(const int lo[3], const int cnt[3], int a, int b, int c) {
   return ((a - lo[0]) * cnt[1] + (b - lo[1])) * cnt[2] + (c - lo[2]);
 }
 
+// the first entry of row (z, y, x) of a tensor product of 1-D patterns: pre[a] entries precede
+// the row's coordinate on axis a, which has cnt[a] entries; nnz1 per 1-D pattern
+__device__ __forceinline__ long tp_row_start(const long pre[3], const int cnt[3], long nnz1) {
+  return (pre[0] * nnz1 + cnt[0] * pre[1]) * nnz1 + (long)cnt[0] * cnt[1] * pre[2];
+}
+
 template <class T>
-__global__ __launch_bounds__(256) void k_rap_q1(int Mc, int M, const int *__restrict__ arp,
-                                                const T *__restrict__ av,
-                                                const int *__restrict__ prp,
+__global__ __launch_bounds__(256) void k_rap_q1(int Mc, int M, const T *__restrict__ av,
                                                 const T *__restrict__ pv,
                                                 const int *__restrict__ crp, double *acc,
                                                 int upper) {
+  // the 1-D patterns: A's row i holds i - 1, i, i + 1 within the lattice (3 M - 2 entries,
+  // 3 i - 1 of them before row i > 0); P's row i holds i / 2 and, for odd i, i / 2 + 1
+  // (3 Mc - 2 entries, i + i / 2 before row i)
+  const long nA = 3L * M - 2, nP = 3L * Mc - 2;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long Nc = (long)Mc * Mc * Mc;
   if (t >= Nc * 27) return;
@@ -2231,39 +2246,53 @@ __global__ __launch_bounds__(256) void k_rap_q1(int Mc, int M, const int *__rest
     for (int ly = l0[1]; ly <= l1[1]; ly++)
       for (int lx = l0[2]; lx <= l1[2]; lx++) {
         const int L[3] = {lz, ly, lx};
-        const long l = ((long)lz * M + ly) * M + lx;
         // P_lk: row l holds floor(l / 2) (and ceil(l / 2) on odd coordinates) per axis
         int plo[3], pcnt[3], alo[3], acnt[3], m0[3], m1[3];
+        long ppre[3], apre[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
           plo[a] = L[a] >> 1;
           pcnt[a] = (L[a] & 1) + 1;
+          ppre[a] = L[a] + (L[a] >> 1);
           alo[a] = L[a] > 0 ? L[a] - 1 : 0;
           acnt[a] = min(L[a] + 1, M - 1) - alo[a] + 1;
+          apre[a] = L[a] > 0 ? 3L * L[a] - 1 : 0;
           m0[a] = max(max(L[a] - 1, 2 * Nn[a] - 1), 0);
           m1[a] = min(min(L[a] + 1, 2 * Nn[a] + 1), M - 1);
         }
-        const double plk = (double)pv[prp[l] + lat_pos3(plo, pcnt, K[0], K[1], K[2])];
-        const int ab = arp[l];
+        const double plk =
+            (double)pv[tp_row_start(ppre, pcnt, nP) + lat_pos3(plo, pcnt, K[0], K[1], K[2])];
+        const long ab = tp_row_start(apre, acnt, nA);
         for (int mz = m0[0]; mz <= m1[0]; mz++)
           for (int my = m0[1]; my <= m1[1]; my++)
             for (int mx = m0[2]; mx <= m1[2]; mx++) {
               const double w = plk * (double)av[ab + lat_pos3(alo, acnt, mz, my, mx)];
-              const long m = ((long)mz * M + my) * M + mx;
               const int qlo[3] = {mz >> 1, my >> 1, mx >> 1};
               const int qcnt[3] = {(mz & 1) + 1, (my & 1) + 1, (mx & 1) + 1};
-              s += w * (double)pv[prp[m] + lat_pos3(qlo, qcnt, Nn[0], Nn[1], Nn[2])];
+              const long qpre[3] = {mz + (mz >> 1), my + (my >> 1), mx + (mx >> 1)};
+              s += w * (double)pv[tp_row_start(qpre, qcnt, nP) +
+                                  lat_pos3(qlo, qcnt, Nn[0], Nn[1], Nn[2])];
             }
       }
   acc[crp[k] + lat_pos3(clo, ccnt, D[0], D[1], D[2])] = s;
 }
 
-// the longest row: a wave's maximum first, then one atomic per wave
-__global__ void k_max_row(int n, const int *__restrict__ rp, int *mx) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int len = i < n ? rp[i + 1] - rp[i] : 0;
+// the longest row: a grid of at most 1024 blocks strides over the rows, each block reduces its
+// maximum in LDS and issues one atomic (atomics to one address are serialised)
+constexpr int kMaxRowBlocks = 1024;
+
+__global__ __launch_bounds__(256) void k_max_row(int n, const int *__restrict__ rp, int *mx) {
+  __shared__ int wm[4];
+  int len = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    len = max(len, rp[i + 1] - rp[i]);
   for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, 64));
-  if ((threadIdx.x & 63) == 0 && len > 0) atomicMax(mx, len);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = len;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int m = max(max(wm[0], wm[1]), max(wm[2], wm[3]));
+    if (m > 0) atomicMax(mx, m);
+  }
 }
 
 // the longest row of A's CSR (A's SELL may not be built yet)
@@ -2271,7 +2300,8 @@ static int max_row_length(const flgpu_mat *A) {
   if (A->nrows == 0) return 0;
   int *dm = (int *)dev_alloc(sizeof(int));
   FLGPU_HIP(hipMemsetAsync(dm, 0, sizeof(int), A->ctx->stream));
-  k_max_row<<<grid_for(A->nrows, 256), 256, 0, A->ctx->stream>>>(A->nrows, A->rowptr, dm);
+  const unsigned g = std::min<unsigned>(grid_for(A->nrows, 256), kMaxRowBlocks);
+  k_max_row<<<g, 256, 0, A->ctx->stream>>>(A->nrows, A->rowptr, dm);
   FLGPU_HIP(hipGetLastError());
   int h = 0;
   FLGPU_HIP(hipMemcpyAsync(&h, dm, sizeof(int), hipMemcpyDeviceToHost, A->ctx->stream));
@@ -2408,12 +2438,12 @@ void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac, boo
     const unsigned gl = (unsigned)((nc + kRapB - 1) / kRapB);
     if (q1 && A->prec == FLGPU_F64)
       k_rap_q1<double><<<grid_for(27L * nc, 256), 256, 0, ctx->stream>>>(
-          (int)ncl + 1, (int)nf + 1, A->rowptr, (const double *)A->vals, P->rowptr,
-          (const double *)P->vals, Ac->rowptr, acc, upper);
+          (int)ncl + 1, (int)nf + 1, (const double *)A->vals, (const double *)P->vals,
+          Ac->rowptr, acc, upper);
     else if (q1)
       k_rap_q1<float><<<grid_for(27L * nc, 256), 256, 0, ctx->stream>>>(
-          (int)ncl + 1, (int)nf + 1, A->rowptr, (const float *)A->vals, P->rowptr,
-          (const float *)P->vals, Ac->rowptr, acc, upper);
+          (int)ncl + 1, (int)nf + 1, (const float *)A->vals, (const float *)P->vals,
+          Ac->rowptr, acc, upper);
     else if (lds && A->prec == FLGPU_F64)
       k_rap_lds<double><<<gl, kRapB, 0, ctx->stream>>>(
           nc, PT->rowptr, PT->colidx, (const double *)PT->vals, A->rowptr, A->colidx,
@@ -2437,7 +2467,10 @@ void galerkin_product(const flgpu_mat *P, const flgpu_mat *A, flgpu_mat *Ac, boo
     FLGPU_HIP(hipGetLastError());
   };
   int hmiss = 1;
-  if (csr_symmetric(A)) {
+  // a lattice matrix's stencil form records whether its values are exactly symmetric (the
+  // same comparison of A(i, j) with A(j, i) over the same entries, made when it was filled)
+  const bool sym = (q1 && A->val27) ? A->st_sym : csr_symmetric(A);
+  if (sym) {
     FLGPU_HIP(hipMemsetAsync(missing, 0, sizeof(int), ctx->stream));
     rap(1);
     k_mirror<<<grid_for(nc, 128), 128, 0, ctx->stream>>>(nc, Ac->rowptr, Ac->colidx, acc,
