@@ -11,7 +11,11 @@ namespace flgpu {
 // system's precision and length) for the residual r it was computed from.
 struct LearnedHook {
   virtual ~LearnedHook() {}
-  virtual void correct(void *r, void *q) = 0;
+  // q holds the cycle's output z1 for the residual r.  The network's correction is added to q
+  // (on the device) when its energy-optimal step adds at least min_gain of the energy of the
+  // cycle's own step, z1 . r; the reduction slot `flag` receives 1.0 when it was added, else
+  // 0.0.  Slots 11..13 hold the step's dots.
+  virtual void correct(void *r, void *q, double min_gain, int flag) = 0;
 };
 
 // The hook of the network attached to mg (error if none is attached).

@@ -908,12 +908,14 @@ extern "C" int flgpu_mg_use_graphs(flgpu_mg *mg, int enable) {
 
 // ---- C-ABI: solver --------------------------------------------------------------------------
 
-// The learned correction is dropped when its step has added less than this fraction of the
-// preconditioned step's energy in this many FCG steps in a row.  One: the network is trained
-// on residuals after one V-cycle, which is what it sees in the first step; a network that adds
-// little there is not expected to add more later, and each application costs about a cycle.
+// The learned correction is added only when its energy-optimal step adds at least this fraction
+// of the energy of the cycle's own step (decided on the device, within the step), and the
+// first step where it does not drops the network for the rest of the solve: the network is
+// trained on residuals after one V-cycle, which is what it sees in the first step; a network
+// that adds little there is not expected to add more later, and each application costs about
+// a cycle.  The decision comes back in reduction slot kLearnedFlag.
 constexpr double kLearnedMinGain = 1e-2;
-constexpr int kLearnedWeakSteps = 1;
+constexpr int kLearnedFlag = 14;
 
 namespace {
 
@@ -955,14 +957,14 @@ static void precondition(flgpu_mg *mg, const flgpu_solve_params *p, SolveWork &W
                          void *q, LearnedHook *learned, bool staged = false) {
   if (!p->alias) {
     mg_run_cached(mg, staged ? 2 : 1, W.prec, q, r, W.w2);
-    if (learned) learned->correct(r, q);
+    if (learned) learned->correct(r, q, kLearnedMinGain, kLearnedFlag);
     return;
   }
   // femlisp's literal (iterate q w w): rhs and residual are one vector
   vcopy(W.ctx, W.N, W.prec, r, W.prec, W.w);
   vfill(W.ctx, W.N, W.prec, q, 0.0);
   mg_apply_cached(mg, W.prec, q, W.w, W.w);
-  if (learned) learned->correct(r, q);
+  if (learned) learned->correct(r, q, kLearnedMinGain, kLearnedFlag);
 }
 
 }  // namespace flgpu
@@ -1022,7 +1024,7 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
     double best_true = -1.0;
     int best_step = 0;
     int count = 0;  // <cg>'s steps for restart-cycle (krylow.lisp:101-105,128-131)
-    int weak_steps = 0;  // consecutive steps whose learned correction added little
+    bool learned_added = false;  // the network's correction has been added in some step
     // <safe-linear-solver> (linsolve.lisp:45-56,75-89): the <lu> fallback's inverse of A
     bool fallback_p = false;
     std::unique_ptr<double, void (*)(double *)> lu_inv(nullptr, [](double *q) { dev_free(q); });
@@ -1120,7 +1122,15 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             if (pc) precondition(mg, p, W, W.r, W.p, learned);
             else vcopy(ctx, N, prec, W.r, prec, W.p);
             vdot_dev(ctx, N, prec, W.p, W.r, 0);
-            fetch_slots(ctx, 1);
+            fetch_slots(ctx, learned ? kLearnedFlag + 1 : 1);
+            if (learned) {
+              // not added: p is the cycle's own output, so the recurrence goes on without it
+              if (ctx->host_out[kLearnedFlag] != 0.0) learned_added = true;
+              else {
+                learned = nullptr;
+                fell_back = 1;
+              }
+            }
             alpha = ctx->host_out[0];
             ca = 0;
             initialized = true;
@@ -1175,8 +1185,8 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             }
             FLGPU_HIP(hipGetLastError());
             ca = na;
-            // with the network on, its step's dots (slots 11, 12) come along
-            fetch_slots_async(ctx, learned ? 13 : 4);
+            // with the network on, its decision (kLearnedFlag) comes along
+            fetch_slots_async(ctx, learned ? kLearnedFlag + 1 : 4);
             if (!spmv_dot(A, W.p, W.a, 1)) {  // the next step's product, ahead of the wait
               spmv(A, 1.0, W.p, 0.0, W.a, W.a);
               vdot_dev(ctx, N, prec, W.a, W.p, 1);
@@ -1185,20 +1195,20 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
             fetch_wait(ctx);
             alpha = ctx->host_out[ca];
             if (learned) {
-              // the energy the network's step adds, (c.r1)^2 / (c.Ac), against that of the
-              // whole preconditioned step, z.r: after kLearnedWeakSteps steps in a row below
-              // kLearnedMinGain the network is dropped and GMG-FCG restarts from the residual
-              // at hand.  Keeping the direction formed with the network was measured worse
-              // (138 against 97 steps at 128^3): flexible CG tolerates the change of
-              // preconditioner, but that direction still slows the steps after it.
-              const double num = ctx->host_out[11], den = ctx->host_out[12];
-              const double gain = (den > 0.0 && alpha > 0.0) ? num * num / den / alpha : 0.0;
-              weak_steps = gain < kLearnedMinGain ? weak_steps + 1 : 0;
-              if (weak_steps >= kLearnedWeakSteps) {
+              if (ctx->host_out[kLearnedFlag] != 0.0) {
+                learned_added = true;
+              } else {
+                // the step went on with the cycle's own output; the network is dropped.  A
+                // direction formed with the network's correction in an earlier step is not
+                // kept: GMG-FCG restarts from the residual at hand (keeping it was measured
+                // worse, 138 against 97 steps at 128^3: flexible CG tolerates the change of
+                // preconditioner, but that direction still slows the steps after it).
                 learned = nullptr;
                 fell_back = 1;
-                initialized = false;
-                ap_ready = false;
+                if (learned_added) {
+                  initialized = false;
+                  ap_ready = false;
+                }
               }
             }
           } else {

@@ -789,22 +789,26 @@ __global__ void k_unpack(int n, const float *__restrict__ y, T *q) {
   q[idx] = interior ? (T)y[((long)zz * n + yy) * n + xx] : (T)0;
 }
 
-// q += beta * c with beta = (c . r1) / (c . A c) when the denominator is
-// positive (the energy-optimal step along c), else 0
+// q += beta c with beta = (c . r1) / (c . A c), the energy-optimal step along c, when that step
+// adds at least min_gain of the energy of the cycle's own step: (c . r1)^2 / (c . A c) >=
+// min_gain (z1 . r).  d = {c . r1, c . A c, z1 . r}; *flag = 1 when added, else 0.
 template <class T>
-__global__ void k_add_opt(long N, const double *num, const double *den, const T *c, T *q) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  double d = *den;
-  T beta = (T)((d > 0.0) ? (*num / d) : 0.0);
+__global__ void k_add_opt(long N, const double *__restrict__ d, double min_gain, double *flag,
+                          const T *c, T *q) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double num = d[0], den = d[1], zr = d[2];
+  const bool add = den > 0.0 && zr > 0.0 && num * num / den >= min_gain * zr;
+  if (i == 0) *flag = add ? 1.0 : 0.0;
+  if (i >= N || !add) return;
+  const T beta = (T)(num / den);
   q[i] += beta * c[i];
 }
 
 // The network's correction inside flexible CG.  After the V-cycle z1 = q, the residual
 // r1 = r - A z1 is taken to the level the network runs on (the cycle's own restrictions, in
 // the cycle's precision), the network maps it to an error estimate there, the cycle's
-// prolongations bring that back (c), and q += beta c with the energy-optimal beta.  On the
-// top level this is z = z1 + beta N(r1).
+// prolongations bring that back (c), and q += beta c with the energy-optimal beta, when that
+// step carries enough energy (k_add_opt).  On the top level this is z = z1 + beta N(r1).
 struct UnetHook : LearnedHook {
   flgpu_ctx *ctx;
   flgpu_mg *mg;
@@ -824,7 +828,7 @@ struct UnetHook : LearnedHook {
     for (void *p : lv) dev_free(p);
     dev_free(xin);
   }
-  void correct(void *r, void *q) override {
+  void correct(void *r, void *q, double min_gain, int flag) override {
     hipStream_t s = ctx->stream;
     FLGPU_REQUIRE(prec == FLGPU_F64, "the learned correction runs inside an FP64 outer iteration");
     spmv(A, -1.0, q, 1.0, r, r1);
@@ -857,10 +861,14 @@ struct UnetHook : LearnedHook {
       vcopy(ctx, N, mprec, lv[top - lc], FLGPU_F64, c);
     }
     spmv(A, 1.0, c, 0.0, Ac, Ac);
-    vdot_dev(ctx, N, prec, c, r1, 11);
-    vdot_dev(ctx, N, prec, c, Ac, 12);
-    k_add_opt<double><<<grid_for(N, 256), 256, 0, s>>>(N, ctx->red_out + 11, ctx->red_out + 12,
-                                                       (const double *)c, (double *)q);
+    {  // c . r1, c . A c and the cycle step's z1 . r, one pass
+      const void *dx[3] = {c, c, q}, *dy[3] = {r1, Ac, r};
+      const int slots[3] = {11, 12, 13};
+      vdot_multi_dev(ctx, N, prec, 3, dx, dy, slots);
+    }
+    k_add_opt<double><<<grid_for(N, 256), 256, 0, s>>>(N, ctx->red_out + 11, min_gain,
+                                                       ctx->red_out + flag, (const double *)c,
+                                                       (double *)q);
     FLGPU_HIP(hipGetLastError());
   }
 };

@@ -359,7 +359,9 @@ int flgpu_mg_reassemble(flgpu_mg *mg, const flgpu_vec *kappa, int nk);
  * accumulation on MFMA; Adam) mapping a residual (and log kappa) on the n^3
  * grid of lattice nodes with coordinates < n to an error estimate.  Inside
  * flexible CG (flgpu_solve with learned = 1) the preconditioner becomes
- * z = V(r) + beta N(r - A V(r)), beta minimising the energy along N. */
+ * z = V(r) + beta N(r - A V(r)), beta minimising the energy along N, when
+ * that step adds at least 1% of the energy of the cycle's own step; the first
+ * step where it does not goes on with z = V(r) and drops the network. */
 int flgpu_unet_create(flgpu_ctx *ctx, int cin, unsigned long seed,
                       flgpu_unet **out);
 int flgpu_unet_destroy(flgpu_unet *u);
@@ -414,8 +416,9 @@ typedef struct flgpu_solve_params {
   int learned;         /* 1: add the learned coarse correction (FCG only) */
   int stall_window;    /* learned: fall back to plain GMG-FCG when the true
                           residual has not dropped by 10x in this many steps, or
-                          as soon as the network's step has added less than 1% of the
-                          preconditioned step's energy (a restart from the residual) */
+                          as soon as the network's step would add less than 1% of the
+                          cycle step's energy (that step goes on without it; FCG restarts
+                          from the residual only if the network was added before) */
   int restart_cycle;   /* <cg>'s restart-cycle (krylow.lisp:68-70,101-105,128-131): 0 never;
                           else every restart_cycle CG/FCG steps the step leaves no
                           :residual-after, the loop recomputes r = b - A x and the next step

@@ -217,6 +217,33 @@ def test_learned_fcg_and_fallback():
     assert fb["fell_back"]
 
 
+def test_learned_correction_not_added_leaves_fcg_unchanged():
+    """A network whose step carries less than 1% of the cycle step's energy (here: all
+    parameters zero, so its correction is zero) is never added to the cycle's output and is
+    dropped in the first step without restarting FCG: the learned solve then equals plain FCG
+    with the same hierarchy bit for bit (steps, defect norms, solution)."""
+    say("learned correction not added")
+    n = 16
+    kap = _problem(n, 31337, "inclusions")
+    kv = GpuVector.from_array(kap.ravel(), F.F64)
+    cs = it.GeometricCS(it.Jacobi(0.9), 2, 2, 1, transfer="blackbox")
+    H = it.Hierarchy.lattice(cs, 3, 1, 1, 5, kappa=(kv, n))
+    A, b = H.top_matrix(), H.rhs
+    plain = it.solve(it.LinearSolver(it.FCG(cs), true_residual=True, failure_if=(">", "step", 200)),
+                     {"matrix": A, "rhs": b, "hierarchy": H})
+    net = learned.UNet(2, 7)
+    net.set_params(np.zeros(net.nparams))
+    learned.attach(H, net, kv, n)
+    lrn = it.solve(it.LinearSolver(it.FCG(cs), true_residual=True, learned=True,
+                                   stall_window=1000, failure_if=(">", "step", 200)),
+                   {"matrix": A, "rhs": b, "hierarchy": H})
+    assert plain["status"] == "success" and lrn["status"] == "success"
+    assert lrn["fell_back"]
+    assert lrn["steps"] == plain["steps"]
+    assert np.array_equal(np.asarray(lrn["defnorms"]), np.asarray(plain["defnorms"]))
+    assert np.array_equal(lrn["solution"].numpy(), plain["solution"].numpy())
+
+
 def test_learned_solves_reuse_the_hook():
     """The hierarchy keeps the learned correction's buffers between solves: two learned solves
     in a row give the same result bit for bit, a solve after another network is attached
