@@ -407,11 +407,33 @@ __device__ __forceinline__ float bf2f(short h) {
   return __uint_as_float(((unsigned)(unsigned short)h) << 16);
 }
 
+// v = h + m + l + e with BF16 parts, each the BF16 rounding of what the previous ones leave
+// (the remainders are exact in FP32), |e| <= 2^-25 |v|: the 24 bits of an FP32 value
+__device__ __forceinline__ void bf16_split3(float v, short &h, short &m, short &l) {
+  h = f2bf(v);
+  const float r = v - bf2f(h);
+  m = f2bf(r);
+  l = f2bf(r - bf2f(m));
+}
+
+// acc += A B for operands split in three BF16 parts: the six products down to 2^-16 of the
+// leading one, the smallest first (al bh, am bm, ah bl, am bh, ah bm, ah bh)
+__device__ __forceinline__ v4f mfma_bf16x3(v4s ah, v4s am, v4s al, v4s bh, v4s bm, v4s bl,
+                                           v4f acc) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bm, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(am, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bm, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+  return acc;
+}
+
 // BF16 MFMA 16x16x16 (FP32 accumulate): lane l supplies A[l%16][4(l/16)+v]
 // and B[4(l/16)+v][l%16] (v = 0..3); it receives D[4(l/16)+v][l%16].
-// Every operand is split into BF16 parts hi + lo (lo = the BF16 rounding of
-// the remainder) and hi*hi + hi*lo + lo*hi is accumulated: three BF16 MFMAs
-// give about 16 significant bits per product instead of 8.
+// Every operand is split into three BF16 parts (bf16_split3) and the six
+// products down to 2^-16 of the leading one are accumulated (mfma_bf16x3):
+// about 24 significant bits per product, as in FP32, instead of 8.
 __global__ __launch_bounds__(256) void k_elem_mfma_bf16(long ncells, int nq, int nloc2,
                                                         const double *__restrict__ S,
                                                         const double *__restrict__ Tt, float *E) {
@@ -425,19 +447,22 @@ __global__ __launch_bounds__(256) void k_elem_mfma_bf16(long ncells, int nq, int
     v4f acc = {0.f, 0.f, 0.f, 0.f};
     int bcol = n0 + i;
     for (int k0 = 0; k0 < nq; k0 += 16) {
-      v4s ah, al, bh, bl;
+      v4s ah, am, al, bh, bm, bl;
       for (int v = 0; v < 4; v++) {
         int k = k0 + 4 * kg + v;
         float av = (float)((arow < ncells && k < nq) ? S[arow * nq + k] : 0.0);
         float bv = (float)((k < nq && bcol < nloc2) ? Tt[(long)k * nloc2 + bcol] : 0.0);
-        ah[v] = f2bf(av);
-        al[v] = f2bf(av - bf2f(ah[v]));
-        bh[v] = f2bf(bv);
-        bl[v] = f2bf(bv - bf2f(bh[v]));
+        short h, m, l;
+        bf16_split3(av, h, m, l);
+        ah[v] = h;
+        am[v] = m;
+        al[v] = l;
+        bf16_split3(bv, h, m, l);
+        bh[v] = h;
+        bm[v] = m;
+        bl[v] = l;
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+      acc = mfma_bf16x3(ah, am, al, bh, bm, bl, acc);
     }
     for (int v = 0; v < 4; v++) {
       long row = c0 + 4 * kg + v;
@@ -570,14 +595,15 @@ __global__ __launch_bounds__(kAsmThreads) void k_asm_q1(int n, const double *__r
                                                 T *rhs) {
   // E in the precision the two-pass path stores it in: FP64 for FP64 matrices, FP32 otherwise;
   // the rows of E are kAsmEsW apart, so that the nodes of a wave, whose cells differ, read
-  // different banks.  S is kept in FP64 (FMA and FP64 MFMA) or as its BF16 high and low parts
-  // (BF16 MFMA), formed once per value.
+  // different banks.  S is kept in FP64 (FMA and FP64 MFMA) or as its three BF16 parts
+  // (BF16 MFMA, bf16_split3), formed once per value.
   constexpr bool kBF = MODE == ASM_MFMA_BF16;
-  constexpr int kSBytes = kBF ? kAsmNCP * 8 * 2 * (int)sizeof(short) : kAsmNCP * 8 * (int)sizeof(double);
+  constexpr int kSBytes = kBF ? kAsmNCP * 8 * 3 * (int)sizeof(short) : kAsmNCP * 8 * (int)sizeof(double);
   __shared__ __align__(16) unsigned char smem[kSBytes + kAsmNC * kAsmEsW * sizeof(T)];
   double(*Ss)[8] = (double(*)[8])smem;
-  short(*Sh)[8] = (short(*)[8])smem;                   // BF16 high parts
-  short(*Sl)[8] = (short(*)[8])(smem + kSBytes / 2);   // BF16 low parts
+  short(*Sh)[8] = (short(*)[8])smem;                       // BF16 parts: high
+  short(*Sm)[8] = (short(*)[8])(smem + kSBytes / 3);       // middle
+  short(*Sl)[8] = (short(*)[8])(smem + 2 * kSBytes / 3);   // low
   T(*Es)[kAsmEsW] = (T(*)[kAsmEsW])(smem + kSBytes);
   const int M = n + 1;
   const int nbpa = (M + kAsmB - 1) / kAsmB;
@@ -618,10 +644,11 @@ __global__ __launch_bounds__(kAsmThreads) void k_asm_q1(int n, const double *__r
       const int e = tid + u * kAsmThreads;
       const double sv = in[u] ? scale * kv[u] : 0.0;
       if (kBF) {
-        const float av = (float)sv;
-        const short h = f2bf(av);
+        short h, m, l;
+        bf16_split3((float)sv, h, m, l);
         Sh[e >> 3][e & 7] = h;
-        Sl[e >> 3][e & 7] = f2bf(av - bf2f(h));
+        Sm[e >> 3][e & 7] = m;
+        Sl[e >> 3][e & 7] = l;
       } else {
         Ss[e >> 3][e & 7] = sv;
       }
@@ -657,22 +684,23 @@ __global__ __launch_bounds__(kAsmThreads) void k_asm_q1(int n, const double *__r
           if (c0 + kg + 4 * v < kAsmNC) Es[c0 + kg + 4 * v][bcol] = (T)acc[v];
       }
     } else {
-      v4s bh, bl;
+      v4s bh, bm, bl;
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int k = 4 * kg + v;
-        const float bv = (float)(k < 8 ? Tt[k * 64 + bcol] : 0.0);
-        bh[v] = f2bf(bv);
-        bl[v] = f2bf(bv - bf2f(bh[v]));
+        short h, m, l;
+        bf16_split3((float)(k < 8 ? Tt[k * 64 + bcol] : 0.0), h, m, l);
+        bh[v] = h;
+        bm[v] = m;
+        bl[v] = l;
       }
       for (int c0 = 0; c0 < kAsmNCP; c0 += 16) {
         v4f acc = {0.f, 0.f, 0.f, 0.f};
         const v4s zero = {0, 0, 0, 0};  // k >= 8: the padding of K to 16
         const v4s ah = kg < 2 ? *(const v4s *)&Sh[c0 + i][4 * kg] : zero;
+        const v4s am = kg < 2 ? *(const v4s *)&Sm[c0 + i][4 * kg] : zero;
         const v4s al = kg < 2 ? *(const v4s *)&Sl[c0 + i][4 * kg] : zero;
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, acc, 0, 0, 0);
+        acc = mfma_bf16x3(ah, am, al, bh, bm, bl, acc);
         for (int v = 0; v < 4; v++)
           if (c0 + 4 * kg + v < kAsmNC) Es[c0 + 4 * kg + v][bcol] = (T)acc[v];
       }
