@@ -192,12 +192,13 @@ static double *dense_inverse_dev(const flgpu_mat *A) {
     FLGPU_HIP(hipGetLastError());
     std::vector<int> trivial(std::max(n, 1), 0);
     if (n > 0) {
-      int *dt = (int *)dev_alloc(sizeof(int) * n);
-      k_dense_trivial<<<n, 256, 0, s>>>(n, M0, dt);
+      std::unique_ptr<int, void (*)(int *)> dt((int *)dev_alloc(sizeof(int) * n),
+                                               [](int *q) { dev_free(q); });
+      k_dense_trivial<<<n, 256, 0, s>>>(n, M0, dt.get());
       FLGPU_HIP(hipGetLastError());
-      FLGPU_HIP(hipMemcpyAsync(trivial.data(), dt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+      FLGPU_HIP(hipMemcpyAsync(trivial.data(), dt.get(), sizeof(int) * n, hipMemcpyDeviceToHost,
+                               s));
       FLGPU_HIP(hipStreamSynchronize(s));
-      dev_free(dt);
     }
     for (int k = 0; k < n; k++) {
       if (trivial[k]) continue;
