@@ -915,7 +915,10 @@ extern "C" int flgpu_mg_use_graphs(flgpu_mg *mg, int enable) {
 // trained on residuals after one V-cycle, which is what it sees in the first step; a network
 // that adds little there is not expected to add more later, and each application costs about
 // a cycle.  The decision comes back in reduction slot kLearnedFlag.
-constexpr double kLearnedMinGain = 1e-2;
+#ifndef FLGPU_LEARNED_MIN_GAIN
+#define FLGPU_LEARNED_MIN_GAIN 1e-2
+#endif
+constexpr double kLearnedMinGain = FLGPU_LEARNED_MIN_GAIN;
 constexpr int kLearnedFlag = 14;
 
 namespace {
