@@ -255,6 +255,7 @@ void mat_finish(flgpu_mat *A) {
   if (is_q1_lattice(A)) {
     A->val27 = dev_alloc(prec_bytes(A->prec) * 27 * (size_t)A->nrows);
     stencil_refresh(A);
+    if (A->sor_strips == 0) sor_strips_alloc(A);
   }
 }
 
@@ -277,14 +278,20 @@ void mat_free(flgpu_mat *A) {
   delete A;
 }
 
+// the strip <sor> sweeps' boundary copies, once neither lattice form is left
+static void drop_sor_strips(flgpu_mat *A) {
+  if (A->alpha || A->val27) return;
+  dev_free(A->sor_ghost);
+  A->sor_ghost = nullptr;
+  A->sor_strips = A->sor_h = 0;
+}
+
 // values no longer follow the per-cell form (new values or another ordering)
 void drop_alpha(flgpu_mat *A) {
   if (A->alpha) layout_changed(A);
   dev_free(A->alpha);
   A->alpha = nullptr;
-  dev_free(A->sor_ghost);
-  A->sor_ghost = nullptr;
-  A->sor_strips = A->sor_h = 0;
+  drop_sor_strips(A);
 }
 
 void mat_refresh_values(flgpu_mat *A) {
@@ -1256,6 +1263,192 @@ __global__ __launch_bounds__(256) void k_st27_sor(int M, int cz, int cy, int cx,
   st27_sor_row<T, SYM>(M, cz, cy, cx, nx, ny, t, val27, omega, x, b, r);
 }
 
+// ---- the stencil-form <sor> sweep in plane strips --------------------------------------------
+// The decomposition of k_q1_sor_strips for matrices in 27-point stencil storage: one launch per
+// z parity sweeps the four in-plane colours of its planes (all of them, the boundary planes
+// included, since a stencil row is swept like any other), a block owns the whole lines
+// [y0, y1) of one plane and recomputes colour j on 3 - j lines beyond them, and the lines of a
+// neighbouring strip are read from the copies k_st27_sor_prep takes before the sweep.  Each row
+// is st27_row's sum in its order: b, then the terms k = 0..8 (plane z - 1), 9..17 (plane z),
+// 18..26 (plane z + 1), then the division by the diagonal entry.  The planes z - 1 and z + 1
+// have the other parity and do not change during the launch, so the first nine terms are
+// summed for every staged row before the colour phases, and the last nine are loaded from
+// device memory in the row's colour phase.  A term whose neighbour lies outside the lattice
+// takes the row's own x, as st27_row does, which is the staged value: it still holds x_i
+// before the row's update.
+template <class T>
+__global__ __launch_bounds__(256) void k_st27_sor_prep(int M, int H, int S, FastDiv fM,
+                                                       FastDiv fS1, const T *x, const T *b,
+                                                       T *gx, T *gb) {
+  const unsigned uM = (unsigned)M;
+  const unsigned nb = uM * (unsigned)(S - 1) * 8u * uM;  // boundary lines of every plane
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < nb; t += gridDim.x * blockDim.x) {
+    unsigned u = fdiv(t, fM);
+    const unsigned xc = t - u * uM, k = u & 7u;
+    u >>= 3;  // z (S - 1) + sb - 1
+    const unsigned z = S > 2 ? fdiv(u, fS1) : u;
+    const int sb = (int)(u - z * (unsigned)(S - 1)) + 1;
+    const int y = sb * H - 4 + (int)k;
+    if (y < 0 || y >= M) continue;
+    const unsigned g = (u * 8u + k) * uM + xc;  // the layout k_st27_sor_strips reads
+    const unsigned i = (z * uM + (unsigned)y) * uM + xc;
+    gx[g] = x[i];
+    gb[g] = b[i];
+  }
+}
+
+// rows per thread and load batch of the plane z - 1 terms
+template <class T> constexpr int sor_batch_p() { return sizeof(T) == 8 ? 2 : 4; }
+
+// st27_coef with 32-bit indices (27 N < 2^32) and every load issued (the neighbour's index
+// replaced by the row's own when the neighbour is outside the lattice), the value selected
+// afterwards
+template <class T, bool SYM>
+__device__ __forceinline__ T st27_coef_sel(const T *__restrict__ val27, unsigned N, int k,
+                                           unsigned i, int off, bool live) {
+  if (SYM && k < 13) {
+    const T v = val27[(unsigned)(26 - k) * N + (live ? i + (unsigned)off : i)];
+    return live ? v : (T)0;
+  }
+  return val27[(unsigned)k * N + i];
+}
+
+template <class T, bool SYM, int CAP>
+__global__ __launch_bounds__(1024) void k_st27_sor_strips(int M, int pz, int H, int S,
+                                                          int backwards, FastDiv fM, FastDiv fx0,
+                                                          FastDiv fx1,
+                                                          const T *__restrict__ val27, T omega,
+                                                          T *__restrict__ x, const T *b, T *r,
+                                                          const T *__restrict__ gx,
+                                                          const T *__restrict__ gb) {
+  __shared__ T lds[CAP];
+  T *xs = lds;               // x of plane z, line y0 - 4 + l
+  T *ps = xs + (H + 8) * M;  // b plus the plane z - 1 terms, line y0 - 3 + l
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+  const int s = (int)(lb % S);
+  const int z = pz + 2 * (int)(lb / S);
+  if (z > M - 1) return;  // the grid's padding (whole blocks)
+  const int y0 = s * H, y1 = min(M, y0 + H);
+  const int M2 = M * M;
+  const unsigned N = (unsigned)M2 * (unsigned)M, zb = (unsigned)z * (unsigned)M2;
+  // the copies of boundary s (lines y0 - 4 ..) and s + 1 (lines y1 - 4 ..); only read where
+  // that boundary exists
+  const unsigned ol = (unsigned)(z * (S - 1) + s - 1) * 8u * (unsigned)M, ou = ol + 8u * M;
+  const int nxs = (H + 8) * M;
+  for (int e0 = threadIdx.x; e0 < nxs; e0 += kSorBatch * blockDim.x) {
+    T xv[kSorBatch];
+#pragma unroll
+    for (int u = 0; u < kSorBatch; u++) {
+      const int e = e0 + u * blockDim.x;
+      xv[u] = (T)0;
+      if (e < nxs) {
+        const int l = (int)fdiv((unsigned)e, fM), xc = e - l * M, y = y0 - 4 + l;
+        if (y >= 0 && y < M) {
+          if (y < y0) xv[u] = gx[ol + (unsigned)((y - y0 + 4) * M + xc)];
+          else if (y >= y1) xv[u] = gx[ou + (unsigned)((y - y1 + 4) * M + xc)];
+          else xv[u] = x[zb + (unsigned)(y * M + xc)];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSorBatch; u++) {
+      const int e = e0 + u * blockDim.x;
+      if (e < nxs) xs[e] = xv[u];
+    }
+  }
+  __syncthreads();
+  // b plus the terms k = 0..8 for the rows of the lines [y0 - 3, y1 + 3)
+  const int nps = (H + 6) * M;
+  constexpr int BP = sor_batch_p<T>();
+  for (int e0 = threadIdx.x; e0 < nps; e0 += BP * blockDim.x) {
+    T acc[BP], av[BP][9], xn[BP][9];
+    int yq[BP], xq[BP];
+#pragma unroll
+    for (int u = 0; u < BP; u++) {
+      const int e = e0 + u * blockDim.x;
+      const int l = (int)fdiv((unsigned)min(e, nps - 1), fM);
+      xq[u] = min(e, nps - 1) - l * M;
+      const int y = y0 - 3 + l;
+      yq[u] = e < nps && y >= 0 && y < M ? y : -1;  // -1: no row
+      const int yc = min(max(y, 0), M - 1), xc = xq[u];
+      const unsigned i = zb + (unsigned)(yc * M + xc);
+      acc[u] = (T)0;
+      if (yq[u] >= 0) {
+        if (y < y0) acc[u] = gb[ol + (unsigned)((y - y0 + 4) * M + xc)];
+        else if (y >= y1) acc[u] = gb[ou + (unsigned)((y - y1 + 4) * M + xc)];
+        else acc[u] = b[i];
+      }
+      const bool ly[3] = {yc > 0, true, yc < M - 1}, lx[3] = {xc > 0, true, xc < M - 1};
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const int dy = k / 3 - 1, dx = k % 3 - 1;
+        const bool live = z > 0 && ly[dy + 1] && lx[dx + 1];
+        const int off = -M2 + dy * M + dx;
+        av[u][k] = st27_coef_sel<T, SYM>(val27, N, k, i, off, live);
+        xn[u][k] = x[live ? i + (unsigned)off : i];  // not used when the neighbour is outside
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < BP; u++) {
+      const int y = yq[u], xc = xq[u];
+      if (y < 0) continue;
+      const bool ly[3] = {y > 0, true, y < M - 1}, lx[3] = {xc > 0, true, xc < M - 1};
+      const T xi = xs[(y - y0 + 4) * M + xc];
+      T a = acc[u];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const int dy = k / 3 - 1, dx = k % 3 - 1;
+        const bool live = z > 0 && ly[dy + 1] && lx[dx + 1];
+        a += ((T)-1 * (live ? xn[u][k] : xi)) * av[u][k];
+      }
+      ps[e0 + u * blockDim.x] = a;
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < 4; j++) {
+    const int c = backwards ? 3 - j : j;
+    const int cy = c >> 1, cx = c & 1;
+    int lo = max(0, y0 - (3 - j));
+    const int hi = min(M, y1 + (3 - j));
+    lo += (lo - cy) & 1;  // the first line of parity cy
+    const int nyl = hi > lo ? (hi - lo + 1) / 2 : 0, nxl = (M - cx + 1) / 2;
+    const FastDiv fx = cx ? fx1 : fx0;
+    for (int e = threadIdx.x; e < nyl * nxl; e += blockDim.x) {
+      const int ey = nxl > 1 ? (int)fdiv((unsigned)e, fx) : e;
+      const int y = lo + 2 * ey, xc = cx + 2 * (e - ey * nxl);
+      const int li = (y - y0 + 4) * M + xc;
+      const unsigned i = zb + (unsigned)(y * M + xc);
+      const bool ly[3] = {y > 0, true, y < M - 1}, lx[3] = {xc > 0, true, xc < M - 1};
+      T av[18], xp[9];
+#pragma unroll
+      for (int k = 9; k < 27; k++) {
+        const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+        const bool live = (dz == 0 || z < M - 1) && ly[dy + 1] && lx[dx + 1];
+        const int off = dz * M2 + dy * M + dx;
+        av[k - 9] = st27_coef_sel<T, SYM>(val27, N, k, i, off, live);
+        if (dz > 0) xp[k - 18] = x[live ? i + (unsigned)off : i];  // not used when outside
+      }
+      const T xi = xs[li];
+      T acc = ps[(y - y0 + 3) * M + xc];
+#pragma unroll
+      for (int k = 9; k < 27; k++) {
+        const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+        const bool live = (dz == 0 || z < M - 1) && ly[dy + 1] && lx[dx + 1];
+        const T xk = dz == 0 ? xs[live ? li + dy * M + dx : li] : (live ? xp[k - 18] : xi);
+        acc += ((T)-1 * xk) * av[k - 9];
+      }
+      const T corr = acc / av[13 - 9];
+      const T xnew = xi + omega * corr;
+      xs[li] = xnew;
+      if (y >= y0 && y < y1) {
+        r[i] = corr;
+        x[i] = xnew;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 bool is_q1_lattice(const flgpu_mat *A) {
   return A->lat_dim == 3 && A->lat_order == 1 &&
          (long)A->nrows == (long)(A->lat_n + 1) * (A->lat_n + 1) * (A->lat_n + 1) &&
@@ -1325,6 +1518,7 @@ void drop_stencil(flgpu_mat *A) {
   dev_free(A->val27);
   A->val27 = nullptr;
   A->st_sym = false;
+  drop_sor_strips(A);
 }
 
 static Q1K q1k(const flgpu_mat *A) {
@@ -2034,6 +2228,38 @@ static void q1_sor_strips(const flgpu_mat *A, T omega, int backwards, T *x, cons
   }
 }
 
+// the stencil-form sweep in strips: the copies of the strip boundaries of every plane, then one
+// launch per z parity (the same strips as the per-cell sweep, whose LDS need is larger)
+template <class T, int CAP>
+static void st27_sor_strips(const flgpu_mat *A, T omega, int backwards, T *x, const T *b, T *r) {
+  const int M = A->lat_n + 1, S = A->sor_strips, H = A->sor_h;
+  FLGPU_REQUIRE((long)(2 * H + 14) * M <= CAP, "strip sweep: LDS size");
+  FLGPU_REQUIRE(27L * M * M * M < (1L << 32), "strip sweep: lattice too large");
+  T *gx = (T *)A->sor_ghost, *gb = gx ? gx + (long)M * (S - 1) * 8 * M : nullptr;
+  const FastDiv fM = fast_div((unsigned)M);
+  if (S > 1) {
+    const long prep = (long)M * (S - 1) * 8 * M;
+    const FastDiv fS1 = S > 2 ? fast_div((unsigned)(S - 1)) : FastDiv{1, 0, 0};
+    k_st27_sor_prep<T><<<grid_for(prep, 256), 256, 0, A->ctx->stream>>>(M, H, S, fM, fS1, x, b,
+                                                                        gx, gb);
+    FLGPU_HIP(hipGetLastError());
+  }
+  const FastDiv fx0 = fast_div((unsigned)((M + 1) / 2));
+  const FastDiv fx1 = M / 2 > 1 ? fast_div((unsigned)(M / 2)) : FastDiv{1, 0, 0};
+  for (int k = 0; k < 2; k++) {
+    const int pz = backwards ? 1 - k : k;
+    const int planes = (M - 1 - pz) / 2 + 1;
+    const unsigned g = xcd_grid((unsigned)(planes * S));
+    if (A->st_sym)
+      k_st27_sor_strips<T, true, CAP><<<g, 1024, 0, A->ctx->stream>>>(
+          M, pz, H, S, backwards, fM, fx0, fx1, (const T *)A->val27, omega, x, b, r, gx, gb);
+    else
+      k_st27_sor_strips<T, false, CAP><<<g, 1024, 0, A->ctx->stream>>>(
+          M, pz, H, S, backwards, fM, fx0, fx1, (const T *)A->val27, omega, x, b, r, gx, gb);
+    FLGPU_HIP(hipGetLastError());
+  }
+}
+
 void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const void *b, void *r) {
   if (A->alpha && A->sor_strips > 0) {
     if (A->prec == FLGPU_F64)
@@ -2042,6 +2268,15 @@ void sor_sweep(const flgpu_mat *A, double omega, int backwards, void *x, const v
     else
       q1_sor_strips<float, kSorCapF32>(A, (float)omega, backwards, (float *)x,
                                        (const float *)b, (float *)r);
+    return;
+  }
+  if (!A->alpha && A->val27 && A->sor_strips > 0 && 27L * A->nrows < (1L << 32)) {
+    if (A->prec == FLGPU_F64)
+      st27_sor_strips<double, kSorCapF64>(A, omega, backwards, (double *)x, (const double *)b,
+                                          (double *)r);
+    else
+      st27_sor_strips<float, kSorCapF32>(A, (float)omega, backwards, (float *)x,
+                                         (const float *)b, (float *)r);
     return;
   }
   if (A->alpha || A->val27) {
