@@ -119,8 +119,8 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int n, const double *__res
 // the <lu> fallback's system -- are SPD after the Dirichlet elimination), in place in the
 // textbook form: for each pivot k, row k is divided by a_kk (a_kk becomes 1 / a_kk) and
 // a_ik-multiples of it are subtracted from the other rows (a_ik becoming -a_ik / a_kk).
-// After the last pivot the array holds A^-1.  One launch per pivot reads one buffer and
-// writes the other, so that no element is read after it was overwritten.
+// After the last pivot the array holds A^-1.  A launch (one pivot, or two in a row) reads one
+// buffer and writes the other, so that no element is read after it was overwritten.
 
 __global__ void k_dense_zero(long nn, double *M) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,17 +136,48 @@ __global__ void k_dense_scatter(int n, const int *__restrict__ rowptr,
   for (int e = rowptr[r]; e < rowptr[r + 1]; e++) M[(long)r * n + colidx[e]] = (double)vals[e];
 }
 
+// entry b of the new row k, and entry (a, b) after the pivot k, from the values before it (one
+// expression each, shared by both kernels below so that they round alike)
+__device__ __forceinline__ double gj_row(double skb, double skk, bool b_is_k) {
+  return (b_is_k ? 1.0 : skb) / skk;
+}
+__device__ __forceinline__ double gj_entry(double sab, double sak, double rkb, bool a_is_k,
+                                           bool b_is_k) {
+  return a_is_k ? rkb : (b_is_k ? 0.0 : sab) - sak * rkb;
+}
+
 __global__ __launch_bounds__(256) void k_gauss_jordan(int n, int k, const double *__restrict__ S,
                                                       double *__restrict__ D, int *bad) {
   // n <= 8192: every index fits 32 bits
   unsigned idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (unsigned)n * (unsigned)n) return;
   unsigned i = idx / (unsigned)n, j = idx - i * (unsigned)n;
-  double piv = S[(long)k * n + k];
+  const double piv = S[(long)k * n + k];
   if (idx == 0 && piv == 0.0) *bad = 1;
-  double rk = (j == (unsigned)k ? 1.0 : S[(long)k * n + j]) / piv;  // the new row k
-  double v = (i == (unsigned)k) ? rk : (j == (unsigned)k ? 0.0 : S[idx]) - S[(long)i * n + k] * rk;
-  D[idx] = v;
+  const double rk = gj_row(S[(long)k * n + j], piv, j == (unsigned)k);  // the new row k
+  D[idx] = gj_entry(S[idx], S[(long)i * n + k], rk, i == (unsigned)k, j == (unsigned)k);
+}
+
+// Two pivots k1 then k2 in one launch: each entry forms the values of the first step it needs
+// (its own, its row's in column k2, and row k2's) with the first kernel's expressions, then
+// takes the second step, so the result equals two launches of k_gauss_jordan bit for bit.
+__global__ __launch_bounds__(256) void k_gauss_jordan2(int n, int k1, int k2,
+                                                       const double *__restrict__ S,
+                                                       double *__restrict__ D, int *bad) {
+  unsigned idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (unsigned)n * (unsigned)n) return;
+  const unsigned i = idx / (unsigned)n, j = idx - i * (unsigned)n;
+  const unsigned u1 = (unsigned)k1, u2 = (unsigned)k2;
+  const double p1 = S[(long)k1 * n + k1];
+  const double r1j = gj_row(S[(long)k1 * n + j], p1, j == u1);
+  const double r1k2 = gj_row(S[(long)k1 * n + k2], p1, false);
+  const double tij = gj_entry(S[idx], S[(long)i * n + k1], r1j, i == u1, j == u1);
+  const double tik2 = gj_entry(S[(long)i * n + k2], S[(long)i * n + k1], r1k2, i == u1, false);
+  const double tk2j = gj_entry(S[(long)k2 * n + j], S[(long)k2 * n + k1], r1j, false, j == u1);
+  const double p2 = gj_entry(S[(long)k2 * n + k2], S[(long)k2 * n + k1], r1k2, false, false);
+  if (idx == 0 && (p1 == 0.0 || p2 == 0.0)) *bad = 1;
+  const double r2j = gj_row(tk2j, p2, j == u2);
+  D[idx] = gj_entry(tij, tik2, r2j, i == u2, j == u2);
 }
 
 // trivial[k] = 1 when row k is the unit row e_k and column k is +0 apart from its diagonal
@@ -200,9 +231,16 @@ static double *dense_inverse_dev(const flgpu_mat *A) {
                                s));
       FLGPU_HIP(hipStreamSynchronize(s));
     }
-    for (int k = 0; k < n; k++) {
-      if (trivial[k]) continue;
-      k_gauss_jordan<<<grid_for(nn, 256), 256, 0, s>>>(n, k, M0, M1, bad);
+    std::vector<int> piv;  // the pivots that change anything, in order
+    for (int k = 0; k < n; k++)
+      if (!trivial[k]) piv.push_back(k);
+    size_t q = 0;
+    for (; q + 1 < piv.size(); q += 2) {  // two pivots per launch
+      k_gauss_jordan2<<<grid_for(nn, 256), 256, 0, s>>>(n, piv[q], piv[q + 1], M0, M1, bad);
+      std::swap(M0, M1);
+    }
+    if (q < piv.size()) {
+      k_gauss_jordan<<<grid_for(nn, 256), 256, 0, s>>>(n, piv[q], M0, M1, bad);
       std::swap(M0, M1);
     }
     FLGPU_HIP(hipGetLastError());
