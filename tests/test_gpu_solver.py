@@ -334,6 +334,33 @@ def test_cell_sor_sweeps_over_many_tiles(n, prec, form):
             assert relerr(rs[0].numpy(), rs[1].numpy()) < 10 * tol, (bw, alias)
 
 
+@pytest.mark.parametrize("n", [24, 40])
+@pytest.mark.parametrize("prec", [F.F64, F.F32])
+def test_stencil_sor_strips_nonsymmetric(prec, n):
+    """the stencil-form <sor> sweep in strips on a lattice matrix whose values are not
+    symmetric (all 27 planes streamed) equals the SELL colour sweep, forwards and backwards"""
+    say(f"stencil sor strips, nonsymmetric, n={n} prec={prec}")
+    rng = np.random.default_rng(77 + n)
+    A, b = lat.assemble(3, 1, n, prec=prec)
+    rp, ci, v = A.csr()
+    v = v * (1.0 + 0.1 * rng.uniform(-1, 1, v.size))  # unequal A(i, j) and A(j, i)
+    B, _ = lat.assemble(3, 1, n, prec=prec)
+    A.update_values(v)
+    B.update_values(v)
+    assert A.storage(F.STORAGE_STENCIL) == F.STORAGE_STENCIL and A.stencil_planes() == 27
+    assert B.storage(F.STORAGE_SELL) == F.STORAGE_SELL
+    tol = 1e-13 if prec == F.F64 else 2e-6
+    y = rng.uniform(-1, 1, A.nrows)
+    for bw in (0, 1):
+        xs = [GpuVector.from_array(y, prec) for _ in range(2)]
+        rs = [GpuVector(A.nrows, prec) for _ in range(2)]
+        bs = GpuVector.from_array(b.numpy(), prec)
+        for M, xv, rv in ((A, xs[0], rs[0]), (B, xs[1], rs[1])):
+            check(lib().flgpu_sor_iterate(M.h, 1.1, bw, xv.h, bs.h, rv.h))
+        assert relerr(xs[0].numpy(), xs[1].numpy()) < tol, bw
+        assert relerr(rs[0].numpy(), rs[1].numpy()) < 10 * tol, bw
+
+
 def _gather_host(E, n, M):
     """femlisp's scatter of element matrices (global-local-matrix-operation) restated: for every
     row P and lattice neighbour Q, the sum in ascending cell order, in FP64, of E over the cells
