@@ -96,7 +96,9 @@ static long ipow(long b, int e) {
 struct RefElement {
   int dim, order, s, nq, nl1, nloc, nloc2;
   double pts[8], wts[8];
-  std::vector<double> T;  // [nq][nloc*nloc]: w_q sum_m dphi_a/dx_m dphi_b/dx_m (unit cube)
+  // [nq + 1][nloc*nloc]: row q < nq: w_q sum_m dphi_a/dx_m dphi_b/dx_m (unit cube); row nq:
+  // the sum of rows 0 .. nq-1 in ascending q (the reference stiffness matrix)
+  std::vector<double> T;
   std::vector<double> m;  // [nloc]: sum_q w_q phi_a(q)  (load vector of f = 1)
 };
 
@@ -112,7 +114,7 @@ static RefElement ref_element(int dim, int order) {
   gauss_rule(R.s, R.pts, R.wts);
   double phi[8][8], dphi[8][8];
   for (int q = 0; q < R.s; q++) lagrange1d(order, R.pts[q], phi[q], dphi[q]);
-  R.T.assign((size_t)R.nq * R.nloc2, 0.0);
+  R.T.assign((size_t)(R.nq + 1) * R.nloc2, 0.0);
   R.m.assign(R.nloc, 0.0);
   std::vector<double> g((size_t)R.nloc * kMaxDim), v(R.nloc);
   for (int q = 0; q < R.nq; q++) {
@@ -140,6 +142,12 @@ static RefElement ref_element(int dim, int order) {
         R.T[(size_t)q * R.nloc2 + (size_t)a * R.nloc + b] = wq * gg;
       }
     }
+  }
+  R.T.resize((size_t)(R.nq + 1) * R.nloc2, 0.0);
+  for (int b = 0; b < R.nloc2; b++) {
+    double sum = 0.0;
+    for (int q = 0; q < R.nq; q++) sum += R.T[(size_t)q * R.nloc2 + b];
+    R.T[(size_t)R.nq * R.nloc2 + b] = sum;
   }
   return R;
 }
@@ -434,6 +442,11 @@ __device__ __forceinline__ v4f mfma_bf16x3(v4s ah, v4s am, v4s al, v4s bh, v4s b
 // Every operand is split into three BF16 parts (bf16_split3) and the six
 // products down to 2^-16 of the leading one are accumulated (mfma_bf16x3):
 // about 24 significant bits per product, as in FP32, instead of 8.
+// The contraction is E = S_0 K + sum_{q>0} (S_q - S_0) T_q, K = sum_q T_q (row nq of the
+// table; the same sum, regrouped): quadrature point 0 carries the cell's value against K and
+// the others only its variation, so a cell with one coefficient value gets E = S_0 K with
+// one rounding per entry, and the entries where K vanishes (the axis neighbours of 3D Q1)
+// are exact zeros instead of the FP32 accumulator's cancellation error.
 __global__ __launch_bounds__(256) void k_elem_mfma_bf16(long ncells, int nq, int nloc2,
                                                         const double *__restrict__ S,
                                                         const double *__restrict__ Tt, float *E) {
@@ -450,8 +463,11 @@ __global__ __launch_bounds__(256) void k_elem_mfma_bf16(long ncells, int nq, int
       v4s ah, am, al, bh, bm, bl;
       for (int v = 0; v < 4; v++) {
         int k = k0 + 4 * kg + v;
-        float av = (float)((arow < ncells && k < nq) ? S[arow * nq + k] : 0.0);
-        float bv = (float)((k < nq && bcol < nloc2) ? Tt[(long)k * nloc2 + bcol] : 0.0);
+        const double s0 = arow < ncells ? S[arow * nq] : 0.0;
+        float av = (float)((arow < ncells && k < nq) ? (k == 0 ? s0 : S[arow * nq + k] - s0)
+                                                     : 0.0);
+        float bv = (float)((k < nq && bcol < nloc2) ? Tt[(long)(k == 0 ? nq : k) * nloc2 + bcol]
+                                                    : 0.0);
         short h, m, l;
         bf16_split3(av, h, m, l);
         ah[v] = h;
@@ -644,8 +660,11 @@ __global__ __launch_bounds__(kAsmThreads) void k_asm_q1(int n, const double *__r
       const int e = tid + u * kAsmThreads;
       const double sv = in[u] ? scale * kv[u] : 0.0;
       if (kBF) {
+        // k_elem_mfma_bf16's operand: S_0 at q = 0, S_q - S_0 otherwise (the 8 values of a
+        // cell are in 8 consecutive lanes, q = lane mod 8)
+        const double s0 = __shfl(sv, (tid & 63) & ~7, 64);
         short h, m, l;
-        bf16_split3((float)sv, h, m, l);
+        bf16_split3((float)((e & 7) == 0 ? s0 : sv - s0), h, m, l);
         Sh[e >> 3][e & 7] = h;
         Sm[e >> 3][e & 7] = m;
         Sl[e >> 3][e & 7] = l;
@@ -689,7 +708,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_asm_q1(int n, const double *__r
       for (int v = 0; v < 4; v++) {
         const int k = 4 * kg + v;
         short h, m, l;
-        bf16_split3((float)(k < 8 ? Tt[k * 64 + bcol] : 0.0), h, m, l);
+        bf16_split3((float)(k < 8 ? Tt[(k == 0 ? 8 : k) * 64 + bcol] : 0.0), h, m, l);
         bh[v] = h;
         bm[v] = m;
         bl[v] = l;
@@ -935,11 +954,7 @@ static void lattice_set_alpha(flgpu_mat *A, const flgpu_vec *kappa, int nk) {
     return;
   }
   RefElement R = ref_element(3, 1);
-  for (int b = 0; b < 8; b++) {
-    double s = 0.0;
-    for (int q = 0; q < R.nq; q++) s += R.T[(size_t)q * R.nloc2 + b];
-    A->kt[b] = s;
-  }
+  for (int b = 0; b < 8; b++) A->kt[b] = R.T[(size_t)R.nq * R.nloc2 + b];  // sum_q T_q
   long ncells = (long)n * n * n;
   if (!A->alpha) {
     A->alpha = dev_alloc(prec_bytes(A->prec) * ncells);

@@ -79,6 +79,25 @@ def test_element_kernel_parity(dim, order, n):
     err = np.abs(E32 - E64).max() / np.abs(E64).max()
     say(f"  BF16x3 element error {err:.2e} of the largest entry")
     assert err <= 2e-6
+    # one coefficient value per cell: the contraction is S_0 K (quadrature point 0 against the
+    # reference stiffness K, the others against the zero variation), so every entry is within a
+    # few FP32 roundings of its own size, and the entries where K vanishes keep the FP64 sum's
+    # residue (about 1e-17 of the largest) instead of the FP32 accumulator's cancellation error
+    emax = np.abs(E64).max(axis=(1, 2), keepdims=True)
+    big = np.abs(E64) > 1e-12 * emax
+    zero = np.abs(E32) / emax
+    assert np.all(zero[~big] <= 1e-15), zero[~big].max()
+    rel = (np.abs(E32 - E64)[big] / np.abs(E64)[big]).max()
+    say(f"  per-entry relative error {rel:.2e}; {np.count_nonzero(~big)} vanishing entries, "
+        f"at most {zero[~big].max() if (~big).any() else 0.0:.1e} of the largest")
+    assert rel <= 1e-6
+    # coefficient varying inside the cells (4 samples per cell and axis): the bound above
+    kv = np.exp(rng.normal(size=(4 * n,) * dim))
+    E32 = lat.element_matrices(dim, order, n, kv, prec=F.F32, mode=F.ASM_MFMA)
+    E64 = lat.element_matrices(dim, order, n, kv, prec=F.F64, mode=F.ASM_F64)
+    err = np.abs(E32 - E64).max() / np.abs(E64).max()
+    say(f"  BF16x3 element error, varying coefficient: {err:.2e} of the largest entry")
+    assert err <= 2e-6
 
 
 @pytest.mark.parametrize("dim,order,nc", [(1, 1, 4), (2, 1, 4), (3, 1, 4), (2, 2, 3), (3, 2, 2)])
