@@ -543,6 +543,16 @@ struct Q1K {
   double k[8];
 };
 
+// The neighbours along one axis (one nonzero offset): the Q1 stiffness of a cube couples the
+// nodes of a common edge with exactly 0, and kt holds the FP64 Gauss sum's residue, about
+// 1e-17 of kt[0]: such an entry is 1e-17 of the row's diagonal, far below the FP32 resolution
+// (6e-8), so the FP32 per-cell passes skip those 6 terms; FP64 keeps them, as the assembled
+// values do.
+template <class T>
+__device__ __forceinline__ constexpr bool q1_axis(int dz, int dy, int dx) {
+  return sizeof(T) == 4 && (dz != 0) + (dy != 0) + (dx != 0) == 1;
+}
+
 // acc + sum over interior neighbours j of row i (ascending j) of (coef x_j) A_ij
 // with A_ij = kt[|dz|*4 + |dy|*2 + |dx|] * (sum of alpha over the cells shared
 // by i and j); *dii receives A_ii.  Row i = (z,y,x) must be interior.  All
@@ -574,6 +584,7 @@ __device__ __forceinline__ T q1_row(int M, int n, long i, int z, int y, int x,
 #pragma unroll
   for (int k = 0; k < 27; k++) {
     int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+    if (q1_axis<T>(dz, dy, dx)) continue;
     int pz0 = dz > 0 ? 1 : 0, pz1 = dz < 0 ? 0 : 1;
     int py0 = dy > 0 ? 1 : 0, py1 = dy < 0 ? 0 : 1;
     int px0 = dx > 0 ? 1 : 0, px1 = dx < 0 ? 0 : 1;
@@ -760,6 +771,7 @@ __global__ __launch_bounds__(256) void k_q1_zblock(int M, int n, const T *__rest
 #pragma unroll
     for (int k = 0; k < 27; k++) {
       int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+      if (q1_axis<T>(dz, dy, dx)) continue;
       T aij;
       if (dz != 0) {
         aij = pl[r + (dz > 0)][dy + 1][dx + 1];
@@ -1063,6 +1075,7 @@ __global__ __launch_bounds__(1024) void k_q1_sor_strips(int M, int n, int pz, in
           for (int q = q0; q <= q1; q++)
 #pragma unroll
             for (int c = c0c; c <= c1c; c++) sum += a[p][q][c];
+          if (q1_axis<T>(1, dy, dx)) continue;
           const T aij = (T)kt.k[4 + (dy != 0) * 2 + (dx != 0)] * sum;
           const bool live = lz[p] && ly[dy + 1] && lx[dx + 1];
           acc += ((T)-1 * xn[p][dy + 1][dx + 1]) * (live ? aij : (T)0);
@@ -1103,6 +1116,7 @@ __global__ __launch_bounds__(1024) void k_q1_sor_strips(int M, int n, int pz, in
           for (int q = q0; q <= q1; q++)
 #pragma unroll
             for (int cc = c0; cc <= c1; cc++) sum += be[q][cc];
+          if (q1_axis<T>(0, dy, dx)) continue;
           const T aij = (T)kt.k[(dy != 0) * 2 + (dx != 0)] * sum;
           if (k == 4) d = aij;
           const bool live = ly[dy + 1] && lx[dx + 1];
