@@ -31,12 +31,13 @@ PATH_DESC = {
                "coefficient problems, maps the residual left by the V-cycle, restricted to the "
                "32^3 level, to a correction there, prolongated and added along the energy-optimal "
                "step inside FP64 flexible CG; plain GMG-FCG takes over on stall",
-    "mixed-bb-jac": "FP32 damped-Jacobi V(2,2) V-cycle, omega 0.9, black-box transfers, Galerkin levels, "
-                    "in FP64 flexible CG",
-    "plain-bb-jac": "FP64 damped-Jacobi V(2,2) V-cycle, omega 0.9, black-box transfers, Galerkin levels, "
-                    "in CG",
-    "mixed-bb": "FP32 multicolour GS V(2,2) V-cycle, black-box transfers, in FP64 flexible CG",
-    "plain-bb": "FP64 multicolour GS V(2,2) V-cycle, black-box transfers, in CG",
+    "mixed-bb-jac": "FP32 damped-Jacobi V(2,2) V-cycle, omega 0.9, black-box transfers, Galerkin levels "
+                    "down to 16^3 and an exact solve there, in FP64 flexible CG",
+    "plain-bb-jac": "FP64 damped-Jacobi V(2,2) V-cycle, omega 0.9, black-box transfers, Galerkin levels "
+                    "down to 16^3 and an exact solve there, in CG",
+    "mixed-bb": "FP32 multicolour GS V(2,2) V-cycle, black-box transfers, exact 16^3 solve, in FP64 "
+                "flexible CG",
+    "plain-bb": "FP64 multicolour GS V(2,2) V-cycle, black-box transfers, exact 16^3 solve, in CG",
     "mixed": "FP32 multicolour GS V(2,2) V-cycle in FP64 flexible CG",
 }
 # the value path's dominant kernel as the profilers name it: the FP32 per-cell <jacobi> pass
@@ -185,7 +186,12 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="cells per axis (power of two)")
-    ap.add_argument("--base", type=int, default=3, help="coarse-solve level (2^base cells/axis)")
+    ap.add_argument("--base", type=int, default=3,
+                    help="coarse-solve level (2^base cells/axis) of the geometric paths (plain, "
+                         "mixed: femlisp's GMG, compared with the oracle at the same level)")
+    ap.add_argument("--bb-base", type=int, default=4,
+                    help="coarse-solve level of the black-box paths (-bb, -bb-jac, learned): the "
+                         "16^3 Galerkin level solved exactly (a 3375 x 3375 inverse)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-problems", type=int, default=2048,
@@ -470,10 +476,11 @@ def main():
         if name not in paths and not (name == "mixed-bb-jac" and "learned" in paths):
             continue
         t = time.perf_counter()
+        base = args.bb_base if tr == "blackbox" else args.base
         if sm == "jac":
-            cs = it.GeometricCS(it.Jacobi(0.9), 2, 2, args.base, prec=prec, transfer=tr)
+            cs = it.GeometricCS(it.Jacobi(0.9), 2, 2, base, prec=prec, transfer=tr)
         else:
-            cs = it.GeometricCS(it.SOR(1.0), 2, 2, args.base, symmetric=True, prec=prec,
+            cs = it.GeometricCS(it.SOR(1.0), 2, 2, base, symmetric=True, prec=prec,
                                 transfer=tr)
         hier[name] = (cs, it.Hierarchy.lattice(cs, 3, 1, n0, levels, kappa=(kv, n), mode=mode))
         if args.no_graphs:
@@ -689,7 +696,8 @@ def main():
         "dtype": "f64 outer / f32 V-cycle / bf16-mfma coarse assembly",
         "data": "synthetic: two-phase random inclusions (SplitMix64 seed 2024)",
         "config": {"workload": f"3D Q1 diffusion {n}^3 cells, N={N}, nnz={nnz}, contrast 1e4, "
-                               f"30% inclusions r 3-8 cells, base level {args.base}; value = "
+                               f"30% inclusions r 3-8 cells, base level {args.base} "
+                               f"(black-box paths {args.bb_base}); value = "
                                f"'{fast}' path ({PATH_DESC.get(fast, fast)}); plain = femlisp's "
                                f"FP64 GMG-CG, V(2,2) multicolour GS",
                    "global_batch": 1, "seq_len": N, "parallelism": "single GPU"},

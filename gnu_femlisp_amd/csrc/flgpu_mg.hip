@@ -25,6 +25,28 @@
 
 namespace flgpu {
 
+// A^-1 of a small square matrix, compressed: a DOF whose row is the unit row e_k and whose
+// column is zero apart from its diagonal (a Dirichlet DOF of a lattice matrix) is trivial, and
+// A^-1 keeps e_k there (x_k = r_k).  The other m DOFs (cidx, ascending) form the m x m matrix
+// C whose inverse is kept (FP64, row-major).
+struct DenseInverse {
+  double *inv = nullptr;
+  float *inv32 = nullptr;  // for an FP32 cycle: the FP64 inverse rounded to FP32 (inv freed)
+  int *cidx = nullptr;     // the m non-trivial DOFs
+  int *tidx = nullptr;     // the nt trivial DOFs
+  int m = 0, nt = 0;
+  void release() {
+    dev_free(inv);
+    dev_free(inv32);
+    dev_free(cidx);
+    dev_free(tidx);
+    inv = nullptr;
+    inv32 = nullptr;
+    cidx = tidx = nullptr;
+    m = nt = 0;
+  }
+};
+
 struct MGLevel {
   flgpu_mat *A = nullptr;  // owned
   flgpu_mat *I = nullptr;  // owned: interpolation level -> level+1 (transpose attached)
@@ -36,7 +58,7 @@ struct MGLevel {
                         // writes it instead of updating it
   bool step1 = false;   // with x_zero: tmp already holds the first <jacobi> step from zero
                         // (written by the pass that produced rhs)
-  double *cinv = nullptr;  // base level: dense inverse (FP64, row-major)
+  DenseInverse cinv;  // base level: dense inverse (FP64, compressed)
   long N = 0;
 };
 
@@ -80,47 +102,65 @@ struct flgpu_mg {
 
 namespace flgpu {
 
-// ---- coarse solve: x += A^-1 r (FP64 dense inverse) -----------------------------------------
-// One wave per row, its loads issued in batches of 16 per lane before the products are summed
-// (lane l sums j = l, l + 64, ... in order), then a wave reduction; lane 0 updates x.
+// ---- coarse solve: x += A^-1 r (FP64 dense inverse, compressed) ------------------------------
+// A block stages r at the m non-trivial DOFs in LDS (FP64), then each of its waves takes one
+// row: loads issued in batches of 16 per lane before the products are summed (lane l sums
+// j = l, l + 64, ... in order), a wave reduction, and lane 0 updates x at the row's DOF.  The
+// blocks after the dense rows take the trivial DOFs (x_k = r_k).
 // femlisp's <lu> iterate also leaves A^-1 r in r; nothing reads r afterwards (residual-p is
 // cleared), and r is read by every row of this pass, so it keeps the restricted residual.
-template <class T>
-__global__ __launch_bounds__(256) void k_coarse_solve(int n, const double *__restrict__ Ainv,
+template <class T, class TA, int U>
+__global__ __launch_bounds__(256) void k_coarse_solve(int m, const TA *__restrict__ Ainv,
+                                                      const int *__restrict__ cidx, int nt,
+                                                      const int *__restrict__ tidx,
                                                       const T *__restrict__ r, T *__restrict__ x,
                                                       int x_zero) {
+  extern __shared__ double rc[];  // m values
+  const int nbd = (m + 3) / 4;    // blocks of the dense rows
+  if ((int)blockIdx.x >= nbd) {
+    const int t = ((int)blockIdx.x - nbd) * 256 + (int)threadIdx.x;
+    if (t < nt) {
+      const int k = tidx[t];
+      const T v = r[k];
+      x[k] = x_zero ? v : x[k] + v;
+    }
+    return;
+  }
+  for (int j = threadIdx.x; j < m; j += 256) rc[j] = (double)r[cidx[j]];
+  __syncthreads();
   int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= n) return;
+  if (row >= m) return;
   int lane = threadIdx.x & 63;
-  constexpr int U = 16;  // one batch up to 1024 columns (the 8^3 base level: 729)
   double acc = 0.0;
-  for (int j0 = 0; j0 < n; j0 += 64 * U) {
-    double a[U], v[U];
+  for (int j0 = 0; j0 < m; j0 += 64 * U) {  // batches of 64 U columns
+    TA a[U];
     // every load unconditional (the index clamped into the row) so that all are in flight
     // together; the terms beyond the row are dropped from the sum, which keeps its order
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      int j = min(j0 + lane + 64 * u, n - 1);
-      a[u] = Ainv[(long)row * n + j];
-      v[u] = (double)r[j];
-    }
+    for (int u = 0; u < U; u++) a[u] = Ainv[(long)row * m + min(j0 + lane + 64 * u, m - 1)];
 #pragma unroll
-    for (int u = 0; u < U; u++) acc = (j0 + lane + 64 * u < n) ? acc + a[u] * v[u] : acc;
+    for (int u = 0; u < U; u++) {
+      const int j = j0 + lane + 64 * u;
+      acc = j < m ? acc + (double)a[u] * rc[j] : acc;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if (lane == 0) {
+    const int k = cidx[row];
     T v = (T)acc;
-    x[row] = x_zero ? v : x[row] + v;
+    x[k] = x_zero ? v : x[k] + v;
   }
 }
 
 // ---- dense FP64 inverse on the GPU -------------------------------------------------------
 // Gauss-Jordan elimination without pivoting (the matrices inverted here -- coarse levels and
-// the <lu> fallback's system -- are SPD after the Dirichlet elimination), in place in the
-// textbook form: for each pivot k, row k is divided by a_kk (a_kk becomes 1 / a_kk) and
-// a_ik-multiples of it are subtracted from the other rows (a_ik becoming -a_ik / a_kk).
-// After the last pivot the array holds A^-1.  A launch (one pivot, or two in a row) reads one
-// buffer and writes the other, so that no element is read after it was overwritten.
+// the <lu> fallback's system -- are SPD after the Dirichlet elimination) in the textbook form:
+// for each pivot k, row k is divided by a_kk (a_kk becomes 1 / a_kk) and a_ik-multiples of it
+// are subtracted from the other rows (a_ik becoming -a_ik / a_kk); after the last pivot the
+// array holds the inverse.  Blocked by kGjB pivots: with K the pivot block and D = C_KK^-1
+// (the unblocked elimination of the block alone), the pivots of K together take
+//   C_iK <- -C_iK D,  C_KJ <- D C_KJ,  C_iJ <- C_iJ - C_iK D C_KJ,  C_KK <- D
+// for rows i and columns J outside K, the rank-kGjB update on FP64 MFMA.
 
 __global__ void k_dense_zero(long nn, double *M) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -146,40 +186,6 @@ __device__ __forceinline__ double gj_entry(double sab, double sak, double rkb, b
   return a_is_k ? rkb : (b_is_k ? 0.0 : sab) - sak * rkb;
 }
 
-__global__ __launch_bounds__(256) void k_gauss_jordan(int n, int k, const double *__restrict__ S,
-                                                      double *__restrict__ D, int *bad) {
-  // n <= 8192: every index fits 32 bits
-  unsigned idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (unsigned)n * (unsigned)n) return;
-  unsigned i = idx / (unsigned)n, j = idx - i * (unsigned)n;
-  const double piv = S[(long)k * n + k];
-  if (idx == 0 && piv == 0.0) *bad = 1;
-  const double rk = gj_row(S[(long)k * n + j], piv, j == (unsigned)k);  // the new row k
-  D[idx] = gj_entry(S[idx], S[(long)i * n + k], rk, i == (unsigned)k, j == (unsigned)k);
-}
-
-// Two pivots k1 then k2 in one launch: each entry forms the values of the first step it needs
-// (its own, its row's in column k2, and row k2's) with the first kernel's expressions, then
-// takes the second step, so the result equals two launches of k_gauss_jordan bit for bit.
-__global__ __launch_bounds__(256) void k_gauss_jordan2(int n, int k1, int k2,
-                                                       const double *__restrict__ S,
-                                                       double *__restrict__ D, int *bad) {
-  unsigned idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (unsigned)n * (unsigned)n) return;
-  const unsigned i = idx / (unsigned)n, j = idx - i * (unsigned)n;
-  const unsigned u1 = (unsigned)k1, u2 = (unsigned)k2;
-  const double p1 = S[(long)k1 * n + k1];
-  const double r1j = gj_row(S[(long)k1 * n + j], p1, j == u1);
-  const double r1k2 = gj_row(S[(long)k1 * n + k2], p1, false);
-  const double tij = gj_entry(S[idx], S[(long)i * n + k1], r1j, i == u1, j == u1);
-  const double tik2 = gj_entry(S[(long)i * n + k2], S[(long)i * n + k1], r1k2, i == u1, false);
-  const double tk2j = gj_entry(S[(long)k2 * n + j], S[(long)k2 * n + k1], r1j, false, j == u1);
-  const double p2 = gj_entry(S[(long)k2 * n + k2], S[(long)k2 * n + k1], r1k2, false, false);
-  if (idx == 0 && (p1 == 0.0 || p2 == 0.0)) *bad = 1;
-  const double r2j = gj_row(tk2j, p2, j == u2);
-  D[idx] = gj_entry(tij, tik2, r2j, i == u2, j == u2);
-}
-
 // trivial[k] = 1 when row k is the unit row e_k and column k is +0 apart from its diagonal
 // (a Dirichlet row and column of a lattice matrix).  Such a pivot is skipped: row k stays e_k
 // under every pivot, every other row subtracts zero multiples of it, and its column keeps
@@ -199,20 +205,148 @@ __global__ __launch_bounds__(256) void k_dense_trivial(int n, const double *__re
   if (threadIdx.x == 0) trivial[k] = t ? 1 : 0;
 }
 
-// A^-1 of a square matrix of at most 8192 rows as a new n x n FP64 device array (row-major)
-static double *dense_inverse_dev(const flgpu_mat *A) {
+__global__ void k_dense_gather(int m, int n, const int *__restrict__ cidx,
+                               const double *__restrict__ M, double *__restrict__ C) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)m * m) return;
+  const int a = (int)(e / m), b = (int)(e - (long)a * m);
+  C[e] = M[(long)cidx[a] * n + cidx[b]];
+}
+
+constexpr int kGjB = 64;  // pivots per block
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// D = C_KK^-1 (bs x bs, bs <= kGjB, zero-padded to kGjB x kGjB) by the unblocked elimination
+// in LDS, pivot by pivot; one workgroup
+__global__ __launch_bounds__(256) void k_gj_diag(int m, int k0, int bs,
+                                                 const double *__restrict__ C,
+                                                 double *__restrict__ D, int *bad) {
+  __shared__ double sm[kGjB][kGjB + 1];
+  constexpr int kPer = kGjB * kGjB / 256;
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int e = threadIdx.x + 256 * u, a = e / kGjB, b = e % kGjB;
+    sm[a][b] = (a < bs && b < bs) ? C[(long)(k0 + a) * m + k0 + b] : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < bs; k++) {
+    const double piv = sm[k][k];
+    if (threadIdx.x == 0 && piv == 0.0) *bad = 1;
+    double nv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = threadIdx.x + 256 * u, a = e / kGjB, b = e % kGjB;
+      nv[u] = gj_entry(sm[a][b], sm[a][k], gj_row(sm[k][b], piv, b == k), a == k, b == k);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = threadIdx.x + 256 * u;
+      sm[e / kGjB][e % kGjB] = nv[u];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int e = threadIdx.x + 256 * u;
+    D[e] = sm[e / kGjB][e % kGjB];
+  }
+}
+
+// U (m x kGjB) = C_iK D for the rows i outside K and D's rows inside K; R (kGjB x m) = C_KJ
+// before the update (rows and columns beyond bs are zero)
+__global__ __launch_bounds__(256) void k_gj_panels(int m, int k0, int bs,
+                                                   const double *__restrict__ C,
+                                                   const double *__restrict__ D,
+                                                   double *__restrict__ U, double *__restrict__ R) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nu = (long)m * kGjB;
+  if (e < nu) {
+    const int i = (int)(e / kGjB), c = (int)(e % kGjB);
+    double v = 0.0;
+    if (i >= k0 && i < k0 + bs) {
+      v = D[(i - k0) * kGjB + c];
+    } else if (c < bs) {
+      for (int t = 0; t < bs; t++) v += C[(long)i * m + k0 + t] * D[t * kGjB + c];
+    }
+    U[e] = v;
+  } else if (e < 2 * nu) {
+    const long f = e - nu;
+    const int t = (int)(f / m), j = (int)(f - (long)t * m);
+    R[f] = t < bs ? C[(long)(k0 + t) * m + j] : 0.0;
+  }
+}
+
+// one kGjB x kGjB tile (I, J) of C per workgroup, K = the pivot block:
+//   J == K: C_iK = D_i (i in K) or -U_i (i outside K)
+//   J != K: C_iJ = D_i R_J (i in K) or C_iJ - U_i R_J (i outside K)
+// U_I R_J on FP64 MFMA 16x16x4 from LDS (lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and receives C[l / 16 + 4 v][l % 16]); wave w forms the 32 x 32 quarter
+// (w / 2, w % 2) of the tile
+__global__ __launch_bounds__(256) void k_gj_update(int m, int k0, const double *__restrict__ U,
+                                                   const double *__restrict__ R,
+                                                   double *__restrict__ C) {
+  __shared__ double us[kGjB][kGjB + 1];
+  __shared__ double rs[kGjB][kGjB + 2];
+  const int ntile = (m + kGjB - 1) / kGjB;
+  const int I = (int)blockIdx.x / ntile, J = (int)blockIdx.x % ntile, kb = k0 / kGjB;
+  const int i0 = I * kGjB, j0 = J * kGjB;
+  constexpr int kPer = kGjB * kGjB / 256;
+#pragma unroll
+  for (int u = 0; u < kPer; u++) {
+    const int e = threadIdx.x + 256 * u, a = e / kGjB, b = e % kGjB;
+    us[a][b] = i0 + a < m ? U[(long)(i0 + a) * kGjB + b] : 0.0;
+    rs[a][b] = j0 + b < m ? R[(long)a * m + j0 + b] : 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, kg = lane >> 4;
+  const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+  v4d acc[2][2];
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+#pragma unroll
+    for (int q = 0; q < 2; q++) acc[p][q] = v4d{0.0, 0.0, 0.0, 0.0};
+  if (J != kb) {
+    for (int k = 0; k < kGjB; k += 4)
+#pragma unroll
+      for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+          acc[p][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              us[r0 + 16 * p + li][k + kg], rs[k + kg][c0 + 16 * q + li], acc[p][q], 0, 0, 0);
+  }
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int a = r0 + 16 * p + kg + 4 * v, b = c0 + 16 * q + li;
+        const int i = i0 + a, j = j0 + b;
+        if (i < m && j < m) {
+          double out;
+          if (J == kb) out = I == kb ? us[a][b] : -us[a][b];
+          else out = I == kb ? acc[p][q][v] : C[(long)i * m + j] - acc[p][q][v];
+          C[(long)i * m + j] = out;
+        }
+      }
+}
+
+// A^-1 of a square matrix of at most 8192 rows, compressed
+static DenseInverse dense_inverse_dev(const flgpu_mat *A) {
   FLGPU_REQUIRE(A->nrows == A->ncols, "dense inverse of a non-square matrix");
   FLGPU_REQUIRE(A->nrows <= 8192, "matrix too large for the dense inverse (> 8192 rows)");
-  int n = A->nrows;
-  long nn = (long)n * n;
+  const int n = A->nrows;
+  const long nn = (long)n * n;
   hipStream_t s = A->ctx->stream;
-  double *M0 = (double *)dev_alloc(sizeof(double) * std::max<long>(nn, 1));
-  double *M1 = nullptr;
-  int *bad = nullptr;
+  DenseInverse out;
+  double *M0 = nullptr, *D = nullptr, *U = nullptr, *R = nullptr;
+  int *bad = nullptr, *dt = nullptr;
   try {
-    M1 = (double *)dev_alloc(sizeof(double) * std::max<long>(nn, 1));
-    bad = (int *)dev_alloc(sizeof(int));
-    FLGPU_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+    if (n == 0) return out;
+    M0 = (double *)dev_alloc(sizeof(double) * nn);
+    dt = (int *)dev_alloc(sizeof(int) * n);
     k_dense_zero<<<grid_for(nn, 256), 256, 0, s>>>(nn, M0);
     if (A->prec == FLGPU_F64)
       k_dense_scatter<double><<<grid_for(n, 256), 256, 0, s>>>(n, A->rowptr, A->colidx,
@@ -220,61 +354,112 @@ static double *dense_inverse_dev(const flgpu_mat *A) {
     else
       k_dense_scatter<float><<<grid_for(n, 256), 256, 0, s>>>(n, A->rowptr, A->colidx,
                                                               (const float *)A->vals, M0);
+    k_dense_trivial<<<n, 256, 0, s>>>(n, M0, dt);
     FLGPU_HIP(hipGetLastError());
-    std::vector<int> trivial(std::max(n, 1), 0);
-    if (n > 0) {
-      std::unique_ptr<int, void (*)(int *)> dt((int *)dev_alloc(sizeof(int) * n),
-                                               [](int *q) { dev_free(q); });
-      k_dense_trivial<<<n, 256, 0, s>>>(n, M0, dt.get());
-      FLGPU_HIP(hipGetLastError());
-      FLGPU_HIP(hipMemcpyAsync(trivial.data(), dt.get(), sizeof(int) * n, hipMemcpyDeviceToHost,
-                               s));
-      FLGPU_HIP(hipStreamSynchronize(s));
-    }
-    std::vector<int> piv;  // the pivots that change anything, in order
-    for (int k = 0; k < n; k++)
-      if (!trivial[k]) piv.push_back(k);
-    size_t q = 0;
-    for (; q + 1 < piv.size(); q += 2) {  // two pivots per launch
-      k_gauss_jordan2<<<grid_for(nn, 256), 256, 0, s>>>(n, piv[q], piv[q + 1], M0, M1, bad);
-      std::swap(M0, M1);
-    }
-    if (q < piv.size()) {
-      k_gauss_jordan<<<grid_for(nn, 256), 256, 0, s>>>(n, piv[q], M0, M1, bad);
-      std::swap(M0, M1);
-    }
-    FLGPU_HIP(hipGetLastError());
-    int hbad = 0;
-    FLGPU_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    std::vector<int> trivial(n, 0), piv, triv;
+    FLGPU_HIP(hipMemcpyAsync(trivial.data(), dt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     FLGPU_HIP(hipStreamSynchronize(s));
-    FLGPU_REQUIRE(hbad == 0, "matrix is singular (zero pivot in the dense inverse)");
+    for (int k = 0; k < n; k++) (trivial[k] ? triv : piv).push_back(k);
+    const int m = (int)piv.size();
+    out.m = m;
+    out.nt = (int)triv.size();
+    if (out.nt > 0) {
+      out.tidx = (int *)dev_alloc(sizeof(int) * out.nt);
+      FLGPU_HIP(hipMemcpyAsync(out.tidx, triv.data(), sizeof(int) * out.nt,
+                               hipMemcpyHostToDevice, s));
+    }
+    if (m > 0) {
+      const long mm = (long)m * m;
+      out.cidx = (int *)dev_alloc(sizeof(int) * m);
+      FLGPU_HIP(hipMemcpyAsync(out.cidx, piv.data(), sizeof(int) * m, hipMemcpyHostToDevice, s));
+      out.inv = (double *)dev_alloc(sizeof(double) * mm);
+      k_dense_gather<<<grid_for(mm, 256), 256, 0, s>>>(m, n, out.cidx, M0, out.inv);
+      FLGPU_HIP(hipGetLastError());
+      D = (double *)dev_alloc(sizeof(double) * kGjB * kGjB);
+      U = (double *)dev_alloc(sizeof(double) * (size_t)m * kGjB);
+      R = (double *)dev_alloc(sizeof(double) * (size_t)m * kGjB);
+      bad = (int *)dev_alloc(sizeof(int));
+      FLGPU_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+      const int ntile = (m + kGjB - 1) / kGjB;
+      for (int k0 = 0; k0 < m; k0 += kGjB) {
+        const int bs = std::min(kGjB, m - k0);
+        k_gj_diag<<<1, 256, 0, s>>>(m, k0, bs, out.inv, D, bad);
+        k_gj_panels<<<grid_for(2L * m * kGjB, 256), 256, 0, s>>>(m, k0, bs, out.inv, D, U, R);
+        k_gj_update<<<(unsigned)(ntile * ntile), 256, 0, s>>>(m, k0, U, R, out.inv);
+      }
+      FLGPU_HIP(hipGetLastError());
+      int hbad = 0;
+      FLGPU_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+      FLGPU_HIP(hipStreamSynchronize(s));
+      FLGPU_REQUIRE(hbad == 0, "matrix is singular (zero pivot in the dense inverse)");
+    }
+    FLGPU_HIP(hipStreamSynchronize(s));
   } catch (...) {
     dev_free(M0);
-    dev_free(M1);
+    dev_free(dt);
+    dev_free(D);
+    dev_free(U);
+    dev_free(R);
     dev_free(bad);
+    out.release();
     throw;
   }
-  dev_free(M1);
+  dev_free(M0);
+  dev_free(dt);
+  dev_free(D);
+  dev_free(U);
+  dev_free(R);
   dev_free(bad);
-  return M0;
+  return out;
 }
 
+// x = A^-1 r (x_zero) or x += A^-1 r
+static void dense_inverse_apply(hipStream_t s, const DenseInverse &D, int prec, const void *r,
+                                void *x, int x_zero) {
+  const unsigned g = (unsigned)((D.m + 3) / 4 + (D.nt + 255) / 256);
+  if (g == 0) return;
+  const size_t lds = sizeof(double) * (size_t)D.m;  // m <= 8192: at most 64 KiB
+  // one or two batches of loads per row: 64 registers of the inverse per lane
+  if (D.inv32) {
+    FLGPU_REQUIRE(prec == FLGPU_F32, "the FP32 copy of the inverse is for FP32 vectors");
+    k_coarse_solve<float, float, 64><<<g, 256, lds, s>>>(D.m, D.inv32, D.cidx, D.nt, D.tidx,
+                                                         (const float *)r, (float *)x, x_zero);
+  } else if (prec == FLGPU_F64) {
+    k_coarse_solve<double, double, 32><<<g, 256, lds, s>>>(D.m, D.inv, D.cidx, D.nt, D.tidx,
+                                                           (const double *)r, (double *)x,
+                                                           x_zero);
+  } else {
+    k_coarse_solve<float, double, 32><<<g, 256, lds, s>>>(D.m, D.inv, D.cidx, D.nt, D.tidx,
+                                                          (const float *)r, (float *)x,
+                                                          x_zero);
+  }
+  FLGPU_HIP(hipGetLastError());
+}
+
+__global__ void k_to_f32(long n, const double *__restrict__ a, float *__restrict__ b) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// the base level's inverse, computed in FP64; an FP32 cycle applies it rounded to FP32 (its
+// products summed in FP64), half the bytes per application
 static void coarse_setup(flgpu_mg *mg, MGLevel &L) {
   FLGPU_REQUIRE(L.A->nrows <= 8192,
                 "base level too large for the dense coarse solve (> 8192 DOFs)");
   L.cinv = dense_inverse_dev(L.A);
+  if (mg->p.prec == FLGPU_F32 && L.cinv.m > 0) {
+    const long mm = (long)L.cinv.m * L.cinv.m;
+    L.cinv.inv32 = (float *)dev_alloc(sizeof(float) * mm);
+    k_to_f32<<<grid_for(mm, 256), 256, 0, mg->ctx->stream>>>(mm, L.cinv.inv, L.cinv.inv32);
+    FLGPU_HIP(hipGetLastError());
+    FLGPU_HIP(hipStreamSynchronize(mg->ctx->stream));
+    dev_free(L.cinv.inv);
+    L.cinv.inv = nullptr;
+  }
 }
 
 static void coarse_apply(flgpu_mg *mg, MGLevel &L, void *x, void *r) {
-  int n = (int)L.N;
-  hipStream_t s = mg->ctx->stream;
-  if (mg->p.prec == FLGPU_F64)
-    k_coarse_solve<double><<<grid_for(n, 4), 256, 0, s>>>(n, L.cinv, (const double *)r,
-                                                          (double *)x, L.x_zero);
-  else
-    k_coarse_solve<float><<<grid_for(n, 4), 256, 0, s>>>(n, L.cinv, (const float *)r, (float *)x,
-                                                         L.x_zero);
-  FLGPU_HIP(hipGetLastError());
+  dense_inverse_apply(mg->ctx->stream, L.cinv, mg->p.prec, r, x, L.x_zero);
 }
 
 // ---- V-cycle ---------------------------------------------------------------------------------
@@ -664,7 +849,7 @@ void mg_free(flgpu_mg *mg) {
     dev_free(L.rhs);
     dev_free(L.res);
     dev_free(L.tmp);
-    dev_free(L.cinv);
+    L.cinv.release();
   }
   dev_free(mg->xs);
   dev_free(mg->bs);
@@ -859,8 +1044,7 @@ extern "C" int flgpu_mg_reassemble(flgpu_mg *mg, const flgpu_vec *kappa, int nk)
     }
     graphs_clear(mg);  // the coarse inverse is reallocated below
     MGLevel &B = mg->lv[mg->p.base_level];
-    dev_free(B.cinv);
-    B.cinv = nullptr;
+    B.cinv.release();
     coarse_setup(mg, B);
     FLGPU_HIP(hipStreamSynchronize(mg->ctx->stream));
     return 0;
@@ -1069,7 +1253,10 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
     bool learned_added = false;  // the network's correction has been added in some step
     // <safe-linear-solver> (linsolve.lisp:45-56,75-89): the <lu> fallback's inverse of A
     bool fallback_p = false;
-    std::unique_ptr<double, void (*)(double *)> lu_inv(nullptr, [](double *q) { dev_free(q); });
+    struct InverseHolder {
+      DenseInverse d;
+      ~InverseHolder() { d.release(); }
+    } lu_inv;
     for (;;) {
       // intermediate :before: defect and its norm (solve.lisp:139-155)
       if (!residual_p) {
@@ -1112,18 +1299,10 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
         }
         // the iterator becomes <lu>'s; the step count and the residual at hand stay
         fallback_p = true;
-        lu_inv.reset(dense_inverse_dev(A));
+        lu_inv.d = dense_inverse_dev(A);
       }
       if (fallback_p) {  // <lu> iterate (linit.lisp:150-160): x += A^-1 r, residual-after nil
-        if (prec == FLGPU_F64)
-          k_coarse_solve<double><<<grid_for(N, 4), 256, 0, s>>>((int)N, lu_inv.get(),
-                                                                (const double *)W.r,
-                                                                (double *)x->d, 0);
-        else
-          k_coarse_solve<float><<<grid_for(N, 4), 256, 0, s>>>((int)N, lu_inv.get(),
-                                                               (const float *)W.r,
-                                                               (float *)x->d, 0);
-        FLGPU_HIP(hipGetLastError());
+        dense_inverse_apply(s, lu_inv.d, prec, W.r, x->d, 0);
         residual_p = false;
         ap_ready = false;
         step++;

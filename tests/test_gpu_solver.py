@@ -9,6 +9,7 @@ import threading
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 from gnu_femlisp_amd import flgpu as F
 from gnu_femlisp_amd import iteration as it
@@ -359,6 +360,34 @@ def test_stencil_sor_strips_nonsymmetric(prec, n):
             check(lib().flgpu_sor_iterate(M.h, 1.1, bw, xv.h, bs.h, rv.h))
         assert relerr(xs[0].numpy(), xs[1].numpy()) < tol, bw
         assert relerr(rs[0].numpy(), rs[1].numpy()) < 10 * tol, bw
+
+
+@pytest.mark.parametrize("n,prec", [(16, F.F64), (16, F.F32), (7, F.F64)])
+def test_base_level_dense_solve(n, prec):
+    """a one-level hierarchy is its base-level solve: x = A^-1 b from the compressed dense
+    inverse (the Dirichlet DOFs kept as unit rows; at n = 16 the other 3375 DOFs are inverted in
+    53 blocks of 64 pivots, the last of 47; at n = 7, 216 in 4, the last of 24), against scipy's
+    sparse LU on the same matrix values"""
+    say(f"base-level dense solve n={n} prec={prec}")
+    rng = np.random.default_rng(n)
+    kap = np.exp(rng.normal(scale=1.5, size=(n,) * 3))
+    cs = it.GeometricCS(it.Jacobi(0.9), 2, 2, 0, prec=prec)
+    H = it.Hierarchy.lattice(cs, 3, 1, n, 1, kappa=kap)
+    rp, ci, v = H.top_matrix().csr()
+    M = sp.csr_matrix((v, ci, rp)).tocsc()
+    bh = rng.uniform(-1, 1, M.shape[0])
+    ref = spla.spsolve(M, bh.astype(np.float32).astype(np.float64) if prec == F.F32 else bh)
+    x = GpuVector(M.shape[0], prec)
+    b = GpuVector.from_array(bh, prec)
+    r = GpuVector.from_array(bh, prec)
+    H.apply(x, b, r)
+    assert relerr(x.numpy(), ref) < (1e-11 if prec == F.F64 else 1e-5)
+    # from x0 with r = b - A x0: x0 + A^-1 r is the solution again
+    x0 = rng.uniform(-1, 1, M.shape[0])
+    x = GpuVector.from_array(x0, prec)
+    r = GpuVector.from_array(bh - M @ x0, prec)
+    H.apply(x, b, r)
+    assert relerr(x.numpy(), ref) < (1e-11 if prec == F.F64 else 1e-5)
 
 
 def _gather_host(E, n, M):
