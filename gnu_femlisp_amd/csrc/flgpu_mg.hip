@@ -126,7 +126,21 @@ __global__ __launch_bounds__(256) void k_coarse_solve(int m, const TA *__restric
     }
     return;
   }
-  for (int j = threadIdx.x; j < m; j += 256) rc[j] = (double)r[cidx[j]];
+  // 16 values per thread and round, their index loads and then their value loads all in
+  // flight together (the index clamped into the list, the values beyond it not stored)
+  for (int j0 = 0; j0 < m; j0 += 256 * 16) {
+    int ci[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) ci[u] = cidx[min(j0 + (int)threadIdx.x + 256 * u, m - 1)];
+    T rv[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) rv[u] = r[ci[u]];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int j = j0 + (int)threadIdx.x + 256 * u;
+      if (j < m) rc[j] = (double)rv[u];
+    }
+  }
   __syncthreads();
   int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= m) return;
@@ -216,41 +230,46 @@ __global__ void k_dense_gather(int m, int n, const int *__restrict__ cidx,
 constexpr int kGjB = 64;  // pivots per block
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// D = C_KK^-1 (bs x bs, bs <= kGjB, zero-padded to kGjB x kGjB) by the unblocked elimination
-// in LDS, pivot by pivot; one workgroup
+// D = C_KK^-1 (bs x bs, bs <= kGjB, zero-padded to kGjB x kGjB) by the unblocked elimination,
+// pivot by pivot; one workgroup.  Thread t holds the entries (t / 64 + 4 u, t % 64), u < 16, in
+// registers; per pivot k the holders of column k and row k publish their values before it
+// (LDS, two buffers by the parity of k, so one barrier per pivot), and every thread forms its
+// new entries with gj_row / gj_entry.
 __global__ __launch_bounds__(256) void k_gj_diag(int m, int k0, int bs,
                                                  const double *__restrict__ C,
                                                  double *__restrict__ D, int *bad) {
-  __shared__ double sm[kGjB][kGjB + 1];
+  __shared__ double colk[2][kGjB], rowk[2][kGjB];
   constexpr int kPer = kGjB * kGjB / 256;
+  const int b = threadIdx.x % kGjB, a0 = threadIdx.x / kGjB;
+  double v[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; u++) {
-    const int e = threadIdx.x + 256 * u, a = e / kGjB, b = e % kGjB;
-    sm[a][b] = (a < bs && b < bs) ? C[(long)(k0 + a) * m + k0 + b] : 0.0;
+    const int a = a0 + 4 * u;
+    v[u] = (a < bs && b < bs) ? C[(long)(k0 + a) * m + k0 + b] : 0.0;
   }
-  __syncthreads();
   for (int k = 0; k < bs; k++) {
-    const double piv = sm[k][k];
+    const int par = k & 1;
+    if (b == k) {
+#pragma unroll
+      for (int u = 0; u < kPer; u++) colk[par][a0 + 4 * u] = v[u];
+    }
+    if (a0 == (k & 3)) {
+#pragma unroll
+      for (int u = 0; u < kPer; u++)
+        if (u == (k >> 2)) rowk[par][b] = v[u];
+    }
+    __syncthreads();
+    const double piv = colk[par][k];
     if (threadIdx.x == 0 && piv == 0.0) *bad = 1;
-    double nv[kPer];
+    const double rkb = gj_row(rowk[par][b], piv, b == k);
 #pragma unroll
     for (int u = 0; u < kPer; u++) {
-      const int e = threadIdx.x + 256 * u, a = e / kGjB, b = e % kGjB;
-      nv[u] = gj_entry(sm[a][b], sm[a][k], gj_row(sm[k][b], piv, b == k), a == k, b == k);
+      const int a = a0 + 4 * u;
+      v[u] = gj_entry(v[u], colk[par][a], rkb, a == k, b == k);
     }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < kPer; u++) {
-      const int e = threadIdx.x + 256 * u;
-      sm[e / kGjB][e % kGjB] = nv[u];
-    }
-    __syncthreads();
   }
 #pragma unroll
-  for (int u = 0; u < kPer; u++) {
-    const int e = threadIdx.x + 256 * u;
-    D[e] = sm[e / kGjB][e % kGjB];
-  }
+  for (int u = 0; u < kPer; u++) D[(a0 + 4 * u) * kGjB + b] = v[u];
 }
 
 // U (m x kGjB) = C_iK D for the rows i outside K and D's rows inside K; R (kGjB x m) = C_KJ
@@ -292,6 +311,22 @@ __global__ __launch_bounds__(256) void k_gj_update(int m, int k0, const double *
   const int I = (int)blockIdx.x / ntile, J = (int)blockIdx.x % ntile, kb = k0 / kGjB;
   const int i0 = I * kGjB, j0 = J * kGjB;
   constexpr int kPer = kGjB * kGjB / 256;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, kg = lane >> 4;
+  const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+  // this thread's 16 entries of the tile, loaded first (index clamped into C) so that their
+  // loads overlap the staging below
+  double cv[2][2][4];
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = min(i0 + r0 + 16 * p + kg + 4 * v, m - 1);
+        const int j = min(j0 + c0 + 16 * q + li, m - 1);
+        cv[p][q][v] = C[(long)i * m + j];
+      }
 #pragma unroll
   for (int u = 0; u < kPer; u++) {
     const int e = threadIdx.x + 256 * u, a = e / kGjB, b = e % kGjB;
@@ -299,9 +334,6 @@ __global__ __launch_bounds__(256) void k_gj_update(int m, int k0, const double *
     rs[a][b] = j0 + b < m ? R[(long)a * m + j0 + b] : 0.0;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int li = lane & 15, kg = lane >> 4;
-  const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
   v4d acc[2][2];
 #pragma unroll
   for (int p = 0; p < 2; p++)
@@ -327,7 +359,7 @@ __global__ __launch_bounds__(256) void k_gj_update(int m, int k0, const double *
         if (i < m && j < m) {
           double out;
           if (J == kb) out = I == kb ? us[a][b] : -us[a][b];
-          else out = I == kb ? acc[p][q][v] : C[(long)i * m + j] - acc[p][q][v];
+          else out = I == kb ? acc[p][q][v] : cv[p][q][v] - acc[p][q][v];
           C[(long)i * m + j] = out;
         }
       }
