@@ -154,8 +154,8 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partial_multi(long n, const
 
 // The CG/FCG step's dots q.r, r.r (and q.a for K = 3) with each vector loaded once: the same
 // products summed in the same order as k_dot_partial_multi over (q, r, q) x (r, r, a)
-template <typename T, int K>
-__global__ __launch_bounds__(kRedThreads) void k_dot_partial_qr(long n, const T *q, const T *r,
+template <typename T, int K, typename TQ = T>
+__global__ __launch_bounds__(kRedThreads) void k_dot_partial_qr(long n, const TQ *q, const T *r,
                                                                 const T *a, double *partial) {
   __shared__ double s[K][kRedThreads / 64];
   double acc[K];
@@ -164,7 +164,8 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partial_qr(long n, const T 
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long stride = (long)gridDim.x * blockDim.x;
   for (; i + 3 * stride < n; i += 4 * stride) {
-    T qv[4], rv[4], av[4];
+    TQ qv[4];
+    T rv[4], av[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       qv[u] = q[i + u * stride];
@@ -344,6 +345,19 @@ void vdot_multi_dev(flgpu_ctx *ctx, long n, int prec, int count, const void *con
   k_dot_final_multi<<<count, kRedBlocks, 0, st>>>(ctx->red_partial, kRedBlocks, ctx->red_out,
                                                   slots[0], count > 1 ? slots[1] : slots[0],
                                                   count > 2 ? slots[2] : slots[0]);
+  FLGPU_HIP(hipGetLastError());
+}
+
+// the FCG step's dots z.r, r.r, z.a with z in FP32 (an FP32 cycle's output) and r, a in FP64:
+// the same products, summed in the same order, as vdot_multi_dev over the FP64 copy of z
+void vdot_qr_f32(flgpu_ctx *ctx, long n, const float *z, const double *r, const double *a,
+                 const int *slots) {
+  for (int k = 0; k < 3; k++)
+    FLGPU_REQUIRE(slots[k] >= 0 && slots[k] < kRedSlots, "reduction slot out of range");
+  k_dot_partial_qr<double, 3, float><<<kRedBlocks, kRedThreads, 0, ctx->stream>>>(
+      n, z, r, a, ctx->red_partial);
+  k_dot_final_multi<<<3, kRedBlocks, 0, ctx->stream>>>(ctx->red_partial, kRedBlocks, ctx->red_out,
+                                                       slots[0], slots[1], slots[2]);
   FLGPU_HIP(hipGetLastError());
 }
 

@@ -287,6 +287,8 @@ void dot_final(flgpu_ctx *ctx, const double *partial, int np, int slot);
 // 1 to 3 dots in one pass, each identical to vdot_dev's
 void vdot_multi_dev(flgpu_ctx *ctx, long n, int prec, int count, const void *const *x,
                     const void *const *y, const int *slots);
+void vdot_qr_f32(flgpu_ctx *ctx, long n, const float *z, const double *r, const double *a,
+                 const int *slots);
 double vdot_host(flgpu_ctx *ctx, long n, int prec, const void *x,
                  const void *y);
 void fetch_slots(flgpu_ctx *ctx, int nslots);  // red_out[0..n) -> host_out

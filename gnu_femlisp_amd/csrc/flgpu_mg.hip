@@ -695,8 +695,9 @@ static bool mg_stages_first(const flgpu_mg *mg, int vprec) {
 
 // staged: the caller has already written bs (r converted) and the top level's tmp (the first
 // <jacobi> step from zero) as stage_first_step would (only when mg_stages_first)
+// keep32 (an FP32 cycle under FP64 vectors): V(b) is left in FP32 in mg->xs, q not written
 static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work,
-                          bool staged = false) {
+                          bool staged = false, bool keep32 = false) {
   const flgpu_mg_params &p = mg->p;
   MGLevel &T = mg->lv[mg->top];
   long N = T.N;
@@ -725,21 +726,22 @@ static void mg_apply_zero(flgpu_mg *mg, int vprec, void *q, void *r, void *work,
   if (zero) T.x_zero = true;
   else vfill(mg->ctx, N, p.prec, mg->xs, 0.0);
   if (reads_r) vcopy(mg->ctx, N, p.prec, mg->bs, p.prec, mg->rs);
-  mg->x64 = (vprec == FLGPU_F64) ? (double *)q : nullptr;
+  mg->x64 = (vprec == FLGPU_F64 && !keep32) ? (double *)q : nullptr;
   mg->x64_done = false;
   mg_cycle(mg, mg->xs, mg->bs, mg->rs);
-  if (!mg->x64_done) vcopy(mg->ctx, N, p.prec, mg->xs, vprec, q);
+  if (!keep32 && !mg->x64_done) vcopy(mg->ctx, N, p.prec, mg->xs, vprec, q);
   mg->x64 = nullptr;
   mg->x64_done = false;
 }
 
 static void mg_run(flgpu_mg *mg, int mode, int vprec, void *x, void *b, void *r) {
-  if (mode == 1 || mode == 2) mg_apply_zero(mg, vprec, x, b, r, mode == 2);
+  if (mode >= 1 && mode <= 4) mg_apply_zero(mg, vprec, x, b, r, mode == 2 || mode == 4, mode >= 3);
   else mg_apply_ptr(mg, vprec, x, b, r);
 }
 
 // mode 0: x <- x + V(r) on (x, b, r); mode 1: x <- V(b) from zero, b read only, r work; mode
-// 2: mode 1 with b already staged (mg_apply_zero's `staged`).
+// 2: mode 1 with b already staged (mg_apply_zero's `staged`); modes 3 and 4: modes 1 and 2
+// with V(b) left in FP32 in mg->xs (mg_apply_zero's `keep32`).
 // With graphs on, the first call on a (mode, x, b, r, precision) is captured and replayed.
 // Sum of the layout counts of every matrix a cycle reads.  The counts only grow, so the sum
 // moves whenever one of them does: a level matrix handed out by flgpu_mg_level_matrix whose
@@ -902,13 +904,13 @@ __global__ void k_cg_p(long n, const double *a, const double *b, T *p, const T *
 
 // p <- q + beta p, beta = -(q.a)/(a.p) (flexible CG): Polak-Ribiere's q.(r - r_old)/(q_old.r_old)
 // with r - r_old = -lam a and lam = (q_old.r_old)/(a.p), so that r_old need not be kept
-template <class T>
+template <class T, class TQ = T>
 __global__ void k_fcg_p(long n, const double *qa, const double *ap, T *p,
-                        const T *__restrict__ q) {
+                        const TQ *__restrict__ q) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   T beta = (T)(-*qa / *ap);
-  p[i] = q[i] + beta * p[i];
+  p[i] = (T)q[i] + beta * p[i];
 }
 
 // <cg>'s update in one pass: x <- x + lam p, r <- r - lam a with lam = num / den rounded to T
@@ -1212,9 +1214,11 @@ namespace flgpu {
 // preconditioner call of <cg> (krylow.lisp:118-121): (copy! r w) (x<-0 q)
 // (funcall iterate q w w); alias = 0 passes a second copy as the residual
 static void precondition(flgpu_mg *mg, const flgpu_solve_params *p, SolveWork &W, void *r,
-                         void *q, LearnedHook *learned, bool staged = false) {
+                         void *q, LearnedHook *learned, bool staged = false,
+                         bool keep32 = false) {
   if (!p->alias) {
-    mg_run_cached(mg, staged ? 2 : 1, W.prec, q, r, W.w2);
+    FLGPU_REQUIRE(!keep32 || !learned, "the learned correction reads q in FP64");
+    mg_run_cached(mg, keep32 ? (staged ? 4 : 3) : (staged ? 2 : 1), W.prec, q, r, W.w2);
     if (learned) learned->correct(r, q, kLearnedMinGain, kLearnedFlag);
     return;
   }
@@ -1413,15 +1417,26 @@ extern "C" int flgpu_solve(flgpu_mg *mg, const flgpu_mat *A, const flgpu_solve_p
                                                    (const float *)W.a, (float *)x->d,
                                                    (float *)W.r);
             FLGPU_HIP(hipGetLastError());
-            if (pc) precondition(mg, p, W, W.r, W.q, learned, staged);
+            // FCG with an FP32 cycle under FP64 vectors reads the cycle's output in FP32 (the
+            // same values as their FP64 copy, which is then not written)
+            const bool z32 = pc && fcg && !learned && !p->alias && prec == FLGPU_F64 &&
+                             mg->p.prec == FLGPU_F32;
+            if (pc) precondition(mg, p, W, W.r, W.q, learned, staged, z32);
             else vcopy(ctx, N, prec, W.r, prec, W.q);
             const int na = 2 - ca;  // the new alpha's slot
-            {  // new alpha = q.r, defect norm^2 = r.r, FCG's q.a: one pass
+            if (z32) {  // new alpha = z.r, defect norm^2 = r.r, FCG's z.a: one pass
+              const int slots[3] = {na, 3, 4};
+              vdot_qr_f32(ctx, N, (const float *)mg->xs, (const double *)W.r,
+                          (const double *)W.a, slots);
+            } else {  // new alpha = q.r, defect norm^2 = r.r, FCG's q.a: one pass
               const void *dx[3] = {W.q, W.r, W.q}, *dy[3] = {W.r, W.r, W.a};
               const int slots[3] = {na, 3, 4};
               vdot_multi_dev(ctx, N, prec, fcg ? 3 : 2, dx, dy, slots);
             }
-            if (fcg) {
+            if (z32) {
+              k_fcg_p<double, float><<<g, 256, 0, s>>>(N, slot + 4, slot + 1, (double *)W.p,
+                                                       (const float *)mg->xs);
+            } else if (fcg) {
               if (prec == FLGPU_F64)
                 k_fcg_p<double><<<g, 256, 0, s>>>(N, slot + 4, slot + 1, (double *)W.p,
                                                   (const double *)W.q);
