@@ -29,21 +29,31 @@ namespace flgpu {
 // column is zero apart from its diagonal (a Dirichlet DOF of a lattice matrix) is trivial, and
 // A^-1 keeps e_k there (x_k = r_k).  The other m DOFs (cidx, ascending) form the m x m matrix
 // C whose inverse is kept (FP64, row-major).
+// For an FP32 cycle the inverse is kept as FP32 64 x 64 tiles of its upper triangle (tiles
+// (I, J), I <= J, in the order of tI / tJ; inv freed), applied by two passes through part.
 struct DenseInverse {
   double *inv = nullptr;
-  float *inv32 = nullptr;  // for an FP32 cycle: the FP64 inverse rounded to FP32 (inv freed)
-  int *cidx = nullptr;     // the m non-trivial DOFs
-  int *tidx = nullptr;     // the nt trivial DOFs
+  float *tiles = nullptr;
+  int *tI = nullptr, *tJ = nullptr;  // the tiles' block row and column
+  double *part = nullptr;            // ntile x ntile x 64 partial products
+  int ntile = 0;
+  int *cidx = nullptr;  // the m non-trivial DOFs
+  int *tidx = nullptr;  // the nt trivial DOFs
   int m = 0, nt = 0;
   void release() {
     dev_free(inv);
-    dev_free(inv32);
+    dev_free(tiles);
+    dev_free(tI);
+    dev_free(tJ);
+    dev_free(part);
     dev_free(cidx);
     dev_free(tidx);
     inv = nullptr;
-    inv32 = nullptr;
+    tiles = nullptr;
+    tI = tJ = nullptr;
+    part = nullptr;
     cidx = tidx = nullptr;
-    m = nt = 0;
+    m = nt = ntile = 0;
   }
 };
 
@@ -445,18 +455,108 @@ static DenseInverse dense_inverse_dev(const flgpu_mat *A) {
   return out;
 }
 
+// The symmetric inverse from the tiles of its upper triangle, pass 1: one workgroup per tile
+// (I, J) stages it in LDS with r at its rows and columns, and writes T r_J (row sums) to
+// part[I][J] and, for I < J, T^T r_I (column sums) to part[J][I]; four threads per row or
+// column, 16 products each in order, then combined in a fixed order (deterministic)
+constexpr int kSymB = 64;
+template <class T>
+__global__ __launch_bounds__(256) void k_coarse_sym1(int m, int ntile,
+                                                     const float *__restrict__ tiles,
+                                                     const int *__restrict__ tI,
+                                                     const int *__restrict__ tJ,
+                                                     const int *__restrict__ cidx,
+                                                     const T *__restrict__ r,
+                                                     double *__restrict__ part) {
+  __shared__ float tl[kSymB][kSymB + 1];
+  __shared__ double rI[kSymB], rJ[kSymB];
+  const int q = blockIdx.x, I = tI[q], J = tJ[q];
+  const float *tq = tiles + (long)q * kSymB * kSymB;
+#pragma unroll
+  for (int u = 0; u < kSymB * kSymB / 256; u++) {
+    const int e = threadIdx.x + 256 * u;
+    tl[e / kSymB][e % kSymB] = tq[e];
+  }
+  if (threadIdx.x < kSymB) {
+    const int i = I * kSymB + threadIdx.x;
+    rI[threadIdx.x] = i < m ? (double)r[cidx[i]] : 0.0;
+  } else if (threadIdx.x < 2 * kSymB) {
+    const int j = J * kSymB + threadIdx.x - kSymB;
+    rJ[threadIdx.x - kSymB] = j < m ? (double)r[cidx[j]] : 0.0;
+  }
+  __syncthreads();
+  const int a = threadIdx.x >> 2, c = threadIdx.x & 3;
+  double sr = 0.0;
+#pragma unroll
+  for (int b = 16 * c; b < 16 * c + 16; b++) sr += (double)tl[a][b] * rJ[b];
+  sr += __shfl_xor(sr, 1, 64);
+  sr += __shfl_xor(sr, 2, 64);
+  if (c == 0) part[((long)I * ntile + J) * kSymB + a] = sr;
+  if (I < J) {
+    double sc = 0.0;
+#pragma unroll
+    for (int b = 16 * c; b < 16 * c + 16; b++) sc += (double)tl[b][a] * rI[b];
+    sc += __shfl_xor(sc, 1, 64);
+    sc += __shfl_xor(sc, 2, 64);
+    if (c == 0) part[((long)J * ntile + I) * kSymB + a] = sc;
+  }
+}
+
+// pass 2: row i of block row I sums part[I][0..ntile)[i mod 64] in order and updates x at its
+// DOF; the blocks after the dense rows take the trivial DOFs (x_k = r_k)
+template <class T>
+__global__ __launch_bounds__(256) void k_coarse_sym2(int m, int ntile,
+                                                     const double *__restrict__ part,
+                                                     const int *__restrict__ cidx, int nt,
+                                                     const int *__restrict__ tidx,
+                                                     const T *__restrict__ r, T *__restrict__ x,
+                                                     int x_zero) {
+  const int nbd = (m + 255) / 256;
+  if ((int)blockIdx.x >= nbd) {
+    const int t = ((int)blockIdx.x - nbd) * 256 + (int)threadIdx.x;
+    if (t < nt) {
+      const int k = tidx[t];
+      const T v = r[k];
+      x[k] = x_zero ? v : x[k] + v;
+    }
+    return;
+  }
+  const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (i >= m) return;
+  const double *pi = part + (long)(i / kSymB) * ntile * kSymB + i % kSymB;
+  // 16 partials per batch, their loads all in flight (the index clamped), summed in order
+  double acc = 0.0;
+  for (int J0 = 0; J0 < ntile; J0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = pi[(long)min(J0 + u, ntile - 1) * kSymB];
+#pragma unroll
+    for (int u = 0; u < 16; u++) acc = J0 + u < ntile ? acc + v[u] : acc;
+  }
+  const int k = cidx[i];
+  const T v = (T)acc;
+  x[k] = x_zero ? v : x[k] + v;
+}
+
 // x = A^-1 r (x_zero) or x += A^-1 r
 static void dense_inverse_apply(hipStream_t s, const DenseInverse &D, int prec, const void *r,
                                 void *x, int x_zero) {
+  if (D.tiles) {
+    FLGPU_REQUIRE(prec == FLGPU_F32, "the FP32 tiles of the inverse are for FP32 vectors");
+    const int nq = D.ntile * (D.ntile + 1) / 2;
+    k_coarse_sym1<float><<<nq, 256, 0, s>>>(D.m, D.ntile, D.tiles, D.tI, D.tJ, D.cidx,
+                                            (const float *)r, D.part);
+    const unsigned g2 = (unsigned)((D.m + 255) / 256 + (D.nt + 255) / 256);
+    k_coarse_sym2<float><<<g2, 256, 0, s>>>(D.m, D.ntile, D.part, D.cidx, D.nt, D.tidx,
+                                            (const float *)r, (float *)x, x_zero);
+    FLGPU_HIP(hipGetLastError());
+    return;
+  }
   const unsigned g = (unsigned)((D.m + 3) / 4 + (D.nt + 255) / 256);
   if (g == 0) return;
   const size_t lds = sizeof(double) * (size_t)D.m;  // m <= 8192: at most 64 KiB
   // one or two batches of loads per row: 64 registers of the inverse per lane
-  if (D.inv32) {
-    FLGPU_REQUIRE(prec == FLGPU_F32, "the FP32 copy of the inverse is for FP32 vectors");
-    k_coarse_solve<float, float, 64><<<g, 256, lds, s>>>(D.m, D.inv32, D.cidx, D.nt, D.tidx,
-                                                         (const float *)r, (float *)x, x_zero);
-  } else if (prec == FLGPU_F64) {
+  if (prec == FLGPU_F64) {
     k_coarse_solve<double, double, 32><<<g, 256, lds, s>>>(D.m, D.inv, D.cidx, D.nt, D.tidx,
                                                            (const double *)r, (double *)x,
                                                            x_zero);
@@ -468,25 +568,45 @@ static void dense_inverse_apply(hipStream_t s, const DenseInverse &D, int prec, 
   FLGPU_HIP(hipGetLastError());
 }
 
-__global__ void k_to_f32(long n, const double *__restrict__ a, float *__restrict__ b) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (float)a[i];
+// tile q = (I, J) of the upper triangle, rounded to FP32 (zero beyond m)
+__global__ void k_pack_sym(int m, const double *__restrict__ inv, const int *__restrict__ tI,
+                           const int *__restrict__ tJ, float *__restrict__ tiles) {
+  const int q = blockIdx.x;
+  const int I = tI[q], J = tJ[q];
+  for (int e = threadIdx.x; e < kSymB * kSymB; e += blockDim.x) {
+    const int i = I * kSymB + e / kSymB, j = J * kSymB + e % kSymB;
+    tiles[(long)q * kSymB * kSymB + e] = (i < m && j < m) ? (float)inv[(long)i * m + j] : 0.0f;
+  }
 }
 
-// the base level's inverse, computed in FP64; an FP32 cycle applies it rounded to FP32 (its
-// products summed in FP64), half the bytes per application
+// the base level's inverse, computed in FP64; an FP32 cycle applies it rounded to FP32 from
+// the tiles of its upper triangle (its products summed in FP64): a quarter of the bytes
 static void coarse_setup(flgpu_mg *mg, MGLevel &L) {
   FLGPU_REQUIRE(L.A->nrows <= 8192,
                 "base level too large for the dense coarse solve (> 8192 DOFs)");
   L.cinv = dense_inverse_dev(L.A);
-  if (mg->p.prec == FLGPU_F32 && L.cinv.m > 0) {
-    const long mm = (long)L.cinv.m * L.cinv.m;
-    L.cinv.inv32 = (float *)dev_alloc(sizeof(float) * mm);
-    k_to_f32<<<grid_for(mm, 256), 256, 0, mg->ctx->stream>>>(mm, L.cinv.inv, L.cinv.inv32);
+  DenseInverse &D = L.cinv;
+  if (mg->p.prec == FLGPU_F32 && D.m > 0) {
+    hipStream_t s = mg->ctx->stream;
+    D.ntile = (D.m + kSymB - 1) / kSymB;
+    std::vector<int> hI, hJ;
+    for (int I = 0; I < D.ntile; I++)
+      for (int J = I; J < D.ntile; J++) {
+        hI.push_back(I);
+        hJ.push_back(J);
+      }
+    const int nq = (int)hI.size();
+    D.tI = (int *)dev_alloc(sizeof(int) * nq);
+    D.tJ = (int *)dev_alloc(sizeof(int) * nq);
+    D.tiles = (float *)dev_alloc(sizeof(float) * (size_t)nq * kSymB * kSymB);
+    D.part = (double *)dev_alloc(sizeof(double) * (size_t)D.ntile * D.ntile * kSymB);
+    FLGPU_HIP(hipMemcpyAsync(D.tI, hI.data(), sizeof(int) * nq, hipMemcpyHostToDevice, s));
+    FLGPU_HIP(hipMemcpyAsync(D.tJ, hJ.data(), sizeof(int) * nq, hipMemcpyHostToDevice, s));
+    k_pack_sym<<<nq, 256, 0, s>>>(D.m, D.inv, D.tI, D.tJ, D.tiles);
     FLGPU_HIP(hipGetLastError());
-    FLGPU_HIP(hipStreamSynchronize(mg->ctx->stream));
-    dev_free(L.cinv.inv);
-    L.cinv.inv = nullptr;
+    FLGPU_HIP(hipStreamSynchronize(s));
+    dev_free(D.inv);
+    D.inv = nullptr;
   }
 }
 
