@@ -471,11 +471,17 @@ __global__ __launch_bounds__(256) void k_coarse_sym1(int m, int ntile,
   __shared__ float tl[kSymB][kSymB + 1];
   __shared__ double rI[kSymB], rJ[kSymB];
   const int q = blockIdx.x, I = tI[q], J = tJ[q];
-  const float *tq = tiles + (long)q * kSymB * kSymB;
+  const float4 *tq = (const float4 *)(tiles + (long)q * kSymB * kSymB);
+  float4 tv[kSymB * kSymB / 1024];  // 16-byte loads, all in flight
 #pragma unroll
-  for (int u = 0; u < kSymB * kSymB / 256; u++) {
-    const int e = threadIdx.x + 256 * u;
-    tl[e / kSymB][e % kSymB] = tq[e];
+  for (int u = 0; u < kSymB * kSymB / 1024; u++) tv[u] = tq[threadIdx.x + 256 * u];
+#pragma unroll
+  for (int u = 0; u < kSymB * kSymB / 1024; u++) {
+    const int e = 4 * (threadIdx.x + 256 * u), a = e / kSymB, b = e % kSymB;
+    tl[a][b] = tv[u].x;
+    tl[a][b + 1] = tv[u].y;
+    tl[a][b + 2] = tv[u].z;
+    tl[a][b + 3] = tv[u].w;
   }
   if (threadIdx.x < kSymB) {
     const int i = I * kSymB + threadIdx.x;
