@@ -101,7 +101,7 @@ struct flgpu_mg {
   // captured cycles: one HIP graph per (x, b, r, precision) of the caller
   struct Graph {
     void *x, *b, *r;
-    int vprec, mode;  // mode 0: x += V(r); mode 1: x = V(b) from zero
+    int vprec, mode;  // mg_run's mode: 0 x += V(r); 1-4 x = V(b) from zero (staged, FP32 kept)
     hipGraphExec_t exec;
   };
   std::vector<Graph> graphs;
